@@ -1,0 +1,173 @@
+"""Benchmark of the monodomain time step on one MI355X.
+
+Workload (default): the 1M-node TTP cube -- 101^3 nodes (1,030,301),
+6,000,000 Kuhn tetrahedra, edge 0.1 mm, TenTusscherPanfilov ENDO, float32
+storage with float64 PCG reductions, dt = 0.01 ms, fibres rotating -60..+60
+degrees about z, two conductivity regions split at x = 5 mm, 1,000-node
+corner stimulus (100 uA/cm^2, 0..1 ms), Jacobi-PCG atol = rtol = 1e-5,
+max_iter 100, linear guess on.  Synthetic mesh and inputs; random state is
+not used.
+
+A step is one time step of the whole path (ionic kernel, RHS, PCG).  W
+warm-up steps run first (they include the stimulus onset); K steps are timed
+between synchronisations.  value = nodes * K / elapsed (node-timesteps/s).
+
+Extra fields: per-kernel times measured with HIP events on the stream the
+kernels run on, the PCG algorithmic HBM rate, the roofline of the dominant
+kernel, and the CPU oracle timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "node-timesteps/s + PCG HBM GB/s (1M-node TTP); sims/hour & p99 lat batched"
+HBM_PEAK_GBPS = 8000.0
+
+
+def alg_bytes(n, nnz, S):
+    """Algorithmic HBM bytes per launch: FP32 values, int32 indices, FP32 vectors."""
+    return {
+        "ionic": n * 4 * (1 + S + S + 1) + 8 * n,
+        "rhs": 12 * nnz + 4 * (n + 1) + 12 * n,
+        "pcg_init": 8 * nnz + 4 * (n + 1) + 32 * n,
+        "pcg_spmv": 8 * nnz + 4 * (n + 1) + 20 * n,
+        "pcg_update": 28 * n,
+    }
+
+
+def build_sim(N, dev):
+    import torch
+    from torchcor_amd import meshes
+    from torchcor_amd.ionic import TenTusscherPanfilov
+    from torchcor_amd.simulator import Monodomain
+    nodes, elems, regions, fib = meshes.cube_workload(N)
+    sim = Monodomain(TenTusscherPanfilov("ENDO", 0.01, device=dev, dtype=torch.float32), T=100.0,
+                     dt=0.01, device=dev, dtype=torch.float32)
+    sim.load_arrays(nodes, elems, regions, fib)
+    sim.add_conductivity([0], 0.5272, 0.2076)
+    sim.add_conductivity([1], 0.9074, 0.3332)
+    sim.add_stimulus(meshes.corner_nodes(N, 10), 0.0, 1.0, 100)
+    return sim, (nodes, elems, regions, fib)
+
+
+def cpu_baseline(N, mesh, steps):
+    """The float64 CPU oracle on the same workload: `steps` time steps after its
+    own (untimed) assembly, single-threaded numpy/scipy."""
+    import numpy as np
+    from oracle import ionic as oion
+    from oracle import monodomain as omono
+    from torchcor_amd import meshes
+    nodes, elems, regions, fib = mesh
+    m = omono.Monodomain(oion.TenTusscherPanfilov("ENDO", 0.01), T=100.0, dt=0.01)
+    m.load_arrays(nodes, elems, regions, fib)
+    m.add_conductivity([0], 0.5272, 0.2076)
+    m.add_conductivity([1], 0.9074, 0.3332)
+    m.add_stimulus(meshes.corner_nodes(N, 10), 0.0, 1.0, 100)
+    print("[bench] cpu oracle: assembling", flush=True)
+    m.assemble()
+    m.assemble = lambda: None
+    print(f"[bench] cpu oracle: {steps} steps", flush=True)
+    t0 = time.perf_counter()
+    with np.errstate(all="ignore"):
+        m.solve(1e-5, 1e-5, 100, nsteps=steps)
+    el = time.perf_counter() - t0
+    return {"value": nodes.shape[0] * steps / el, "unit": "node-timesteps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{steps} time steps (t = 0.01..{steps * 0.01:.2f} ms) of the same "
+                      f"{N}^3 TTP workload in float64 numpy/scipy, assembly excluded; "
+                      f"{el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=101, help="nodes per cube side")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the simulator runs on a single MI355X).", flush=True)
+        sys.exit(2)
+
+    import torch
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    N = args.n
+    print(f"[bench] building {N}^3 workload", flush=True)
+    sim, mesh = build_sim(N, dev)
+    t0 = time.perf_counter()
+    sim.start(1e-5, 1e-5, 100, calculate_AT_RT=True, linear_guess=True,
+              snapshot_interval=1e9, verbose=False)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    n, nnz = sim.n_nodes, sim.system.nnz
+    print(f"[bench] assembled n={n} nnz={nnz} in {t_setup:.2f} s; warmup {args.warmup}", flush=True)
+    sim.advance(args.warmup)
+    torch.cuda.synchronize()
+    if not args.no_kernel_timing:
+        sim.set_kernel_timing(True)
+    it0 = len(sim.iterations)
+    print(f"[bench] timing {args.steps} steps", flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    iters = sim.iterations[it0:]
+    value = n * args.steps / elapsed
+    kt = {} if args.no_kernel_timing else sim.kernel_times()
+    S = len(sim.ionic_model.STATES)
+    B = alg_bytes(n, nnz, S)
+    kernels = {}
+    for k, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        avg_s = ms / cnt / 1e3
+        kernels[k] = {"launches": cnt, "avg_us": round(avg_s * 1e6, 2),
+                      "total_ms": round(ms, 2), "alg_bytes": B[k],
+                      "GBps": round(B[k] / avg_s / 1e9, 1)}
+    roof = None
+    pcg_rate = None
+    if kernels:
+        dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+        a = kernels[dom]["GBps"]
+        roof = {"kernel": dom, "bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(a / HBM_PEAK_GBPS, 4), "traffic": None}
+        ele = [k for k in ("rhs", "pcg_init", "pcg_spmv", "pcg_update") if k in kernels]
+        ele_bytes = sum(B[k] * kernels[k]["launches"] for k in ele)
+        ele_s = sum(kernels[k]["total_ms"] for k in ele) / 1e3
+        pcg_rate = round(ele_bytes / ele_s / 1e9, 1)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(N, mesh, args.cpu_steps)
+    line = {
+        "metric": METRIC, "value": value, "unit": "node-timesteps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"cube{N}_ttp_endo_f32: {n} nodes, {sim.elems.shape[0]} P1 tets, "
+                               f"nnz {nnz}, dt 0.01 ms, Jacobi-PCG tol 1e-5",
+                   "n_nodes": n, "nnz": nnz, "t_window_ms": [args.warmup * 0.01,
+                                                            (args.warmup + args.steps) * 0.01],
+                   "pcg_iterations_per_step": sum(iters) / max(len(iters), 1) + 1},
+        "pcg_hbm_GBps": pcg_rate,
+        "kernels": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "setup_s": round(t_setup, 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

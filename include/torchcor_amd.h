@@ -1,0 +1,248 @@
+/*
+ * torchcor_amd — C ABI of the MI355X (gfx950) monodomain time-step library.
+ *
+ * Drop-in boundary for the reference's hot path (sanpinnawala/torchcor):
+ * P1 FEM assembly into CSR, the Jacobi-PCG on A = Cm*M + theta*dt*K, and the
+ * fused ionic updates.  The reference is pure Python/PyTorch, so these entry
+ * points replace the PyTorch operations its classes call; each declaration
+ * names the reference function it stands in for.
+ *
+ * Conventions
+ *  - Every function returns 0 (TC_OK) or a negative tc_status; the text of
+ *    the last error on the calling thread is tc_last_error().
+ *  - Array arguments are DEVICE pointers owned by the caller (PyTorch
+ *    tensors' data_ptr()), except where a name ends in _host.
+ *  - `stream` is a hipStream_t passed as void*; work is enqueued
+ *    asynchronously on it.  Only the calls marked "synchronizes" wait.
+ *  - `dtype` selects the storage type of value/vector arrays:
+ *    TC_F32 (float) or TC_F64 (double).  Reductions are always float64.
+ *  - No function allocates device memory in the time-step calls; scratch
+ *    space is passed in as a caller-owned workspace sized by the matching
+ *    *_workspace_size function.
+ */
+#ifndef TORCHCOR_AMD_H
+#define TORCHCOR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum tc_status {
+  TC_OK = 0,
+  TC_ERR_ARG = -1,     /* invalid argument (null pointer, bad size/dtype) */
+  TC_ERR_HIP = -2,     /* HIP runtime or launch error */
+  TC_ERR_SIZE = -3,    /* workspace too small or size limit exceeded */
+  TC_ERR_STATE = -4    /* object used in the wrong state */
+};
+
+enum tc_dtype { TC_F32 = 0, TC_F64 = 1 };
+
+/* Text of the last error on this thread ("" if none). */
+const char* tc_last_error(void);
+/* ABI version of this library (bumped on any signature change). */
+int tc_abi_version(void);
+
+/* ------------------------------------------------------------------------
+ * CSR pattern of the P1 operators.
+ * Replaces the COO listing + coalesce of Matrices3D.assemble_matrices
+ * (torchcor/core/assemble.py:217-248, and :67-96 for triangles): the entries
+ * (elems[e][a], elems[e][b]) for a, b in 0..k-1 listed in the order
+ * (a, b, e) are sorted by (row, col) with a stable sort, so duplicates keep
+ * that order.  The result is the coalesced pattern with explicit zeros kept.
+ * ---------------------------------------------------------------------- */
+int tc_pattern_workspace_size(int64_t n_elems, int nodes_per_elem, size_t* bytes_host);
+/* Sorts the k*k*E entries into `ws`; returns the number of distinct
+ * (row, col) pairs in *nnz_host.  Synchronizes `stream`. */
+int tc_pattern_sort(const int32_t* elems, int64_t n_elems, int nodes_per_elem,
+                    int64_t n_nodes, void* ws, size_t ws_bytes, int64_t* nnz_host,
+                    void* stream);
+/* Writes row_ptr (n+1), col_idx (nnz), seg_ptr (nnz+1) and contrib (k*k*E):
+ * entry s of the CSR sums the element contributions contrib[seg_ptr[s] ..
+ * seg_ptr[s+1]-1], each an id (a*k+b)*E+e.  `ws` is the one given to
+ * tc_pattern_sort. */
+int tc_pattern_finish(int64_t n_elems, int nodes_per_elem, int64_t n_nodes, int64_t nnz,
+                      const void* ws, int32_t* row_ptr, int32_t* col_idx,
+                      int32_t* seg_ptr, int32_t* contrib, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Element matrices and their sum into the CSR values.
+ * Replaces Conductivity.calculate_sigma (core/conductivity.py:49-69) and
+ * Matrices3D.construct_local_matrices / Matrices3DSurface.local_stiffness
+ * (core/assemble.py:163-215, :99-145) plus the value half of
+ * assemble_matrices (:217-248).  Local matrices are formed in float64 from
+ * closed-form cofactors; with round_f32 != 0 each contribution is rounded to
+ * float32 before the sum, as the reference's Python-list COO path does
+ * (assemble.py:233-243).  Sums run in float64 in contrib order.
+ * nodes (n,3) f64 in mm; elems (E,k) int32, k = 4 (tet) or 3 (surface tri).
+ * The conductivity is either a full tensor per element, sigma (E,3,3) f64,
+ * or (sigma == NULL) formed on the fly as st*I + (sl-st) f f^T from
+ * sigma_l, sigma_t (E) f64 and fibres (E,3) f64.
+ * local_ws holds 2*k*k*E f64.
+ * ---------------------------------------------------------------------- */
+int tc_assemble_p1(const double* nodes, const int32_t* elems, int64_t n_elems,
+                   int nodes_per_elem, const double* sigma, const double* sigma_l,
+                   const double* sigma_t, const double* fibres, const int32_t* seg_ptr,
+                   const int32_t* contrib,
+                   int64_t nnz, double* local_ws, int round_f32, double* M_vals,
+                   double* K_vals, void* stream);
+
+/* Working operator values in `dtype` (monodomain.py:93-100,
+ * preconditioner.py:9-27): Kv = K/Chi, Mv = M, Av = Cm*Mv + theta*dt*Kv,
+ * Dinv = 1/diag(Av). */
+int tc_operator_values(const double* M_vals, const double* K_vals, const int32_t* row_ptr,
+                       const int32_t* col_idx, int64_t n, int64_t nnz, double Chi,
+                       double Cm, double theta, double dt, int dtype, void* Mv, void* Kv,
+                       void* Av, void* Dinv, void* stream);
+
+/* ------------------------------------------------------------------------
+ * SELL-64 layout: rows in slices of 64 (one wavefront), each slice padded
+ * to its longest row and stored column-major so that lane l of slice s reads
+ * entry j of row 64*s+l at slice_ptr[s] + 64*j + l.  Padding entries hold
+ * col = row and value 0.  The CSR stays the external format; SELL is the
+ * internal layout of the SpMV kernels.
+ * ---------------------------------------------------------------------- */
+typedef struct tc_sell {
+  int64_t n;              /* rows */
+  int64_t n_slices;       /* ceil(n/64) */
+  const int32_t* slice_ptr; /* n_slices+1 entry offsets */
+  const int32_t* col;     /* padded column indices */
+} tc_sell;
+
+/* Padded entry count; synchronizes `stream`.  slice_ptr (n_slices+1) is
+ * filled as a side effect. */
+int tc_sell_layout(const int32_t* row_ptr, int64_t n, int32_t* slice_ptr,
+                   int64_t* padded_host, void* stream);
+int tc_sell_columns(const int32_t* row_ptr, const int32_t* col_idx, int64_t n,
+                    const int32_t* slice_ptr, int32_t* sell_col, void* stream);
+int tc_sell_values(const int32_t* row_ptr, int64_t n, const int32_t* slice_ptr,
+                   const void* csr_vals, void* sell_vals, int dtype, void* stream);
+/* y = A x over a SELL operator (float64 accumulation), for tests and
+ * callers of Monodomain.M/K/A products. */
+int tc_sell_spmv(const tc_sell* A, const void* vals, const void* x, void* y, int dtype,
+                 void* stream);
+
+/* ------------------------------------------------------------------------
+ * Right-hand side (monodomain.py:130-131):
+ *   b = M w - coef * (K u),   coef = (1-theta)*dt.
+ * M and K share the pattern of A.
+ * ---------------------------------------------------------------------- */
+int tc_rhs(const tc_sell* A, const void* Mv, const void* Kv, const void* w, const void* u,
+           double coef, void* b, int dtype, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Jacobi-PCG (core/solver.py:5-76, core/preconditioner.py:29-43).
+ * A plan binds the operator (SELL pattern + A values + Dinv) and a
+ * caller-owned workspace (scalars, partial sums, r, p, p', Ap).
+ * ---------------------------------------------------------------------- */
+typedef struct tc_pcg tc_pcg;
+int tc_pcg_workspace_size(int64_t n, int dtype, size_t* bytes_host);
+int tc_pcg_create(tc_pcg** plan, const tc_sell* A, const void* Av, const void* Dinv,
+                  int dtype, void* ws, size_t ws_bytes);
+int tc_pcg_destroy(tc_pcg* plan);
+/* One solve of ConjugateGradient.solve (solver.py:40-76) with the reference
+ * stopping rule: after each update stop when ||Dinv r|| < a_tol or
+ * ||Dinv r|| / ||Dinv b|| < r_tol; *n_iter_host counts the iterations that
+ * did not stop (0 when the first iteration converges).
+ * With linear_guess != 0 the start is x_next = 2x - x_prev
+ * (solver.py:35-38) and the solution is left in x_next (the caller then
+ * treats x as the new x_prev and x_next as the new x); otherwise x is
+ * updated in place and x_prev/x_next are not touched.  Synchronizes
+ * `stream` (to return the iteration count). */
+int tc_pcg_solve(tc_pcg* plan, const void* b, void* x, const void* x_prev, void* x_next,
+                 double a_tol, double r_tol, int max_iter, int linear_guess,
+                 int* n_iter_host, void* stream);
+/* Per-kernel timing of tc_pcg_solve with HIP events on the solve's stream
+ * (for measurement; off by default).  set_timing resets the totals;
+ * get_timing returns milliseconds and launch counts for
+ * [0] the start pass, [1] the SpMV pass, [2] the update pass, counting only
+ * iterations that ran.  No counterpart in the reference. */
+int tc_pcg_set_timing(tc_pcg* plan, int on);
+int tc_pcg_get_timing(tc_pcg* plan, double* ms_host, int64_t* count_host);
+
+/* ------------------------------------------------------------------------
+ * Ionic models.  State arrays are one (S, n) row-major block in `dtype`
+ * (state k of node i at states[k*n + i]), in the order listed per model.
+ * Tables are built on the device in float64 on the reference's float32
+ * grids and stored in `dtype`.
+ *
+ * One step (the ionic half of Monodomain.step, monodomain.py:116-128, plus
+ * the stimulus of Stimuli.apply, stimulation.py:51-63):
+ *   dV   = -Iion(V, states)            (model.differentiate)
+ *   w    = Cm*V + dt_rhs*(dV/100) + dt_rhs*(Istim/100)   when w != NULL
+ *   Istim = sum over active stimuli k with node bit k set of amp[k]
+ *   states advance by dt from the old values.
+ * dV may be NULL; w may be NULL (then only dV and the states are written).
+ * ---------------------------------------------------------------------- */
+#define TC_MAX_STIMULI 32
+typedef struct tc_ionic_io {
+  int64_t n;
+  double dt;                 /* ionic model time step (ms): state advance */
+  double dt_rhs;             /* monodomain time step (ms) used in w */
+  double Cm;                 /* membrane capacitance for w (0.01) */
+  const void* V;             /* (n) membrane potential, mV */
+  void* w;                   /* (n) out, or NULL */
+  void* dV;                  /* (n) out, or NULL */
+  const uint32_t* stim_mask; /* (n) bit k = node in stimulus k, or NULL */
+  uint32_t stim_active;      /* bit k = stimulus k is on this step */
+  double stim_amp[TC_MAX_STIMULI]; /* intensity of stimulus k (uA/cm^2) */
+  /* Activation/repolarisation bookkeeping folded into the same pass
+   * (monodomain.py:174-176) for the PREVIOUS step's solution V at time
+   * lat_t: AT[(V>0)&(AT==-1)] = lat_t; RT[(AT>0)&(RT==-1)&(V<-70)] = lat_t.
+   * Disabled when AT == NULL. */
+  void* AT;
+  void* RT;
+  double lat_t;
+} tc_ionic_io;
+
+/* ModifiedMS2v (ionic/ModifiedMS2v.py:4-44).  States: H. */
+typedef struct tc_ms2v_params {
+  double tau_in, tau_out, tau_open, tau_close, u_gate, u_crit, vmin, vmax;
+} tc_ms2v_params;
+int tc_ms2v_step(const tc_ms2v_params* p, void* states, const tc_ionic_io* io, int dtype,
+                 void* stream);
+
+/* TenTusscherPanfilov (ionic/TenTusscherPanfilov.py:6-454).  States:
+ * CaSR CaSS Cai D F F2 FCaSS H J Ki M Nai R R_ S Xr1 Xr2 Xs.
+ * Tables: V (32000 x 27), CaSS (999999 x 3), VEk (32000 x 1). */
+typedef struct tc_ttp_params {
+  int endo;                 /* 1: ENDO tau_S branch (TenTusscherPanfilov.py:258) */
+  double GCaL, GKr, GKs, Gto;
+  double Bufc, Bufsr, Bufss, CAPACITANCE, Cao, D_CaL_off, Fconst, GK1, GNa, GbCa, GbNa,
+      GpCa, GpK, Kbufc, Kbufsr, Kbufss, KmCa, KmK, KmNa, KmNai, Ko, KpCa, Kup, Nao, Rconst,
+      T, Vc, Vleak, Vmaxup, Vrel, Vsr, Vss, knaca, knak, ksat, n, pKNa, scl_tau_f, vHalfXs,
+      xr2_off, EC, Vxfer, k1_, k2_, k3, k4, maxsr, minsr;
+} tc_ttp_params;
+int tc_ttp_tables(const tc_ttp_params* p, double dt, void* V_tab, void* CaSS_tab,
+                  void* VEk_tab, int dtype, void* stream);
+int tc_ttp_step(const tc_ttp_params* p, void* states, const void* V_tab,
+                const void* CaSS_tab, const void* VEk_tab, const tc_ionic_io* io, int dtype,
+                void* stream);
+
+/* CourtemancheRamirezNattel (ionic/CourtemancheRamirezNattel.py:5-444).
+ * States: Ca_rel Ca_up Cai Ki d f f_Ca h j m oa oi u ua ui v w xr xs.
+ * Tables: Cai (100000 x 5), V (4000 x 32), fn (60000 x 3). */
+typedef struct tc_crn_params {
+  double C_B1c, C_B1d, C_B1e, C_Fn1, C_Fn2, C_dCa_rel, C_dCaup, F, K_Q10, K_up, KmCa,
+      KmCmdn, KmCsqn, KmKo, KmNa3, KmNai, KmTrpn, Nai, R, T, Voli, gamma, k_rel, k_sat,
+      maxCmdn, maxTrpn, tau_f_Ca, tau_tr, tau_u, ACh, Cao, GACh, GCaL, GK1, GKr, GKs, GNa,
+      GbCa, GbNa, Gto, Ko, Nao, factorGKur, factorGrel, factorGtr, factorGup,
+      factorhGate, factormGate, factoroaGate, factorxrGate, maxCaup, maxINaCa, maxINaK,
+      maxIpCa, maxIup;
+} tc_crn_params;
+int tc_crn_tables(const tc_crn_params* p, double dt, void* Cai_tab, void* V_tab,
+                  void* fn_tab, int dtype, void* stream);
+int tc_crn_step(const tc_crn_params* p, void* states, const void* Cai_tab,
+                const void* V_tab, const void* fn_tab, const tc_ionic_io* io, int dtype,
+                void* stream);
+
+/* LAT/RT update alone (monodomain.py:174-176), for the last step of a run. */
+int tc_lat_update(const void* u, void* AT, void* RT, int64_t n, double t, int dtype,
+                  void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TORCHCOR_AMD_H */

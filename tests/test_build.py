@@ -1,0 +1,100 @@
+"""The C-ABI library loads and exports every function the header declares;
+host-side logic (mesh files, stimulus schedule) works without a GPU."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_header_symbols():
+    import torchcor_amd._lib as L
+    header = (ROOT / "include" / "torchcor_amd.h").read_text()
+    names = set(re.findall(r"^\s*(?:int|const char\*)\s+(tc_\w+)\s*\(", header, re.M))
+    assert len(names) >= 20
+    for nm in sorted(names):
+        assert hasattr(L.lib, nm), nm
+    assert set(L.EXPORTS) == names
+    assert L.lib.tc_abi_version() == 1
+
+
+def test_error_path_reports_text():
+    import ctypes as C
+    import torchcor_amd._lib as L
+    nb = C.c_size_t(0)
+    rc = L.lib.tc_pattern_workspace_size(0, 4, C.byref(nb))
+    assert rc == -1 and b"bad arguments" in L.lib.tc_last_error()
+    with pytest.raises(L.TorchcorError):
+        L.call("tc_pattern_workspace_size", 10, 5, C.byref(nb))
+
+
+def test_no_cpu_fallback():
+    import torch
+    import torchcor_amd._lib as L
+    with pytest.raises(L.TorchcorError):
+        L.ptr(torch.zeros(3))
+
+
+def test_alias_package_surface():
+    import torchcor as tc
+    from torchcor.simulator import Monodomain  # noqa: F401
+    from torchcor.simulator.monodomain import Monodomain as M2  # noqa: F401
+    from torchcor.ionic import CourtemancheRamirezNattel, ModifiedMS2v, TenTusscherPanfilov  # noqa: F401
+    assert tc.float32 is not None and callable(tc.get_device)
+    m = TenTusscherPanfilov("ENDO", 0.01)
+    assert m.GKs_init == 0.392 and m.Gto_init == 0.073
+    m = TenTusscherPanfilov("EPI", 0.01)
+    assert m.Gto_init == 0.294
+    ms = ModifiedMS2v(0.01)
+    ms.u_gate = 0.1
+    assert ms.u_gate == 0.1 and ms.DV == 100.0
+
+
+def test_mesh_files_round_trip(tmp_path):
+    from torchcor_amd import meshes
+    from torchcor_amd.core.mesh import MeshReader
+    nodes, elems = meshes.kuhn_cube(4)
+    regions = np.arange(len(elems)) % 3
+    fib = meshes.rotating_fibres(nodes, elems)
+    meshes.write_mesh(tmp_path, nodes, elems, regions, fib)
+    n2, e2, r2, f2 = MeshReader(tmp_path).read(unit_conversion=1000)
+    assert np.allclose(n2, nodes) and np.array_equal(e2, elems)
+    assert np.array_equal(r2, regions) and np.allclose(f2, fib)
+
+
+def test_mesh_count_mismatch_raises(tmp_path):
+    from torchcor_amd import meshes
+    from torchcor_amd.core.mesh import MeshReader
+    nodes, elems = meshes.kuhn_cube(3)
+    meshes.write_mesh(tmp_path, nodes, elems, np.zeros(len(elems), int), np.zeros((len(elems), 3)))
+    pts = tmp_path / "mesh.pts"
+    lines = pts.read_text().splitlines()
+    pts.write_text("\n".join([str(len(nodes) + 1)] + lines[1:]) + "\n")
+    with pytest.raises(ValueError):
+        MeshReader(tmp_path).read()
+
+
+def test_stimulus_schedule_and_vtx(tmp_path):
+    import torch
+    from torchcor_amd import meshes
+    from torchcor_amd.core.stimulation import Stimuli, load_stimulus_region
+    p = meshes.write_vtx(tmp_path / "s.vtx", [1, 3, 5])
+    assert load_stimulus_region(p).tolist() == [1, 3, 5]
+    bad = tmp_path / "bad.vtx"
+    bad.write_text("4\nextra\n1\n2\n")
+    with pytest.raises(Exception):
+        load_stimulus_region(bad)
+    s = Stimuli(10, torch.device("cpu"), torch.float32)
+    s.add(p, 0.0, 1.0, 100.0, period=50.0)
+    s.add([0, 1], 5.0, 1.0, 30.0, period=50.0)
+    with pytest.raises(Exception):
+        s.add(p, 0.0, 1.0, 1.0, period=None)
+    assert s.mask.tolist() == [2, 3, 0, 1, 0, 1, 0, 0, 0, 0]
+    t, on = 0, []
+    for n in range(1, 700):
+        t += 0.01
+        on.append(s.active(t))
+    assert on[0] == 1 and on[98] == 1 and on[99] == 0 and on[500] == 2 and on[599] == 2
+    assert s.apply(0.5).tolist() == [0, 100, 0, 100, 0, 100, 0, 0, 0, 0]

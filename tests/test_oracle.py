@@ -1,0 +1,273 @@
+"""Pins the CPU oracle against things other than itself.
+
+Each oracle function is checked against the reference's own single-cell
+traces (tests/golden, written by the reference classes), a library routine
+(scipy), a closed form, or an invariant, chosen so that a dropped term, a
+wrong sign/index or a transposed operand fails.
+"""
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import fem, ionic, monodomain, pcg
+from torchcor_amd import meshes
+
+GOLD = Path(__file__).parent / "golden"
+
+
+# ------------------------------------------------------------------ fem
+def test_tet_local_reference_element():
+    # unit right tetrahedron, sigma = I: V = 1/6, K = (1/6) * [[3,-1,-1,-1],[-1,1,0,0],...]
+    X = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]], dtype=float)
+    Me, Ke = fem.tet_local(X, np.eye(3)[None])
+    Kref = np.array([[3, -1, -1, -1], [-1, 1, 0, 0], [-1, 0, 1, 0], [-1, 0, 0, 1]]) / 6.0
+    Mref = (1 / 6) / 20 * (np.ones((4, 4)) + np.eye(4))
+    assert np.allclose(Ke[0], Kref, atol=1e-15)
+    assert np.allclose(Me[0], Mref, atol=1e-15)
+
+
+def _grad_barycentric(P):
+    """Gradients of the P1 basis by solving [1 x y z] c = e_a (a route not used by the oracle)."""
+    Amat = np.concatenate([np.ones((4, 1)), P], axis=1)
+    C = np.linalg.solve(Amat, np.eye(4))       # columns: coefficients of basis a
+    return C[1:, :].T                           # (4,3)
+
+
+def test_tet_local_anisotropic_against_barycentric_route():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        P = rng.normal(size=(4, 3))
+        f = rng.normal(size=3)
+        f /= np.linalg.norm(f)
+        S = 0.2 * np.eye(3) + 0.7 * np.outer(f, f)
+        Me, Ke = fem.tet_local(P[None], S[None])
+        G = _grad_barycentric(P)
+        vol = abs(np.linalg.det(np.concatenate([np.ones((4, 1)), P], axis=1))) / 6
+        assert np.allclose(Ke[0], vol * G @ S @ G.T, rtol=1e-10, atol=1e-12)
+        assert np.isclose(Me[0].sum(), vol)
+
+
+def test_tri_local_planar_and_rotated():
+    rng = np.random.default_rng(1)
+    P2 = rng.normal(size=(3, 2))
+    P = np.concatenate([P2, np.zeros((3, 1))], axis=1)
+    S = np.diag([0.9, 0.3, 0.5])
+    Me, Ke = fem.tri_local(P[None], S[None])
+    # 2-D closed form with the in-plane block of sigma
+    A = np.concatenate([np.ones((3, 1)), P2], axis=1)
+    G = np.linalg.solve(A, np.eye(3))[1:, :].T
+    area = abs(np.linalg.det(A)) / 2
+    assert np.allclose(Ke[0], area * G @ S[:2, :2] @ G.T, rtol=1e-10, atol=1e-12)
+    assert np.isclose(Me[0].sum(), area)
+    # rigid rotation of points and conductivity leaves Ke unchanged
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    Me2, Ke2 = fem.tri_local((P @ Q.T)[None], (Q @ S @ Q.T)[None])
+    assert np.allclose(Ke2, Ke, rtol=1e-9, atol=1e-12)
+
+
+def test_assemble_pattern_and_invariants_cube():
+    N = 6
+    nodes, elems = meshes.kuhn_cube(N)
+    E = len(elems)
+    rng = np.random.default_rng(2)
+    fib = rng.normal(size=(E, 3))
+    fib /= np.linalg.norm(fib, axis=1, keepdims=True)
+    S = fem.sigma_tensor(np.full(E, 0.5), np.full(E, 0.2), fib)
+    Me, Ke = fem.tet_local(nodes[elems], S)
+    n = len(nodes)
+    rp, col, kv = fem.assemble_csr(elems, Ke, n)
+    _, _, mv = fem.assemble_csr(elems, Me, n)
+    # pattern and values against scipy's COO->CSR (duplicates summed)
+    rows = np.concatenate([elems[:, i] for i in range(4) for j in range(4)])
+    cols = np.concatenate([elems[:, j] for i in range(4) for j in range(4)])
+    vals = np.concatenate([Ke[:, i, j] for i in range(4) for j in range(4)]).astype(np.float32)
+    ref = sp.coo_matrix((vals.astype(np.float64), (rows, cols)), shape=(n, n)).tocsr()
+    ref.sort_indices()
+    assert np.array_equal(rp, ref.indptr) and np.array_equal(col, ref.indices)
+    assert np.allclose(kv, ref.data, rtol=1e-13, atol=1e-16)
+    Kc = sp.csr_matrix((kv, col, rp), shape=(n, n))
+    Mc = sp.csr_matrix((mv, col, rp), shape=(n, n))
+    assert np.isclose(Mc.sum(), 1000.0, rtol=1e-6)         # 10 mm cube volume (f32 entries)
+    assert np.abs(Kc @ np.ones(n)).max() < 1e-6
+    assert abs(Kc - Kc.T).max() < 1e-7
+    assert np.linalg.eigvalsh(Kc.toarray()).min() > -1e-8
+    # nnz of the Kuhn pattern: known count for 21^3
+    if N == 6:
+        n21, e21 = meshes.kuhn_cube(21)
+        r21, c21, _ = fem.assemble_csr(e21, np.zeros((len(e21), 4, 4)), len(n21))
+        assert r21[-1] == 128581
+
+
+def test_conductivity_fields():
+    regions = np.array([0, 1, 2, 1])
+    sl, st = fem.conductivity_fields(regions, [([0], 0.3, 0.06, None, None)])
+    assert sl.tolist() == [0.3, 0, 0, 0] and st.tolist() == [0.06, 0, 0, 0]
+    sl, st = fem.conductivity_fields(regions, [([1, 2], 0.5, 0.2, 1.0, 0.4)])
+    assert np.allclose(sl[1:], 0.5 * 1.0 / 1.5) and np.allclose(st[1:], 0.2 * 0.4 / 0.6)
+
+
+# ------------------------------------------------------------------ pcg
+def _spd(n, seed):
+    rng = np.random.default_rng(seed)
+    B = sp.random(n, n, density=0.05, random_state=seed) + sp.eye(n)
+    A = (B @ B.T + n * sp.eye(n)).tocsr()
+    A.sort_indices()
+    return A, rng.normal(size=n)
+
+
+def test_pcg_solution_vs_spsolve():
+    A, b = _spd(200, 3)
+    P = pcg.PCG(A.indptr, A.indices, A.data, pcg.jacobi(A.indptr, A.indices, A.data))
+    P.initialize(np.zeros(200), linear_guess=False)
+    x, it = P.solve(b, 1e-14, 1e-14, 500)
+    assert np.allclose(x, spla.spsolve(A.tocsc(), b), rtol=1e-10)
+    assert 0 < it < 200
+
+
+def test_pcg_iteration_convention():
+    # diagonal A: Jacobi makes z exact, the first iteration converges -> n_iter 0
+    d = np.arange(1.0, 11.0)
+    A = sp.diags(d).tocsr()
+    P = pcg.PCG(A.indptr, A.indices, A.data, 1 / d)
+    P.initialize(np.zeros(10), linear_guess=False)
+    x, it = P.solve(np.ones(10), 1e-12, 1e-12, 100)
+    assert it == 0 and np.allclose(x, 1 / d)
+    # max_iter reached silently: n_iter == max_iter
+    A, b = _spd(100, 4)
+    P = pcg.PCG(A.indptr, A.indices, A.data, pcg.jacobi(A.indptr, A.indices, A.data))
+    P.initialize(np.zeros(100), linear_guess=False)
+    _, it = P.solve(b, 1e-30, 1e-30, 3)
+    assert it == 3
+    # looser tolerance never needs more iterations
+    its = []
+    for tol in (1e-2, 1e-4, 1e-8):
+        P.initialize(np.zeros(100), linear_guess=False)
+        its.append(P.solve(b, tol, tol, 200)[1])
+    assert its == sorted(its)
+
+
+def test_pcg_linear_guess():
+    A, b = _spd(50, 5)
+    P = pcg.PCG(A.indptr, A.indices, A.data, pcg.jacobi(A.indptr, A.indices, A.data))
+    x0 = np.arange(50.0)
+    P.initialize(x0, linear_guess=True)
+    P.x = x0 + 1.0       # as if one step had moved x by +1
+    P.solve(b, 1e-12, 1e-12, 0)   # max_iter 0: only the extrapolation runs
+    assert np.allclose(P.x, x0 + 2.0) and np.allclose(P.x_prev, x0 + 1.0)
+
+
+# ---------------------------------------------------------------- ionic
+def _single_cell(model, nsteps, dt=0.01, stim=40.0, keep=10):
+    V = model.initialize(1)
+    tr = [V[0]]
+    t = 0.0
+    for s in range(1, nsteps + 1):
+        dV = model.differentiate(V)
+        V = V + dt * dV
+        t += dt
+        if 0.0 <= t <= 2.0:
+            V = V + dt * stim
+        if s % keep == 0:
+            tr.append(V[0])
+    return np.array(tr)
+
+
+@pytest.mark.parametrize("name,make", [
+    ("ttp_endo", lambda: ionic.TenTusscherPanfilov("ENDO", 0.01)),
+    ("crn", lambda: ionic.CourtemancheRamirezNattel(0.01)),
+])
+def test_ionic_vs_reference_golden(name, make):
+    g = np.load(GOLD / f"ionic_{name}_f64.npz")
+    nsteps = 8000                      # first 80 ms: upstroke and plateau
+    tr = _single_cell(make(), nsteps)
+    ref = g["V"][: len(tr)]
+    assert np.abs(tr - ref).max() < 1e-6
+
+
+def test_arange_grid_matches_torch():
+    torch = pytest.importorskip("torch")
+    for (a, b, s) in [(-800., 800., 0.05), (0.00001, 10.0, 0.00001), (-200., 200., 0.1)]:
+        g = ionic.arange_f32(a, b, s)
+        t = torch.arange(a, b, s).numpy()
+        assert g.shape == t.shape
+        assert (g != t).sum() <= 16          # vector-tail differences only, 1 ulp
+        assert np.abs(g.astype(np.float64) - t).max() <= np.spacing(np.float32(max(abs(a), abs(b))))
+
+
+def test_ms2v_against_mms_spec():
+    """benchmark/opencarp/.../mMS.model:26-41 written in its own form."""
+    m = ionic.ModifiedMS2v(0.01)
+    rng = np.random.default_rng(6)
+    V = rng.uniform(-85, 25, 50)
+    m.initialize(50)
+    h = rng.uniform(0, 1, 50)
+    m.H = h.copy()
+    dV = m.differentiate(V)
+    Vmin, Vmax, Uamp = -80.0, 20.0, 100.0
+    U = (V - Vmin) / Uamp
+    J_in = h * U * (U - 0.13) * ((Vmax - V) / Uamp) / 0.1
+    J_out = -(1 - h) * (U / 9.0)
+    Iion = -Uamp * (J_in + J_out)
+    assert np.allclose(dV, -Iion, rtol=1e-12, atol=1e-12)
+    diff_h = np.where(U < 0.13, (1 - h) / 100.0, -h / 120.0)
+    assert np.allclose(m.H, h + 0.01 * diff_h)
+
+
+def test_ionic_rest_and_gates():
+    for m in (ionic.TenTusscherPanfilov("ENDO", 0.01), ionic.CourtemancheRamirezNattel(0.01)):
+        V = m.initialize(1)
+        for _ in range(2000):
+            V = V + 0.01 * m.differentiate(V)
+        assert abs(V[0] - m.V_init) < 2.0          # no stimulus: stays near rest
+        for k in ("H", "J", "M", "D", "F") if "H" in m.s else ("h", "j", "m", "d", "f"):
+            assert 0.0 <= m.s[k][0] <= 1.0
+
+
+# ----------------------------------------------------------- monodomain
+def test_stimulus_windows_follow_double_time():
+    on = []
+    t = 0
+    for n in range(1, 700):
+        t += 0.01
+        if monodomain.stimulus_active(t, 0.0, 1.0, 50.0, 1):
+            on.append(n)
+    assert on[0] == 1 and on[-1] == 99 and len(on) == 99
+    t = 0
+    on = []
+    for n in range(1, 700):
+        t += 0.01
+        if monodomain.stimulus_active(t, 5.0, 1.0, 50.0, 1):
+            on.append(n)
+    assert on[0] == 501 and on[-1] == 600
+
+
+def _cube_model(N, model, T):
+    nodes, elems = meshes.kuhn_cube(N)
+    m = monodomain.Monodomain(model, T=T, dt=0.01)
+    m.load_arrays(nodes, elems, np.zeros(len(elems), int), np.tile([1.0, 0, 0], (len(elems), 1)))
+    m.add_conductivity([0], 0.1, 0.1)
+    return m
+
+
+def test_monodomain_rest_is_stationary():
+    m = _cube_model(6, ionic.ModifiedMS2v(0.01), 2.0)
+    r = m.solve(1e-5, 1e-5, 100)
+    assert np.abs(r["u"] - (-80.0)).max() < 1e-9
+    assert (r["AT"] == -1).all()
+
+
+def test_monodomain_planar_wave():
+    N = 21
+    m = _cube_model(N, ionic.ModifiedMS2v(0.01), 20.0)
+    m.add_stimulus(meshes.face_nodes(N, 0, 0), 0.0, 1.0, 100)
+    r = m.solve(1e-5, 1e-5, 100)
+    AT = r["AT"].reshape(N, N, N)
+    assert (AT[: N // 2] > 0).all()
+    mean_x = np.array([AT[i][AT[i] > 0].mean() for i in range(N // 2)])
+    assert (np.diff(mean_x) > 0).all()                    # activation moves along +x
+    spread = np.array([AT[i].max() - AT[i].min() for i in range(N // 2)])
+    assert spread.max() < 1.0                             # nearly planar (ms)

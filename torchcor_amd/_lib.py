@@ -1,0 +1,149 @@
+"""ctypes binding of libtorchcor_amd.so (the C ABI of include/torchcor_amd.h).
+
+torch is imported first so that the library resolves the HIP runtime torch
+already loaded (one runtime per process, shared streams).  A missing library
+raises at import: there is no CPU or PyTorch fallback for these operations.
+"""
+import ctypes as C
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede loading the HIP library)
+
+LIB_PATH = Path(__file__).resolve().parent / "libtorchcor_amd.so"
+
+TC_F32, TC_F64 = 0, 1
+MAX_STIMULI = 32
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python -m torchcor_amd.build` "
+        "(hipcc, gfx950). torchcor_amd has no fallback path.")
+
+lib = C.CDLL(str(LIB_PATH))
+
+p = C.c_void_p
+i64 = C.c_int64
+i32 = C.c_int
+dbl = C.c_double
+
+
+class Sell(C.Structure):
+    _fields_ = [("n", i64), ("n_slices", i64), ("slice_ptr", p), ("col", p)]
+
+
+class IonicIO(C.Structure):
+    _fields_ = [("n", i64), ("dt", dbl), ("dt_rhs", dbl), ("Cm", dbl), ("V", p), ("w", p),
+                ("dV", p), ("stim_mask", p), ("stim_active", C.c_uint32),
+                ("stim_amp", dbl * MAX_STIMULI), ("AT", p), ("RT", p), ("lat_t", dbl)]
+
+
+MS2V_FIELDS = ["tau_in", "tau_out", "tau_open", "tau_close", "u_gate", "u_crit", "vmin", "vmax"]
+TTP_FIELDS = ["GCaL", "GKr", "GKs", "Gto", "Bufc", "Bufsr", "Bufss", "CAPACITANCE", "Cao",
+              "D_CaL_off", "Fconst", "GK1", "GNa", "GbCa", "GbNa", "GpCa", "GpK", "Kbufc",
+              "Kbufsr", "Kbufss", "KmCa", "KmK", "KmNa", "KmNai", "Ko", "KpCa", "Kup", "Nao",
+              "Rconst", "T", "Vc", "Vleak", "Vmaxup", "Vrel", "Vsr", "Vss", "knaca", "knak",
+              "ksat", "n", "pKNa", "scl_tau_f", "vHalfXs", "xr2_off", "EC", "Vxfer", "k1_",
+              "k2_", "k3", "k4", "maxsr", "minsr"]
+CRN_FIELDS = ["C_B1c", "C_B1d", "C_B1e", "C_Fn1", "C_Fn2", "C_dCa_rel", "C_dCaup", "F", "K_Q10",
+              "K_up", "KmCa", "KmCmdn", "KmCsqn", "KmKo", "KmNa3", "KmNai", "KmTrpn", "Nai", "R",
+              "T", "Voli", "gamma", "k_rel", "k_sat", "maxCmdn", "maxTrpn", "tau_f_Ca", "tau_tr",
+              "tau_u", "ACh", "Cao", "GACh", "GCaL", "GK1", "GKr", "GKs", "GNa", "GbCa", "GbNa",
+              "Gto", "Ko", "Nao", "factorGKur", "factorGrel", "factorGtr", "factorGup",
+              "factorhGate", "factormGate", "factoroaGate", "factorxrGate", "maxCaup", "maxINaCa",
+              "maxINaK", "maxIpCa", "maxIup"]
+
+
+class MS2vParams(C.Structure):
+    _fields_ = [(f, dbl) for f in MS2V_FIELDS]
+
+
+class TTPParams(C.Structure):
+    _fields_ = [("endo", i32)] + [(f, dbl) for f in TTP_FIELDS]
+
+
+class CRNParams(C.Structure):
+    _fields_ = [(f, dbl) for f in CRN_FIELDS]
+
+
+def _proto(name, *args):
+    fn = getattr(lib, name)
+    fn.restype = C.c_int
+    fn.argtypes = list(args)
+    return fn
+
+
+lib.tc_last_error.restype = C.c_char_p
+lib.tc_last_error.argtypes = []
+lib.tc_abi_version.restype = C.c_int
+lib.tc_abi_version.argtypes = []
+
+SIZE_P = C.POINTER(C.c_size_t)
+I64_P = C.POINTER(i64)
+INT_P = C.POINTER(C.c_int)
+SELL_P = C.POINTER(Sell)
+IO_P = C.POINTER(IonicIO)
+
+FUNCS = {
+    "tc_pattern_workspace_size": (i64, i32, SIZE_P),
+    "tc_pattern_sort": (p, i64, i32, i64, p, C.c_size_t, I64_P, p),
+    "tc_pattern_finish": (i64, i32, i64, i64, p, p, p, p, p, p),
+    "tc_assemble_p1": (p, p, i64, i32, p, p, p, p, p, p, i64, p, i32, p, p, p),
+    "tc_operator_values": (p, p, p, p, i64, i64, dbl, dbl, dbl, dbl, i32, p, p, p, p, p),
+    "tc_sell_layout": (p, i64, p, I64_P, p),
+    "tc_sell_columns": (p, p, i64, p, p, p),
+    "tc_sell_values": (p, i64, p, p, p, i32, p),
+    "tc_sell_spmv": (SELL_P, p, p, p, i32, p),
+    "tc_rhs": (SELL_P, p, p, p, p, dbl, p, i32, p),
+    "tc_pcg_workspace_size": (i64, i32, SIZE_P),
+    "tc_pcg_create": (C.POINTER(p), SELL_P, p, p, i32, p, C.c_size_t),
+    "tc_pcg_destroy": (p,),
+    "tc_pcg_solve": (p, p, p, p, p, dbl, dbl, i32, i32, INT_P, p),
+    "tc_pcg_set_timing": (p, i32),
+    "tc_pcg_get_timing": (p, C.POINTER(dbl), I64_P),
+    "tc_ms2v_step": (C.POINTER(MS2vParams), p, IO_P, i32, p),
+    "tc_ttp_tables": (C.POINTER(TTPParams), dbl, p, p, p, i32, p),
+    "tc_ttp_step": (C.POINTER(TTPParams), p, p, p, p, IO_P, i32, p),
+    "tc_crn_tables": (C.POINTER(CRNParams), dbl, p, p, p, i32, p),
+    "tc_crn_step": (C.POINTER(CRNParams), p, p, p, p, IO_P, i32, p),
+    "tc_lat_update": (p, p, p, i64, dbl, i32, p),
+}
+for _n, _a in FUNCS.items():
+    _proto(_n, *_a)
+
+# Every name the header declares, for the export check in the tests.
+EXPORTS = ["tc_last_error", "tc_abi_version", *FUNCS]
+
+
+class TorchcorError(RuntimeError):
+    pass
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib.tc_last_error().decode(errors="replace")
+        raise TorchcorError(f"{what or 'torchcor_amd'} failed ({rc}): {msg}")
+
+
+def call(name, *args):
+    check(getattr(lib, name)(*args), name)
+
+
+def dtype_code(dtype):
+    if dtype == torch.float32:
+        return TC_F32
+    if dtype == torch.float64:
+        return TC_F64
+    raise TypeError(f"unsupported dtype {dtype} (float32 or float64)")
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL); requires a GPU tensor."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise TorchcorError("torchcor_amd kernels need GPU tensors (no CPU path)")
+    return t.data_ptr()
+
+
+def stream_handle(device=None):
+    return torch.cuda.current_stream(device).cuda_stream

@@ -1,0 +1,162 @@
+"""Synthetic meshes and openCARP-style mesh files.
+
+The structured cube is a unit box split into hexahedra, each cut into six
+P1 tetrahedra along its main diagonal (the Kuhn split).  Node (i, j, k) sits at
+``(i*h, j*h, k*h)`` millimetres and has index ``i*N*N + j*N + k``.
+
+The writers produce the text formats the reference's reader consumes
+(core/mesh.py:26-63, core/stimulation.py:27-36): ``.pts`` (count line, then
+x y z in micrometres), ``.elem`` (count line, then ``Tt a b c d region``),
+``.lon`` (header ``1``, then one fibre per element) and ``.vtx`` (count line,
+``extra``, then one node index per line).
+"""
+from itertools import permutations
+from pathlib import Path
+
+import numpy as np
+
+# Hex corner offsets (di, dj, dk) indexed by bit pattern 4*di + 2*dj + dk.
+_CORNERS = np.array([[(c >> 2) & 1, (c >> 1) & 1, c & 1] for c in range(8)])
+
+
+def _kuhn_tets():
+    """Six corner-index paths 0 -> 7 through the unit cube, one per axis order."""
+    tets = []
+    for perm in permutations(range(3)):
+        path = [0]
+        cur = 0
+        for ax in perm:
+            cur |= 4 >> ax
+            path.append(cur)
+        tets.append(path)
+    return np.array(tets)
+
+
+def kuhn_cube(N, edge_mm=None, length_mm=10.0):
+    """Structured N x N x N node cube of side ``length_mm`` (default 10 mm).
+
+    Returns (nodes (N^3,3) float64 mm, elems (6(N-1)^3, 4) int32).
+    """
+    h = length_mm / (N - 1) if edge_mm is None else edge_mm
+    g = np.arange(N)
+    I, J, K = np.meshgrid(g, g, g, indexing="ij")
+    nodes = np.stack([I.ravel() * h, J.ravel() * h, K.ravel() * h], axis=1).astype(np.float64)
+    c = np.arange(N - 1)
+    ci, cj, ck = np.meshgrid(c, c, c, indexing="ij")
+    base = (ci * N * N + cj * N + ck).ravel()
+    off = _CORNERS[:, 0] * N * N + _CORNERS[:, 1] * N + _CORNERS[:, 2]
+    tets = _kuhn_tets()
+    elems = (base[:, None, None] + off[tets][None, :, :]).reshape(-1, 4)
+    return nodes, elems.astype(np.int32)
+
+
+def centroids(nodes, elems):
+    return nodes[elems].mean(axis=1)
+
+
+def rotating_fibres(nodes, elems, length_mm=10.0, theta0_deg=-60.0, span_deg=120.0):
+    """Per-element unit fibres (cos t, sin t, 0), t = theta0 + span * z_c / length."""
+    zc = centroids(nodes, elems)[:, 2] / length_mm
+    th = np.deg2rad(theta0_deg + span_deg * zc)
+    return np.stack([np.cos(th), np.sin(th), np.zeros_like(th)], axis=1)
+
+
+def corner_nodes(N, m):
+    """Indices of the m x m x m corner block of nodes (i, j, k < m)."""
+    g = np.arange(m)
+    I, J, K = np.meshgrid(g, g, g, indexing="ij")
+    return (I * N * N + J * N + K).ravel()
+
+
+def face_nodes(N, axis=0, index=0):
+    """Indices of the nodes on one face of the cube (e.g. x = 0)."""
+    g = np.arange(N)
+    A, B = np.meshgrid(g, g, indexing="ij")
+    idx = [None, None, None]
+    others = [a for a in range(3) if a != axis]
+    idx[axis] = np.full_like(A, index)
+    idx[others[0]] = A
+    idx[others[1]] = B
+    return (idx[0] * N * N + idx[1] * N + idx[2]).ravel()
+
+
+def cube_workload(N, model="TTP"):
+    """Mesh arrays of the large-cube workload at N nodes per side.
+
+    Two conductivity regions split at x = 5 mm (region 0 below, region 1
+    above), fibres rotating about z from -60 to +60 degrees.
+    """
+    nodes, elems = kuhn_cube(N)
+    cen = centroids(nodes, elems)
+    regions = (cen[:, 0] >= 5.0).astype(np.int32)
+    fibres = rotating_fibres(nodes, elems)
+    return nodes, elems, regions, fibres
+
+
+def write_mesh(folder, nodes_mm, elems, regions, fibres, name="mesh", unit=1000):
+    """Write ``.pts`` (micrometres), ``.elem`` and ``.lon`` files into ``folder``."""
+    folder = Path(folder)
+    folder.mkdir(parents=True, exist_ok=True)
+    k = elems.shape[1]
+    tag = "Tt" if k == 4 else "Tr"
+    with open(folder / f"{name}.pts", "w") as f:
+        f.write(f"{nodes_mm.shape[0]}\n")
+        np.savetxt(f, nodes_mm * unit, fmt="%.17g")
+    with open(folder / f"{name}.elem", "w") as f:
+        f.write(f"{elems.shape[0]}\n")
+        rows = np.concatenate([elems.astype(np.int64), np.asarray(regions, dtype=np.int64)[:, None]], axis=1)
+        np.savetxt(f, rows, fmt=tag + " " + " ".join(["%d"] * (k + 1)))
+    with open(folder / f"{name}.lon", "w") as f:
+        f.write("1\n")
+        np.savetxt(f, np.asarray(fibres, dtype=np.float64), fmt="%.17g")
+    return folder
+
+
+def write_vtx(path, node_ids):
+    node_ids = np.asarray(node_ids, dtype=np.int64)
+    with open(path, "w") as f:
+        f.write(f"{node_ids.shape[0]}\nextra\n")
+        np.savetxt(f, node_ids, fmt="%d")
+    return Path(path)
+
+
+def jittered_delaunay(n_side, seed=0, length_mm=10.0, jitter=0.3, permute=True):
+    """Unstructured tetrahedral mesh of a cube: a jittered lattice of
+    ``n_side**3`` points, Delaunay-tetrahedralised (scipy/Qhull), slivers with
+    near-zero volume dropped, nodes optionally randomly renumbered.
+
+    Returns (nodes (n,3) mm, elems (E,4) int32).
+    """
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    h = length_mm / (n_side - 1)
+    g = np.arange(n_side) * h
+    P = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    interior = np.all((P > 0) & (P < length_mm), axis=1)
+    P[interior] += rng.uniform(-jitter * h, jitter * h, size=(interior.sum(), 3))
+    tets = Delaunay(P).simplices.astype(np.int64)
+    X = P[tets]
+    vol = np.abs(np.einsum("ij,ij->i", X[:, 1] - X[:, 0],
+                           np.cross(X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]))) / 6
+    tets = tets[vol > 1e-6 * h ** 3]
+    if permute:
+        perm = rng.permutation(P.shape[0])        # new index of old node i
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(P.shape[0])
+        P = P[inv]
+        tets = perm[tets]
+    return P, tets.astype(np.int32)
+
+
+def surface_of(nodes, elems):
+    """Boundary triangles of a tetrahedral mesh (faces used by one tet),
+    renumbered to the surface nodes.  Returns (nodes, tris int32)."""
+    faces = np.concatenate([elems[:, [0, 1, 2]], elems[:, [0, 1, 3]], elems[:, [0, 2, 3]],
+                            elems[:, [1, 2, 3]]])
+    key = np.sort(faces, axis=1)
+    _, idx, cnt = np.unique(key, axis=0, return_index=True, return_counts=True)
+    tri = faces[idx[cnt == 1]]
+    used = np.unique(tri)
+    remap = -np.ones(nodes.shape[0], dtype=np.int64)
+    remap[used] = np.arange(used.shape[0])
+    return nodes[used], remap[tri].astype(np.int32)
