@@ -43,6 +43,7 @@ TabParam make_tab(double mn, double mx, double res) {
 // each 8-wide vector is formed in double, lanes add k*float(res) with a
 // single rounding.
 __device__ inline double grid_f32(const TabParam& t, int64_t i) {
+#pragma clang fp contract(off)
   const int64_t q = i & ~(int64_t)7;
   const float base = (float)(t.mn + t.res * (double)q);
   const double k = (double)(i - q);
@@ -56,30 +57,35 @@ struct Lerp {
 };
 
 // Reference interpolate() (TenTusscherPanfilov.py:188-195) in float32.
+// Every operation rounds separately, as in the reference's elementwise tensor
+// ops: contraction into fused multiply-adds is switched off here (the HIP
+// *_rn helpers are plain operators and would still be fused).
 __device__ inline Lerp<float> lerp_at(float X, const TabParam& t) {
+#pragma clang fp contract(off)
   const float mn = (float)t.mn, mx = (float)t.mx, res = (float)t.res, st = (float)t.step;
   X = X < mn ? mn : (X > mx ? mx : X);
-  const float s = __fmul_rn(__fsub_rn(X, mn), st);
+  const float s = (X - mn) * st;
   int64_t idx = (int64_t)s;
   idx = idx < 0 ? 0 : (idx > t.mx_idx - 1 ? t.mx_idx - 1 : idx);
-  const float lp = __fadd_rn(__fmul_rn((float)idx, res), mn);
+  const float lp = (float)idx * res + mn;
   Lerp<float> L;
   L.lo = idx;
-  L.w = __fdiv_rn(__fsub_rn(X, lp), res);
+  L.w = __fdiv_rn(X - lp, res);
   return L;
 }
 
 // Same in float64; lower_pos stays float32 as the reference's int64 x float
 // promotion makes it.
 __device__ inline Lerp<double> lerp_at(double X, const TabParam& t) {
+#pragma clang fp contract(off)
   X = X < t.mn ? t.mn : (X > t.mx ? t.mx : X);
-  const double s = __dmul_rn(__dsub_rn(X, t.mn), t.step);
+  const double s = (X - t.mn) * t.step;
   int64_t idx = (int64_t)s;
   idx = idx < 0 ? 0 : (idx > t.mx_idx - 1 ? t.mx_idx - 1 : idx);
-  const double lp = (double)__fadd_rn(__fmul_rn((float)idx, (float)t.res), (float)t.mn);
+  const double lp = (double)((float)idx * (float)t.res + (float)t.mn);
   Lerp<double> L;
   L.lo = idx;
-  L.w = __ddiv_rn(__dsub_rn(X, lp), t.res);
+  L.w = (X - lp) / t.res;
   return L;
 }
 

@@ -6,12 +6,21 @@
 // in float64.  Each PCG iteration is two passes:
 //   spmv pass:   p' = beta p + Dinv r (first iteration: p' = Dinv r),
 //                Ap = A p' with p' formed on the fly for the gathered
-//                neighbours, partial p'.Ap
-//   update pass: x += alpha p', r -= alpha Ap, z = Dinv r,
-//                partials z.z and r.z, and the stopping test.
-// Scalars live in a device control block; the last block of each pass folds
-// the partial sums in block order (deterministic) and writes the scalars, so
-// no host round trip is needed inside an iteration.
+//                neighbours, partial sums of p'.Ap
+//   update pass: x += alpha p', r -= alpha Ap, partial sums of z.z and r.z
+//                with z = Dinv r.
+//
+// Reductions without atomics: the PCG kernels run on a fixed grid of G
+// blocks; each block writes its partial sums to partials[k*G + block], and
+// every block of the NEXT pass folds all G partials itself, in block order,
+// before using the scalar.  All blocks therefore see bit-identical scalars
+// and the result does not depend on scheduling.  The stopping test of
+// iteration k is evaluated by the spmv pass of iteration k+1 (or by a one-
+// block check pass at the end of a launch chunk), exactly with the reference
+// rule, so no host round trip is needed inside an iteration.
+//
+// XCD-aware work split: block b runs on XCD b % 8; blocks of one XCD get one
+// contiguous range of slices so that neighbour gathers stay in that XCD's L2.
 #include <vector>
 
 #include "tc_internal.h"
@@ -19,17 +28,13 @@
 namespace tc {
 
 struct PcgCtl {
-  double rz;      // r.z of the current residual
-  double alpha;
-  double beta;
+  double rz[2];   // r.z of iteration k in rz[k & 1]
   double bnorm;   // ||Dinv b||
-  double znorm;   // ||Dinv r|| after the last update
   double a_tol, r_tol;
   int converged;  // stop rule met
   int n_iter;     // iterations that did not stop (reference convention)
   int iter;       // iterations executed
   int pad;
-  unsigned ticket[4];
 };
 
 }  // namespace tc
@@ -40,9 +45,9 @@ struct tc_pcg {
   const void* Dinv;
   int dtype;
   int64_t n;
-  int grid;
+  int G;                 // blocks of every PCG pass
   tc::PcgCtl* ctl;       // device
-  double* partials;      // device, 2*grid
+  double* part[3];       // device partial sums: init (2G), spmv (G), update (2G)
   void* r;
   void* p[2];
   void* Ap;
@@ -60,6 +65,9 @@ namespace tc {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxBlocks = 2048;
+constexpr int kXcd = 8;
 
 // ------------------------------------------------------------------ SpMV
 template <class T>
@@ -103,20 +111,81 @@ k_rhs(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ s
   if (i < n) bout[i] = (T)(am - coef * ak);
 }
 
+// ------------------------------------------------------- PCG work split
+struct Split {
+  int64_t s0, s1;  // this block's slice range
+};
+
+__device__ inline Split block_slices(int64_t ns) {
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  int vb = b;
+  if (G % kXcd == 0) vb = (b % kXcd) * (G / kXcd) + b / kXcd;
+  const int64_t per = (ns + G - 1) / G;
+  Split sp;
+  sp.s0 = (int64_t)vb * per;
+  sp.s1 = sp.s0 + per < ns ? sp.s0 + per : ns;
+  return sp;
+}
+
+// Block partial sums -> partials[k*G + blockIdx.x].
+template <int NV>
+__device__ inline void store_partials(double (&v)[NV], double* partials) {
+  block_sum<NV>(v);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = v[k];
+  }
+}
+
+// Sum of the previous pass's G partials, in a fixed order, returned to
+// every thread of the block.
+template <int NV>
+__device__ inline void fold_partials(const double* __restrict__ partials, int G,
+                                     double (&out)[NV]) {
+  __shared__ double bc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < G; b += blockDim.x) s += partials[k * G + b];
+    out[k] = s;
+  }
+  block_sum<NV>(out);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) bc[k] = out[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) out[k] = bc[k];
+}
+
+// Block-uniform read of the converged flag (one load, shared by all waves).
+__device__ inline bool block_converged(const PcgCtl* ctl) {
+  __shared__ int conv;
+  if (threadIdx.x == 0) conv = ctl->converged;
+  __syncthreads();
+  return conv != 0;
+}
+
+__device__ inline bool stop_rule(double zn, const PcgCtl* ctl) {
+  return zn < ctl->a_tol || zn / ctl->bnorm < ctl->r_tol;
+}
+
 // PCG start: x0 = 2x - x_prev (linear guess) or x; r = b - A x0;
-// partials r.z and (Dinv b)^2.
+// partial sums of r.z and (Dinv b)^2.
 template <class T, bool LG>
 __global__ void __launch_bounds__(kBlock)
 k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
            const T* __restrict__ Av, const T* __restrict__ Dinv, const T* __restrict__ bvec,
            const T* __restrict__ x, const T* __restrict__ xp, T* __restrict__ xnext,
-           T* __restrict__ r, PcgCtl* ctl, double* partials) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t s = i >> 6;
-  const int lane = threadIdx.x & 63;
+           T* __restrict__ r, PcgCtl* ctl, double a_tol, double r_tol, double* part0) {
   const int64_t ns = (n + 63) >> 6;
+  const Split sp = block_slices(ns);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double red[2] = {0.0, 0.0};
-  if (s < ns) {
+  for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
+    const int64_t i = s * 64 + lane;
     const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
     const int32_t* c = scol + b + lane;
     const T* a = Av + b + lane;
@@ -134,41 +203,68 @@ k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
       const double bi = (double)bvec[i];
       const T ri = (T)(bi - ax);
       r[i] = ri;
-      red[0] = (double)ri * (d * (double)ri);
+      red[0] += (double)ri * (d * (double)ri);
       const double db = d * bi;
-      red[1] = db * db;
+      red[1] += db * db;
     }
   }
-  if (grid_sum_last<2>(red, partials, &ctl->ticket[0])) {
-    ctl->rz = red[0];
-    ctl->bnorm = sqrt(red[1]);
+  store_partials<2>(red, part0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->converged = 0;
     ctl->n_iter = 0;
     ctl->iter = 0;
+    ctl->a_tol = a_tol;
+    ctl->r_tol = r_tol;
   }
 }
 
-// spmv pass of one iteration (see file comment).  pin is the previous p
-// (unused on the first iteration), pout receives p'.
+// spmv pass of iteration k.  pin is the previous p (unused when k == 0),
+// pout receives p'.  Iteration k-1's stopping test happens here first.
 template <class T>
 __global__ void __launch_bounds__(kBlock)
 k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
            const T* __restrict__ Av, const T* __restrict__ Dinv, const T* __restrict__ r,
-           const T* __restrict__ pin, T* __restrict__ pout, T* __restrict__ Ap, int first,
-           PcgCtl* ctl, double* partials) {
-  if (__hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  const double beta = ctl->beta;
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const int64_t s = i >> 6;
-  const int lane = threadIdx.x & 63;
+           const T* __restrict__ pin, T* __restrict__ pout, T* __restrict__ Ap, int k,
+           PcgCtl* ctl, const double* __restrict__ part0, const double* __restrict__ part2,
+           double* part1) {
+  if (block_converged(ctl)) return;
+  const int G = gridDim.x;
+  double beta = 0.0, rz;
+  if (k == 0) {
+    double v[2];
+    fold_partials<2>(part0, G, v);
+    rz = v[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->bnorm = sqrt(v[1]);
+  } else {
+    double v[2];
+    fold_partials<2>(part2, G, v);
+    if (stop_rule(sqrt(v[0]), ctl)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->converged = 1;
+        ctl->n_iter = k - 1;
+        ctl->iter = k;
+      }
+      return;
+    }
+    rz = v[1];
+    beta = rz / ctl->rz[(k - 1) & 1];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->rz[k & 1] = rz;
+    ctl->n_iter = k;
+    ctl->iter = k;
+  }
   const int64_t ns = (n + 63) >> 6;
+  const Split sp = block_slices(ns);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double red[1] = {0.0};
-  if (s < ns) {
+  for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
+    const int64_t i = s * 64 + lane;
     const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
     const int32_t* c = scol + b + lane;
     const T* a = Av + b + lane;
     double acc = 0.0;
-    if (first) {
+    if (k == 0) {
       for (int j = 0; j < wd; ++j) {
         const int32_t cj = c[j * 64];
         const T pj = (T)((double)Dinv[cj] * (double)r[cj]);
@@ -183,58 +279,70 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
     }
     if (i < n) {
       T pi;
-      if (first) pi = (T)((double)Dinv[i] * (double)r[i]);
+      if (k == 0) pi = (T)((double)Dinv[i] * (double)r[i]);
       else pi = (T)(beta * (double)pin[i] + (double)Dinv[i] * (double)r[i]);
       const T api = (T)acc;
       pout[i] = pi;
       Ap[i] = api;
-      red[0] = (double)pi * (double)api;
+      red[0] += (double)pi * (double)api;
     }
   }
-  if (grid_sum_last<1>(red, partials, &ctl->ticket[1])) {
-    ctl->alpha = ctl->rz / red[0];
-  }
+  store_partials<1>(red, part1);
 }
 
 template <class T>
 __global__ void __launch_bounds__(kBlock)
 k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
-             const T* __restrict__ Ap, T* __restrict__ x, T* __restrict__ r, PcgCtl* ctl,
-             double* partials) {
-  if (__hip_atomic_load(&ctl->converged, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  const double alpha = ctl->alpha;
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+             const T* __restrict__ Ap, T* __restrict__ x, T* __restrict__ r, int k, PcgCtl* ctl,
+             const double* __restrict__ part1, double* part2) {
+  if (block_converged(ctl)) return;
+  double v[1];
+  fold_partials<1>(part1, gridDim.x, v);
+  const double alpha = ctl->rz[k & 1] / v[0];
+  const int64_t ns = (n + 63) >> 6;
+  const Split sp = block_slices(ns);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double red[2] = {0.0, 0.0};
-  if (i < n) {
-    const T xi = (T)((double)x[i] + alpha * (double)p[i]);
-    const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
-    x[i] = xi;
-    r[i] = ri;
-    const double z = (double)Dinv[i] * (double)ri;
-    red[0] = z * z;
-    red[1] = (double)ri * z;
-  }
-  if (grid_sum_last<2>(red, partials, &ctl->ticket[2])) {
-    const double zn = sqrt(red[0]);
-    ctl->znorm = zn;
-    ctl->iter += 1;
-    if (zn < ctl->a_tol || zn / ctl->bnorm < ctl->r_tol) {
-      ctl->converged = 1;
-    } else {
-      ctl->beta = red[1] / ctl->rz;
-      ctl->rz = red[1];
-      ctl->n_iter += 1;
+  for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
+    const int64_t i = s * 64 + lane;
+    if (i < n) {
+      const T xi = (T)((double)x[i] + alpha * (double)p[i]);
+      const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
+      x[i] = xi;
+      r[i] = ri;
+      const double z = (double)Dinv[i] * (double)ri;
+      red[0] += z * z;
+      red[1] += (double)ri * z;
     }
   }
+  store_partials<2>(red, part2);
 }
 
-__global__ void k_set_tol(PcgCtl* ctl, double a_tol, double r_tol) {
-  ctl->a_tol = a_tol;
-  ctl->r_tol = r_tol;
+// One block: stopping test of the last update of a launch chunk (iteration
+// k), so the host can read the state.
+__global__ void __launch_bounds__(kBlock)
+k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2) {
+  if (block_converged(ctl)) return;
+  double v[2];
+  fold_partials<2>(part2, G, v);
+  if (threadIdx.x == 0) {
+    const bool stop = stop_rule(sqrt(v[0]), ctl);
+    ctl->converged = stop ? 1 : 0;
+    ctl->n_iter = stop ? k : k + 1;
+    ctl->iter = k + 1;
+  }
 }
 
 size_t vec_bytes(int64_t n, int dtype) {
   return align_up((size_t)n * (dtype == TC_F64 ? 8 : 4), 256);
+}
+
+int pcg_blocks(int64_t n) {
+  const int64_t ns = ceil_div(n, 64);
+  int64_t g = ceil_div(ns, kWaves);
+  if (g > kMaxBlocks) g = kMaxBlocks;
+  if (g >= kXcd) g -= g % kXcd;
+  return (int)(g < 1 ? 1 : g);
 }
 
 // Event pair k brackets launch k of the current solve.
@@ -251,7 +359,7 @@ template <class T>
 int pcg_solve(tc_pcg* P, const T* b, T* x, const T* xp, T* xn, double a_tol, double r_tol,
               int max_iter, int lg, int* n_iter_host, hipStream_t s) {
   const int64_t n = P->n;
-  const dim3 g(P->grid), blk(kBlock);
+  const dim3 g(P->G), blk(kBlock);
   const int32_t* sp = P->A.slice_ptr;
   const int32_t* sc = P->A.col;
   const T* Av = (const T*)P->Av;
@@ -259,8 +367,6 @@ int pcg_solve(tc_pcg* P, const T* b, T* x, const T* xp, T* xn, double a_tol, dou
   T* r = (T*)P->r;
   T* Ap = (T*)P->Ap;
   T* pp[2] = {(T*)P->p[0], (T*)P->p[1]};
-  hipLaunchKernelGGL(k_set_tol, dim3(1), dim3(1), 0, s, P->ctl, a_tol, r_tol);
-  TC_LAUNCH();
   if (P->timing) {
     int rc = ev_reserve(P, 1 + 2 * (size_t)(max_iter > 0 ? max_iter : 0));
     if (rc) return rc;
@@ -269,11 +375,11 @@ int pcg_solve(tc_pcg* P, const T* b, T* x, const T* xp, T* xn, double a_tol, dou
   T* xs;
   if (lg) {
     hipLaunchKernelGGL((k_pcg_init<T, true>), g, blk, 0, s, n, sp, sc, Av, D, b, x, xp, xn, r,
-                       P->ctl, P->partials);
+                       P->ctl, a_tol, r_tol, P->part[0]);
     xs = xn;
   } else {
     hipLaunchKernelGGL((k_pcg_init<T, false>), g, blk, 0, s, n, sp, sc, Av, D, b, x, xp,
-                       (T*)nullptr, r, P->ctl, P->partials);
+                       (T*)nullptr, r, P->ctl, a_tol, r_tol, P->part[0]);
     xs = x;
   }
   TC_LAUNCH();
@@ -290,19 +396,21 @@ int pcg_solve(tc_pcg* P, const T* b, T* x, const T* xp, T* xn, double a_tol, dou
       T* pin = pp[k & 1];
       T* pout = pp[(k + 1) & 1];
       if (P->timing) TC_HIP(hipEventRecord(P->ev[2 + 4 * k], s));
-      hipLaunchKernelGGL(k_pcg_spmv<T>, g, blk, 0, s, n, sp, sc, Av, D, r, pin, pout, Ap,
-                         k == 0 ? 1 : 0, P->ctl, P->partials);
+      hipLaunchKernelGGL(k_pcg_spmv<T>, g, blk, 0, s, n, sp, sc, Av, D, r, pin, pout, Ap, k,
+                         P->ctl, P->part[0], P->part[2], P->part[1]);
       TC_LAUNCH();
       if (P->timing) {
         TC_HIP(hipEventRecord(P->ev[3 + 4 * k], s));
         TC_HIP(hipEventRecord(P->ev[4 + 4 * k], s));
       }
-      hipLaunchKernelGGL(k_pcg_update<T>, g, blk, 0, s, n, D, pout, Ap, xs, r, P->ctl,
-                         P->partials);
+      hipLaunchKernelGGL(k_pcg_update<T>, g, blk, 0, s, n, D, pout, Ap, xs, r, k, P->ctl,
+                         P->part[1], P->part[2]);
       TC_LAUNCH();
       if (P->timing) TC_HIP(hipEventRecord(P->ev[5 + 4 * k], s));
     }
     it += c;
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), blk, 0, s, it - 1, P->G, P->ctl, P->part[2]);
+    TC_LAUNCH();
     TC_HIP(hipMemcpyAsync(P->host_ctl, P->ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, s));
     TC_HIP(hipStreamSynchronize(s));
     if (P->host_ctl->converged) break;
@@ -373,8 +481,8 @@ extern "C" int tc_rhs(const tc_sell* A, const void* Mv, const void* Kv, const vo
 extern "C" int tc_pcg_workspace_size(int64_t n, int dtype, size_t* bytes) {
   TC_CHECK(bytes && n > 0 && (dtype == TC_F32 || dtype == TC_F64), TC_ERR_ARG,
            "tc_pcg_workspace_size: bad arguments");
-  const int64_t grid = ceil_div(ceil_div(n, 64) * 64, kBlock);
-  *bytes = align_up(sizeof(PcgCtl), 256) + align_up(sizeof(double) * 2 * grid, 256) +
+  const int G = pcg_blocks(n);
+  *bytes = align_up(sizeof(PcgCtl), 256) + 3 * align_up(sizeof(double) * 2 * G, 256) +
            4 * vec_bytes(n, dtype);
   return TC_OK;
 }
@@ -393,13 +501,15 @@ extern "C" int tc_pcg_create(tc_pcg** plan, const tc_sell* A, const void* Av, co
   P->Dinv = Dinv;
   P->dtype = dtype;
   P->n = A->n;
-  P->grid = (int)ceil_div(A->n_slices * 64, kBlock);
+  P->G = pcg_blocks(A->n);
   char* b = (char*)ws;
   size_t off = 0;
   P->ctl = (PcgCtl*)(b + off);
   off += align_up(sizeof(PcgCtl), 256);
-  P->partials = (double*)(b + off);
-  off += align_up(sizeof(double) * 2 * P->grid, 256);
+  for (int k = 0; k < 3; ++k) {
+    P->part[k] = (double*)(b + off);
+    off += align_up(sizeof(double) * 2 * P->G, 256);
+  }
   P->r = b + off; off += vec_bytes(P->n, dtype);
   P->p[0] = b + off; off += vec_bytes(P->n, dtype);
   P->p[1] = b + off; off += vec_bytes(P->n, dtype);
@@ -416,8 +526,7 @@ extern "C" int tc_pcg_create(tc_pcg** plan, const tc_sell* A, const void* Av, co
     set_error("hipHostMalloc: %s", hipGetErrorString(e));
     return TC_ERR_HIP;
   }
-  // Zero the control block (tickets must start at 0); plan creation is a
-  // set-up call, never captured.
+  // Zero the control block; plan creation is a set-up call, never captured.
   e = hipMemset(P->ctl, 0, sizeof(PcgCtl));
   if (e != hipSuccess) {
     (void)hipHostFree(P->host_ctl);
