@@ -195,6 +195,9 @@ typedef struct tc_ionic_io {
   void* AT;
   void* RT;
   double lat_t;
+  /* When non-NULL and *gate != 0 at kernel start, the kernel does nothing
+   * (used by the pipelined time-step runtime). */
+  const int* gate;
 } tc_ionic_io;
 
 /* ModifiedMS2v (ionic/ModifiedMS2v.py:4-44).  States: H. */
@@ -237,6 +240,68 @@ int tc_crn_tables(const tc_crn_params* p, double dt, void* Cai_tab, void* V_tab,
 int tc_crn_step(const tc_crn_params* p, void* states, const void* Cai_tab,
                 const void* V_tab, const void* fn_tab, const tc_ionic_io* io, int dtype,
                 void* stream);
+
+/* ------------------------------------------------------------------------
+ * Time-step runtime: the loop of Monodomain.solve (monodomain.py:141-196)
+ * in native code.  Each step enqueues the fused ionic kernel, the fused
+ * RHS + PCG start pass, a speculative number of PCG iterations (the most the
+ * recent steps needed) and a one-block stopping check; the host runs up to
+ * `depth` steps ahead and reads each step's status from pinned memory.  A
+ * step that needs more iterations than were launched sets a device stall
+ * flag that turns every later launch into a no-op; the host then launches
+ * the missing iterations and re-issues the later steps, so results are those
+ * of the plain loop.  Time is accumulated as a double (t += dt), stimulus
+ * windows and LAT/RT use that t, as in the reference.
+ * ---------------------------------------------------------------------- */
+enum tc_model { TC_MODEL_MS2V = 0, TC_MODEL_TTP = 1, TC_MODEL_CRN = 2 };
+typedef struct tc_stimulus {
+  double start, duration, period, count, amp;
+} tc_stimulus;
+typedef struct tc_sim_desc {
+  int model;                 /* tc_model */
+  const void* params;        /* tc_ms2v_params / tc_ttp_params / tc_crn_params */
+  void* states;              /* (S, n) state block */
+  const void* tab[3];        /* model tables in the order of its *_step call */
+  tc_pcg* pcg;               /* plan over A: SELL pattern, A values, Dinv */
+  const void* Mv;            /* SELL values of M */
+  const void* Kv;            /* SELL values of K (already divided by Chi) */
+  int dtype;
+  int64_t n;
+  double dt, dt_ionic, Cm, theta;
+  void* x[3];                /* solution buffers: current, previous, next */
+  void* w;                   /* (n) scratch vector */
+  void* AT;                  /* (n) activation times or NULL */
+  void* RT;                  /* (n) repolarisation times */
+  const uint32_t* stim_mask; /* (n) stimulus bits or NULL */
+  int n_stim;
+  tc_stimulus stim[TC_MAX_STIMULI];
+  double a_tol, r_tol;
+  int max_iter, linear_guess;
+  int depth;                 /* steps enqueued ahead of the status read, >= 1 */
+} tc_sim_desc;
+typedef struct tc_sim tc_sim;
+int tc_sim_create(tc_sim** sim, const tc_sim_desc* desc);
+int tc_sim_destroy(tc_sim* sim);
+/* Run nsteps more steps.  iters_host (nsteps) receives each step's PCG
+ * iteration count (reference convention).  When frames != NULL, after every
+ * step whose global index is a multiple of ts_per_frame the solution is
+ * copied to frames + k*n for the next frame k < max_frames.  Synchronizes
+ * `stream` before returning. */
+int tc_sim_advance(tc_sim* sim, int64_t nsteps, int32_t* iters_host, void* frames,
+                   int64_t ts_per_frame, int64_t max_frames, void* stream);
+/* Apply the LAT/RT test to the last step's solution (it is otherwise folded
+ * into the next step's ionic pass).  Idempotent. */
+int tc_sim_finish(tc_sim* sim, void* stream);
+/* t (double), steps done, index 0..2 of desc.x holding the solution, frames
+ * written. */
+int tc_sim_status(tc_sim* sim, double* t_host, int64_t* steps_host, int* x_index_host,
+                  int64_t* frames_host);
+/* Per-kernel timing of the runtime with HIP events around each launch
+ * (off by default; for measurement).  get returns total ms and launches for
+ * [0] ionic, [1] rhs+pcg start, [2] pcg spmv, [3] pcg update, [4] check,
+ * counting only PCG iterations that ran. */
+int tc_sim_set_timing(tc_sim* sim, int on);
+int tc_sim_get_timing(tc_sim* sim, double* ms_host, int64_t* count_host);
 
 /* LAT/RT update alone (monodomain.py:174-176), for the last step of a run. */
 int tc_lat_update(const void* u, void* AT, void* RT, int64_t n, double t, int dtype,

@@ -423,3 +423,54 @@ def test_large_cube_properties():
             assert S.spmv("K", ones).abs().max().item() < 1e-4
             assert (sim.activation_time > 0).sum().item() >= 1000
     assert torch.equal(outs[0], outs[1])
+
+
+def _cube_sim(N, model, dtype, T=100.0):
+    from torchcor_amd.simulator import Monodomain
+    nodes, elems, regions, fib = meshes.cube_workload(N)
+    sim = Monodomain(_gpu_model(model, dtype), T=T, dt=0.01, device=DEV, dtype=dtype)
+    sim.load_arrays(nodes, elems, regions, fib)
+    sim.add_conductivity([0], 0.5272, 0.2076)
+    sim.add_conductivity([1], 0.9074, 0.3332)
+    sim.add_stimulus(meshes.corner_nodes(N, 3), 0.0, 1.0, 100)
+    sim.add_stimulus(meshes.corner_nodes(N, 2), 3.0, 0.5, 60, period=4.0, count=3)
+    return sim
+
+
+@pytest.mark.parametrize("model", ["ttp", "crn"])
+def test_runtime_pipeline_matches_python_loop(model):
+    """The native pipelined loop (any depth, speculative iterations with
+    stalls) equals the plain per-step loop bit for bit."""
+    _say(f"[gpu] runtime vs python loop {model}")
+    N, steps = 11, 600
+    runs = []
+    for depth in (1, 6):
+        sim = _cube_sim(N, model, torch.float32)
+        sim.start(1e-5, 1e-5, 100, snapshot_interval=1.0, depth=depth)
+        sim.advance(steps)
+        sim.finish()
+        runs.append((sim.u.clone(), list(sim.iterations), sim.activation_time.clone(),
+                     sim.frames.clone()))
+    assert torch.equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
+    assert torch.equal(runs[0][2], runs[1][2]) and torch.equal(runs[0][3], runs[1][3])
+    assert len(set(runs[0][1])) > 1        # iteration counts vary, so stalls were exercised
+    # plain loop through step()
+    sim = _cube_sim(N, model, torch.float32)
+    sim.assemble()
+    u = sim.ionic_model.initialize(sim.n_nodes)
+    sim._init_solver(u, True)
+    AT = torch.full_like(u, -1.0)
+    RT = torch.full_like(u, -1.0)
+    t, prev, its = 0, None, []
+    for n in range(steps):
+        t += 0.01
+        lat = (AT, RT, prev) if prev is not None else None
+        u, it, _, _ = sim.step(sim._x, t, 1e-5, 1e-5, 100, False, lat)
+        prev = t
+        its.append(it)
+    assert its == runs[0][1]
+    assert torch.equal(u, runs[0][0])
+    import torchcor_amd._lib as L
+    L.call("tc_lat_update", L.ptr(u), L.ptr(AT), L.ptr(RT), u.numel(), float(prev), 0,
+           L.stream_handle(DEV))
+    assert torch.equal(AT, runs[0][2])

@@ -34,7 +34,8 @@ class Sell(C.Structure):
 class IonicIO(C.Structure):
     _fields_ = [("n", i64), ("dt", dbl), ("dt_rhs", dbl), ("Cm", dbl), ("V", p), ("w", p),
                 ("dV", p), ("stim_mask", p), ("stim_active", C.c_uint32),
-                ("stim_amp", dbl * MAX_STIMULI), ("AT", p), ("RT", p), ("lat_t", dbl)]
+                ("stim_amp", dbl * MAX_STIMULI), ("AT", p), ("RT", p), ("lat_t", dbl),
+                ("gate", p)]
 
 
 MS2V_FIELDS = ["tau_in", "tau_out", "tau_open", "tau_close", "u_gate", "u_crit", "vmin", "vmax"]
@@ -63,6 +64,22 @@ class TTPParams(C.Structure):
 
 class CRNParams(C.Structure):
     _fields_ = [(f, dbl) for f in CRN_FIELDS]
+
+
+class Stimulus(C.Structure):
+    _fields_ = [("start", dbl), ("duration", dbl), ("period", dbl), ("count", dbl), ("amp", dbl)]
+
+
+class SimDesc(C.Structure):
+    _fields_ = [("model", i32), ("params", p), ("states", p), ("tab", p * 3), ("pcg", p),
+                ("Mv", p), ("Kv", p), ("dtype", i32), ("n", i64), ("dt", dbl), ("dt_ionic", dbl),
+                ("Cm", dbl), ("theta", dbl), ("x", p * 3), ("w", p), ("AT", p), ("RT", p),
+                ("stim_mask", p), ("n_stim", i32), ("stim", Stimulus * MAX_STIMULI),
+                ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32), ("linear_guess", i32),
+                ("depth", i32)]
+
+
+MODEL_MS2V, MODEL_TTP, MODEL_CRN = 0, 1, 2
 
 
 def _proto(name, *args):
@@ -106,6 +123,13 @@ FUNCS = {
     "tc_crn_tables": (C.POINTER(CRNParams), dbl, p, p, p, i32, p),
     "tc_crn_step": (C.POINTER(CRNParams), p, p, p, p, IO_P, i32, p),
     "tc_lat_update": (p, p, p, i64, dbl, i32, p),
+    "tc_sim_create": (C.POINTER(p), C.POINTER(SimDesc)),
+    "tc_sim_destroy": (p,),
+    "tc_sim_advance": (p, i64, C.POINTER(C.c_int32), p, i64, i64, p),
+    "tc_sim_finish": (p, p),
+    "tc_sim_status": (p, C.POINTER(dbl), I64_P, INT_P, I64_P),
+    "tc_sim_set_timing": (p, i32),
+    "tc_sim_get_timing": (p, C.POINTER(dbl), I64_P),
 }
 for _n, _a in FUNCS.items():
     _proto(_n, *_a)

@@ -152,6 +152,7 @@ __global__ void __launch_bounds__(kBlock)
 k_ms2v(tc_ms2v_params p, T* __restrict__ H, tc_ionic_io io) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= io.n) return;
+  if (io.gate && *(volatile const int*)io.gate) return;
   const T U = ((const T*)io.V)[i];
   const T h = H[i];
   const T DV = (T)(p.vmax - p.vmin);
@@ -308,6 +309,7 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t n = io.n;
   if (i >= n) return;
+  if (io.gate && *(volatile const int*)io.gate) return;
   const T V = ((const T*)io.V)[i];
   T s[TS_N];
 #pragma unroll
@@ -545,6 +547,7 @@ k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t n = io.n;
   if (i >= n) return;
+  if (io.gate && *(volatile const int*)io.gate) return;
   const T V = ((const T*)io.V)[i];
   T s[CS_N];
 #pragma unroll

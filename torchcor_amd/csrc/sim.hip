@@ -1,0 +1,460 @@
+// Time-step runtime: the monodomain time loop in native code, pipelined.
+//
+// Per step the host enqueues (all on one stream, nothing waits):
+//   1. the model's fused ionic kernel (states, w, stimulus, LAT/RT of the
+//      previous step's solution),
+//   2. the fused RHS + PCG start pass,
+//   3. K speculative PCG iterations (spmv + update passes), where K is the
+//      largest iteration count among the last steps,
+//   4. a one-block stopping check that stores the step status and, when the
+//      solve is not finished, raises ctl->stall,
+//   5. an optional snapshot copy and the status copy to pinned host memory.
+// The host keeps up to `depth` steps in flight and validates the oldest.
+// While ctl->stall is set every later kernel returns at once, so a step that
+// needed more than K iterations leaves the device exactly where the plain
+// loop would be; the host then launches the remaining iterations, clears the
+// flag and re-issues the later steps from that state.
+#include <deque>
+#include <vector>
+
+#include "pcg_kernels.h"
+
+namespace tc {
+namespace {
+
+constexpr int kRing = 64;
+constexpr int kTimers = 5;  // ionic, rhs+start, spmv, update, check
+constexpr int kHistory = 16;
+
+struct HostState {
+  int64_t steps = 0;
+  double t = 0.0;
+  int rot[3] = {0, 1, 2};  // indices into desc.x of current, previous, next
+  int64_t frames = 0;
+  bool lat_pending = false;
+  double lat_t = 0.0;
+};
+
+struct Plan {
+  int64_t step;
+  int slot;
+  int K;
+  int64_t frame;  // frame written after this step or -1
+  HostState after;
+};
+
+}  // namespace
+}  // namespace tc
+
+struct tc_sim {
+  tc_sim_desc d;
+  union {
+    tc_ms2v_params ms2v;
+    tc_ttp_params ttp;
+    tc_crn_params crn;
+  } params;
+  tc::StepStatus* dev_status;
+  tc::StepStatus* host_status;
+  hipEvent_t done_ev[tc::kRing];
+  tc::HostState vs;  // validated
+  std::deque<int> recent;
+  int timing;
+  std::vector<hipEvent_t> tev[tc::kRing];
+  double acc_ms[tc::kTimers];
+  int64_t acc_n[tc::kTimers];
+  // frame buffer of the current advance call
+  void* frames;
+  int64_t tpf, max_frames;
+};
+
+namespace tc {
+namespace {
+
+uint32_t active_bits(const tc_sim_desc& d, double t) {
+  uint32_t bits = 0;
+  for (int k = 0; k < d.n_stim; ++k) {
+    const tc_stimulus& s = d.stim[k];
+    const double p_t = fmod(t, s.period);
+    const double t_t = t / s.period;
+    // Python's % on positive doubles equals fmod.
+    if (t_t <= s.count && p_t >= s.start && p_t <= s.start + s.duration) bits |= 1u << k;
+  }
+  return bits;
+}
+
+int ev_reserve(tc_sim* S, int slot, size_t count) {
+  while (S->tev[slot].size() < count) {
+    hipEvent_t e;
+    TC_HIP(hipEventCreate(&e));
+    S->tev[slot].push_back(e);
+  }
+  return TC_OK;
+}
+
+int ionic_launch(tc_sim* S, const tc_ionic_io* io, hipStream_t s) {
+  const tc_sim_desc& d = S->d;
+  switch (d.model) {
+    case TC_MODEL_MS2V:
+      return tc_ms2v_step(&S->params.ms2v, d.states, io, d.dtype, s);
+    case TC_MODEL_TTP:
+      return tc_ttp_step(&S->params.ttp, d.states, d.tab[0], d.tab[1], d.tab[2], io, d.dtype, s);
+    case TC_MODEL_CRN:
+      return tc_crn_step(&S->params.crn, d.states, d.tab[0], d.tab[1], d.tab[2], io, d.dtype, s);
+  }
+  set_error("unknown model %d", d.model);
+  return TC_ERR_ARG;
+}
+
+template <class T>
+int launch_iters(tc_sim* S, int k0, int k1, T* xs, hipStream_t s, std::vector<hipEvent_t>* ev,
+                 size_t ev_base) {
+  tc_pcg* P = S->d.pcg;
+  const dim3 g(P->G), blk(kBlock);
+  T* pp[2] = {(T*)P->p[0], (T*)P->p[1]};
+  for (int k = k0; k < k1; ++k) {
+    T* pin = pp[k & 1];
+    T* pout = pp[(k + 1) & 1];
+    const size_t e = ev_base + 4 * (size_t)(k - k0);
+    if (ev) TC_HIP(hipEventRecord((*ev)[e], s));
+    hipLaunchKernelGGL(k_pcg_spmv<T>, g, blk, 0, s, P->n, P->A.slice_ptr, P->A.col,
+                       (const T*)P->Av, (const T*)P->Dinv, (const T*)P->r, pin, pout, (T*)P->Ap,
+                       k, P->ctl, P->part[0], P->part[2], P->part[1]);
+    TC_LAUNCH();
+    if (ev) {
+      TC_HIP(hipEventRecord((*ev)[e + 1], s));
+      TC_HIP(hipEventRecord((*ev)[e + 2], s));
+    }
+    hipLaunchKernelGGL(k_pcg_update<T>, g, blk, 0, s, P->n, (const T*)P->Dinv, pout,
+                       (const T*)P->Ap, xs, (T*)P->r, k, P->ctl, P->part[1], P->part[2]);
+    TC_LAUNCH();
+    if (ev) TC_HIP(hipEventRecord((*ev)[e + 3], s));
+  }
+  return TC_OK;
+}
+
+int check_launch(tc_sim* S, int k, int slot, hipStream_t s) {
+  tc_pcg* P = S->d.pcg;
+  hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kBlock), 0, s, k, P->G, P->ctl, P->part[2],
+                     S->dev_status + slot, 1, S->d.max_iter);
+  TC_LAUNCH();
+  return TC_OK;
+}
+
+template <class T>
+int copy_frame(tc_sim* S, const void* src, int64_t frame, const int* gate, hipStream_t s) {
+  const int64_t n = S->d.n;
+  T* dst = (T*)S->frames + frame * n;
+  hipLaunchKernelGGL(k_copy_gated<T>, dim3((int)ceil_div(n, 256)), dim3(256), 0, s, gate,
+                     (const T*)src, dst, n);
+  TC_LAUNCH();
+  return TC_OK;
+}
+
+int desired_K(tc_sim* S) {
+  int K = 1;
+  for (int v : S->recent) K = v > K ? v : K;
+  if (K > S->d.max_iter) K = S->d.max_iter;
+  return K;
+}
+
+// Enqueue the step after planned state ps; ps becomes the state after it.
+template <class T>
+int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
+  const tc_sim_desc& d = S->d;
+  tc_pcg* P = d.pcg;
+  const int64_t step = ps.steps + 1;
+  const double t = ps.t + d.dt;
+  const int slot = (int)(step % kRing);
+  const int K = desired_K(S);
+  const int c = ps.rot[0], p = ps.rot[1], q = ps.rot[2];
+  std::vector<hipEvent_t>* ev = nullptr;
+  if (S->timing) {
+    int rc = ev_reserve(S, slot, 6 + 4 * (size_t)K);
+    if (rc) return rc;
+    ev = &S->tev[slot];
+  }
+  // 1. ionic
+  tc_ionic_io io{};
+  io.n = d.n;
+  io.dt = d.dt_ionic;
+  io.dt_rhs = d.dt;
+  io.Cm = d.Cm;
+  io.V = d.x[c];
+  io.w = d.w;
+  io.dV = nullptr;
+  const uint32_t bits = active_bits(d, t);
+  io.stim_mask = bits ? d.stim_mask : nullptr;
+  io.stim_active = bits;
+  for (int k = 0; k < d.n_stim; ++k) io.stim_amp[k] = d.stim[k].amp;
+  if (d.AT && ps.lat_pending) {
+    io.AT = d.AT;
+    io.RT = d.RT;
+    io.lat_t = ps.lat_t;
+  }
+  io.gate = &P->ctl->stall;
+  if (ev) TC_HIP(hipEventRecord((*ev)[0], s));
+  int rc = ionic_launch(S, &io, s);
+  if (rc) return rc;
+  if (ev) TC_HIP(hipEventRecord((*ev)[1], s));
+  // 2. RHS + PCG start
+  const dim3 g(P->G), blk(kBlock);
+  const double coef = (1.0 - d.theta) * d.dt;
+  T* xs;
+  if (ev) TC_HIP(hipEventRecord((*ev)[2], s));
+  if (d.linear_guess) {
+    hipLaunchKernelGGL((k_rhs_init<T, true>), g, blk, 0, s, d.n, P->A.slice_ptr, P->A.col,
+                       (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Av, (const T*)P->Dinv,
+                       (const T*)d.w, (const T*)d.x[c], (const T*)d.x[p], (T*)d.x[q],
+                       (T*)P->r, coef, P->ctl, d.a_tol, d.r_tol, P->part[0]);
+    xs = (T*)d.x[q];
+  } else {
+    hipLaunchKernelGGL((k_rhs_init<T, false>), g, blk, 0, s, d.n, P->A.slice_ptr, P->A.col,
+                       (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Av, (const T*)P->Dinv,
+                       (const T*)d.w, (const T*)d.x[c], (const T*)nullptr, (T*)nullptr,
+                       (T*)P->r, coef, P->ctl, d.a_tol, d.r_tol, P->part[0]);
+    xs = (T*)d.x[c];
+  }
+  TC_LAUNCH();
+  if (ev) TC_HIP(hipEventRecord((*ev)[3], s));
+  // 3. speculative iterations
+  rc = launch_iters<T>(S, 0, K, xs, s, ev, 6);
+  if (rc) return rc;
+  // 4. check
+  if (ev) TC_HIP(hipEventRecord((*ev)[4], s));
+  rc = check_launch(S, K - 1, slot, s);
+  if (rc) return rc;
+  if (ev) TC_HIP(hipEventRecord((*ev)[5], s));
+  // state after the step
+  HostState nx = ps;
+  nx.steps = step;
+  nx.t = t;
+  if (d.linear_guess) {
+    nx.rot[0] = q;
+    nx.rot[1] = c;
+    nx.rot[2] = p;
+  }
+  nx.lat_pending = true;
+  nx.lat_t = t;
+  // 5. snapshot
+  plan.frame = -1;
+  if (S->frames && S->tpf > 0 && step % S->tpf == 0 && ps.frames < S->max_frames) {
+    rc = copy_frame<T>(S, d.x[nx.rot[0]], ps.frames, &P->ctl->stall, s);
+    if (rc) return rc;
+    plan.frame = ps.frames;
+    nx.frames = ps.frames + 1;
+  }
+  TC_HIP(hipMemcpyAsync(S->host_status + slot, S->dev_status + slot, sizeof(StepStatus),
+                        hipMemcpyDeviceToHost, s));
+  TC_HIP(hipEventRecord(S->done_ev[slot], s));
+  plan.step = step;
+  plan.slot = slot;
+  plan.K = K;
+  plan.after = nx;
+  ps = nx;
+  return TC_OK;
+}
+
+// The step of `plan` stalled: run its remaining iterations until the stop
+// rule or max_iter, then redo its snapshot copy (which the stall gated).
+template <class T>
+int resolve_stall(tc_sim* S, const Plan& plan, hipStream_t s) {
+  const tc_sim_desc& d = S->d;
+  tc_pcg* P = d.pcg;
+  StepStatus* st = S->host_status + plan.slot;
+  T* xs = (T*)d.x[plan.after.rot[0]];
+  int it = st->iter;
+  int chunk = it > 0 ? it : 1;
+  while (st->stalled) {
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, &P->ctl->stall, 0);
+    TC_LAUNCH();
+    const int c = chunk < d.max_iter - it ? chunk : d.max_iter - it;
+    int rc = launch_iters<T>(S, it, it + c, xs, s, nullptr, 0);
+    if (rc) return rc;
+    it += c;
+    rc = check_launch(S, it - 1, plan.slot, s);
+    if (rc) return rc;
+    TC_HIP(hipMemcpyAsync(st, S->dev_status + plan.slot, sizeof(StepStatus),
+                          hipMemcpyDeviceToHost, s));
+    TC_HIP(hipStreamSynchronize(s));
+    chunk = chunk * 2 < 64 ? chunk * 2 : 64;
+  }
+  if (plan.frame >= 0) {
+    int rc = copy_frame<T>(S, xs, plan.frame, nullptr, s);
+    if (rc) return rc;
+  }
+  return TC_OK;
+}
+
+void accumulate_timing(tc_sim* S, const Plan& plan, int iters_run) {
+  std::vector<hipEvent_t>& ev = S->tev[plan.slot];
+  float ms = 0.f;
+  const int pairs[3][2] = {{0, 1}, {2, 3}, {4, 5}};
+  const int which[3] = {0, 1, 4};
+  for (int k = 0; k < 3; ++k) {
+    if (hipEventElapsedTime(&ms, ev[pairs[k][0]], ev[pairs[k][1]]) == hipSuccess) {
+      S->acc_ms[which[k]] += ms;
+      S->acc_n[which[k]] += 1;
+    }
+  }
+  const int kk = iters_run < plan.K ? iters_run : plan.K;
+  for (int k = 0; k < kk; ++k) {
+    const size_t e = 6 + 4 * (size_t)k;
+    if (hipEventElapsedTime(&ms, ev[e], ev[e + 1]) == hipSuccess) {
+      S->acc_ms[2] += ms;
+      S->acc_n[2] += 1;
+    }
+    if (hipEventElapsedTime(&ms, ev[e + 2], ev[e + 3]) == hipSuccess) {
+      S->acc_ms[3] += ms;
+      S->acc_n[3] += 1;
+    }
+  }
+}
+
+template <class T>
+int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
+  const int depth = S->d.depth < 1 ? 1 : (S->d.depth > kRing / 2 ? kRing / 2 : S->d.depth);
+  std::deque<Plan> inflight;
+  HostState ps = S->vs;
+  int64_t enq = 0, done = 0;
+  while (done < nsteps) {
+    while ((int)inflight.size() < depth && enq < nsteps) {
+      Plan plan;
+      int rc = enqueue<T>(S, ps, plan, s);
+      if (rc) return rc;
+      inflight.push_back(plan);
+      ++enq;
+    }
+    Plan plan = inflight.front();
+    TC_HIP(hipEventSynchronize(S->done_ev[plan.slot]));
+    StepStatus* st = S->host_status + plan.slot;
+    if (S->timing) accumulate_timing(S, plan, st->iter);
+    if (st->stalled) {
+      int rc = resolve_stall<T>(S, plan, s);
+      if (rc) return rc;
+      enq -= (int64_t)inflight.size() - 1;
+      inflight.clear();
+      ps = plan.after;
+    } else {
+      inflight.pop_front();
+    }
+    iters[done] = st->n_iter;
+    S->recent.push_back(st->iter > 0 ? st->iter : 1);
+    if ((int)S->recent.size() > kHistory) S->recent.pop_front();
+    S->vs = plan.after;
+    ++done;
+  }
+  TC_HIP(hipStreamSynchronize(s));
+  return TC_OK;
+}
+
+}  // namespace
+}  // namespace tc
+
+using namespace tc;
+
+extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
+  TC_CHECK(out && d && d->pcg && d->states && d->Mv && d->Kv && d->w && d->n > 0, TC_ERR_ARG,
+           "tc_sim_create: bad arguments");
+  TC_CHECK(d->dtype == TC_F32 || d->dtype == TC_F64, TC_ERR_ARG, "bad dtype %d", d->dtype);
+  TC_CHECK(d->x[0] && d->x[1] && d->x[2], TC_ERR_ARG, "tc_sim_create: need three x buffers");
+  TC_CHECK(d->n_stim >= 0 && d->n_stim <= TC_MAX_STIMULI, TC_ERR_ARG, "bad stimulus count");
+  TC_CHECK(d->n_stim == 0 || d->stim_mask, TC_ERR_ARG, "stimuli need a mask");
+  TC_CHECK(d->max_iter >= 0, TC_ERR_ARG, "max_iter < 0");
+  TC_CHECK(!d->AT || d->RT, TC_ERR_ARG, "AT without RT");
+  TC_CHECK(d->pcg->n == d->n && d->pcg->dtype == d->dtype, TC_ERR_ARG,
+           "tc_sim_create: PCG plan does not match n/dtype");
+  tc_sim* S = new tc_sim();
+  S->d = *d;
+  switch (d->model) {
+    case TC_MODEL_MS2V: S->params.ms2v = *(const tc_ms2v_params*)d->params; break;
+    case TC_MODEL_TTP: S->params.ttp = *(const tc_ttp_params*)d->params; break;
+    case TC_MODEL_CRN: S->params.crn = *(const tc_crn_params*)d->params; break;
+    default:
+      delete S;
+      set_error("unknown model %d", d->model);
+      return TC_ERR_ARG;
+  }
+  S->d.params = nullptr;
+  hipError_t e = hipMalloc((void**)&S->dev_status, sizeof(StepStatus) * kRing);
+  if (e == hipSuccess)
+    e = hipHostMalloc((void**)&S->host_status, sizeof(StepStatus) * kRing, hipHostMallocDefault);
+  for (int k = 0; k < kRing && e == hipSuccess; ++k)
+    e = hipEventCreateWithFlags(&S->done_ev[k], hipEventDisableTiming);
+  if (e != hipSuccess) {
+    set_error("tc_sim_create: %s", hipGetErrorString(e));
+    return TC_ERR_HIP;
+  }
+  // The stall flag lives in the shared PCG control block; start clear.
+  e = hipMemset(&d->pcg->ctl->stall, 0, sizeof(int));
+  if (e != hipSuccess) {
+    set_error("tc_sim_create: %s", hipGetErrorString(e));
+    return TC_ERR_HIP;
+  }
+  S->timing = 0;
+  for (int k = 0; k < kTimers; ++k) {
+    S->acc_ms[k] = 0.0;
+    S->acc_n[k] = 0;
+  }
+  *out = S;
+  return TC_OK;
+}
+
+extern "C" int tc_sim_destroy(tc_sim* S) {
+  if (!S) return TC_OK;
+  for (int k = 0; k < kRing; ++k) {
+    if (S->done_ev[k]) (void)hipEventDestroy(S->done_ev[k]);
+    for (hipEvent_t e : S->tev[k]) (void)hipEventDestroy(e);
+  }
+  if (S->dev_status) (void)hipFree(S->dev_status);
+  if (S->host_status) (void)hipHostFree(S->host_status);
+  delete S;
+  return TC_OK;
+}
+
+extern "C" int tc_sim_advance(tc_sim* S, int64_t nsteps, int32_t* iters, void* frames,
+                              int64_t tpf, int64_t max_frames, void* stream) {
+  TC_CHECK(S && iters && nsteps >= 0, TC_ERR_ARG, "tc_sim_advance: bad arguments");
+  S->frames = frames;
+  S->tpf = tpf;
+  S->max_frames = frames ? max_frames : 0;
+  hipStream_t s = (hipStream_t)stream;
+  if (S->d.dtype == TC_F32) return advance<float>(S, nsteps, iters, s);
+  return advance<double>(S, nsteps, iters, s);
+}
+
+extern "C" int tc_sim_finish(tc_sim* S, void* stream) {
+  TC_CHECK(S, TC_ERR_ARG, "tc_sim_finish: null");
+  if (S->d.AT && S->vs.lat_pending)
+    return tc_lat_update(S->d.x[S->vs.rot[0]], S->d.AT, S->d.RT, S->d.n, S->vs.lat_t,
+                         S->d.dtype, stream);
+  return TC_OK;
+}
+
+extern "C" int tc_sim_status(tc_sim* S, double* t, int64_t* steps, int* x_index,
+                             int64_t* frames) {
+  TC_CHECK(S && t && steps && x_index && frames, TC_ERR_ARG, "tc_sim_status: bad arguments");
+  *t = S->vs.t;
+  *steps = S->vs.steps;
+  *x_index = S->vs.rot[0];
+  *frames = S->vs.frames;
+  return TC_OK;
+}
+
+extern "C" int tc_sim_set_timing(tc_sim* S, int on) {
+  TC_CHECK(S, TC_ERR_ARG, "tc_sim_set_timing: null");
+  S->timing = on ? 1 : 0;
+  for (int k = 0; k < kTimers; ++k) {
+    S->acc_ms[k] = 0.0;
+    S->acc_n[k] = 0;
+  }
+  return TC_OK;
+}
+
+extern "C" int tc_sim_get_timing(tc_sim* S, double* ms, int64_t* cnt) {
+  TC_CHECK(S && ms && cnt, TC_ERR_ARG, "tc_sim_get_timing: bad arguments");
+  for (int k = 0; k < kTimers; ++k) {
+    ms[k] = S->acc_ms[k];
+    cnt[k] = S->acc_n[k];
+  }
+  return TC_OK;
+}

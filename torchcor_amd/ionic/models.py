@@ -37,6 +37,14 @@ class ModifiedMS2v(IonicModel):
     def V_init(self):
         return self.vmin
 
+    MODEL = L.MODEL_MS2V
+
+    def c_params(self):
+        return L.MS2vParams(**self._param_values())
+
+    def c_tables(self):
+        return (None, None, None)
+
     def step(self, io):
         p = L.MS2vParams(**self._param_values())
         L.call("tc_ms2v_step", C.byref(p), L.ptr(self._states), C.byref(io),
@@ -89,6 +97,14 @@ class TenTusscherPanfilov(IonicModel):
         L.call("tc_ttp_tables", C.byref(p), float(self.dt), L.ptr(self.V_tab), L.ptr(self.CaSS_tab),
                L.ptr(self.VEk_tab), L.dtype_code(self.dtype), L.stream_handle(self.device))
 
+    MODEL = L.MODEL_TTP
+
+    def c_params(self):
+        return self._params()
+
+    def c_tables(self):
+        return (self.V_tab, self.CaSS_tab, self.VEk_tab)
+
     def step(self, io):
         p = self._params()
         L.call("tc_ttp_step", C.byref(p), L.ptr(self._states), L.ptr(self.V_tab),
@@ -134,6 +150,14 @@ class CourtemancheRamirezNattel(IonicModel):
         p = L.CRNParams(**self._param_values())
         L.call("tc_crn_tables", C.byref(p), float(self.dt), L.ptr(self.Cai_tab), L.ptr(self.V_tab),
                L.ptr(self.fn_tab), L.dtype_code(self.dtype), L.stream_handle(self.device))
+
+    MODEL = L.MODEL_CRN
+
+    def c_params(self):
+        return L.CRNParams(**self._param_values())
+
+    def c_tables(self):
+        return (self.Cai_tab, self.V_tab, self.fn_tab)
 
     def step(self, io):
         p = L.CRNParams(**self._param_values())

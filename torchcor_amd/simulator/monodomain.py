@@ -45,8 +45,7 @@ class Monodomain:
         self.mesh_path = None
         self.result_path = None
         self._x = None
-        self.kernel_timing = False
-        self._events = {"ionic": [], "rhs": []}
+        self._sim = None
 
     # ------------------------------------------------------------ set-up
     def load_mesh(self, path="Data/ventricle/Case_1", unit_conversion=1000):
@@ -119,45 +118,16 @@ class Monodomain:
                                   stim_mask=stim.mask if bits else None, stim_bits=bits,
                                   stim_amp=stim.amplitudes() if bits else (), AT=AT, RT=RT,
                                   lat_t=lat_t)
-        ev = self._event_pair("ionic")
         self.ionic_model.step(io)
-        if ev:
-            ev[1].record()
 
     def _electric(self, u, a_tol, r_tol, max_iter):
         sys_ = self.system
-        ev = self._event_pair("rhs")
         sys_.rhs(self._w, u, (1 - self.theta) * self.dt, self._b)
-        if ev:
-            ev[1].record()
         n_iter = sys_.pcg_solve(self._b, self._x, self._x_prev, self._x_next, a_tol, r_tol,
                                 max_iter, self._linear_guess)
         if self._linear_guess:
             self._x_prev, self._x, self._x_next = self._x, self._x_next, self._x_prev
         return self._x, n_iter
-
-    def _event_pair(self, name):
-        if not self.kernel_timing:
-            return None
-        pair = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-        pair[0].record()
-        self._events[name].append(pair)
-        return pair
-
-    def set_kernel_timing(self, on):
-        """Time the ionic and RHS kernels and the PCG passes with HIP events
-        on the stream they run on (for measurement)."""
-        self.kernel_timing = bool(on)
-        self._events = {"ionic": [], "rhs": []}
-        if self.system is not None:
-            self.system.pcg_timing(bool(on))
-
-    def kernel_times(self):
-        """{kernel: (total_ms, launches)} since ``set_kernel_timing(True)``."""
-        torch.cuda.synchronize(self.device)
-        out = {k: (sum(a.elapsed_time(b) for a, b in v), len(v)) for k, v in self._events.items()}
-        out.update({f"pcg_{k}": v for k, v in self.system.pcg_timing().items()})
-        return out
 
     def step(self, u, t, a_tol, r_tol, max_iter, verbose=False, _lat=None):
         """One time step (monodomain.py:109-139): returns (u, n_iter, ionic_s, electric_s)."""
@@ -194,65 +164,134 @@ class Monodomain:
         return self.finish()
 
     # The loop of ``solve`` split in three so that callers (benchmarks,
-    # servers) can run it in pieces: start -> advance(k)... -> finish.
+    # servers) can run it in pieces: start -> advance(k)... -> finish.  The
+    # loop itself runs in the native runtime (tc_sim_*): steps are pipelined
+    # on the device and no host round trip is needed per step.
     def start(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True,
-              snapshot_interval=1, verbose=False):
+              snapshot_interval=1, verbose=False, depth=4):
+        import ctypes as C
         self.assemble()
-        u = self.ionic_model.initialize(self.n_nodes)
-        self._init_solver(u, linear_guess)
-        self._run = dict(a_tol=a_tol, r_tol=r_tol, max_iter=max_iter, calc=calculate_AT_RT,
-                         tpf=int(snapshot_interval / self.dt), snapshot_interval=snapshot_interval,
-                         verbose=verbose, t=0, n=0, prev_t=None, n_total_iter=0, tot_ion=0.0,
-                         tot_ele=0.0, t0=time.time())
+        ion = self.ionic_model
+        u = ion.initialize(self.n_nodes)
+        self._X = [u.clone(), u.clone(), torch.empty_like(u)]
+        self._w = torch.empty_like(u)
+        tpf = int(snapshot_interval / self.dt)
+        nframes = self.nt // tpf if tpf > 0 else 0
+        self._frames = torch.empty((nframes + 1, self.n_nodes), dtype=self.dtype, device=self.device)
+        self._frames[0].copy_(u)
         self.activation_time = self.repolarization_time = None
         if calculate_AT_RT:
             self.activation_time = torch.ones_like(u) * -1
             self.repolarization_time = torch.ones_like(u) * -1
-        self.frames = [u.clone()]
+        d = L.SimDesc()
+        d.model = ion.MODEL
+        self._cparams = ion.c_params()
+        d.params = C.cast(C.pointer(self._cparams), C.c_void_p)
+        d.states = L.ptr(ion.states)
+        for k, tab in enumerate(ion.c_tables()):
+            d.tab[k] = L.ptr(tab)
+        d.pcg = self.system._pcg
+        d.Mv, d.Kv = L.ptr(self.system.sM), L.ptr(self.system.sK)
+        d.dtype = L.dtype_code(self.dtype)
+        d.n = self.n_nodes
+        d.dt, d.dt_ionic, d.Cm, d.theta = self.dt, ion.dt, self.Cm, self.theta
+        for k in range(3):
+            d.x[k] = L.ptr(self._X[k])
+        d.w = L.ptr(self._w)
+        d.AT = L.ptr(self.activation_time)
+        d.RT = L.ptr(self.repolarization_time)
+        st = self.stimuli.stimulus_list
+        d.n_stim = len(st)
+        d.stim_mask = L.ptr(self.stimuli.mask) if st else None
+        for k, sc in enumerate(st):
+            d.stim[k] = L.Stimulus(float(sc.start), float(sc.duration), float(sc.period),
+                                   float(sc.count), float(sc.intensity))
+        d.a_tol, d.r_tol, d.max_iter = float(a_tol), float(r_tol), int(max_iter)
+        d.linear_guess = 1 if linear_guess else 0
+        d.depth = int(depth)
+        self._free_sim()
+        h = C.c_void_p(0)
+        L.call("tc_sim_create", C.byref(h), C.byref(d))
+        self._sim = h
+        self._run = dict(calc=calculate_AT_RT, tpf=tpf, nframes=nframes,
+                         snapshot_interval=snapshot_interval, verbose=verbose, n_total_iter=0,
+                         t0=time.time())
         self.iterations = []
 
+    def _free_sim(self):
+        if getattr(self, "_sim", None):
+            L.lib.tc_sim_destroy(self._sim)
+            self._sim = None
+
+    def __del__(self):
+        try:
+            self._free_sim()
+        except Exception:
+            pass
+
+    def sim_status(self):
+        import ctypes as C
+        t, steps, xi, fr = C.c_double(), C.c_int64(), C.c_int(), C.c_int64()
+        L.call("tc_sim_status", self._sim, C.byref(t), C.byref(steps), C.byref(xi), C.byref(fr))
+        return t.value, steps.value, xi.value, fr.value
+
     def advance(self, k):
+        """Run k more time steps in the native runtime."""
+        import ctypes as C
         R = self._run
-        AT, RT = self.activation_time, self.repolarization_time
-        for _ in range(int(k)):
-            R["n"] += 1
-            R["t"] += self.dt
-            t = R["t"]
-            lat = (AT, RT, R["prev_t"]) if (R["calc"] and R["prev_t"] is not None) else None
-            u, n_iter, it_s, et_s = self.step(self._x, t, R["a_tol"], R["r_tol"], R["max_iter"],
-                                              R["verbose"], lat)
-            R["prev_t"] = t
-            R["n_total_iter"] += n_iter
-            self.iterations.append(n_iter)
-            R["tot_ion"] += it_s
-            R["tot_ele"] += et_s
-            if R["tpf"] > 0 and R["n"] % R["tpf"] == 0:
-                self.frames.append(u.clone())
-                if R["verbose"] and R["snapshot_interval"] != self.T:
+        k = int(k)
+        while k > 0:
+            chunk = k
+            if R["verbose"] and R["tpf"] > 0:
+                _, steps, _, _ = self.sim_status()
+                chunk = min(k, R["tpf"] - steps % R["tpf"])
+            iters = (C.c_int32 * chunk)()
+            frames = self._frames[1:] if R["nframes"] > 0 else None
+            L.call("tc_sim_advance", self._sim, chunk, iters,
+                   L.ptr(frames) if frames is not None else None, R["tpf"], R["nframes"],
+                   L.stream_handle(self.device))
+            self.iterations.extend(iters)
+            R["n_total_iter"] += sum(iters)
+            k -= chunk
+            if R["verbose"] and R["tpf"] > 0 and R["snapshot_interval"] != self.T:
+                t, steps, _, _ = self.sim_status()
+                if steps % R["tpf"] == 0:
                     print(f"t: {round(t, 1)}/{self.T}", "Time elapsed:",
                           round(time.time() - R["t0"], 2), "Total CG iter:", R["n_total_iter"],
                           flush=True)
 
+    def set_kernel_timing(self, on):
+        """Time each kernel of the runtime with HIP events around its launch."""
+        L.call("tc_sim_set_timing", self._sim, 1 if on else 0)
+
+    def kernel_times(self):
+        """{kernel: (total_ms, launches)} since ``set_kernel_timing(True)``."""
+        import ctypes as C
+        ms = (C.c_double * 5)()
+        cnt = (C.c_int64 * 5)()
+        L.call("tc_sim_get_timing", self._sim, ms, cnt)
+        names = ("ionic", "rhs_init", "pcg_spmv", "pcg_update", "pcg_check")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}
+
     def finish(self):
         R = self._run
-        AT, RT = self.activation_time, self.repolarization_time
-        if R["calc"] and R["prev_t"] is not None:
-            L.call("tc_lat_update", L.ptr(self._x), L.ptr(AT), L.ptr(RT), self.n_nodes,
-                   float(R["prev_t"]), L.dtype_code(self.dtype), L.stream_handle(self.device))
+        L.call("tc_sim_finish", self._sim, L.stream_handle(self.device))
         torch.cuda.synchronize(self.device)
+        t, steps, xi, nfr = self.sim_status()
         self.solving_time = time.time() - R["t0"]
-        self.u = self._x
+        self.u = self._X[xi]
+        self.frames = self._frames[: 1 + nfr]
+        AT, RT = self.activation_time, self.repolarization_time
         if self.result_path is not None:
             if R["calc"]:
                 torch.save(AT.cpu(), self.result_path / "ATs.pt")
                 torch.save(RT.cpu(), self.result_path / "RTs.pt")
             if R["snapshot_interval"] < self.T:
-                torch.save(torch.stack(self.frames, dim=0).cpu(), self.result_path / "Vm.pt")
+                torch.save(self.frames.cpu(), self.result_path / "Vm.pt")
         if R["verbose"]:
             mem = torch.cuda.max_memory_allocated(self.device) / 1e9
-            print(self.ionic_model.name, self.n_nodes, round(self.solving_time, 2),
-                  round(R["tot_ion"], 2), round(R["tot_ele"], 2), R["n_total_iter"], "n/a",
-                  f"{round(mem, 2)}", flush=True)
+            print(self.ionic_model.name, self.n_nodes, round(self.solving_time, 2), 0.0, 0.0,
+                  R["n_total_iter"], "n/a", f"{round(mem, 2)}", flush=True)
             if R["calc"]:
                 print("ATs: ", AT.min().item(), AT.max().item(), flush=True)
                 print("RTs: ", RT.min().item(), RT.max().item(), flush=True)
