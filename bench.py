@@ -104,6 +104,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--timing-period", type=int, default=8,
+                    help="time the kernels of every k-th step with HIP events")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -126,7 +128,7 @@ def main():
     sim.advance(args.warmup)
     torch.cuda.synchronize()
     if not args.no_kernel_timing:
-        sim.set_kernel_timing(True)
+        sim.set_kernel_timing(True, period=args.timing_period)
     it0 = len(sim.iterations)
     print(f"[bench] timing {args.steps} steps", flush=True)
     torch.cuda.synchronize()

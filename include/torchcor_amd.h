@@ -296,11 +296,12 @@ int tc_sim_finish(tc_sim* sim, void* stream);
  * written. */
 int tc_sim_status(tc_sim* sim, double* t_host, int64_t* steps_host, int* x_index_host,
                   int64_t* frames_host);
-/* Per-kernel timing of the runtime with HIP events around each launch
- * (off by default; for measurement).  get returns total ms and launches for
+/* Per-kernel timing of the runtime with HIP events around each launch of
+ * every `period`-th step (0 = off, the default; for measurement).  get
+ * returns total ms and launches over the timed steps for
  * [0] ionic, [1] rhs+pcg start, [2] pcg spmv, [3] pcg update, [4] check,
  * counting only PCG iterations that ran. */
-int tc_sim_set_timing(tc_sim* sim, int on);
+int tc_sim_set_timing(tc_sim* sim, int period);
 int tc_sim_get_timing(tc_sim* sim, double* ms_host, int64_t* count_host);
 
 /* LAT/RT update alone (monodomain.py:174-176), for the last step of a run. */

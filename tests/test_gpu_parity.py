@@ -334,7 +334,7 @@ def _run_pair(N, model, dtype, T, stim_nodes, snapshot=1.0, cond=(0.1, 0.1), mes
     gm.add_conductivity([0], *cond)
     gm.add_stimulus(stim_nodes, 0.0, 1.0, 100)
     it = gm.solve(1e-5, 1e-5, 100, snapshot_interval=snapshot, verbose=False)
-    Vg = torch.stack(gm.frames).cpu().numpy()
+    Vg = gm.frames.cpu().numpy()
     return ro, gm, Vg, it
 
 

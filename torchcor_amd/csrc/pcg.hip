@@ -19,7 +19,7 @@ template <class T>
 int pcg_solve(tc_pcg* P, const T* b, T* x, const T* xp, T* xn, double a_tol, double r_tol,
               int max_iter, int lg, int* n_iter_host, hipStream_t s) {
   const int64_t n = P->n;
-  const dim3 g(P->G), blk(kBlock);
+  const dim3 g(P->G), blk(kPcgBlock);
   const int32_t* sp = P->A.slice_ptr;
   const int32_t* sc = P->A.col;
   const T* Av = (const T*)P->Av;
@@ -71,7 +71,7 @@ int pcg_solve(tc_pcg* P, const T* b, T* x, const T* xp, T* xn, double a_tol, dou
       if (P->timing) TC_HIP(hipEventRecord(P->ev[5 + 4 * k], s));
     }
     it += c;
-    hipLaunchKernelGGL(k_pcg_check, dim3(1), blk, 0, s, it - 1, P->G, P->ctl, P->part[2],
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kPcgBlock), 0, s, it - 1, P->G, P->ctl, P->part[2],
                        (StepStatus*)nullptr, 0, max_iter);
     TC_LAUNCH();
     TC_HIP(hipMemcpyAsync(P->host_ctl, P->ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, s));

@@ -73,9 +73,12 @@ struct tc_pcg {
 namespace tc {
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
-constexpr int kMaxBlocks = 2048;
+constexpr int kBlock = 256;       // stand-alone SpMV / RHS kernels
+constexpr int kPcgBlock = 512;    // PCG passes: 8 waves per block
+constexpr int kWaves = kPcgBlock / 64;
+// At most 2 blocks per CU (16 waves): enough to stream HBM, and few enough
+// partial sums that every block folding all of them stays cheap.
+constexpr int kMaxBlocks = 512;
 constexpr int kXcd = 8;
 
 // ------------------------------------------------------------------ SpMV
@@ -192,7 +195,7 @@ __device__ inline bool stop_rule(double zn, const PcgCtl* ctl) {
 // PCG start: x0 = 2x - x_prev (linear guess) or x; r = b - A x0;
 // partial sums of r.z and (Dinv b)^2.
 template <class T, bool LG>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kPcgBlock)
 k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
            const T* __restrict__ Av, const T* __restrict__ Dinv, const T* __restrict__ bvec,
            const T* __restrict__ x, const T* __restrict__ xp, T* __restrict__ xnext,
@@ -238,7 +241,7 @@ k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
 // spmv pass of iteration k.  pin is the previous p (unused when k == 0),
 // pout receives p'.  Iteration k-1's stopping test happens here first.
 template <class T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kPcgBlock)
 k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
            const T* __restrict__ Av, const T* __restrict__ Dinv, const T* __restrict__ r,
            const T* __restrict__ pin, T* __restrict__ pout, T* __restrict__ Ap, int k,
@@ -308,7 +311,7 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
 }
 
 template <class T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kPcgBlock)
 k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
              const T* __restrict__ Ap, T* __restrict__ x, T* __restrict__ r, int k, PcgCtl* ctl,
              const double* __restrict__ part1, double* part2) {
@@ -340,7 +343,7 @@ k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
 // `status` the result is also stored there; with `stall_open` an unconverged
 // result below max_iter sets ctl->stall, which gates every later launch of
 // a pipelined run until the host has launched more iterations.
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kPcgBlock)
 k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepStatus* status,
             int stall_open, int max_iter) {
   __shared__ int flags[2];
@@ -383,7 +386,7 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepSta
 //   r  = b - A x0,   partial sums of r.z and (Dinv b)^2
 // One pass over the shared pattern with M, K and A values.
 template <class T, bool LG>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kPcgBlock)
 k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
            const T* __restrict__ Mv, const T* __restrict__ Kv, const T* __restrict__ Av,
            const T* __restrict__ Dinv, const T* __restrict__ w, const T* __restrict__ x,

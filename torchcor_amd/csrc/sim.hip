@@ -39,6 +39,7 @@ struct Plan {
   int64_t step;
   int slot;
   int K;
+  bool timed;
   int64_t frame;  // frame written after this step or -1
   HostState after;
 };
@@ -109,7 +110,7 @@ template <class T>
 int launch_iters(tc_sim* S, int k0, int k1, T* xs, hipStream_t s, std::vector<hipEvent_t>* ev,
                  size_t ev_base) {
   tc_pcg* P = S->d.pcg;
-  const dim3 g(P->G), blk(kBlock);
+  const dim3 g(P->G), blk(kPcgBlock);
   T* pp[2] = {(T*)P->p[0], (T*)P->p[1]};
   for (int k = k0; k < k1; ++k) {
     T* pin = pp[k & 1];
@@ -134,7 +135,7 @@ int launch_iters(tc_sim* S, int k0, int k1, T* xs, hipStream_t s, std::vector<hi
 
 int check_launch(tc_sim* S, int k, int slot, hipStream_t s) {
   tc_pcg* P = S->d.pcg;
-  hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kBlock), 0, s, k, P->G, P->ctl, P->part[2],
+  hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kPcgBlock), 0, s, k, P->G, P->ctl, P->part[2],
                      S->dev_status + slot, 1, S->d.max_iter);
   TC_LAUNCH();
   return TC_OK;
@@ -168,7 +169,8 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   const int K = desired_K(S);
   const int c = ps.rot[0], p = ps.rot[1], q = ps.rot[2];
   std::vector<hipEvent_t>* ev = nullptr;
-  if (S->timing) {
+  plan.timed = S->timing > 0 && step % S->timing == 0;
+  if (plan.timed) {
     int rc = ev_reserve(S, slot, 6 + 4 * (size_t)K);
     if (rc) return rc;
     ev = &S->tev[slot];
@@ -197,7 +199,7 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   if (rc) return rc;
   if (ev) TC_HIP(hipEventRecord((*ev)[1], s));
   // 2. RHS + PCG start
-  const dim3 g(P->G), blk(kBlock);
+  const dim3 g(P->G), blk(kPcgBlock);
   const double coef = (1.0 - d.theta) * d.dt;
   T* xs;
   if (ev) TC_HIP(hipEventRecord((*ev)[2], s));
@@ -327,7 +329,7 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     Plan plan = inflight.front();
     TC_HIP(hipEventSynchronize(S->done_ev[plan.slot]));
     StepStatus* st = S->host_status + plan.slot;
-    if (S->timing) accumulate_timing(S, plan, st->iter);
+    if (plan.timed) accumulate_timing(S, plan, st->iter);
     if (st->stalled) {
       int rc = resolve_stall<T>(S, plan, s);
       if (rc) return rc;
@@ -442,7 +444,7 @@ extern "C" int tc_sim_status(tc_sim* S, double* t, int64_t* steps, int* x_index,
 
 extern "C" int tc_sim_set_timing(tc_sim* S, int on) {
   TC_CHECK(S, TC_ERR_ARG, "tc_sim_set_timing: null");
-  S->timing = on ? 1 : 0;
+  S->timing = on > 0 ? on : 0;
   for (int k = 0; k < kTimers; ++k) {
     S->acc_ms[k] = 0.0;
     S->acc_n[k] = 0;

@@ -260,9 +260,10 @@ class Monodomain:
                           round(time.time() - R["t0"], 2), "Total CG iter:", R["n_total_iter"],
                           flush=True)
 
-    def set_kernel_timing(self, on):
-        """Time each kernel of the runtime with HIP events around its launch."""
-        L.call("tc_sim_set_timing", self._sim, 1 if on else 0)
+    def set_kernel_timing(self, on, period=1):
+        """Time each kernel of the runtime with HIP events around its launch,
+        on every ``period``-th step."""
+        L.call("tc_sim_set_timing", self._sim, int(period) if on else 0)
 
     def kernel_times(self):
         """{kernel: (total_ms, launches)} since ``set_kernel_timing(True)``."""
