@@ -158,7 +158,8 @@ def main():
                 "unit": "GB/s", "frac": round(a / HBM_PEAK_GBPS, 4), "traffic": None}
         b_rhs, b_init, b_iter = survey_electric_bytes(n, nnz)
         n_it = kernels.get("pcg_spmv", {}).get("launches", 0)
-        ele_bytes = args.steps * (b_rhs + b_init) + n_it * b_iter
+        n_st = kernels.get("rhs_init", {}).get("launches", 0)
+        ele_bytes = n_st * (b_rhs + b_init) + n_it * b_iter
         ele_s = sum(kernels[k]["total_ms"] for k in ("rhs_init", "pcg_spmv", "pcg_update",
                                                      "pcg_check") if k in kernels) / 1e3
         pcg_rate = round(ele_bytes / ele_s / 1e9, 1)

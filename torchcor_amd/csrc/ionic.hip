@@ -302,11 +302,8 @@ __global__ void k_ttp_vektab(TabParam t, void* out, int f64) {
 }
 
 // One TTP step per node (TenTusscherPanfilov.py:353-454).
-// At most 128 VGPRs (4 waves per SIMD): more waves in flight hide the
-// dependent state -> table-row load chain better than keeping every
-// interpolated value in registers.
 template <class T>
-__global__ void __launch_bounds__(kBlock, 4)
+__global__ void __launch_bounds__(kBlock)
 k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
       const T* __restrict__ Ctab, const T* __restrict__ Ktab, TtpTabs tb, tc_ionic_io io) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;

@@ -41,9 +41,13 @@ struct PcgCtl {
                   // every later gated kernel then returns at once
 };
 
-// Per-step result written by the check pass.
+// Per-step result written by the check pass; `seq` (the step number) is
+// written last, after a system-scope fence, so a host polling pinned memory
+// sees a complete record.
 struct StepStatus {
   int n_iter, iter, converged, stalled;
+  long long seq;
+  long long pad;
 };
 
 }  // namespace tc
@@ -76,9 +80,9 @@ namespace {
 constexpr int kBlock = 256;       // stand-alone SpMV / RHS kernels
 constexpr int kPcgBlock = 512;    // PCG passes: 8 waves per block
 constexpr int kWaves = kPcgBlock / 64;
-// At most 2 blocks per CU (16 waves): enough to stream HBM, and few enough
-// partial sums that every block folding all of them stays cheap.
-constexpr int kMaxBlocks = 512;
+// At most 4 blocks per CU (32 waves, full occupancy), and few enough partial
+// sums (<= 1024) that every block folding all of them stays cheap.
+constexpr int kMaxBlocks = 1024;
 constexpr int kXcd = 8;
 
 // ------------------------------------------------------------------ SpMV
@@ -345,7 +349,7 @@ k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
 // a pipelined run until the host has launched more iterations.
 __global__ void __launch_bounds__(kPcgBlock)
 k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepStatus* status,
-            int stall_open, int max_iter) {
+            int stall_open, int max_iter, long long seq) {
   __shared__ int flags[2];
   if (threadIdx.x == 0) {
     flags[0] = ctl->stall;
@@ -353,30 +357,35 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepSta
   }
   __syncthreads();
   if (flags[0]) return;
+  int n_iter, iter, conv, stalled = 0;
   if (flags[1] || k < 0) {
-    if (threadIdx.x == 0 && status) {
-      status->n_iter = k < 0 ? 0 : ctl->n_iter;
-      status->iter = k < 0 ? 0 : ctl->iter;
-      status->converged = k < 0 ? 0 : 1;
-      status->stalled = 0;
-    }
-    return;
-  }
-  double v[2];
-  fold_partials<2>(part2, G, v);
-  if (threadIdx.x == 0) {
+    n_iter = k < 0 ? 0 : ctl->n_iter;
+    iter = k < 0 ? 0 : ctl->iter;
+    conv = k < 0 ? 0 : 1;
+  } else {
+    double v[2];
+    fold_partials<2>(part2, G, v);
     const bool stop = stop_rule(sqrt(v[0]), ctl);
-    ctl->converged = stop ? 1 : 0;
-    ctl->n_iter = stop ? k : k + 1;
-    ctl->iter = k + 1;
-    const int stalled = (!stop && stall_open && k + 1 < max_iter) ? 1 : 0;
-    if (stalled) ctl->stall = 1;
-    if (status) {
-      status->n_iter = ctl->n_iter;
-      status->iter = k + 1;
-      status->converged = stop ? 1 : 0;
-      status->stalled = stalled;
+    conv = stop ? 1 : 0;
+    n_iter = stop ? k : k + 1;
+    iter = k + 1;
+    stalled = (!stop && stall_open && k + 1 < max_iter) ? 1 : 0;
+    if (threadIdx.x == 0) {
+      ctl->converged = conv;
+      ctl->n_iter = n_iter;
+      ctl->iter = iter;
+      if (stalled) ctl->stall = 1;
     }
+  }
+  if (threadIdx.x == 0 && status) {
+    volatile StepStatus* st = status;
+    st->n_iter = n_iter;
+    st->iter = iter;
+    st->converged = conv;
+    st->stalled = stalled;
+    __threadfence_system();
+    st->seq = seq;
+    __threadfence_system();
   }
 }
 

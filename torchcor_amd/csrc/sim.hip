@@ -8,14 +8,21 @@
 //      largest iteration count among the last steps,
 //   4. a one-block stopping check that stores the step status and, when the
 //      solve is not finished, raises ctl->stall,
-//   5. an optional snapshot copy and the status copy to pinned host memory.
+//   5. an optional snapshot copy.
+// The check kernel writes the step status straight into pinned host memory
+// (with a sequence number written last), so no per-step event or copy is
+// enqueued; the host polls that record.
 // The host keeps up to `depth` steps in flight and validates the oldest.
 // While ctl->stall is set every later kernel returns at once, so a step that
 // needed more than K iterations leaves the device exactly where the plain
 // loop would be; the host then launches the remaining iterations, clears the
 // flag and re-issues the later steps from that state.
+#include <chrono>
 #include <deque>
+#include <thread>
 #include <vector>
+
+#include <atomic>
 
 #include "pcg_kernels.h"
 
@@ -39,6 +46,7 @@ struct Plan {
   int64_t step;
   int slot;
   int K;
+  long long seq;  // sequence number the check pass writes for this step
   bool timed;
   int64_t frame;  // frame written after this step or -1
   HostState after;
@@ -54,9 +62,9 @@ struct tc_sim {
     tc_ttp_params ttp;
     tc_crn_params crn;
   } params;
-  tc::StepStatus* dev_status;
-  tc::StepStatus* host_status;
-  hipEvent_t done_ev[tc::kRing];
+  tc::StepStatus* host_status;  // pinned, written by the device
+  tc::StepStatus* dev_view;     // device address of host_status
+  long long seq;
   tc::HostState vs;  // validated
   std::deque<int> recent;
   int timing;
@@ -133,11 +141,28 @@ int launch_iters(tc_sim* S, int k0, int k1, T* xs, hipStream_t s, std::vector<hi
   return TC_OK;
 }
 
-int check_launch(tc_sim* S, int k, int slot, hipStream_t s) {
+int check_launch(tc_sim* S, int k, int slot, long long seq, hipStream_t s) {
   tc_pcg* P = S->d.pcg;
   hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kPcgBlock), 0, s, k, P->G, P->ctl, P->part[2],
-                     S->dev_status + slot, 1, S->d.max_iter);
+                     S->dev_view + slot, 1, S->d.max_iter, seq);
   TC_LAUNCH();
+  return TC_OK;
+}
+
+// Wait until the check pass has published record `seq` in slot `slot`.
+int wait_status(tc_sim* S, int slot, long long seq) {
+  volatile StepStatus* st = S->host_status + slot;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (long spins = 0; st->seq != seq; ++spins) {
+    if ((spins & 1023) == 1023) {
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(120)) {
+        set_error("time-step runtime: no status for record %lld after 120 s", seq);
+        return TC_ERR_STATE;
+      }
+      std::this_thread::yield();
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
   return TC_OK;
 }
 
@@ -223,7 +248,8 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   if (rc) return rc;
   // 4. check
   if (ev) TC_HIP(hipEventRecord((*ev)[4], s));
-  rc = check_launch(S, K - 1, slot, s);
+  plan.seq = ++S->seq;
+  rc = check_launch(S, K - 1, slot, plan.seq, s);
   if (rc) return rc;
   if (ev) TC_HIP(hipEventRecord((*ev)[5], s));
   // state after the step
@@ -245,9 +271,6 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
     plan.frame = ps.frames;
     nx.frames = ps.frames + 1;
   }
-  TC_HIP(hipMemcpyAsync(S->host_status + slot, S->dev_status + slot, sizeof(StepStatus),
-                        hipMemcpyDeviceToHost, s));
-  TC_HIP(hipEventRecord(S->done_ev[slot], s));
   plan.step = step;
   plan.slot = slot;
   plan.K = K;
@@ -273,11 +296,11 @@ int resolve_stall(tc_sim* S, const Plan& plan, hipStream_t s) {
     int rc = launch_iters<T>(S, it, it + c, xs, s, nullptr, 0);
     if (rc) return rc;
     it += c;
-    rc = check_launch(S, it - 1, plan.slot, s);
+    const long long seq = ++S->seq;
+    rc = check_launch(S, it - 1, plan.slot, seq, s);
     if (rc) return rc;
-    TC_HIP(hipMemcpyAsync(st, S->dev_status + plan.slot, sizeof(StepStatus),
-                          hipMemcpyDeviceToHost, s));
-    TC_HIP(hipStreamSynchronize(s));
+    rc = wait_status(S, plan.slot, seq);
+    if (rc) return rc;
     chunk = chunk * 2 < 64 ? chunk * 2 : 64;
   }
   if (plan.frame >= 0) {
@@ -289,6 +312,7 @@ int resolve_stall(tc_sim* S, const Plan& plan, hipStream_t s) {
 
 void accumulate_timing(tc_sim* S, const Plan& plan, int iters_run) {
   std::vector<hipEvent_t>& ev = S->tev[plan.slot];
+  (void)hipEventSynchronize(ev[5]);
   float ms = 0.f;
   const int pairs[3][2] = {{0, 1}, {2, 3}, {4, 5}};
   const int which[3] = {0, 1, 4};
@@ -327,7 +351,8 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
       ++enq;
     }
     Plan plan = inflight.front();
-    TC_HIP(hipEventSynchronize(S->done_ev[plan.slot]));
+    int rc = wait_status(S, plan.slot, plan.seq);
+    if (rc) return rc;
     StepStatus* st = S->host_status + plan.slot;
     if (plan.timed) accumulate_timing(S, plan, st->iter);
     if (st->stalled) {
@@ -377,15 +402,16 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
       return TC_ERR_ARG;
   }
   S->d.params = nullptr;
-  hipError_t e = hipMalloc((void**)&S->dev_status, sizeof(StepStatus) * kRing);
-  if (e == hipSuccess)
-    e = hipHostMalloc((void**)&S->host_status, sizeof(StepStatus) * kRing, hipHostMallocDefault);
-  for (int k = 0; k < kRing && e == hipSuccess; ++k)
-    e = hipEventCreateWithFlags(&S->done_ev[k], hipEventDisableTiming);
+  hipError_t e = hipHostMalloc((void**)&S->host_status, sizeof(StepStatus) * kRing,
+                               hipHostMallocMapped);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&S->dev_view, S->host_status, 0);
   if (e != hipSuccess) {
     set_error("tc_sim_create: %s", hipGetErrorString(e));
+    tc_sim_destroy(S);
     return TC_ERR_HIP;
   }
+  for (int k = 0; k < kRing; ++k) S->host_status[k].seq = -1;
+  S->seq = 0;
   // The stall flag lives in the shared PCG control block; start clear.
   e = hipMemset(&d->pcg->ctl->stall, 0, sizeof(int));
   if (e != hipSuccess) {
@@ -403,11 +429,8 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
 
 extern "C" int tc_sim_destroy(tc_sim* S) {
   if (!S) return TC_OK;
-  for (int k = 0; k < kRing; ++k) {
-    if (S->done_ev[k]) (void)hipEventDestroy(S->done_ev[k]);
+  for (int k = 0; k < kRing; ++k)
     for (hipEvent_t e : S->tev[k]) (void)hipEventDestroy(e);
-  }
-  if (S->dev_status) (void)hipFree(S->dev_status);
   if (S->host_status) (void)hipHostFree(S->host_status);
   delete S;
   return TC_OK;
