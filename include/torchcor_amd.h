@@ -198,6 +198,9 @@ typedef struct tc_ionic_io {
   /* When non-NULL and *gate != 0 at kernel start, the kernel does nothing
    * (used by the pipelined time-step runtime). */
   const int* gate;
+  /* V is float64 (rounded to the state dtype on load): the float64 PCG
+   * iterate of a float32 run. */
+  int v_f64;
 } tc_ionic_io;
 
 /* ModifiedMS2v (ionic/ModifiedMS2v.py:4-44).  States: H. */
@@ -268,7 +271,9 @@ typedef struct tc_sim_desc {
   int dtype;
   int64_t n;
   double dt, dt_ionic, Cm, theta;
-  void* x[3];                /* solution buffers: current, previous, next */
+  int x_f64;                 /* iterate stored in float64 (float32 runs) */
+  void* x[2];                /* (n) iterate buffers (float64 when x_f64), current first */
+  void* xp[2];               /* (n) x_prev buffers in dtype, current first */
   void* w;                   /* (n) scratch vector */
   void* AT;                  /* (n) activation times or NULL */
   void* RT;                  /* (n) repolarisation times */
@@ -284,15 +289,15 @@ int tc_sim_create(tc_sim** sim, const tc_sim_desc* desc);
 int tc_sim_destroy(tc_sim* sim);
 /* Run nsteps more steps.  iters_host (nsteps) receives each step's PCG
  * iteration count (reference convention).  When frames != NULL, after every
- * step whose global index is a multiple of ts_per_frame the solution is
- * copied to frames + k*n for the next frame k < max_frames.  Synchronizes
+ * step whose global index is a multiple of ts_per_frame the solution (in
+ * dtype) is copied to frames + k*n for the next frame k < max_frames.  Synchronizes
  * `stream` before returning. */
 int tc_sim_advance(tc_sim* sim, int64_t nsteps, int32_t* iters_host, void* frames,
                    int64_t ts_per_frame, int64_t max_frames, void* stream);
 /* Apply the LAT/RT test to the last step's solution (it is otherwise folded
  * into the next step's ionic pass).  Idempotent. */
 int tc_sim_finish(tc_sim* sim, void* stream);
-/* t (double), steps done, index 0..2 of desc.x holding the solution, frames
+/* t (double), steps done, index 0..1 of desc.x holding the solution, frames
  * written. */
 int tc_sim_status(tc_sim* sim, double* t_host, int64_t* steps_host, int* x_index_host,
                   int64_t* frames_host);

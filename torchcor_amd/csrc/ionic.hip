@@ -153,7 +153,7 @@ k_ms2v(tc_ms2v_params p, T* __restrict__ H, tc_ionic_io io) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= io.n) return;
   if (io.gate && *(volatile const int*)io.gate) return;
-  const T U = ((const T*)io.V)[i];
+  const T U = io.v_f64 ? (T)((const double*)io.V)[i] : ((const T*)io.V)[i];
   const T h = H[i];
   const T DV = (T)(p.vmax - p.vmin);
   const T Uad = (U - (T)p.vmin) / DV;
@@ -310,7 +310,7 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
   const int64_t n = io.n;
   if (i >= n) return;
   if (io.gate && *(volatile const int*)io.gate) return;
-  const T V = ((const T*)io.V)[i];
+  const T V = io.v_f64 ? (T)((const double*)io.V)[i] : ((const T*)io.V)[i];
   T s[TS_N];
 #pragma unroll
   for (int k = 0; k < TS_N; ++k) s[k] = st[k * n + i];
@@ -548,7 +548,7 @@ k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
   const int64_t n = io.n;
   if (i >= n) return;
   if (io.gate && *(volatile const int*)io.gate) return;
-  const T V = ((const T*)io.V)[i];
+  const T V = io.v_f64 ? (T)((const double*)io.V)[i] : ((const T*)io.V)[i];
   T s[CS_N];
 #pragma unroll
   for (int k = 0; k < CS_N; ++k) s[k] = st[k * n + i];

@@ -314,10 +314,12 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
   store_partials<1>(red, part1);
 }
 
-template <class T>
+// XT is the storage type of the iterate x (float64 in float32 runs of the
+// time-step runtime, as the reference keeps its CG iterate in float64).
+template <class T, class XT = T>
 __global__ void __launch_bounds__(kPcgBlock)
 k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
-             const T* __restrict__ Ap, T* __restrict__ x, T* __restrict__ r, int k, PcgCtl* ctl,
+             const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, int k, PcgCtl* ctl,
              const double* __restrict__ part1, double* part2) {
   if (block_halted(ctl)) return;
   double v[1];
@@ -330,7 +332,7 @@ k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
   for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
     if (i < n) {
-      const T xi = (T)((double)x[i] + alpha * (double)p[i]);
+      const XT xi = (XT)((double)x[i] + alpha * (double)p[i]);
       const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
       x[i] = xi;
       r[i] = ri;
@@ -390,17 +392,21 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepSta
 }
 
 // Fused right-hand side and PCG start of one time step:
-//   b  = M w - coef K u            (u is the current solution x)
-//   x0 = 2x - x_prev (linear guess) or x
-//   r  = b - A x0,   partial sums of r.z and (Dinv b)^2
-// One pass over the shared pattern with M, K and A values.
-template <class T, bool LG>
+//   u  = x rounded to T            (the solution the ionic pass saw)
+//   b  = M w - coef K u
+//   x0 = 2x - x_prev (linear guess) or x, in float64
+//   r  = b - (Cm M + tdt K) x0,   partial sums of r.z and (Dinv b)^2
+// One pass over the shared pattern reading only the M and K values: A x0 is
+// formed from the same loads (A = Cm M + theta dt K).  With the linear guess
+// the pass also writes x_next = x0 and x_prev_next = x rounded to T
+// (solver.py:35-38: the reference keeps x_prev in the dtype of u).
+template <class T, class XT, bool LG>
 __global__ void __launch_bounds__(kPcgBlock)
 k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
-           const T* __restrict__ Mv, const T* __restrict__ Kv, const T* __restrict__ Av,
-           const T* __restrict__ Dinv, const T* __restrict__ w, const T* __restrict__ x,
-           const T* __restrict__ xp, T* __restrict__ xnext, T* __restrict__ r, double coef,
-           PcgCtl* ctl, double a_tol, double r_tol, double* part0) {
+           const T* __restrict__ Mv, const T* __restrict__ Kv, const T* __restrict__ Dinv,
+           const T* __restrict__ w, const XT* __restrict__ x, const T* __restrict__ xp,
+           XT* __restrict__ xnext, T* __restrict__ xpnext, T* __restrict__ r, double coef,
+           double Cm, double tdt, PcgCtl* ctl, double a_tol, double r_tol, double* part0) {
   {
     __shared__ int st;
     if (threadIdx.x == 0) st = ctl->stall;
@@ -417,23 +423,27 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
     const int32_t* c = scol + b + lane;
     const T* m = Mv + b + lane;
     const T* kk = Kv + b + lane;
-    const T* a = Av + b + lane;
-    double am = 0.0, ak = 0.0, ax = 0.0;
+    double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
     for (int j = 0; j < wd; ++j) {
       const int32_t cj = c[j * 64];
-      const T xj = x[cj];
-      T x0;
-      if (LG) x0 = (T)(2.0 * (double)xj - (double)xp[cj]);
-      else x0 = xj;
-      am += (double)m[j * 64] * (double)w[cj];
-      ak += (double)kk[j * 64] * (double)xj;
-      ax += (double)a[j * 64] * (double)x0;
+      const XT xj = x[cj];
+      double x0;
+      if (LG) x0 = (double)(XT)(2.0 * (double)xj - (double)xp[cj]);
+      else x0 = (double)xj;
+      const double mj = (double)m[j * 64], kj = (double)kk[j * 64];
+      am += mj * (double)w[cj];
+      ak += kj * (double)(T)xj;
+      mx += mj * x0;
+      kx += kj * x0;
     }
     if (i < n) {
-      if (LG) xnext[i] = (T)(2.0 * (double)x[i] - (double)xp[i]);
+      if (LG) {
+        xnext[i] = (XT)(2.0 * (double)x[i] - (double)xp[i]);
+        xpnext[i] = (T)x[i];
+      }
       const T bi = (T)(am - coef * ak);
       const double d = (double)Dinv[i];
-      const T ri = (T)((double)bi - ax);
+      const T ri = (T)((double)bi - (Cm * mx + tdt * kx));
       r[i] = ri;
       red[0] += (double)ri * (d * (double)ri);
       const double db = d * (double)bi;
@@ -450,13 +460,28 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
   }
 }
 
-// dst = src unless the gate is set.
-template <class T>
-__global__ void k_copy_gated(const int* gate, const T* __restrict__ src, T* __restrict__ dst,
+// dst = src (converted to T) unless the gate is set.
+template <class T, class XT>
+__global__ void k_copy_gated(const int* gate, const XT* __restrict__ src, T* __restrict__ dst,
                              int64_t n) {
   if (gate && *(volatile const int*)gate) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
+  if (i < n) dst[i] = (T)src[i];
+}
+
+// LAT/RT test (monodomain.py:174-176) on u = x rounded to T.
+template <class T, class XT>
+__global__ void k_lat_x(const XT* __restrict__ x, T* __restrict__ AT, T* __restrict__ RT,
+                        int64_t n, double t) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T V = (T)x[i];
+  T at = AT[i];
+  if (V > T(0) && at == T(-1)) {
+    at = (T)t;
+    AT[i] = at;
+  }
+  if (at > T(0) && V < T(-70) && RT[i] == T(-1)) RT[i] = (T)t;
 }
 
 __global__ void k_set_int(int* p, int v) { *p = v; }

@@ -36,7 +36,8 @@ constexpr int kHistory = 16;
 struct HostState {
   int64_t steps = 0;
   double t = 0.0;
-  int rot[3] = {0, 1, 2};  // indices into desc.x of current, previous, next
+  int xc = 0;  // desc.x[xc] holds the current iterate
+  int pc = 0;  // desc.xp[pc] holds x_prev
   int64_t frames = 0;
   bool lat_pending = false;
   double lat_t = 0.0;
@@ -114,8 +115,8 @@ int ionic_launch(tc_sim* S, const tc_ionic_io* io, hipStream_t s) {
   return TC_ERR_ARG;
 }
 
-template <class T>
-int launch_iters(tc_sim* S, int k0, int k1, T* xs, hipStream_t s, std::vector<hipEvent_t>* ev,
+template <class T, class XT>
+int launch_iters(tc_sim* S, int k0, int k1, XT* xs, hipStream_t s, std::vector<hipEvent_t>* ev,
                  size_t ev_base) {
   tc_pcg* P = S->d.pcg;
   const dim3 g(P->G), blk(kPcgBlock);
@@ -133,7 +134,7 @@ int launch_iters(tc_sim* S, int k0, int k1, T* xs, hipStream_t s, std::vector<hi
       TC_HIP(hipEventRecord((*ev)[e + 1], s));
       TC_HIP(hipEventRecord((*ev)[e + 2], s));
     }
-    hipLaunchKernelGGL(k_pcg_update<T>, g, blk, 0, s, P->n, (const T*)P->Dinv, pout,
+    hipLaunchKernelGGL((k_pcg_update<T, XT>), g, blk, 0, s, P->n, (const T*)P->Dinv, pout,
                        (const T*)P->Ap, xs, (T*)P->r, k, P->ctl, P->part[1], P->part[2]);
     TC_LAUNCH();
     if (ev) TC_HIP(hipEventRecord((*ev)[e + 3], s));
@@ -166,12 +167,12 @@ int wait_status(tc_sim* S, int slot, long long seq) {
   return TC_OK;
 }
 
-template <class T>
+template <class T, class XT>
 int copy_frame(tc_sim* S, const void* src, int64_t frame, const int* gate, hipStream_t s) {
   const int64_t n = S->d.n;
   T* dst = (T*)S->frames + frame * n;
-  hipLaunchKernelGGL(k_copy_gated<T>, dim3((int)ceil_div(n, 256)), dim3(256), 0, s, gate,
-                     (const T*)src, dst, n);
+  hipLaunchKernelGGL((k_copy_gated<T, XT>), dim3((int)ceil_div(n, 256)), dim3(256), 0, s, gate,
+                     (const XT*)src, dst, n);
   TC_LAUNCH();
   return TC_OK;
 }
@@ -184,7 +185,7 @@ int desired_K(tc_sim* S) {
 }
 
 // Enqueue the step after planned state ps; ps becomes the state after it.
-template <class T>
+template <class T, class XT>
 int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   const tc_sim_desc& d = S->d;
   tc_pcg* P = d.pcg;
@@ -192,7 +193,7 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   const double t = ps.t + d.dt;
   const int slot = (int)(step % kRing);
   const int K = desired_K(S);
-  const int c = ps.rot[0], p = ps.rot[1], q = ps.rot[2];
+  const int xc = ps.xc, pc = ps.pc;
   std::vector<hipEvent_t>* ev = nullptr;
   plan.timed = S->timing > 0 && step % S->timing == 0;
   if (plan.timed) {
@@ -206,7 +207,8 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   io.dt = d.dt_ionic;
   io.dt_rhs = d.dt;
   io.Cm = d.Cm;
-  io.V = d.x[c];
+  io.V = d.x[xc];
+  io.v_f64 = (sizeof(XT) == 8 && sizeof(T) == 4) ? 1 : 0;
   io.w = d.w;
   io.dV = nullptr;
   const uint32_t bits = active_bits(d, t);
@@ -226,25 +228,27 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   // 2. RHS + PCG start
   const dim3 g(P->G), blk(kPcgBlock);
   const double coef = (1.0 - d.theta) * d.dt;
-  T* xs;
+  const double tdt = d.theta * d.dt;
+  XT* xs;
   if (ev) TC_HIP(hipEventRecord((*ev)[2], s));
   if (d.linear_guess) {
-    hipLaunchKernelGGL((k_rhs_init<T, true>), g, blk, 0, s, d.n, P->A.slice_ptr, P->A.col,
-                       (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Av, (const T*)P->Dinv,
-                       (const T*)d.w, (const T*)d.x[c], (const T*)d.x[p], (T*)d.x[q],
-                       (T*)P->r, coef, P->ctl, d.a_tol, d.r_tol, P->part[0]);
-    xs = (T*)d.x[q];
+    hipLaunchKernelGGL((k_rhs_init<T, XT, true>), g, blk, 0, s, d.n, P->A.slice_ptr, P->A.col,
+                       (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const T*)d.w,
+                       (const XT*)d.x[xc], (const T*)d.xp[pc], (XT*)d.x[1 - xc],
+                       (T*)d.xp[1 - pc], (T*)P->r, coef, d.Cm, tdt, P->ctl, d.a_tol, d.r_tol,
+                       P->part[0]);
+    xs = (XT*)d.x[1 - xc];
   } else {
-    hipLaunchKernelGGL((k_rhs_init<T, false>), g, blk, 0, s, d.n, P->A.slice_ptr, P->A.col,
-                       (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Av, (const T*)P->Dinv,
-                       (const T*)d.w, (const T*)d.x[c], (const T*)nullptr, (T*)nullptr,
-                       (T*)P->r, coef, P->ctl, d.a_tol, d.r_tol, P->part[0]);
-    xs = (T*)d.x[c];
+    hipLaunchKernelGGL((k_rhs_init<T, XT, false>), g, blk, 0, s, d.n, P->A.slice_ptr, P->A.col,
+                       (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const T*)d.w,
+                       (const XT*)d.x[xc], (const T*)nullptr, (XT*)nullptr, (T*)nullptr,
+                       (T*)P->r, coef, d.Cm, tdt, P->ctl, d.a_tol, d.r_tol, P->part[0]);
+    xs = (XT*)d.x[xc];
   }
   TC_LAUNCH();
   if (ev) TC_HIP(hipEventRecord((*ev)[3], s));
   // 3. speculative iterations
-  rc = launch_iters<T>(S, 0, K, xs, s, ev, 6);
+  rc = launch_iters<T, XT>(S, 0, K, xs, s, ev, 6);
   if (rc) return rc;
   // 4. check
   if (ev) TC_HIP(hipEventRecord((*ev)[4], s));
@@ -257,16 +261,15 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   nx.steps = step;
   nx.t = t;
   if (d.linear_guess) {
-    nx.rot[0] = q;
-    nx.rot[1] = c;
-    nx.rot[2] = p;
+    nx.xc = 1 - xc;
+    nx.pc = 1 - pc;
   }
   nx.lat_pending = true;
   nx.lat_t = t;
   // 5. snapshot
   plan.frame = -1;
   if (S->frames && S->tpf > 0 && step % S->tpf == 0 && ps.frames < S->max_frames) {
-    rc = copy_frame<T>(S, d.x[nx.rot[0]], ps.frames, &P->ctl->stall, s);
+    rc = copy_frame<T, XT>(S, d.x[nx.xc], ps.frames, &P->ctl->stall, s);
     if (rc) return rc;
     plan.frame = ps.frames;
     nx.frames = ps.frames + 1;
@@ -281,19 +284,19 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
 
 // The step of `plan` stalled: run its remaining iterations until the stop
 // rule or max_iter, then redo its snapshot copy (which the stall gated).
-template <class T>
+template <class T, class XT>
 int resolve_stall(tc_sim* S, const Plan& plan, hipStream_t s) {
   const tc_sim_desc& d = S->d;
   tc_pcg* P = d.pcg;
   StepStatus* st = S->host_status + plan.slot;
-  T* xs = (T*)d.x[plan.after.rot[0]];
+  XT* xs = (XT*)d.x[plan.after.xc];
   int it = st->iter;
   int chunk = it > 0 ? it : 1;
   while (st->stalled) {
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, &P->ctl->stall, 0);
     TC_LAUNCH();
     const int c = chunk < d.max_iter - it ? chunk : d.max_iter - it;
-    int rc = launch_iters<T>(S, it, it + c, xs, s, nullptr, 0);
+    int rc = launch_iters<T, XT>(S, it, it + c, xs, s, nullptr, 0);
     if (rc) return rc;
     it += c;
     const long long seq = ++S->seq;
@@ -304,7 +307,7 @@ int resolve_stall(tc_sim* S, const Plan& plan, hipStream_t s) {
     chunk = chunk * 2 < 64 ? chunk * 2 : 64;
   }
   if (plan.frame >= 0) {
-    int rc = copy_frame<T>(S, xs, plan.frame, nullptr, s);
+    int rc = copy_frame<T, XT>(S, xs, plan.frame, nullptr, s);
     if (rc) return rc;
   }
   return TC_OK;
@@ -336,7 +339,7 @@ void accumulate_timing(tc_sim* S, const Plan& plan, int iters_run) {
   }
 }
 
-template <class T>
+template <class T, class XT>
 int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
   const int depth = S->d.depth < 1 ? 1 : (S->d.depth > kRing / 2 ? kRing / 2 : S->d.depth);
   std::deque<Plan> inflight;
@@ -345,7 +348,7 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
   while (done < nsteps) {
     while ((int)inflight.size() < depth && enq < nsteps) {
       Plan plan;
-      int rc = enqueue<T>(S, ps, plan, s);
+      int rc = enqueue<T, XT>(S, ps, plan, s);
       if (rc) return rc;
       inflight.push_back(plan);
       ++enq;
@@ -356,7 +359,7 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     StepStatus* st = S->host_status + plan.slot;
     if (plan.timed) accumulate_timing(S, plan, st->iter);
     if (st->stalled) {
-      int rc = resolve_stall<T>(S, plan, s);
+      int rc = resolve_stall<T, XT>(S, plan, s);
       if (rc) return rc;
       enq -= (int64_t)inflight.size() - 1;
       inflight.clear();
@@ -383,7 +386,8 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
   TC_CHECK(out && d && d->pcg && d->states && d->Mv && d->Kv && d->w && d->n > 0, TC_ERR_ARG,
            "tc_sim_create: bad arguments");
   TC_CHECK(d->dtype == TC_F32 || d->dtype == TC_F64, TC_ERR_ARG, "bad dtype %d", d->dtype);
-  TC_CHECK(d->x[0] && d->x[1] && d->x[2], TC_ERR_ARG, "tc_sim_create: need three x buffers");
+  TC_CHECK(d->x[0] && d->x[1] && (!d->linear_guess || (d->xp[0] && d->xp[1])), TC_ERR_ARG,
+           "tc_sim_create: need two x buffers (and two x_prev buffers with the linear guess)");
   TC_CHECK(d->n_stim >= 0 && d->n_stim <= TC_MAX_STIMULI, TC_ERR_ARG, "bad stimulus count");
   TC_CHECK(d->n_stim == 0 || d->stim_mask, TC_ERR_ARG, "stimuli need a mask");
   TC_CHECK(d->max_iter >= 0, TC_ERR_ARG, "max_iter < 0");
@@ -443,15 +447,28 @@ extern "C" int tc_sim_advance(tc_sim* S, int64_t nsteps, int32_t* iters, void* f
   S->tpf = tpf;
   S->max_frames = frames ? max_frames : 0;
   hipStream_t s = (hipStream_t)stream;
-  if (S->d.dtype == TC_F32) return advance<float>(S, nsteps, iters, s);
-  return advance<double>(S, nsteps, iters, s);
+  if (S->d.dtype == TC_F64) return advance<double, double>(S, nsteps, iters, s);
+  if (S->d.x_f64) return advance<float, double>(S, nsteps, iters, s);
+  return advance<float, float>(S, nsteps, iters, s);
 }
 
 extern "C" int tc_sim_finish(tc_sim* S, void* stream) {
   TC_CHECK(S, TC_ERR_ARG, "tc_sim_finish: null");
-  if (S->d.AT && S->vs.lat_pending)
-    return tc_lat_update(S->d.x[S->vs.rot[0]], S->d.AT, S->d.RT, S->d.n, S->vs.lat_t,
-                         S->d.dtype, stream);
+  if (!(S->d.AT && S->vs.lat_pending)) return TC_OK;
+  const int64_t n = S->d.n;
+  const dim3 g((int)ceil_div(n, 256)), b(256);
+  hipStream_t s = (hipStream_t)stream;
+  const void* x = S->d.x[S->vs.xc];
+  if (S->d.dtype == TC_F64)
+    hipLaunchKernelGGL((k_lat_x<double, double>), g, b, 0, s, (const double*)x,
+                       (double*)S->d.AT, (double*)S->d.RT, n, S->vs.lat_t);
+  else if (S->d.x_f64)
+    hipLaunchKernelGGL((k_lat_x<float, double>), g, b, 0, s, (const double*)x,
+                       (float*)S->d.AT, (float*)S->d.RT, n, S->vs.lat_t);
+  else
+    hipLaunchKernelGGL((k_lat_x<float, float>), g, b, 0, s, (const float*)x,
+                       (float*)S->d.AT, (float*)S->d.RT, n, S->vs.lat_t);
+  TC_LAUNCH();
   return TC_OK;
 }
 
@@ -460,7 +477,7 @@ extern "C" int tc_sim_status(tc_sim* S, double* t, int64_t* steps, int* x_index,
   TC_CHECK(S && t && steps && x_index && frames, TC_ERR_ARG, "tc_sim_status: bad arguments");
   *t = S->vs.t;
   *steps = S->vs.steps;
-  *x_index = S->vs.rot[0];
+  *x_index = S->vs.xc;
   *frames = S->vs.frames;
   return TC_OK;
 }

@@ -168,12 +168,17 @@ class Monodomain:
     # loop itself runs in the native runtime (tc_sim_*): steps are pipelined
     # on the device and no host round trip is needed per step.
     def start(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True,
-              snapshot_interval=1, verbose=False, depth=4):
+              snapshot_interval=1, verbose=False, depth=4, x_f64=True):
+        """Set up a run.  ``x_f64`` keeps the PCG iterate in float64 in float32
+        runs, as the reference's ConjugateGradient does (solver.py:24-26);
+        x_prev stays in the working dtype."""
         import ctypes as C
         self.assemble()
         ion = self.ionic_model
         u = ion.initialize(self.n_nodes)
-        self._X = [u.clone(), u.clone(), torch.empty_like(u)]
+        x_dtype = torch.float64 if (x_f64 or self.dtype == torch.float64) else self.dtype
+        self._X = [u.to(x_dtype).clone(), torch.empty_like(u, dtype=x_dtype)]
+        self._XP = [u.clone(), torch.empty_like(u)]
         self._w = torch.empty_like(u)
         tpf = int(snapshot_interval / self.dt)
         nframes = self.nt // tpf if tpf > 0 else 0
@@ -195,8 +200,10 @@ class Monodomain:
         d.dtype = L.dtype_code(self.dtype)
         d.n = self.n_nodes
         d.dt, d.dt_ionic, d.Cm, d.theta = self.dt, ion.dt, self.Cm, self.theta
-        for k in range(3):
+        d.x_f64 = 1 if x_dtype == torch.float64 else 0
+        for k in range(2):
             d.x[k] = L.ptr(self._X[k])
+            d.xp[k] = L.ptr(self._XP[k])
         d.w = L.ptr(self._w)
         d.AT = L.ptr(self.activation_time)
         d.RT = L.ptr(self.repolarization_time)
@@ -280,7 +287,7 @@ class Monodomain:
         torch.cuda.synchronize(self.device)
         t, steps, xi, nfr = self.sim_status()
         self.solving_time = time.time() - R["t0"]
-        self.u = self._X[xi]
+        self.u = self._X[xi].to(self.dtype)
         self.frames = self._frames[: 1 + nfr]
         AT, RT = self.activation_time, self.repolarization_time
         if self.result_path is not None:
