@@ -487,3 +487,31 @@ def test_runtime_matches_python_step_loop(model):
     L.call("tc_lat_update", L.ptr(u), L.ptr(AT), L.ptr(RT), u.numel(), float(prev), 1,
            L.stream_handle(DEV))
     assert torch.equal(AT, ref_at)
+
+
+@pytest.mark.parametrize("name", ["ttp_endo", "crn"])
+def test_single_cell_float32_vs_reference_float32(name):
+    """float32 kernels against the reference's own float32 single-cell trace
+    (both against its float64 trace for scale)."""
+    _say(f"[gpu] single cell f32 {name}")
+    from pathlib import Path
+    g32 = np.load(Path(__file__).parent / "golden" / f"ionic_{name}_f32.npz")["V"]
+    g64 = np.load(Path(__file__).parent / "golden" / f"ionic_{name}_f64.npz")["V"]
+    g = _gpu_model("ttp" if name == "ttp_endo" else "crn", torch.float32)
+    V = g.initialize(1)
+    tr = [V.item()]
+    t = 0.0
+    nsteps = 6000
+    for s in range(1, nsteps + 1):
+        V = V + 0.01 * g.differentiate(V)
+        t += 0.01
+        if 0.0 <= t <= 2.0:
+            V = V + 0.01 * 40.0
+        if s % 10 == 0:
+            tr.append(V.item())
+    tr = np.array(tr)
+    e32 = np.abs(tr - g32[: len(tr)]).max()
+    e64 = np.abs(tr - g64[: len(tr)]).max()
+    ref = np.abs(g32[: len(tr)] - g64[: len(tr)]).max()
+    _say(f"[gpu] {name}: |gpu32-ref32| {e32:.3e}  |gpu32-ref64| {e64:.3e}  |ref32-ref64| {ref:.3e}")
+    assert e32 < 0.05 and e64 < 0.05
