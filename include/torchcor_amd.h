@@ -201,6 +201,9 @@ typedef struct tc_ionic_io {
   /* V is float64 (rounded to the state dtype on load): the float64 PCG
    * iterate of a float32 run. */
   int v_f64;
+  /* w is stored in float64 and formed in float64 from the model's dV
+   * (float32 runs in precise-RHS mode). */
+  int w_f64;
 } tc_ionic_io;
 
 /* ModifiedMS2v (ionic/ModifiedMS2v.py:4-44).  States: H. */
@@ -272,9 +275,11 @@ typedef struct tc_sim_desc {
   int64_t n;
   double dt, dt_ionic, Cm, theta;
   int x_f64;                 /* iterate stored in float64 (float32 runs) */
+  int w_f64;                 /* precise RHS: w in float64, b never rounded, K u
+                                with the float64 iterate (needs x_f64) */
   void* x[2];                /* (n) iterate buffers (float64 when x_f64), current first */
   void* xp[2];               /* (n) x_prev buffers in dtype, current first */
-  void* w;                   /* (n) scratch vector */
+  void* w;                   /* (n) scratch vector (float64 when w_f64) */
   void* AT;                  /* (n) activation times or NULL */
   void* RT;                  /* (n) repolarisation times */
   const uint32_t* stim_mask; /* (n) stimulus bits or NULL */

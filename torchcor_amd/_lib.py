@@ -35,7 +35,7 @@ class IonicIO(C.Structure):
     _fields_ = [("n", i64), ("dt", dbl), ("dt_rhs", dbl), ("Cm", dbl), ("V", p), ("w", p),
                 ("dV", p), ("stim_mask", p), ("stim_active", C.c_uint32),
                 ("stim_amp", dbl * MAX_STIMULI), ("AT", p), ("RT", p), ("lat_t", dbl),
-                ("gate", p), ("v_f64", i32)]
+                ("gate", p), ("v_f64", i32), ("w_f64", i32)]
 
 
 MS2V_FIELDS = ["tau_in", "tau_out", "tau_open", "tau_close", "u_gate", "u_crit", "vmin", "vmax"]
@@ -73,7 +73,8 @@ class Stimulus(C.Structure):
 class SimDesc(C.Structure):
     _fields_ = [("model", i32), ("params", p), ("states", p), ("tab", p * 3), ("pcg", p),
                 ("Mv", p), ("Kv", p), ("dtype", i32), ("n", i64), ("dt", dbl), ("dt_ionic", dbl),
-                ("Cm", dbl), ("theta", dbl), ("x_f64", i32), ("x", p * 2), ("xp", p * 2),
+                ("Cm", dbl), ("theta", dbl), ("x_f64", i32), ("w_f64", i32), ("x", p * 2),
+                ("xp", p * 2),
                 ("w", p), ("AT", p), ("RT", p),
                 ("stim_mask", p), ("n_stim", i32), ("stim", Stimulus * MAX_STIMULI),
                 ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32), ("linear_guess", i32),

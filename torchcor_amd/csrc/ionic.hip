@@ -120,18 +120,29 @@ template <class T>
 __device__ inline void finish_node(const tc_ionic_io& io, int64_t i, T V, T dV) {
   if (io.dV) ((T*)io.dV)[i] = dV;
   if (io.w) {
-    const T du = dV / T(100);
-    T b = V * (T)io.Cm + (T)io.dt_rhs * du;
-    if (io.stim_active && io.stim_mask) {
-      const uint32_t bits = io.stim_mask[i] & io.stim_active;
+    const uint32_t bits = (io.stim_active && io.stim_mask) ? io.stim_mask[i] & io.stim_active : 0u;
+    if (io.w_f64) {
+      // float64 right-hand-side input from the model's dV
+      double b = (double)V * io.Cm + io.dt_rhs * ((double)dV / 100.0);
+      if (bits) {
+        double ist = 0.0;
+        for (int k = 0; k < TC_MAX_STIMULI; ++k)
+          if (bits & (1u << k)) ist += io.stim_amp[k];
+        b = b + io.dt_rhs * (ist / 100.0);
+      }
+      ((double*)io.w)[i] = b;
+    } else {
+      // the reference's order of operations in the working dtype
+      const T du = dV / T(100);
+      T b = V * (T)io.Cm + (T)io.dt_rhs * du;
       if (bits) {
         T ist = T(0);
         for (int k = 0; k < TC_MAX_STIMULI; ++k)
           if (bits & (1u << k)) ist += (T)io.stim_amp[k];
         b = b + (T)io.dt_rhs * (ist / T(100));
       }
+      ((T*)io.w)[i] = b;
     }
-    ((T*)io.w)[i] = b;
   }
   if (io.AT) {
     T* AT = (T*)io.AT;

@@ -400,11 +400,13 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepSta
 // formed from the same loads (A = Cm M + theta dt K).  With the linear guess
 // the pass also writes x_next = x0 and x_prev_next = x rounded to T
 // (solver.py:35-38: the reference keeps x_prev in the dtype of u).
-template <class T, class XT, bool LG>
+// WT is the type of w; with a float64 w (precise RHS) b is not rounded and
+// K u uses the float64 iterate itself.
+template <class T, class XT, class WT, bool LG>
 __global__ void __launch_bounds__(kPcgBlock)
 k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
            const T* __restrict__ Mv, const T* __restrict__ Kv, const T* __restrict__ Dinv,
-           const T* __restrict__ w, const XT* __restrict__ x, const T* __restrict__ xp,
+           const WT* __restrict__ w, const XT* __restrict__ x, const T* __restrict__ xp,
            XT* __restrict__ xnext, T* __restrict__ xpnext, T* __restrict__ r, double coef,
            double Cm, double tdt, PcgCtl* ctl, double a_tol, double r_tol, double* part0) {
   {
@@ -432,7 +434,8 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
       else x0 = (double)xj;
       const double mj = (double)m[j * 64], kj = (double)kk[j * 64];
       am += mj * (double)w[cj];
-      ak += kj * (double)(T)xj;
+      if (sizeof(WT) == 8) ak += kj * (double)xj;
+      else ak += kj * (double)(T)xj;
       mx += mj * x0;
       kx += kj * x0;
     }
@@ -441,12 +444,13 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
         xnext[i] = (XT)(2.0 * (double)x[i] - (double)xp[i]);
         xpnext[i] = (T)x[i];
       }
-      const T bi = (T)(am - coef * ak);
+      double bi = am - coef * ak;
+      if (sizeof(WT) != 8) bi = (double)(T)bi;
       const double d = (double)Dinv[i];
-      const T ri = (T)((double)bi - (Cm * mx + tdt * kx));
+      const T ri = (T)(bi - (Cm * mx + tdt * kx));
       r[i] = ri;
       red[0] += (double)ri * (d * (double)ri);
-      const double db = d * (double)bi;
+      const double db = d * bi;
       red[1] += db * db;
     }
   }

@@ -185,7 +185,7 @@ int desired_K(tc_sim* S) {
 }
 
 // Enqueue the step after planned state ps; ps becomes the state after it.
-template <class T, class XT>
+template <class T, class XT, class WT>
 int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   const tc_sim_desc& d = S->d;
   tc_pcg* P = d.pcg;
@@ -209,6 +209,7 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   io.Cm = d.Cm;
   io.V = d.x[xc];
   io.v_f64 = (sizeof(XT) == 8 && sizeof(T) == 4) ? 1 : 0;
+  io.w_f64 = (sizeof(WT) == 8 && sizeof(T) == 4) ? 1 : 0;
   io.w = d.w;
   io.dV = nullptr;
   const uint32_t bits = active_bits(d, t);
@@ -232,15 +233,15 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   XT* xs;
   if (ev) TC_HIP(hipEventRecord((*ev)[2], s));
   if (d.linear_guess) {
-    hipLaunchKernelGGL((k_rhs_init<T, XT, true>), g, blk, 0, s, d.n, P->A.slice_ptr, P->A.col,
-                       (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const T*)d.w,
+    hipLaunchKernelGGL((k_rhs_init<T, XT, WT, true>), g, blk, 0, s, d.n, P->A.slice_ptr,
+                       P->A.col, (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const WT*)d.w,
                        (const XT*)d.x[xc], (const T*)d.xp[pc], (XT*)d.x[1 - xc],
                        (T*)d.xp[1 - pc], (T*)P->r, coef, d.Cm, tdt, P->ctl, d.a_tol, d.r_tol,
                        P->part[0]);
     xs = (XT*)d.x[1 - xc];
   } else {
-    hipLaunchKernelGGL((k_rhs_init<T, XT, false>), g, blk, 0, s, d.n, P->A.slice_ptr, P->A.col,
-                       (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const T*)d.w,
+    hipLaunchKernelGGL((k_rhs_init<T, XT, WT, false>), g, blk, 0, s, d.n, P->A.slice_ptr,
+                       P->A.col, (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const WT*)d.w,
                        (const XT*)d.x[xc], (const T*)nullptr, (XT*)nullptr, (T*)nullptr,
                        (T*)P->r, coef, d.Cm, tdt, P->ctl, d.a_tol, d.r_tol, P->part[0]);
     xs = (XT*)d.x[xc];
@@ -339,7 +340,7 @@ void accumulate_timing(tc_sim* S, const Plan& plan, int iters_run) {
   }
 }
 
-template <class T, class XT>
+template <class T, class XT, class WT>
 int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
   const int depth = S->d.depth < 1 ? 1 : (S->d.depth > kRing / 2 ? kRing / 2 : S->d.depth);
   std::deque<Plan> inflight;
@@ -348,7 +349,7 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
   while (done < nsteps) {
     while ((int)inflight.size() < depth && enq < nsteps) {
       Plan plan;
-      int rc = enqueue<T, XT>(S, ps, plan, s);
+      int rc = enqueue<T, XT, WT>(S, ps, plan, s);
       if (rc) return rc;
       inflight.push_back(plan);
       ++enq;
@@ -391,6 +392,7 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
   TC_CHECK(d->n_stim >= 0 && d->n_stim <= TC_MAX_STIMULI, TC_ERR_ARG, "bad stimulus count");
   TC_CHECK(d->n_stim == 0 || d->stim_mask, TC_ERR_ARG, "stimuli need a mask");
   TC_CHECK(d->max_iter >= 0, TC_ERR_ARG, "max_iter < 0");
+  TC_CHECK(!d->w_f64 || d->x_f64 || d->dtype == TC_F64, TC_ERR_ARG, "w_f64 needs x_f64");
   TC_CHECK(!d->AT || d->RT, TC_ERR_ARG, "AT without RT");
   TC_CHECK(d->pcg->n == d->n && d->pcg->dtype == d->dtype, TC_ERR_ARG,
            "tc_sim_create: PCG plan does not match n/dtype");
@@ -447,9 +449,10 @@ extern "C" int tc_sim_advance(tc_sim* S, int64_t nsteps, int32_t* iters, void* f
   S->tpf = tpf;
   S->max_frames = frames ? max_frames : 0;
   hipStream_t s = (hipStream_t)stream;
-  if (S->d.dtype == TC_F64) return advance<double, double>(S, nsteps, iters, s);
-  if (S->d.x_f64) return advance<float, double>(S, nsteps, iters, s);
-  return advance<float, float>(S, nsteps, iters, s);
+  if (S->d.dtype == TC_F64) return advance<double, double, double>(S, nsteps, iters, s);
+  if (S->d.x_f64 && S->d.w_f64) return advance<float, double, double>(S, nsteps, iters, s);
+  if (S->d.x_f64) return advance<float, double, float>(S, nsteps, iters, s);
+  return advance<float, float, float>(S, nsteps, iters, s);
 }
 
 extern "C" int tc_sim_finish(tc_sim* S, void* stream) {

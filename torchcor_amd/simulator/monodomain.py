@@ -168,10 +168,12 @@ class Monodomain:
     # loop itself runs in the native runtime (tc_sim_*): steps are pipelined
     # on the device and no host round trip is needed per step.
     def start(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True,
-              snapshot_interval=1, verbose=False, depth=4, x_f64=True):
+              snapshot_interval=1, verbose=False, depth=4, x_f64=True, precise_rhs=True):
         """Set up a run.  ``x_f64`` keeps the PCG iterate in float64 in float32
         runs, as the reference's ConjugateGradient does (solver.py:24-26);
-        x_prev stays in the working dtype."""
+        x_prev stays in the working dtype.  ``precise_rhs`` also keeps the
+        right-hand side in float64 (w stored in float64, b never rounded);
+        with it off a float32 run rounds w and b as the reference does."""
         import ctypes as C
         self.assemble()
         ion = self.ionic_model
@@ -179,7 +181,8 @@ class Monodomain:
         x_dtype = torch.float64 if (x_f64 or self.dtype == torch.float64) else self.dtype
         self._X = [u.to(x_dtype).clone(), torch.empty_like(u, dtype=x_dtype)]
         self._XP = [u.clone(), torch.empty_like(u)]
-        self._w = torch.empty_like(u)
+        w_f64 = bool(precise_rhs and x_dtype == torch.float64)
+        self._w = torch.empty_like(u, dtype=torch.float64 if w_f64 else self.dtype)
         tpf = int(snapshot_interval / self.dt)
         nframes = self.nt // tpf if tpf > 0 else 0
         self._frames = torch.empty((nframes + 1, self.n_nodes), dtype=self.dtype, device=self.device)
@@ -201,6 +204,7 @@ class Monodomain:
         d.n = self.n_nodes
         d.dt, d.dt_ionic, d.Cm, d.theta = self.dt, ion.dt, self.Cm, self.theta
         d.x_f64 = 1 if x_dtype == torch.float64 else 0
+        d.w_f64 = 1 if w_f64 else 0
         for k in range(2):
             d.x[k] = L.ptr(self._X[k])
             d.xp[k] = L.ptr(self._XP[k])
