@@ -30,18 +30,20 @@ METRIC = "node-timesteps/s + PCG HBM GB/s (1M-node TTP); sims/hour & p99 lat bat
 HBM_PEAK_GBPS = 8000.0
 
 
-def alg_bytes(n, nnz, S):
-    """Algorithmic HBM bytes per launch of each kernel: FP32 values, int32
-    indices, FP32 vectors (each array read or written once)."""
+def alg_bytes(n, nnz, S, ib=4, xb=4, wb=4):
+    """Algorithmic HBM bytes per launch of each kernel (each array read or
+    written once; slice pointers ignored): float32 values and states, `ib`-byte
+    column indices (2 with int16 offsets), `xb`-byte iterate x and `wb`-byte w
+    (8 in float32 runs with the float64 iterate and right-hand side)."""
     return {
-        # V, S states in; S states, w out; AT, RT read
-        "ionic": n * 4 * (1 + S + S + 1) + 8 * n,
-        # M, K, A values + columns, row pointers; w, x, x_prev, Dinv in; x_next, r out
-        "rhs_init": 16 * nnz + 4 * (n + 1) + 24 * n,
-        # A values + columns, row pointers; r, Dinv, p in; p', Ap out
-        "pcg_spmv": 8 * nnz + 4 * (n + 1) + 20 * n,
-        # x, p', r, Ap, Dinv in; x, r out
-        "pcg_update": 28 * n,
+        # V (= x) and S states in; S states, w out; AT, RT read
+        "ionic": n * (xb + 8 * S + wb + 8),
+        # M, K values + columns; w, x, x_prev, Dinv in; x_next, x_prev_next, r out
+        "rhs_init": (8 + ib) * nnz + n * (wb + xb + 4 + 4 + xb + 4 + 4),
+        # A values + columns; r, Dinv, p in; p', Ap out
+        "pcg_spmv": (4 + ib) * nnz + 20 * n,
+        # p', Ap, Dinv, r, x in; x, r out
+        "pcg_update": n * (12 + 4 + xb + xb + 4),
     }
 
 
@@ -104,7 +106,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
-    ap.add_argument("--timing-period", type=int, default=8,
+    ap.add_argument("--timing-period", type=int, default=16,
                     help="time the kernels of every k-th step with HIP events")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -140,7 +142,8 @@ def main():
     value = n * args.steps / elapsed
     kt = {} if args.no_kernel_timing else sim.kernel_times()
     S = len(sim.ionic_model.STATES)
-    B = alg_bytes(n, nnz, S)
+    ib = 2 if sim.system.sell.dcol is not None else 4
+    B = alg_bytes(n, nnz, S, ib=ib, xb=sim._X[0].element_size(), wb=sim._w.element_size())
     kernels = {}
     for k, (ms, cnt) in kt.items():
         if cnt == 0:

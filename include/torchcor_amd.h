@@ -109,6 +109,9 @@ typedef struct tc_sell {
   int64_t n_slices;       /* ceil(n/64) */
   const int32_t* slice_ptr; /* n_slices+1 entry offsets */
   const int32_t* col;     /* padded column indices */
+  /* optional (NULL: unused): col - row as int16, from tc_sell_deltas when
+   * every offset fits; the PCG and RHS passes then read these instead */
+  const int16_t* dcol;
 } tc_sell;
 
 /* Padded entry count; synchronizes `stream`.  slice_ptr (n_slices+1) is
@@ -119,6 +122,10 @@ int tc_sell_columns(const int32_t* row_ptr, const int32_t* col_idx, int64_t n,
                     const int32_t* slice_ptr, int32_t* sell_col, void* stream);
 int tc_sell_values(const int32_t* row_ptr, int64_t n, const int32_t* slice_ptr,
                    const void* csr_vals, void* sell_vals, int dtype, void* stream);
+/* dcol[e] = col[e] - row(e) as int16 for every padded entry; *max_abs_dev
+ * (one device int32) receives the largest |col - row|, so dcol is usable
+ * when it is <= 32767.  Stream-ordered, does not synchronize. */
+int tc_sell_deltas(const tc_sell* A, int16_t* dcol, int32_t* max_abs_dev, void* stream);
 /* y = A x over a SELL operator (float64 accumulation), for tests and
  * callers of Monodomain.M/K/A products. */
 int tc_sell_spmv(const tc_sell* A, const void* vals, const void* x, void* y, int dtype,

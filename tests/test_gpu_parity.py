@@ -515,3 +515,34 @@ def test_single_cell_float32_vs_reference_float32(name):
     ref = np.abs(g32[: len(tr)] - g64[: len(tr)]).max()
     _say(f"[gpu] {name}: |gpu32-ref32| {e32:.3e}  |gpu32-ref64| {e64:.3e}  |ref32-ref64| {ref:.3e}")
     assert e32 < 0.05 and e64 < 0.05
+
+
+def test_runtime_int16_columns_identical(monkeypatch):
+    """The hot passes on int16 column offsets give bit-identical results to
+    the int32 columns, and a permuted mesh whose offsets do not fit falls
+    back to int32 columns."""
+    from torchcor_amd.core.system import P1System
+    _say("[gpu] runtime int16 vs int32 columns")
+    runs = []
+    for flag in (True, False):
+        monkeypatch.setattr(P1System, "int16_columns", flag)
+        sim = _cube_sim(11, "ttp", torch.float32)
+        sim.start(1e-5, 1e-5, 100, snapshot_interval=1.0, depth=4)
+        assert (sim.system.sell.dcol is not None) == flag
+        sim.advance(300)
+        sim.finish()
+        runs.append((sim.u.clone(), list(sim.iterations), sim.frames.clone()))
+    assert torch.equal(runs[0][0], runs[1][0]) and runs[0][1] == runs[1][1]
+    assert torch.equal(runs[0][2], runs[1][2])
+    monkeypatch.setattr(P1System, "int16_columns", True)
+    nodes, elems = meshes.kuhn_cube(41)
+    perm = np.random.default_rng(0).permutation(len(nodes))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    S = _sys(nodes[perm], inv[elems].astype(elems.dtype))
+    E = len(elems)
+    M, K = S.assemble(sigma_l=torch.full((E,), 0.3, dtype=torch.float64),
+                      sigma_t=torch.full((E,), 0.1, dtype=torch.float64),
+                      fibres=torch.tensor(_rand_fibres(E, 2)))
+    S.set_operators(M, K, 140, 0.01, 0.5, 0.01, torch.float32)
+    assert S.bandwidth > 32767 and S.sell.dcol is None

@@ -28,7 +28,7 @@ dbl = C.c_double
 
 
 class Sell(C.Structure):
-    _fields_ = [("n", i64), ("n_slices", i64), ("slice_ptr", p), ("col", p)]
+    _fields_ = [("n", i64), ("n_slices", i64), ("slice_ptr", p), ("col", p), ("dcol", p)]
 
 
 class IonicIO(C.Structure):
@@ -111,6 +111,7 @@ FUNCS = {
     "tc_sell_layout": (p, i64, p, I64_P, p),
     "tc_sell_columns": (p, p, i64, p, p, p),
     "tc_sell_values": (p, i64, p, p, p, i32, p),
+    "tc_sell_deltas": (SELL_P, p, p, p),
     "tc_sell_spmv": (SELL_P, p, p, p, i32, p),
     "tc_rhs": (SELL_P, p, p, p, p, dbl, p, i32, p),
     "tc_pcg_workspace_size": (i64, i32, SIZE_P),

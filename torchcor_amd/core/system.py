@@ -22,6 +22,9 @@ class P1System:
     contributions in the reference's listing order.
     """
 
+    #: hot passes read int16 column offsets (col - row) when every offset fits
+    int16_columns = True
+
     def __init__(self, nodes, elems, device=None):
         device = torch.device(device) if device is not None else elems.device
         if device.type != "cuda":
@@ -101,7 +104,16 @@ class P1System:
         L.call("tc_sell_columns", L.ptr(self.row_ptr), L.ptr(self.col), self.n, L.ptr(self.slice_ptr),
                L.ptr(self.scol), s)
         self.sM, self.sK, self.sA = (self.to_sell(v) for v in (self.Mv, self.Kv, self.Av))
-        self.sell = L.Sell(self.n, ns, L.ptr(self.slice_ptr), L.ptr(self.scol))
+        self.sell = L.Sell(self.n, ns, L.ptr(self.slice_ptr), L.ptr(self.scol), None)
+        # int16 column offsets for the hot passes when the bandwidth allows
+        self.dcol = torch.empty(self.padded, dtype=torch.int16, device=dev)
+        mx = torch.empty(1, dtype=torch.int32, device=dev)
+        L.call("tc_sell_deltas", C.byref(self.sell), L.ptr(self.dcol), L.ptr(mx), s)
+        self.bandwidth = int(mx.item())
+        if self.int16_columns and self.bandwidth <= 32767:
+            self.sell.dcol = L.ptr(self.dcol)
+        else:
+            self.dcol = None
         self._make_pcg()
 
     def to_sell(self, vals):

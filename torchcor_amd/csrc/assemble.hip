@@ -385,6 +385,35 @@ __global__ void k_sell_columns(const int32_t* __restrict__ row_ptr,
   for (int j = 0; j < w; ++j) scol[base + (int64_t)j * kWave] = j < len ? col[b + j] : pad;
 }
 
+__global__ void k_zero_int(int32_t* p) { *p = 0; }
+
+// dcol = scol - row (int16, saturated); *max_abs = the largest |scol - row|.
+__global__ void k_sell_deltas(const int32_t* __restrict__ scol, int64_t n_slices,
+                              const int32_t* __restrict__ slice_ptr, int16_t* __restrict__ dcol,
+                              int32_t* max_abs) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int32_t m = 0;
+  if (i < n_slices * kWave) {
+    const int64_t s = i / kWave;
+    const int l = (int)(i - s * kWave);
+    const int w = (slice_ptr[s + 1] - slice_ptr[s]) / kWave;
+    const int64_t base = slice_ptr[s] + l;
+    for (int j = 0; j < w; ++j) {
+      const int64_t e = base + (int64_t)j * kWave;
+      int64_t d = (int64_t)scol[e] - i;
+      const int64_t a = d < 0 ? -d : d;
+      if (a > m) m = a > INT32_MAX ? INT32_MAX : (int32_t)a;
+      d = d > 32767 ? 32767 : (d < -32767 ? -32767 : d);
+      dcol[e] = (int16_t)d;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const int32_t v = __shfl_xor(m, o);
+    m = v > m ? v : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(max_abs, m);
+}
+
 template <class T>
 __global__ void k_sell_values(const int32_t* __restrict__ row_ptr, int64_t n,
                               int64_t n_slices, const int32_t* __restrict__ slice_ptr,
@@ -538,6 +567,19 @@ extern "C" int tc_sell_columns(const int32_t* row_ptr, const int32_t* col, int64
   int64_t ns = ceil_div(n, kWave);
   hipLaunchKernelGGL(k_sell_columns, dim3((int)ceil_div(ns * kWave, kBlock)), dim3(kBlock), 0,
                      (hipStream_t)stream, row_ptr, col, n, ns, slice_ptr, scol);
+  TC_LAUNCH();
+  return TC_OK;
+}
+
+extern "C" int tc_sell_deltas(const tc_sell* A, int16_t* dcol, int32_t* max_abs, void* stream) {
+  TC_CHECK(A && A->slice_ptr && A->col && dcol && max_abs && A->n > 0, TC_ERR_ARG,
+           "tc_sell_deltas: bad arguments");
+  const int64_t ns = ceil_div(A->n, kWave);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, max_abs);
+  TC_LAUNCH();
+  hipLaunchKernelGGL(k_sell_deltas, dim3((int)ceil_div(ns * kWave, kBlock)), dim3(kBlock), 0, s,
+                     A->col, ns, A->slice_ptr, dcol, max_abs);
   TC_LAUNCH();
   return TC_OK;
 }

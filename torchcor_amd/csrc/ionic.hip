@@ -653,19 +653,23 @@ int check_tab(const TabParam& t) {
 
 using namespace tc;
 
-extern "C" int tc_ms2v_step(const tc_ms2v_params* p, void* states, const tc_ionic_io* io,
-                            int dtype, void* stream) {
+int tc::ms2v_step(const tc_ms2v_params* p, void* states, const tc_ionic_io* io, int dtype,
+                  hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   int rc = check_io(io, dtype);
   if (rc) return rc;
   TC_CHECK(p && states, TC_ERR_ARG, "tc_ms2v_step: bad arguments");
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
-  hipStream_t s = (hipStream_t)stream;
   if (dtype == TC_F32)
-    hipLaunchKernelGGL(k_ms2v<float>, g, b, 0, s, *p, (float*)states, *io);
+    launch(k_ms2v<float>, g, b, s, e0, e1, *p, (float*)states, *io);
   else
-    hipLaunchKernelGGL(k_ms2v<double>, g, b, 0, s, *p, (double*)states, *io);
+    launch(k_ms2v<double>, g, b, s, e0, e1, *p, (double*)states, *io);
   TC_LAUNCH();
   return TC_OK;
+}
+
+extern "C" int tc_ms2v_step(const tc_ms2v_params* p, void* states, const tc_ionic_io* io,
+                            int dtype, void* stream) {
+  return ms2v_step(p, states, io, dtype, (hipStream_t)stream, nullptr, nullptr);
 }
 
 extern "C" int tc_ttp_tables(const tc_ttp_params* p, double dt, void* V_tab, void* CaSS_tab,
@@ -687,24 +691,30 @@ extern "C" int tc_ttp_tables(const tc_ttp_params* p, double dt, void* V_tab, voi
   return TC_OK;
 }
 
-extern "C" int tc_ttp_step(const tc_ttp_params* p, void* states, const void* V_tab,
-                           const void* CaSS_tab, const void* VEk_tab, const tc_ionic_io* io,
-                           int dtype, void* stream) {
+int tc::ttp_step(const tc_ttp_params* p, void* states, const void* V_tab, const void* CaSS_tab,
+                 const void* VEk_tab, const tc_ionic_io* io, int dtype, hipStream_t s,
+                 hipEvent_t e0, hipEvent_t e1) {
   int rc = check_io(io, dtype);
   if (rc) return rc;
   TC_CHECK(p && states && V_tab && CaSS_tab && VEk_tab, TC_ERR_ARG, "tc_ttp_step: bad arguments");
   TtpTabs tb = ttp_tabs();
   if ((rc = check_tab(tb.V)) || (rc = check_tab(tb.CaSS)) || (rc = check_tab(tb.VEk))) return rc;
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
-  hipStream_t s = (hipStream_t)stream;
   if (dtype == TC_F32)
-    hipLaunchKernelGGL(k_ttp<float>, g, b, 0, s, *p, (float*)states, (const float*)V_tab,
-                       (const float*)CaSS_tab, (const float*)VEk_tab, tb, *io);
+    launch(k_ttp<float>, g, b, s, e0, e1, *p, (float*)states, (const float*)V_tab,
+           (const float*)CaSS_tab, (const float*)VEk_tab, tb, *io);
   else
-    hipLaunchKernelGGL(k_ttp<double>, g, b, 0, s, *p, (double*)states, (const double*)V_tab,
-                       (const double*)CaSS_tab, (const double*)VEk_tab, tb, *io);
+    launch(k_ttp<double>, g, b, s, e0, e1, *p, (double*)states, (const double*)V_tab,
+           (const double*)CaSS_tab, (const double*)VEk_tab, tb, *io);
   TC_LAUNCH();
   return TC_OK;
+}
+
+extern "C" int tc_ttp_step(const tc_ttp_params* p, void* states, const void* V_tab,
+                           const void* CaSS_tab, const void* VEk_tab, const tc_ionic_io* io,
+                           int dtype, void* stream) {
+  return ttp_step(p, states, V_tab, CaSS_tab, VEk_tab, io, dtype, (hipStream_t)stream, nullptr,
+                  nullptr);
 }
 
 extern "C" int tc_crn_tables(const tc_crn_params* p, double dt, void* Cai_tab, void* V_tab,
@@ -726,24 +736,30 @@ extern "C" int tc_crn_tables(const tc_crn_params* p, double dt, void* Cai_tab, v
   return TC_OK;
 }
 
-extern "C" int tc_crn_step(const tc_crn_params* p, void* states, const void* Cai_tab,
-                           const void* V_tab, const void* fn_tab, const tc_ionic_io* io,
-                           int dtype, void* stream) {
+int tc::crn_step(const tc_crn_params* p, void* states, const void* Cai_tab, const void* V_tab,
+                 const void* fn_tab, const tc_ionic_io* io, int dtype, hipStream_t s,
+                 hipEvent_t e0, hipEvent_t e1) {
   int rc = check_io(io, dtype);
   if (rc) return rc;
   TC_CHECK(p && states && Cai_tab && V_tab && fn_tab, TC_ERR_ARG, "tc_crn_step: bad arguments");
   CrnTabs tb = crn_tabs();
   if ((rc = check_tab(tb.Cai)) || (rc = check_tab(tb.V)) || (rc = check_tab(tb.fn))) return rc;
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
-  hipStream_t s = (hipStream_t)stream;
   if (dtype == TC_F32)
-    hipLaunchKernelGGL(k_crn<float>, g, b, 0, s, *p, (float*)states, (const float*)Cai_tab,
-                       (const float*)V_tab, (const float*)fn_tab, tb, *io);
+    launch(k_crn<float>, g, b, s, e0, e1, *p, (float*)states, (const float*)Cai_tab,
+           (const float*)V_tab, (const float*)fn_tab, tb, *io);
   else
-    hipLaunchKernelGGL(k_crn<double>, g, b, 0, s, *p, (double*)states, (const double*)Cai_tab,
-                       (const double*)V_tab, (const double*)fn_tab, tb, *io);
+    launch(k_crn<double>, g, b, s, e0, e1, *p, (double*)states, (const double*)Cai_tab,
+           (const double*)V_tab, (const double*)fn_tab, tb, *io);
   TC_LAUNCH();
   return TC_OK;
+}
+
+extern "C" int tc_crn_step(const tc_crn_params* p, void* states, const void* Cai_tab,
+                           const void* V_tab, const void* fn_tab, const tc_ionic_io* io,
+                           int dtype, void* stream) {
+  return crn_step(p, states, Cai_tab, V_tab, fn_tab, io, dtype, (hipStream_t)stream, nullptr,
+                  nullptr);
 }
 
 extern "C" int tc_lat_update(const void* u, void* AT, void* RT, int64_t n, double t, int dtype,

@@ -9,19 +9,19 @@ namespace {
 int ev_reserve(tc_pcg* P, size_t pairs) {
   while (P->ev.size() < 2 * pairs) {
     hipEvent_t e;
-    TC_HIP(hipEventCreate(&e));
+    // timing only: no system-scope fence (no cache write-back per record)
+    TC_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     P->ev.push_back(e);
   }
   return TC_OK;
 }
 
-template <class T>
-int pcg_solve(tc_pcg* P, const T* b, T* x, const T* xp, T* xn, double a_tol, double r_tol,
-              int max_iter, int lg, int* n_iter_host, hipStream_t s) {
+template <class T, class CI>
+int pcg_solve(tc_pcg* P, const CI* sc, const T* b, T* x, const T* xp, T* xn, double a_tol,
+              double r_tol, int max_iter, int lg, int* n_iter_host, hipStream_t s) {
   const int64_t n = P->n;
   const dim3 g(P->G), blk(kPcgBlock);
   const int32_t* sp = P->A.slice_ptr;
-  const int32_t* sc = P->A.col;
   const T* Av = (const T*)P->Av;
   const T* D = (const T*)P->Dinv;
   T* r = (T*)P->r;
@@ -30,22 +30,21 @@ int pcg_solve(tc_pcg* P, const T* b, T* x, const T* xp, T* xn, double a_tol, dou
   if (P->timing) {
     int rc = ev_reserve(P, 1 + 2 * (size_t)(max_iter > 0 ? max_iter : 0));
     if (rc) return rc;
-    TC_HIP(hipEventRecord(P->ev[0], s));
   }
+  auto ev = [&](size_t k) { return P->timing ? P->ev[k] : (hipEvent_t) nullptr; };
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, &P->ctl->stall, 0);
   TC_LAUNCH();
   T* xs;
   if (lg) {
-    hipLaunchKernelGGL((k_pcg_init<T, true>), g, blk, 0, s, n, sp, sc, Av, D, b, x, xp, xn, r,
-                       P->ctl, a_tol, r_tol, P->part[0]);
+    launch(k_pcg_init<T, true, CI>, g, blk, s, ev(0), ev(1), n, sp, sc, Av, D, b, (const T*)x, xp,
+           xn, r, P->ctl, a_tol, r_tol, P->part[0]);
     xs = xn;
   } else {
-    hipLaunchKernelGGL((k_pcg_init<T, false>), g, blk, 0, s, n, sp, sc, Av, D, b, x, xp,
-                       (T*)nullptr, r, P->ctl, a_tol, r_tol, P->part[0]);
+    launch(k_pcg_init<T, false, CI>, g, blk, s, ev(0), ev(1), n, sp, sc, Av, D, b, (const T*)x,
+           xp, (T*)nullptr, r, P->ctl, a_tol, r_tol, P->part[0]);
     xs = x;
   }
   TC_LAUNCH();
-  if (P->timing) TC_HIP(hipEventRecord(P->ev[1], s));
   int it = 0;
   int chunk = P->chunk < 1 ? 1 : P->chunk;
   P->host_ctl->converged = 0;
@@ -57,18 +56,13 @@ int pcg_solve(tc_pcg* P, const T* b, T* x, const T* xp, T* xn, double a_tol, dou
       const int k = it + j;
       T* pin = pp[k & 1];
       T* pout = pp[(k + 1) & 1];
-      if (P->timing) TC_HIP(hipEventRecord(P->ev[2 + 4 * k], s));
-      hipLaunchKernelGGL(k_pcg_spmv<T>, g, blk, 0, s, n, sp, sc, Av, D, r, pin, pout, Ap, k,
-                         P->ctl, P->part[0], P->part[2], P->part[1]);
+      launch(k_pcg_spmv<T, CI>, g, blk, s, ev(2 + 4 * k), ev(3 + 4 * k), n, sp, sc, Av, D,
+             (const T*)r, (const T*)pin, pout, Ap, k, P->ctl, (const double*)P->part[0],
+             (const double*)P->part[2], P->part[1]);
       TC_LAUNCH();
-      if (P->timing) {
-        TC_HIP(hipEventRecord(P->ev[3 + 4 * k], s));
-        TC_HIP(hipEventRecord(P->ev[4 + 4 * k], s));
-      }
-      hipLaunchKernelGGL(k_pcg_update<T>, g, blk, 0, s, n, D, pout, Ap, xs, r, k, P->ctl,
-                         P->part[1], P->part[2]);
+      launch(k_pcg_update<T, T>, g, blk, s, ev(4 + 4 * k), ev(5 + 4 * k), n, D, (const T*)pout,
+             (const T*)Ap, xs, r, k, P->ctl, (const double*)P->part[1], P->part[2]);
       TC_LAUNCH();
-      if (P->timing) TC_HIP(hipEventRecord(P->ev[5 + 4 * k], s));
     }
     it += c;
     hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kPcgBlock), 0, s, it - 1, P->G, P->ctl, P->part[2],
@@ -235,9 +229,11 @@ extern "C" int tc_pcg_solve(tc_pcg* P, const void* b, void* x, const void* x_pre
   TC_CHECK(!linear_guess || (x_prev && x_next), TC_ERR_ARG,
            "tc_pcg_solve: linear_guess needs x_prev and x_next");
   hipStream_t s = (hipStream_t)stream;
+#define TC_PCG_SOLVE(T, SC)                                                                  \
+  pcg_solve<T>(P, SC, (const T*)b, (T*)x, (const T*)x_prev, (T*)x_next, a_tol, r_tol, max_iter, \
+               linear_guess, n_iter_host, s)
   if (P->dtype == TC_F32)
-    return pcg_solve<float>(P, (const float*)b, (float*)x, (const float*)x_prev,
-                            (float*)x_next, a_tol, r_tol, max_iter, linear_guess, n_iter_host, s);
-  return pcg_solve<double>(P, (const double*)b, (double*)x, (const double*)x_prev,
-                           (double*)x_next, a_tol, r_tol, max_iter, linear_guess, n_iter_host, s);
+    return P->A.dcol ? TC_PCG_SOLVE(float, P->A.dcol) : TC_PCG_SOLVE(float, P->A.col);
+  return P->A.dcol ? TC_PCG_SOLVE(double, P->A.dcol) : TC_PCG_SOLVE(double, P->A.col);
+#undef TC_PCG_SOLVE
 }

@@ -85,6 +85,12 @@ constexpr int kWaves = kPcgBlock / 64;
 constexpr int kMaxBlocks = 1024;
 constexpr int kXcd = 8;
 
+// Column of a SELL entry in row `row`: int32 absolute columns, or int16
+// offsets col - row when every offset of the operator fits in 16 bits (half
+// the index traffic of the hot passes).
+__device__ inline int32_t col_of(int32_t c, int64_t) { return c; }
+__device__ inline int32_t col_of(int16_t c, int64_t row) { return (int32_t)row + (int32_t)c; }
+
 // ------------------------------------------------------------------ SpMV
 template <class T>
 __global__ void __launch_bounds__(kBlock)
@@ -198,9 +204,9 @@ __device__ inline bool stop_rule(double zn, const PcgCtl* ctl) {
 
 // PCG start: x0 = 2x - x_prev (linear guess) or x; r = b - A x0;
 // partial sums of r.z and (Dinv b)^2.
-template <class T, bool LG>
+template <class T, bool LG, class CI>
 __global__ void __launch_bounds__(kPcgBlock)
-k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
+k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ scol,
            const T* __restrict__ Av, const T* __restrict__ Dinv, const T* __restrict__ bvec,
            const T* __restrict__ x, const T* __restrict__ xp, T* __restrict__ xnext,
            T* __restrict__ r, PcgCtl* ctl, double a_tol, double r_tol, double* part0) {
@@ -211,11 +217,11 @@ k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
   for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
     const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
-    const int32_t* c = scol + b + lane;
+    const CI* c = scol + b + lane;
     const T* a = Av + b + lane;
     double ax = 0.0;
     for (int j = 0; j < wd; ++j) {
-      const int32_t cj = c[j * 64];
+      const int32_t cj = col_of(c[j * 64], i);
       T xj;
       if (LG) xj = (T)(2.0 * (double)x[cj] - (double)xp[cj]);
       else xj = x[cj];
@@ -244,9 +250,9 @@ k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
 
 // spmv pass of iteration k.  pin is the previous p (unused when k == 0),
 // pout receives p'.  Iteration k-1's stopping test happens here first.
-template <class T>
+template <class T, class CI>
 __global__ void __launch_bounds__(kPcgBlock)
-k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
+k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ scol,
            const T* __restrict__ Av, const T* __restrict__ Dinv, const T* __restrict__ r,
            const T* __restrict__ pin, T* __restrict__ pout, T* __restrict__ Ap, int k,
            PcgCtl* ctl, const double* __restrict__ part0, const double* __restrict__ part2,
@@ -285,18 +291,18 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
   for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
     const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
-    const int32_t* c = scol + b + lane;
+    const CI* c = scol + b + lane;
     const T* a = Av + b + lane;
     double acc = 0.0;
     if (k == 0) {
       for (int j = 0; j < wd; ++j) {
-        const int32_t cj = c[j * 64];
+        const int32_t cj = col_of(c[j * 64], i);
         const T pj = (T)((double)Dinv[cj] * (double)r[cj]);
         acc += (double)a[j * 64] * (double)pj;
       }
     } else {
       for (int j = 0; j < wd; ++j) {
-        const int32_t cj = c[j * 64];
+        const int32_t cj = col_of(c[j * 64], i);
         const T pj = (T)(beta * (double)pin[cj] + (double)Dinv[cj] * (double)r[cj]);
         acc += (double)a[j * 64] * (double)pj;
       }
@@ -402,9 +408,9 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepSta
 // (solver.py:35-38: the reference keeps x_prev in the dtype of u).
 // WT is the type of w; with a float64 w (precise RHS) b is not rounded and
 // K u uses the float64 iterate itself.
-template <class T, class XT, class WT, bool LG>
+template <class T, class XT, class WT, bool LG, class CI>
 __global__ void __launch_bounds__(kPcgBlock)
-k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ scol,
+k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ scol,
            const T* __restrict__ Mv, const T* __restrict__ Kv, const T* __restrict__ Dinv,
            const WT* __restrict__ w, const XT* __restrict__ x, const T* __restrict__ xp,
            XT* __restrict__ xnext, T* __restrict__ xpnext, T* __restrict__ r, double coef,
@@ -422,12 +428,12 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restric
   for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
     const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
-    const int32_t* c = scol + b + lane;
+    const CI* c = scol + b + lane;
     const T* m = Mv + b + lane;
     const T* kk = Kv + b + lane;
     double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
     for (int j = 0; j < wd; ++j) {
-      const int32_t cj = c[j * 64];
+      const int32_t cj = col_of(c[j * 64], i);
       const XT xj = x[cj];
       double x0;
       if (LG) x0 = (double)(XT)(2.0 * (double)xj - (double)xp[cj]);

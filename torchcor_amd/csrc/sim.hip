@@ -95,24 +95,36 @@ uint32_t active_bits(const tc_sim_desc& d, double t) {
 int ev_reserve(tc_sim* S, int slot, size_t count) {
   while (S->tev[slot].size() < count) {
     hipEvent_t e;
-    TC_HIP(hipEventCreate(&e));
+    // timing only: no system-scope fence (no cache write-back per record)
+    TC_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     S->tev[slot].push_back(e);
   }
   return TC_OK;
 }
 
-int ionic_launch(tc_sim* S, const tc_ionic_io* io, hipStream_t s) {
+int ionic_launch(tc_sim* S, const tc_ionic_io* io, hipStream_t s, hipEvent_t e0,
+                 hipEvent_t e1) {
   const tc_sim_desc& d = S->d;
   switch (d.model) {
     case TC_MODEL_MS2V:
-      return tc_ms2v_step(&S->params.ms2v, d.states, io, d.dtype, s);
+      return ms2v_step(&S->params.ms2v, d.states, io, d.dtype, s, e0, e1);
     case TC_MODEL_TTP:
-      return tc_ttp_step(&S->params.ttp, d.states, d.tab[0], d.tab[1], d.tab[2], io, d.dtype, s);
+      return ttp_step(&S->params.ttp, d.states, d.tab[0], d.tab[1], d.tab[2], io, d.dtype, s, e0,
+                      e1);
     case TC_MODEL_CRN:
-      return tc_crn_step(&S->params.crn, d.states, d.tab[0], d.tab[1], d.tab[2], io, d.dtype, s);
+      return crn_step(&S->params.crn, d.states, d.tab[0], d.tab[1], d.tab[2], io, d.dtype, s, e0,
+                      e1);
   }
   set_error("unknown model %d", d.model);
   return TC_ERR_ARG;
+}
+
+template <class T, class CI>
+void launch_spmv(tc_pcg* P, const CI* sc, T* pin, T* pout, int k, hipStream_t s, hipEvent_t e0,
+                 hipEvent_t e1) {
+  launch(k_pcg_spmv<T, CI>, dim3(P->G), dim3(kPcgBlock), s, e0, e1, P->n, P->A.slice_ptr, sc,
+         (const T*)P->Av, (const T*)P->Dinv, (const T*)P->r, pin, pout, (T*)P->Ap, k, P->ctl,
+         (const double*)P->part[0], (const double*)P->part[2], P->part[1]);
 }
 
 template <class T, class XT>
@@ -125,27 +137,23 @@ int launch_iters(tc_sim* S, int k0, int k1, XT* xs, hipStream_t s, std::vector<h
     T* pin = pp[k & 1];
     T* pout = pp[(k + 1) & 1];
     const size_t e = ev_base + 4 * (size_t)(k - k0);
-    if (ev) TC_HIP(hipEventRecord((*ev)[e], s));
-    hipLaunchKernelGGL(k_pcg_spmv<T>, g, blk, 0, s, P->n, P->A.slice_ptr, P->A.col,
-                       (const T*)P->Av, (const T*)P->Dinv, (const T*)P->r, pin, pout, (T*)P->Ap,
-                       k, P->ctl, P->part[0], P->part[2], P->part[1]);
+    hipEvent_t ev0 = ev ? (*ev)[e] : nullptr, ev1 = ev ? (*ev)[e + 1] : nullptr;
+    if (P->A.dcol) launch_spmv<T>(P, P->A.dcol, pin, pout, k, s, ev0, ev1);
+    else launch_spmv<T>(P, P->A.col, pin, pout, k, s, ev0, ev1);
     TC_LAUNCH();
-    if (ev) {
-      TC_HIP(hipEventRecord((*ev)[e + 1], s));
-      TC_HIP(hipEventRecord((*ev)[e + 2], s));
-    }
-    hipLaunchKernelGGL((k_pcg_update<T, XT>), g, blk, 0, s, P->n, (const T*)P->Dinv, pout,
-                       (const T*)P->Ap, xs, (T*)P->r, k, P->ctl, P->part[1], P->part[2]);
+    launch(k_pcg_update<T, XT>, g, blk, s, ev ? (*ev)[e + 2] : nullptr,
+           ev ? (*ev)[e + 3] : nullptr, P->n, (const T*)P->Dinv, (const T*)pout,
+           (const T*)P->Ap, xs, (T*)P->r, k, P->ctl, (const double*)P->part[1], P->part[2]);
     TC_LAUNCH();
-    if (ev) TC_HIP(hipEventRecord((*ev)[e + 3], s));
   }
   return TC_OK;
 }
 
-int check_launch(tc_sim* S, int k, int slot, long long seq, hipStream_t s) {
+int check_launch(tc_sim* S, int k, int slot, long long seq, hipStream_t s,
+                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   tc_pcg* P = S->d.pcg;
-  hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kPcgBlock), 0, s, k, P->G, P->ctl, P->part[2],
-                     S->dev_view + slot, 1, S->d.max_iter, seq);
+  launch(k_pcg_check, dim3(1), dim3(kPcgBlock), s, e0, e1, k, P->G, P->ctl,
+         (const double*)P->part[2], S->dev_view + slot, 1, S->d.max_iter, seq);
   TC_LAUNCH();
   return TC_OK;
 }
@@ -182,6 +190,18 @@ int desired_K(tc_sim* S) {
   for (int v : S->recent) K = v > K ? v : K;
   if (K > S->d.max_iter) K = S->d.max_iter;
   return K;
+}
+
+template <class T, class XT, class WT, bool LG, class CI>
+void rhs_init_launch(tc_sim* S, const CI* sc, int xc, int pc, double coef, double tdt, dim3 g,
+                     dim3 blk, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  const tc_sim_desc& d = S->d;
+  tc_pcg* P = d.pcg;
+  launch(k_rhs_init<T, XT, WT, LG, CI>, g, blk, s, e0, e1, d.n, (const int32_t*)P->A.slice_ptr,
+         sc, (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const WT*)d.w,
+         (const XT*)d.x[xc], LG ? (const T*)d.xp[pc] : (const T*)nullptr,
+         LG ? (XT*)d.x[1 - xc] : (XT*)nullptr, LG ? (T*)d.xp[1 - pc] : (T*)nullptr, (T*)P->r,
+         coef, d.Cm, tdt, P->ctl, d.a_tol, d.r_tol, P->part[0]);
 }
 
 // Enqueue the step after planned state ps; ps becomes the state after it.
@@ -222,41 +242,40 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
     io.lat_t = ps.lat_t;
   }
   io.gate = &P->ctl->stall;
-  if (ev) TC_HIP(hipEventRecord((*ev)[0], s));
-  int rc = ionic_launch(S, &io, s);
+  int rc = ionic_launch(S, &io, s, ev ? (*ev)[0] : nullptr, ev ? (*ev)[1] : nullptr);
   if (rc) return rc;
-  if (ev) TC_HIP(hipEventRecord((*ev)[1], s));
   // 2. RHS + PCG start
   const dim3 g(P->G), blk(kPcgBlock);
   const double coef = (1.0 - d.theta) * d.dt;
   const double tdt = d.theta * d.dt;
   XT* xs;
-  if (ev) TC_HIP(hipEventRecord((*ev)[2], s));
+  hipEvent_t r0 = ev ? (*ev)[2] : nullptr, r1 = ev ? (*ev)[3] : nullptr;
   if (d.linear_guess) {
-    hipLaunchKernelGGL((k_rhs_init<T, XT, WT, true>), g, blk, 0, s, d.n, P->A.slice_ptr,
-                       P->A.col, (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const WT*)d.w,
-                       (const XT*)d.x[xc], (const T*)d.xp[pc], (XT*)d.x[1 - xc],
-                       (T*)d.xp[1 - pc], (T*)P->r, coef, d.Cm, tdt, P->ctl, d.a_tol, d.r_tol,
-                       P->part[0]);
+    if (P->A.dcol)
+      rhs_init_launch<T, XT, WT, true>(S, P->A.dcol, xc, pc, coef, tdt, g, blk, s, r0,
+                                           r1);
+    else
+      rhs_init_launch<T, XT, WT, true>(S, P->A.col, xc, pc, coef, tdt, g, blk, s, r0,
+                                           r1);
     xs = (XT*)d.x[1 - xc];
   } else {
-    hipLaunchKernelGGL((k_rhs_init<T, XT, WT, false>), g, blk, 0, s, d.n, P->A.slice_ptr,
-                       P->A.col, (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const WT*)d.w,
-                       (const XT*)d.x[xc], (const T*)nullptr, (XT*)nullptr, (T*)nullptr,
-                       (T*)P->r, coef, d.Cm, tdt, P->ctl, d.a_tol, d.r_tol, P->part[0]);
+    if (P->A.dcol)
+      rhs_init_launch<T, XT, WT, false>(S, P->A.dcol, xc, pc, coef, tdt, g, blk, s, r0,
+                                           r1);
+    else
+      rhs_init_launch<T, XT, WT, false>(S, P->A.col, xc, pc, coef, tdt, g, blk, s, r0,
+                                           r1);
     xs = (XT*)d.x[xc];
   }
   TC_LAUNCH();
-  if (ev) TC_HIP(hipEventRecord((*ev)[3], s));
   // 3. speculative iterations
   rc = launch_iters<T, XT>(S, 0, K, xs, s, ev, 6);
   if (rc) return rc;
   // 4. check
-  if (ev) TC_HIP(hipEventRecord((*ev)[4], s));
   plan.seq = ++S->seq;
-  rc = check_launch(S, K - 1, slot, plan.seq, s);
+  rc = check_launch(S, K - 1, slot, plan.seq, s, ev ? (*ev)[4] : nullptr,
+                    ev ? (*ev)[5] : nullptr);
   if (rc) return rc;
-  if (ev) TC_HIP(hipEventRecord((*ev)[5], s));
   // state after the step
   HostState nx = ps;
   nx.steps = step;

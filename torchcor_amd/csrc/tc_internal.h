@@ -1,6 +1,7 @@
 // Shared helpers for the torchcor_amd HIP library (gfx950, wave64).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 #include <cstdio>
 #include <cstdarg>
@@ -79,45 +80,24 @@ __device__ inline void block_sum(double (&v)[NV]) {
   }
 }
 
-// Deterministic grid reduction: every block stores its NV partial sums in
-// partials[k*nblocks + blockIdx.x]; the last block to arrive (atomic ticket)
-// adds them in block order and returns true in thread 0 with the totals in v.
-// The ticket is reset by that block, so the counter can be reused by the
-// next launch on the same stream.
-template <int NV>
-__device__ inline bool grid_sum_last(double (&v)[NV], double* partials,
-                                     unsigned* ticket) {
-  block_sum<NV>(v);
-  __shared__ bool last;
-  const unsigned nb = gridDim.x;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) partials[k * nb + blockIdx.x] = v[k];
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
-    last = (t == nb - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  // Sum partials in fixed block order: each thread takes a strided subset,
-  // then a fixed-shape block reduction; the order depends only on nb.
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (unsigned b = threadIdx.x; b < nb; b += blockDim.x)
-      s += __hip_atomic_load(&partials[k * nb + b], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    acc[k] = s;
-  }
-  block_sum<NV>(acc);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = acc[k];
-    *ticket = 0u;
-  }
-  return threadIdx.x == 0;
+// Kernel launch.  With a start/stop event pair the events take the
+// kernel's own begin and end timestamps (hipExtLaunchKernel), so per-kernel
+// timing adds no packet to the stream; null events launch plainly.
+template <class F, class... Args>
+inline void launch(F kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                   Args... args) {
+  hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, args...);
 }
+
+// Fused ionic steps (the tc_*_step entry points), the kernel optionally
+// bracketed by timing events.
+int ms2v_step(const tc_ms2v_params* p, void* states, const tc_ionic_io* io, int dtype,
+              hipStream_t s, hipEvent_t e0, hipEvent_t e1);
+int ttp_step(const tc_ttp_params* p, void* states, const void* V_tab, const void* CaSS_tab,
+             const void* VEk_tab, const tc_ionic_io* io, int dtype, hipStream_t s, hipEvent_t e0,
+             hipEvent_t e1);
+int crn_step(const tc_crn_params* p, void* states, const void* Cai_tab, const void* V_tab,
+             const void* fn_tab, const tc_ionic_io* io, int dtype, hipStream_t s, hipEvent_t e0,
+             hipEvent_t e1);
 
 }  // namespace tc
