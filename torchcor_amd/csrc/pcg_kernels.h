@@ -133,6 +133,25 @@ k_rhs(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ s
   if (i < n) bout[i] = (T)(am - coef * ak);
 }
 
+// ------------------------------------------------- row chunks (PCG passes)
+// The hot passes walk a row in chunks of kChunk entries: all column and
+// value loads of a chunk are issued before its gathers, and all gathers
+// before the (in-order) float64 accumulation, so that each wave keeps many
+// loads in flight.  Entries past the row's padded width repeat the last
+// entry (a valid address) and are masked out of the sums.
+constexpr int kChunk = 4;
+
+template <class CI, class T>
+__device__ inline void load_chunk(const CI* c, const T* v, int j0, int wd, int64_t i,
+                                  int32_t (&cj)[kChunk], T (&vj)[kChunk]) {
+#pragma unroll
+  for (int u = 0; u < kChunk; ++u) {
+    const int jj = j0 + u < wd ? j0 + u : wd - 1;
+    cj[u] = col_of(c[jj * 64], i);
+    vj[u] = v[jj * 64];
+  }
+}
+
 // ------------------------------------------------------- PCG work split
 struct Split {
   int64_t s0, s1;  // this block's slice range
@@ -220,12 +239,20 @@ k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
     const CI* c = scol + b + lane;
     const T* a = Av + b + lane;
     double ax = 0.0;
-    for (int j = 0; j < wd; ++j) {
-      const int32_t cj = col_of(c[j * 64], i);
-      T xj;
-      if (LG) xj = (T)(2.0 * (double)x[cj] - (double)xp[cj]);
-      else xj = x[cj];
-      ax += (double)a[j * 64] * (double)xj;
+    for (int j0 = 0; j0 < wd; j0 += kChunk) {
+      int32_t cj[kChunk];
+      T av[kChunk], xj[kChunk];
+      load_chunk(c, a, j0, wd, i, cj, av);
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        if (LG) xj[u] = (T)(2.0 * (double)x[cj[u]] - (double)xp[cj[u]]);
+        else xj[u] = x[cj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const double t = ax + (double)av[u] * (double)xj[u];
+        ax = j0 + u < wd ? t : ax;
+      }
     }
     if (i < n) {
       if (LG) xnext[i] = (T)(2.0 * (double)x[i] - (double)xp[i]);
@@ -294,17 +321,22 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
     const CI* c = scol + b + lane;
     const T* a = Av + b + lane;
     double acc = 0.0;
-    if (k == 0) {
-      for (int j = 0; j < wd; ++j) {
-        const int32_t cj = col_of(c[j * 64], i);
-        const T pj = (T)((double)Dinv[cj] * (double)r[cj]);
-        acc += (double)a[j * 64] * (double)pj;
+    for (int j0 = 0; j0 < wd; j0 += kChunk) {
+      int32_t cj[kChunk];
+      T av[kChunk], pj[kChunk];
+      load_chunk(c, a, j0, wd, i, cj, av);
+      if (k == 0) {
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) pj[u] = (T)((double)Dinv[cj[u]] * (double)r[cj[u]]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+          pj[u] = (T)(beta * (double)pin[cj[u]] + (double)Dinv[cj[u]] * (double)r[cj[u]]);
       }
-    } else {
-      for (int j = 0; j < wd; ++j) {
-        const int32_t cj = col_of(c[j * 64], i);
-        const T pj = (T)(beta * (double)pin[cj] + (double)Dinv[cj] * (double)r[cj]);
-        acc += (double)a[j * 64] * (double)pj;
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const double t = acc + (double)av[u] * (double)pj[u];
+        acc = j0 + u < wd ? t : acc;
       }
     }
     if (i < n) {
@@ -432,18 +464,34 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
     const T* m = Mv + b + lane;
     const T* kk = Kv + b + lane;
     double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
-    for (int j = 0; j < wd; ++j) {
-      const int32_t cj = col_of(c[j * 64], i);
-      const XT xj = x[cj];
-      double x0;
-      if (LG) x0 = (double)(XT)(2.0 * (double)xj - (double)xp[cj]);
-      else x0 = (double)xj;
-      const double mj = (double)m[j * 64], kj = (double)kk[j * 64];
-      am += mj * (double)w[cj];
-      if (sizeof(WT) == 8) ak += kj * (double)xj;
-      else ak += kj * (double)(T)xj;
-      mx += mj * x0;
-      kx += kj * x0;
+    for (int j0 = 0; j0 < wd; j0 += kChunk) {
+      int32_t cj[kChunk];
+      T mv[kChunk], kv[kChunk];
+      XT xj[kChunk];
+      WT wj[kChunk];
+      T pj[kChunk];
+      load_chunk(c, m, j0, wd, i, cj, mv);
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) kv[u] = kk[(j0 + u < wd ? j0 + u : wd - 1) * 64];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        xj[u] = x[cj[u]];
+        wj[u] = w[cj[u]];
+        if (LG) pj[u] = xp[cj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        if (j0 + u >= wd) break;
+        double x0;
+        if (LG) x0 = (double)(XT)(2.0 * (double)xj[u] - (double)pj[u]);
+        else x0 = (double)xj[u];
+        const double mj = (double)mv[u], kj = (double)kv[u];
+        am += mj * (double)wj[u];
+        if (sizeof(WT) == 8) ak += kj * (double)xj[u];
+        else ak += kj * (double)(T)xj[u];
+        mx += mj * x0;
+        kx += kj * x0;
+      }
     }
     if (i < n) {
       if (LG) {
