@@ -211,6 +211,12 @@ typedef struct tc_ionic_io {
   /* w is stored in float64 and formed in float64 from the model's dV
    * (float32 runs in precise-RHS mode). */
   int w_f64;
+  /* Batch layout (lanes > 0): the n entries are node-instances of one model
+   * group of a batch, i = node * lanes + lane; V, w, dV, stim_mask, AT and
+   * RT are rows of lane_stride entries per node and entry i lives at
+   * node * lane_stride + lane0 + lane.  States stay (S, n).  lanes = 0:
+   * one entry per node. */
+  int32_t lanes, lane_stride, lane0;
 } tc_ionic_io;
 
 /* ModifiedMS2v (ionic/ModifiedMS2v.py:4-44).  States: H. */
@@ -320,6 +326,75 @@ int tc_sim_status(tc_sim* sim, double* t_host, int64_t* steps_host, int* x_index
  * counting only PCG iterations that ran. */
 int tc_sim_set_timing(tc_sim* sim, int period);
 int tc_sim_get_timing(tc_sim* sim, double* ms_host, int64_t* count_host);
+
+/* ------------------------------------------------------------------------
+ * Batched simulations on one shared mesh (no counterpart in the reference:
+ * many Monodomain runs, monodomain.py:141-226, advanced together).
+ *
+ * B <= 64 instances share the mesh and differ in conductivity scale, fibre
+ * angle, stimulus sites and ionic model.  Their stiffness matrices are
+ * combinations of NK shared component matrices (same CSR pattern as M):
+ *   K_b = sum_k coef[k * lanes + b] K_k          (K_k already divided by Chi)
+ *   A_b = Cm M + theta dt K_b, formed in float64 from the stored values.
+ * Per-instance vectors are instance-minor: entry (node i, instance b) at
+ * i * lanes + b, lanes = the smallest power of two >= B.  One wavefront
+ * covers 64/lanes rows and every instance of them, so the pattern and the
+ * shared values are read once for all instances (multi-right-hand-side
+ * Jacobi-PCG, per-instance scalars and stopping tests: an instance that has
+ * met the reference rule stops while the others iterate).
+ * Model groups: instances b0 .. b0+count-1 run one ionic model whose states
+ * are (S, n*count) with entry node * count + (b - b0).
+ * ---------------------------------------------------------------------- */
+#define TC_BATCH_MAX 64
+#define TC_BATCH_MAX_K 5
+#define TC_BATCH_MAX_GROUPS 8
+typedef struct tc_batch_group {
+  int model;                 /* tc_model */
+  const void* params;        /* tc_*_params */
+  void* states;              /* (S, n * count) */
+  const void* tab[3];        /* model tables */
+  int b0, count;
+} tc_batch_group;
+typedef struct tc_batch_desc {
+  int64_t n, nnz;
+  int B, lanes, NK;
+  int dtype;
+  int x_f64, w_f64;          /* as in tc_sim_desc */
+  const int32_t* row_ptr;    /* (n+1) CSR pattern */
+  const int32_t* col;        /* (nnz) */
+  const void* Mv;            /* (nnz) M values, dtype */
+  const void* Kv[TC_BATCH_MAX_K]; /* (nnz) component values, dtype */
+  const double* coef;        /* (NK, lanes) device */
+  double dt, dt_ionic, Cm, theta;
+  void* x[2];                /* (n, lanes) iterate buffers, current first */
+  void* xp[2];               /* (n, lanes) x_prev buffers in dtype */
+  void* w;                   /* (n, lanes) scratch (float64 when w_f64) */
+  void* Dinv;                /* (n, lanes) dtype, filled by tc_batch_create */
+  void* AT;                  /* (n, lanes) or NULL */
+  void* RT;
+  const uint32_t* stim_mask; /* (n, lanes) bit k: stimulus k applies there */
+  int n_stim;
+  tc_stimulus stim[TC_MAX_STIMULI];
+  int n_groups;
+  tc_batch_group groups[TC_BATCH_MAX_GROUPS];
+  double a_tol, r_tol;
+  int max_iter, linear_guess;
+  int depth;                 /* steps enqueued ahead of the status read */
+  void* ws;                  /* workspace of tc_batch_workspace_size bytes */
+  size_t ws_bytes;
+} tc_batch_desc;
+typedef struct tc_batch tc_batch;
+int tc_batch_workspace_size(int64_t n, int lanes, int dtype, size_t* bytes_host);
+/* Validates the description, fills Dinv, clears the control block. */
+int tc_batch_create(tc_batch** batch, const tc_batch_desc* desc, void* stream);
+int tc_batch_destroy(tc_batch* batch);
+/* nsteps time steps; iters[k] (host) = the largest n_iter over the
+ * instances in step k.  Synchronizes at the end. */
+int tc_batch_advance(tc_batch* batch, int64_t nsteps, int32_t* iters_host, void* stream);
+/* LAT/RT test of the last solution; per-instance sums of n_iter (B). */
+int tc_batch_finish(tc_batch* batch, int64_t* total_iters_host, void* stream);
+/* t, steps done, index 0..1 of desc.x holding the solution. */
+int tc_batch_status(tc_batch* batch, double* t_host, int64_t* steps_host, int* x_index_host);
 
 /* LAT/RT update alone (monodomain.py:174-176), for the last step of a run. */
 int tc_lat_update(const void* u, void* AT, void* RT, int64_t n, double t, int dtype,

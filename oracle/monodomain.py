@@ -25,6 +25,9 @@ class Monodomain:
     Chi = 140
     Cm = 0.01
     theta = 0.5
+    #: the reference rounds each element contribution to float32
+    #: (assemble.py:233-243); False assembles in plain float64
+    round_f32 = True
 
     def __init__(self, ionic_model, T, dt):
         self.ionic = ionic_model
@@ -67,8 +70,8 @@ class Monodomain:
             Me, Ke = fem.tri_local(coords, sig)
         else:
             Me, Ke = fem.tet_local(coords, sig)
-        rp, col, kv = fem.assemble_csr(self.elems, Ke, self.n)
-        rp2, col2, mv = fem.assemble_csr(self.elems, Me, self.n)
+        rp, col, kv = fem.assemble_csr(self.elems, Ke, self.n, round_f32=self.round_f32)
+        rp2, col2, mv = fem.assemble_csr(self.elems, Me, self.n, round_f32=self.round_f32)
         assert np.array_equal(rp, rp2) and np.array_equal(col, col2)
         self.row_ptr, self.col = rp, col
         self.K = kv / self.Chi

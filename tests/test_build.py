@@ -98,3 +98,51 @@ def test_stimulus_schedule_and_vtx(tmp_path):
         on.append(s.active(t))
     assert on[0] == 1 and on[98] == 1 and on[99] == 0 and on[500] == 2 and on[599] == 2
     assert s.apply(0.5).tolist() == [0, 100, 0, 100, 0, 100, 0, 0, 0, 0]
+
+
+def test_ctypes_structs_match_c_layout(tmp_path):
+    """Every ctypes mirror of a header struct has the C size and field offsets."""
+    import ctypes as C
+    import subprocess
+    import torchcor_amd._lib as L
+    pairs = {"tc_sell": L.Sell, "tc_ionic_io": L.IonicIO, "tc_ms2v_params": L.MS2vParams,
+             "tc_ttp_params": L.TTPParams, "tc_crn_params": L.CRNParams,
+             "tc_stimulus": L.Stimulus, "tc_sim_desc": L.SimDesc,
+             "tc_batch_group": L.BatchGroup, "tc_batch_desc": L.BatchDesc}
+    lines = ['#include <stdio.h>', '#include <stddef.h>',
+             f'#include "{ROOT / "include" / "torchcor_amd.h"}"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", str(src), "-o", str(exe)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                         text=True).stdout.splitlines())
+    for cname, py in pairs.items():
+        assert int(out[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(out[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
+
+
+def test_sigma_split_reassembles_rotated_tensor():
+    """The scalar/vector/tensor split of the conductivity tensor reproduces
+    sigma_t I + (sigma_l - sigma_t) f' f'^T for fibres rotated about z."""
+    import math
+    import torch
+    from torchcor_amd.batch import rotate_z, sigma_components
+    rng = np.random.default_rng(5)
+    E = 50
+    f = rng.normal(size=(E, 3))
+    f /= np.linalg.norm(f, axis=1, keepdims=True)
+    sl, st = rng.uniform(0.2, 1.0, E), rng.uniform(0.05, 0.2, E)
+    S = [c.numpy() for c in sigma_components(torch.tensor(sl), torch.tensor(st), torch.tensor(f))]
+    for deg in (0.0, 17.0, -30.0, 90.0):
+        fr = rotate_z(f, deg)
+        ref = st[:, None, None] * np.eye(3) + (sl - st)[:, None, None] * np.einsum("ei,ej->eij", fr, fr)
+        d = math.radians(deg)
+        got = S[0] + math.cos(d) * S[1] + math.sin(d) * S[2] + math.cos(2 * d) * S[3] + math.sin(2 * d) * S[4]
+        assert np.abs(got - ref).max() < 1e-14

@@ -35,7 +35,8 @@ class IonicIO(C.Structure):
     _fields_ = [("n", i64), ("dt", dbl), ("dt_rhs", dbl), ("Cm", dbl), ("V", p), ("w", p),
                 ("dV", p), ("stim_mask", p), ("stim_active", C.c_uint32),
                 ("stim_amp", dbl * MAX_STIMULI), ("AT", p), ("RT", p), ("lat_t", dbl),
-                ("gate", p), ("v_f64", i32), ("w_f64", i32)]
+                ("gate", p), ("v_f64", i32), ("w_f64", i32), ("lanes", i32),
+                ("lane_stride", i32), ("lane0", i32)]
 
 
 MS2V_FIELDS = ["tau_in", "tau_out", "tau_open", "tau_close", "u_gate", "u_crit", "vmin", "vmax"]
@@ -79,6 +80,27 @@ class SimDesc(C.Structure):
                 ("stim_mask", p), ("n_stim", i32), ("stim", Stimulus * MAX_STIMULI),
                 ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32), ("linear_guess", i32),
                 ("depth", i32)]
+
+
+BATCH_MAX_K = 5
+BATCH_MAX_GROUPS = 8
+
+
+class BatchGroup(C.Structure):
+    _fields_ = [("model", i32), ("params", p), ("states", p), ("tab", p * 3), ("b0", i32),
+                ("count", i32)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("n", i64), ("nnz", i64), ("B", i32), ("lanes", i32), ("NK", i32),
+                ("dtype", i32), ("x_f64", i32), ("w_f64", i32), ("row_ptr", p), ("col", p),
+                ("Mv", p), ("Kv", p * BATCH_MAX_K), ("coef", p), ("dt", dbl), ("dt_ionic", dbl),
+                ("Cm", dbl), ("theta", dbl), ("x", p * 2), ("xp", p * 2), ("w", p), ("Dinv", p),
+                ("AT", p), ("RT", p), ("stim_mask", p), ("n_stim", i32),
+                ("stim", Stimulus * MAX_STIMULI), ("n_groups", i32),
+                ("groups", BatchGroup * BATCH_MAX_GROUPS), ("a_tol", dbl), ("r_tol", dbl),
+                ("max_iter", i32), ("linear_guess", i32), ("depth", i32), ("ws", p),
+                ("ws_bytes", C.c_size_t)]
 
 
 MODEL_MS2V, MODEL_TTP, MODEL_CRN = 0, 1, 2
@@ -133,6 +155,12 @@ FUNCS = {
     "tc_sim_status": (p, C.POINTER(dbl), I64_P, INT_P, I64_P),
     "tc_sim_set_timing": (p, i32),
     "tc_sim_get_timing": (p, C.POINTER(dbl), I64_P),
+    "tc_batch_workspace_size": (i64, i32, i32, SIZE_P),
+    "tc_batch_create": (C.POINTER(p), C.POINTER(BatchDesc), p),
+    "tc_batch_destroy": (p,),
+    "tc_batch_advance": (p, i64, C.POINTER(C.c_int32), p),
+    "tc_batch_finish": (p, I64_P, p),
+    "tc_batch_status": (p, C.POINTER(dbl), I64_P, INT_P),
 }
 for _n, _a in FUNCS.items():
     _proto(_n, *_a)

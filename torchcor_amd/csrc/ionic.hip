@@ -115,7 +115,22 @@ __device__ inline double t_log(double x) { return log(x); }
 __device__ inline float t_expm1(float x) { return expm1f(x); }
 __device__ inline double t_expm1(double x) { return expm1(x); }
 
-// Stimulus, RHS input and LAT/RT epilogue shared by all models.
+// Entry of node-instance i in the per-node arrays (V, w, dV, stim_mask,
+// AT, RT): i itself, or in a batch group (lanes > 0) lane lane0 + i % lanes
+// of node i / lanes, whose row holds lane_stride entries.
+__device__ inline int64_t io_index(const tc_ionic_io& io, int64_t i) {
+  if (io.lanes <= 0) return i;
+  const int64_t node = i / io.lanes;
+  return node * io.lane_stride + io.lane0 + (i - node * io.lanes);
+}
+
+template <class T>
+__device__ inline T load_v(const tc_ionic_io& io, int64_t vi) {
+  return io.v_f64 ? (T)((const double*)io.V)[vi] : ((const T*)io.V)[vi];
+}
+
+// Stimulus, RHS input and LAT/RT epilogue shared by all models; i is the
+// entry in the per-node arrays.
 template <class T>
 __device__ inline void finish_node(const tc_ionic_io& io, int64_t i, T V, T dV) {
   if (io.dV) ((T*)io.dV)[i] = dV;
@@ -164,7 +179,8 @@ k_ms2v(tc_ms2v_params p, T* __restrict__ H, tc_ionic_io io) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= io.n) return;
   if (io.gate && *(volatile const int*)io.gate) return;
-  const T U = io.v_f64 ? (T)((const double*)io.V)[i] : ((const T*)io.V)[i];
+  const int64_t vi = io_index(io, i);
+  const T U = load_v<T>(io, vi);
   const T h = H[i];
   const T DV = (T)(p.vmax - p.vmin);
   const T Uad = (U - (T)p.vmin) / DV;
@@ -173,7 +189,7 @@ k_ms2v(tc_ms2v_params p, T* __restrict__ H, tc_ionic_io io) {
   const T dU = -(DV * (J_in + J_out));
   const T dH = Uad > (T)p.u_gate ? -h / (T)p.tau_close : (T(1) - h) / (T)p.tau_open;
   H[i] = h + (T)io.dt * dH;
-  finish_node<T>(io, i, U, dU);
+  finish_node<T>(io, vi, U, dU);
 }
 
 // ---------------------------------------------------------------- TTP
@@ -321,7 +337,8 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
   const int64_t n = io.n;
   if (i >= n) return;
   if (io.gate && *(volatile const int*)io.gate) return;
-  const T V = io.v_f64 ? (T)((const double*)io.V)[i] : ((const T*)io.V)[i];
+  const int64_t vi = io_index(io, i);
+  const T V = load_v<T>(io, vi);
   T s[TS_N];
 #pragma unroll
   for (int k = 0; k < TS_N; ++k) s[k] = st[k * n + i];
@@ -408,7 +425,7 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
   o[TS_Xs] = vr(TTP_Xs_A, TTP_VCOLS) + vr(TTP_Xs_B, TTP_VCOLS) * s[TS_Xs];
 #pragma unroll
   for (int k = 0; k < TS_N; ++k) st[k * n + i] = o[k];
-  finish_node<T>(io, i, V, -Iion);
+  finish_node<T>(io, vi, V, -Iion);
 }
 
 // ---------------------------------------------------------------- CRN
@@ -559,7 +576,8 @@ k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
   const int64_t n = io.n;
   if (i >= n) return;
   if (io.gate && *(volatile const int*)io.gate) return;
-  const T V = io.v_f64 ? (T)((const double*)io.V)[i] : ((const T*)io.V)[i];
+  const int64_t vi = io_index(io, i);
+  const T V = load_v<T>(io, vi);
   T s[CS_N];
 #pragma unroll
   for (int k = 0; k < CS_N; ++k) s[k] = st[k * n + i];
@@ -618,7 +636,7 @@ k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
 #undef RLU
 #pragma unroll
   for (int k = 0; k < CS_N; ++k) st[k * n + i] = o[k];
-  finish_node<T>(io, i, V, -Iion);
+  finish_node<T>(io, vi, V, -Iion);
 }
 
 template <class T>

@@ -80,17 +80,7 @@ struct tc_sim {
 namespace tc {
 namespace {
 
-uint32_t active_bits(const tc_sim_desc& d, double t) {
-  uint32_t bits = 0;
-  for (int k = 0; k < d.n_stim; ++k) {
-    const tc_stimulus& s = d.stim[k];
-    const double p_t = fmod(t, s.period);
-    const double t_t = t / s.period;
-    // Python's % on positive doubles equals fmod.
-    if (t_t <= s.count && p_t >= s.start && p_t <= s.start + s.duration) bits |= 1u << k;
-  }
-  return bits;
-}
+uint32_t active_bits(const tc_sim_desc& d, double t) { return stimulus_bits(d.stim, d.n_stim, t); }
 
 int ev_reserve(tc_sim* S, int slot, size_t count) {
   while (S->tev[slot].size() < count) {

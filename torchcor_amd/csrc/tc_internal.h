@@ -80,6 +80,20 @@ __device__ inline void block_sum(double (&v)[NV]) {
   }
 }
 
+// Bit k set when stimulus k is on at time t (stimulation.py:51-63): on iff
+// t/period <= count and start <= t % period <= start + duration, with
+// Python's % on positive doubles equal to fmod.
+inline uint32_t stimulus_bits(const tc_stimulus* st, int n_stim, double t) {
+  uint32_t bits = 0;
+  for (int k = 0; k < n_stim; ++k) {
+    const tc_stimulus& s = st[k];
+    const double p_t = fmod(t, s.period);
+    const double t_t = t / s.period;
+    if (t_t <= s.count && p_t >= s.start && p_t <= s.start + s.duration) bits |= 1u << k;
+  }
+  return bits;
+}
+
 // Kernel launch.  With a start/stop event pair the events take the
 // kernel's own begin and end timestamps (hipExtLaunchKernel), so per-kernel
 // timing adds no packet to the stream; null events launch plainly.
