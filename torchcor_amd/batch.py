@@ -219,7 +219,7 @@ class SharedMeshBatch:
             gdesc.append((model, b0, len(members)))
             b0 += len(members)
         self._X = [X0, mk(x_dtype)]
-        self._XP = [X0.to(dt_), mk(dt_)]
+        self._XP = [X0.to(dt_, copy=True), mk(dt_)]
         self._w = mk(torch.float64 if w_f64 else dt_)
         self.Dinv = mk(dt_)
         self.activation_time = self.repolarization_time = None
