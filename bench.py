@@ -97,6 +97,76 @@ def cpu_baseline(N, mesh, steps):
                       f"{el:.1f} s"}
 
 
+def batched_config(dev, steps, warmup):
+    """64 simulations sharing the 41^3 cube (68,921 nodes): 32 ModifiedMS2v
+    and 32 TTP ENDO, conductivity scales U[0.5, 1.5], fibre rotations
+    U[-30, 30] degrees, one seeded 200-node boundary patch stimulus each
+    (100 uA/cm^2, 0..1 ms), float32, T = 100 ms.  The batch and back-to-back
+    single runs of the same models are timed over the same step window;
+    sims/hour extrapolates the per-step time to the 10,000 steps of a run."""
+    import numpy as np
+    import torch
+    from torchcor_amd import meshes
+    from torchcor_amd.batch import SharedMeshBatch
+    from torchcor_amd.ionic import ModifiedMS2v, TenTusscherPanfilov
+    from torchcor_amd.simulator import Monodomain
+    N, B, T, dt = 41, 64, 100.0, 0.01
+    nt = int(T / dt)
+    f32 = torch.float32
+    nodes, elems, regions, fib = meshes.cube_workload(N)
+    rng = np.random.default_rng(2024)
+    surf = meshes.cube_surface_nodes(N)
+    cond = [([0], 0.5272, 0.2076), ([1], 0.9074, 0.3332)]
+    bat = SharedMeshBatch(T=T, dt=dt, device=dev, dtype=f32)
+    bat.load_arrays(nodes, elems, regions, fib)
+    for c in cond:
+        bat.add_conductivity(*c)
+    ms = ModifiedMS2v(dt, device=dev, dtype=f32)
+    ttp = TenTusscherPanfilov("ENDO", dt, device=dev, dtype=f32)
+    patches = []
+    for k in range(B):
+        patch = meshes.surface_patch(nodes, surf, 200, rng)
+        patches.append(patch)
+        bat.add_instance(ms if k < B // 2 else ttp, rng.uniform(0.5, 1.5), rng.uniform(-30, 30),
+                         [(patch, 0.0, 1.0, 100.0)])
+    print(f"[bench] batch: {B} x {N}^3, warmup {warmup}", flush=True)
+    bat.start(1e-5, 1e-5, 100)
+    bat.advance(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bat.advance(steps)
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    plain = {}
+    for name, make in (("ms2v", lambda: ModifiedMS2v(dt, device=dev, dtype=f32)),
+                       ("ttp", lambda: TenTusscherPanfilov("ENDO", dt, device=dev, dtype=f32))):
+        sim = Monodomain(make(), T=T, dt=dt, device=dev, dtype=f32)
+        sim.load_arrays(nodes, elems, regions, fib)
+        for c in cond:
+            sim.add_conductivity(*c)
+        sim.add_stimulus(patches[0], 0.0, 1.0, 100.0)
+        sim.start(1e-5, 1e-5, 100, snapshot_interval=1e9)
+        sim.advance(warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.advance(steps)
+        torch.cuda.synchronize()
+        plain[name] = (time.perf_counter() - t0) / steps
+    t_plain = 0.5 * (plain["ms2v"] + plain["ttp"])
+    sph = B * 3600.0 / (t_batch * nt)
+    sph_plain = 3600.0 / (t_plain * nt)
+    return {"workload": f"{B} sims on cube{N} ({N ** 3} nodes): 32 MS2v + 32 TTP ENDO, f32, "
+                        f"sigma scale U[0.5,1.5], fibre rotation U[-30,30] deg, 200-node patches",
+            "steps_timed": steps, "t_window_ms": [warmup * dt, (warmup + steps) * dt],
+            "ms_per_step": round(t_batch * 1e3, 4),
+            "node_timesteps_per_s": N ** 3 * B / t_batch,
+            "sims_per_hour": round(sph, 1),
+            "plain_ms_per_step": {k: round(v * 1e3, 4) for k, v in plain.items()},
+            "plain_sims_per_hour": round(sph_plain, 1),
+            "speedup_vs_back_to_back": round(sph / sph_plain, 2),
+            "pcg_iterations_per_step_max": sum(bat.iterations[warmup:]) / steps + 1}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +176,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--batch-warmup", type=int, default=200)
     ap.add_argument("--timing-period", type=int, default=16,
                     help="time the kernels of every k-th step with HIP events")
     args = ap.parse_args()
@@ -166,6 +239,9 @@ def main():
         ele_s = sum(kernels[k]["total_ms"] for k in ("rhs_init", "pcg_spmv", "pcg_update",
                                                      "pcg_check") if k in kernels) / 1e3
         pcg_rate = round(ele_bytes / ele_s / 1e9, 1)
+    batched = None
+    if not args.no_batched:
+        batched = batched_config(dev, args.batch_steps, args.batch_warmup)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(N, mesh, args.cpu_steps)
@@ -183,6 +259,7 @@ def main():
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "batched": batched,
         "setup_s": round(t_setup, 2),
     }
     print(json.dumps(line), flush=True)

@@ -34,7 +34,7 @@ namespace tc {
 namespace {
 
 constexpr int kLanes = TC_BATCH_MAX;  // 64
-constexpr int kBB = 256;              // threads per block of the passes
+constexpr int kBB = 512;              // threads per block of the passes
 constexpr int kBW = kBB / 64;         // waves per block
 constexpr int kBMaxBlocks = 1024;
 constexpr int kBRing = 64;
@@ -71,6 +71,18 @@ struct BatOp {
 struct RowRange {
   int64_t g0, g1;
 };
+
+// Row entries per chunk: the column and shared-value loads of a chunk are
+// issued before its gathers (several loads in flight per wavefront).
+constexpr int kBChunk = 4;
+// Entries per gather group of the 64-lane passes (all gathers of a group in
+// flight together).
+constexpr int kBSeg = 8;
+
+// Wavefront index in the block, as a wave-uniform (scalar) value, so that
+// with 64 lanes the row index and every pattern / shared-value load of the
+// passes are scalar.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 // Row groups (64/lanes rows each, one per wavefront pass) of this block:
 // one contiguous range per block, blocks of one XCD adjacent.
@@ -123,6 +135,34 @@ __device__ inline void bat_partials(double (&v)[NV], double* part, int lanes) {
   }
 }
 
+// Value of lane `src` (wave-uniform) in every lane.
+__device__ inline int32_t bcast(int32_t v, int src) { return __builtin_amdgcn_readlane(v, src); }
+__device__ inline float bcast(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ inline double bcast(double v, int src) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// One row segment of up to 64 entries held across the wave (lane t: entry
+// j + t): column, M and the NK component values.
+template <class T, int NK>
+struct RowSeg {
+  int32_t col;
+  T m;
+  T k[NK];
+  __device__ inline void load(const BatOp<T>& op, int32_t j, int len, int lane) {
+    const int32_t jj = j + (lane < len ? lane : len - 1);
+    col = op.col[jj];
+    m = op.M[jj];
+#pragma unroll
+    for (int q = 0; q < NK; ++q) k[q] = op.K[q][jj];
+  }
+};
+
 // a_j of instance b: Cm M_j + sum_k tc[k] K_k,j (tc = tdt * coef).
 template <class T, int NK>
 __device__ inline double a_val(const BatOp<T>& op, const double (&tcf)[NK], int32_t j) {
@@ -159,14 +199,14 @@ __global__ void k_bat_dinv(BatOp<T> op, T* __restrict__ Dinv) {
 // Fused RHS + PCG start (the batched form of k_rhs_init):
 //   b = M w - ck K_b u,  x0 = 2x - x_prev (LG) or x,  r = b - A_b x0,
 // partial sums of r.z and (Dinv b)^2 per instance.
-template <class T, class XT, class WT, bool LG, int NK>
+template <class T, class XT, class WT, bool LG, int NK, bool WU>
 __global__ void __launch_bounds__(kBB)
 k_bat_rhs_init(BatOp<T> op, const WT* __restrict__ w, const XT* __restrict__ x,
                const T* __restrict__ xp, XT* __restrict__ xnext, T* __restrict__ xpnext,
                T* __restrict__ r, const T* __restrict__ Dinv, BatCtl* ctl, double* part) {
   if (bat_halted(ctl, op.B, false)) return;
   const int lanes = op.lanes, rpw = 64 >> op.lg;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = wave_id();
   const int b = lane & (lanes - 1), rsub = lane >> op.lg;
   double cf[NK];
 #pragma unroll
@@ -175,23 +215,78 @@ k_bat_rhs_init(BatOp<T> op, const WT* __restrict__ w, const XT* __restrict__ x,
   const RowRange rr = block_rows(ng);
   double red[2] = {0.0, 0.0};
   for (int64_t g = rr.g0 + wv; g < rr.g1; g += kBW) {
-    const int64_t i = g * rpw + rsub;
-    if (i >= op.n) continue;
+    const int64_t i = WU ? g : g * rpw + rsub;
+    if (!WU && i >= op.n) continue;
     const int32_t j0 = op.rp[i], j1 = op.rp[i + 1];
     double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
-    for (int32_t j = j0; j < j1; ++j) {
-      const int64_t e = (int64_t)op.col[j] * lanes + b;
-      const double mj = (double)op.M[j];
-      const double kj = k_val<T, NK>(op, cf, j);
-      const XT xj = x[e];
-      double x0;
-      if (LG) x0 = (double)(XT)(2.0 * (double)xj - (double)xp[e]);
-      else x0 = (double)xj;
-      am += mj * (double)w[e];
-      if (sizeof(WT) == 8) ak += kj * (double)xj;
-      else ak += kj * (double)(T)xj;
-      mx += mj * x0;
-      kx += kj * x0;
+    if (WU) {
+      for (int32_t js = j0; js < j1; js += 64) {
+        const int len = j1 - js < 64 ? j1 - js : 64;
+        RowSeg<T, NK> rs;
+        rs.load(op, js, len, lane);
+        for (int u0 = 0; u0 < len; u0 += kBSeg) {
+          XT xj[kBSeg];
+          WT wj[kBSeg];
+          T pj[kBSeg];
+#pragma unroll
+          for (int u = 0; u < kBSeg; ++u) {
+            const int uu = u0 + u < len ? u0 + u : len - 1;
+            const int64_t e = ((int64_t)bcast(rs.col, uu) << 6) + lane;
+            xj[u] = x[e];
+            wj[u] = w[e];
+            if (LG) pj[u] = xp[e];
+          }
+#pragma unroll
+          for (int u = 0; u < kBSeg; ++u) {
+            if (u0 + u >= len) break;
+            const double mj = (double)bcast(rs.m, u0 + u);
+            double kj = 0.0;
+#pragma unroll
+            for (int q = 0; q < NK; ++q) kj += cf[q] * (double)bcast(rs.k[q], u0 + u);
+            double x0;
+            if (LG) x0 = (double)(XT)(2.0 * (double)xj[u] - (double)pj[u]);
+            else x0 = (double)xj[u];
+            am += mj * (double)wj[u];
+            if (sizeof(WT) == 8) ak += kj * (double)xj[u];
+            else ak += kj * (double)(T)xj[u];
+            mx += mj * x0;
+            kx += kj * x0;
+          }
+        }
+      }
+    } else
+    for (int32_t jc = j0; jc < j1; jc += kBChunk) {
+      int32_t cj[kBChunk];
+      double mv[kBChunk], kv[kBChunk];
+      XT xj[kBChunk];
+      WT wj[kBChunk];
+      T pj[kBChunk];
+#pragma unroll
+      for (int u = 0; u < kBChunk; ++u) {
+        const int32_t jj = jc + u < j1 ? jc + u : j1 - 1;
+        cj[u] = op.col[jj];
+        mv[u] = (double)op.M[jj];
+        kv[u] = k_val<T, NK>(op, cf, jj);
+      }
+#pragma unroll
+      for (int u = 0; u < kBChunk; ++u) {
+        const int64_t e = (int64_t)cj[u] * lanes + b;
+        xj[u] = x[e];
+        wj[u] = w[e];
+        if (LG) pj[u] = xp[e];
+      }
+#pragma unroll
+      for (int u = 0; u < kBChunk; ++u) {
+        if (jc + u >= j1) break;
+        double x0;
+        if (LG) x0 = (double)(XT)(2.0 * (double)xj[u] - (double)pj[u]);
+        else x0 = (double)xj[u];
+        am += mv[u] * (double)wj[u];
+        if (sizeof(WT) == 8) ak += kv[u] * (double)xj[u];
+        else ak += kv[u] * (double)(T)xj[u];
+        mx += mv[u] * x0;
+        kx += kv[u] * x0;
+      }
     }
     const int64_t e = i * lanes + b;
     if (LG) {
@@ -210,45 +305,105 @@ k_bat_rhs_init(BatOp<T> op, const WT* __restrict__ w, const XT* __restrict__ x,
   bat_partials<2>(red, part, lanes);
 }
 
-// spmv pass of iteration k: p' = beta p + Dinv r (k == 0: Dinv r), Ap',
-// partial sums of p'.Ap' (active instances only).
-template <class T, int NK>
+// p' = beta p + Dinv r (k == 0: Dinv r) for the active instances,
+// elementwise, so that the spmv pass gathers one vector.
+template <class T>
+__global__ void __launch_bounds__(256)
+k_bat_pform(int64_t n, int lanes, int B, const T* __restrict__ Dinv, const T* __restrict__ r,
+            const T* __restrict__ pin, T* __restrict__ pout, int k, BatCtl* ctl) {
+  if (bat_halted(ctl, B, true)) return;
+  const int64_t m = n * lanes;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e & (lanes - 1));
+    if (b >= B || !ctl->active[b]) continue;
+    if (k == 0) pout[e] = (T)((double)Dinv[e] * (double)r[e]);
+    else pout[e] = (T)(ctl->beta[b] * (double)pin[e] + (double)Dinv[e] * (double)r[e]);
+  }
+}
+
+// spmv pass of iteration k: Ap' = A_b p' (p' from k_bat_pform), partial
+// sums of p'.Ap' (active instances only).
+template <class T, int NK, bool WU>
 __global__ void __launch_bounds__(kBB)
-k_bat_spmv(BatOp<T> op, const T* __restrict__ Dinv, const T* __restrict__ r,
-           const T* __restrict__ pin, T* __restrict__ pout, T* __restrict__ Ap, int k,
-           BatCtl* ctl, double* part) {
+k_bat_spmv(BatOp<T> op, const T* __restrict__ p, T* __restrict__ Ap, int k, BatCtl* ctl,
+           double* part) {
   if (bat_halted(ctl, op.B, true)) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->iter = k + 1;
   const int lanes = op.lanes, rpw = 64 >> op.lg;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = wave_id();
   const int b = lane & (lanes - 1), rsub = lane >> op.lg;
   const bool act = b < op.B && ctl->active[b] != 0;
-  const double beta = k == 0 ? 0.0 : ctl->beta[b];
   double tcf[NK];
 #pragma unroll
   for (int q = 0; q < NK; ++q) tcf[q] = op.tdt * op.coef[q * lanes + b];
   const int64_t ng = (op.n + rpw - 1) / rpw;
   const RowRange rr = block_rows(ng);
   double red[1] = {0.0};
-  if (act) {
+  if (WU) {
+    // 64 lanes = 64 instances of one row: the row's pattern and shared
+    // values are read once, spread over the lanes, and broadcast.
+    for (int64_t g = rr.g0 + wv; g < rr.g1; g += kBW) {
+      const int64_t i = g;
+      const int32_t j0 = op.rp[i], j1 = op.rp[i + 1];
+      double acc = 0.0;
+      for (int32_t js = j0; js < j1; js += 64) {
+        const int len = j1 - js < 64 ? j1 - js : 64;
+        RowSeg<T, NK> rs;
+        rs.load(op, js, len, lane);
+        for (int u0 = 0; u0 < len; u0 += kBSeg) {
+          T pj[kBSeg];
+#pragma unroll
+          for (int u = 0; u < kBSeg; ++u) {
+            const int uu = u0 + u < len ? u0 + u : len - 1;
+            pj[u] = p[((int64_t)bcast(rs.col, uu) << 6) + lane];
+          }
+#pragma unroll
+          for (int u = 0; u < kBSeg; ++u) {
+            const int uu = u0 + u < len ? u0 + u : len - 1;
+            double a = op.Cm * (double)bcast(rs.m, uu);
+#pragma unroll
+            for (int q = 0; q < NK; ++q) a += tcf[q] * (double)bcast(rs.k[q], uu);
+            const double t = acc + a * (double)pj[u];
+            acc = u0 + u < len ? t : acc;
+          }
+        }
+      }
+      if (act) {
+        const int64_t e = (i << 6) + lane;
+        const T pi = p[e];
+        const T api = (T)acc;
+        Ap[e] = api;
+        red[0] += (double)pi * (double)api;
+      }
+    }
+  } else if (act) {
     for (int64_t g = rr.g0 + wv; g < rr.g1; g += kBW) {
       const int64_t i = g * rpw + rsub;
       if (i >= op.n) continue;
       const int32_t j0 = op.rp[i], j1 = op.rp[i + 1];
       double acc = 0.0;
-      for (int32_t j = j0; j < j1; ++j) {
-        const int64_t e = (int64_t)op.col[j] * lanes + b;
-        T pj;
-        if (k == 0) pj = (T)((double)Dinv[e] * (double)r[e]);
-        else pj = (T)(beta * (double)pin[e] + (double)Dinv[e] * (double)r[e]);
-        acc += a_val<T, NK>(op, tcf, j) * (double)pj;
+      for (int32_t jc = j0; jc < j1; jc += kBChunk) {
+        int32_t cj[kBChunk];
+        double av[kBChunk];
+        T pj[kBChunk];
+#pragma unroll
+        for (int u = 0; u < kBChunk; ++u) {
+          const int32_t jj = jc + u < j1 ? jc + u : j1 - 1;
+          cj[u] = op.col[jj];
+          av[u] = a_val<T, NK>(op, tcf, jj);
+        }
+#pragma unroll
+        for (int u = 0; u < kBChunk; ++u) pj[u] = p[(int64_t)cj[u] * lanes + b];
+#pragma unroll
+        for (int u = 0; u < kBChunk; ++u) {
+          const double t = acc + av[u] * (double)pj[u];
+          acc = jc + u < j1 ? t : acc;
+        }
       }
       const int64_t e = i * lanes + b;
-      T pi;
-      if (k == 0) pi = (T)((double)Dinv[e] * (double)r[e]);
-      else pi = (T)(beta * (double)pin[e] + (double)Dinv[e] * (double)r[e]);
+      const T pi = p[e];
       const T api = (T)acc;
-      pout[e] = pi;
       Ap[e] = api;
       red[0] += (double)pi * (double)api;
     }
@@ -264,7 +419,7 @@ k_bat_update(int64_t n, int lanes, int lg, int B, const T* __restrict__ Dinv,
              T* __restrict__ r, BatCtl* ctl, double* part) {
   if (bat_halted(ctl, B, true)) return;
   const int rpw = 64 >> lg;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = wave_id();
   const int b = lane & (lanes - 1), rsub = lane >> lg;
   const bool act = b < B && ctl->active[b] != 0;
   const double alpha = act ? ctl->alpha[b] : 0.0;
@@ -418,7 +573,7 @@ struct tc_batch {
     tc_ttp_params ttp;
     tc_crn_params crn;
   } gp[TC_BATCH_MAX_GROUPS];
-  int G, lg;
+  int G, Gp, lg;  // blocks of the row passes and of the elementwise p' pass
   tc::BatCtl* ctl;
   double* part[3];  // start (2), spmv (1), update (2); each block of G*lanes
   void* r;
@@ -500,8 +655,15 @@ int bat_iters(tc_batch* S, int k0, int k1, XT* xs, hipStream_t s) {
   for (int k = k0; k < k1; ++k) {
     T* pin = pp[k & 1];
     T* pout = pp[(k + 1) & 1];
-    hipLaunchKernelGGL((k_bat_spmv<T, NK>), g, blk, 0, s, op, (const T*)d.Dinv, (const T*)S->r,
-                       (const T*)pin, pout, (T*)S->Ap, k, S->ctl, S->part[1]);
+    hipLaunchKernelGGL(k_bat_pform<T>, dim3(S->Gp), dim3(256), 0, s, d.n, d.lanes, d.B,
+                       (const T*)d.Dinv, (const T*)S->r, (const T*)pin, pout, k, S->ctl);
+    TC_LAUNCH();
+    if (d.lanes == 64)
+      hipLaunchKernelGGL((k_bat_spmv<T, NK, true>), g, blk, 0, s, op, (const T*)pout,
+                         (T*)S->Ap, k, S->ctl, S->part[1]);
+    else
+      hipLaunchKernelGGL((k_bat_spmv<T, NK, false>), g, blk, 0, s, op, (const T*)pout,
+                         (T*)S->Ap, k, S->ctl, S->part[1]);
     TC_LAUNCH();
     hipLaunchKernelGGL(k_bat_fold, dim3(d.B), fb, 0, s, 1, k, S->G, d.lanes, d.B, S->ctl,
                        (const double*)S->part[1]);
@@ -538,6 +700,23 @@ int bat_wait(tc_batch* S, int slot, long long seq) {
   }
   std::atomic_thread_fence(std::memory_order_acquire);
   return TC_OK;
+}
+
+template <class T, class XT, class WT, bool LG, int NK>
+void bat_rhs_launch(tc_batch* S, const BatOp<T>& op, int xc, int pc, hipStream_t s) {
+  const tc_batch_desc& d = S->d;
+  const WT* w = (const WT*)d.w;
+  const XT* x = (const XT*)d.x[xc];
+  const T* xp = LG ? (const T*)d.xp[pc] : nullptr;
+  XT* xn = LG ? (XT*)d.x[1 - xc] : nullptr;
+  T* xpn = LG ? (T*)d.xp[1 - pc] : nullptr;
+  const dim3 g(S->G), blk(kBB);
+  if (d.lanes == 64)
+    hipLaunchKernelGGL((k_bat_rhs_init<T, XT, WT, LG, NK, true>), g, blk, 0, s, op, w, x, xp, xn,
+                       xpn, (T*)S->r, (const T*)d.Dinv, S->ctl, S->part[0]);
+  else
+    hipLaunchKernelGGL((k_bat_rhs_init<T, XT, WT, LG, NK, false>), g, blk, 0, s, op, w, x, xp,
+                       xn, xpn, (T*)S->r, (const T*)d.Dinv, S->ctl, S->part[0]);
 }
 
 template <class T, class XT, class WT, int NK>
@@ -583,14 +762,10 @@ int bat_enqueue(tc_batch* S, BHost& ps, BPlan& plan, hipStream_t s) {
   const dim3 g(S->G), blk(kBB);
   XT* xs;
   if (d.linear_guess) {
-    hipLaunchKernelGGL((k_bat_rhs_init<T, XT, WT, true, NK>), g, blk, 0, s, op, (const WT*)d.w,
-                       (const XT*)d.x[xc], (const T*)d.xp[pc], (XT*)d.x[1 - xc],
-                       (T*)d.xp[1 - pc], (T*)S->r, (const T*)d.Dinv, S->ctl, S->part[0]);
+    bat_rhs_launch<T, XT, WT, true, NK>(S, op, xc, pc, s);
     xs = (XT*)d.x[1 - xc];
   } else {
-    hipLaunchKernelGGL((k_bat_rhs_init<T, XT, WT, false, NK>), g, blk, 0, s, op,
-                       (const WT*)d.w, (const XT*)d.x[xc], (const T*)nullptr, (XT*)nullptr,
-                       (T*)nullptr, (T*)S->r, (const T*)d.Dinv, S->ctl, S->part[0]);
+    bat_rhs_launch<T, XT, WT, false, NK>(S, op, xc, pc, s);
     xs = (XT*)d.x[xc];
   }
   TC_LAUNCH();
@@ -781,6 +956,10 @@ extern "C" int tc_batch_create(tc_batch** out, const tc_batch_desc* d, void* str
   }
   S->lg = log2_exact(d->lanes);
   S->G = bat_blocks(d->n, d->lanes);
+  {
+    const int64_t gp = ceil_div(d->n * d->lanes, 256 * 4);
+    S->Gp = (int)(gp < 1 ? 1 : (gp > 8192 ? 8192 : gp));
+  }
   const WsLayout L = ws_layout(d->n, d->lanes, d->dtype);
   char* base = (char*)d->ws;
   S->ctl = (BatCtl*)(base + L.ctl);

@@ -80,6 +80,22 @@ def face_nodes(N, axis=0, index=0):
     return (idx[0] * N * N + idx[1] * N + idx[2]).ravel()
 
 
+def cube_surface_nodes(N):
+    """Indices of the nodes on the boundary of the N^3 node cube."""
+    g = np.arange(N)
+    I, J, K = np.meshgrid(g, g, g, indexing="ij")
+    on = (I == 0) | (I == N - 1) | (J == 0) | (J == N - 1) | (K == 0) | (K == N - 1)
+    return (I * N * N + J * N + K)[on].ravel()
+
+
+def surface_patch(nodes, surface_ids, size, rng):
+    """``size`` boundary nodes nearest to a boundary node drawn with ``rng``."""
+    surface_ids = np.asarray(surface_ids)
+    c = nodes[surface_ids[rng.integers(surface_ids.shape[0])]]
+    d = np.linalg.norm(nodes[surface_ids] - c, axis=1)
+    return np.sort(surface_ids[np.argsort(d, kind="stable")[:size]])
+
+
 def cube_workload(N, model="TTP"):
     """Mesh arrays of the large-cube workload at N nodes per side.
 
