@@ -184,6 +184,13 @@ int tc_pcg_get_timing(tc_pcg* plan, double* ms_host, int64_t* count_host);
  * dV may be NULL; w may be NULL (then only dV and the states are written).
  * ---------------------------------------------------------------------- */
 #define TC_MAX_STIMULI 32
+/* One stimulus: on iff t/period <= count and start <= t % period <=
+ * start + duration (stimulation.py:51-63), intensity amp (uA/cm^2). */
+typedef struct tc_stimulus {
+  double start, duration, period, count, amp;
+} tc_stimulus;
+#define TC_POOL_STIM 4
+
 typedef struct tc_ionic_io {
   int64_t n;
   double dt;                 /* ionic model time step (ms): state advance */
@@ -217,6 +224,23 @@ typedef struct tc_ionic_io {
    * node * lane_stride + lane0 + lane.  States stay (S, n).  lanes = 0:
    * one entry per node. */
   int32_t lanes, lane_stride, lane0;
+  /* Pool layout (pool_slices != NULL, the served-stream engine): the launch
+   * covers the 64-row slices listed in pool_slices (n = 64 * their count);
+   * entry i is row pool_slices[i/64]*64 + i%64 of every per-row array, states
+   * are (S, state_stride), and the system owning the slice supplies the valid
+   * row range, the stimuli on this step (pool_sys_bits, amplitudes from its
+   * TC_POOL_STIM-entry table pool_sys_stim) and the LAT/RT time (pool_sys_t,
+   * once pool_sys_steps > 0).  stim_mask then holds each row's bits into its
+   * system's stimulus table. */
+  const int32_t* pool_slices;
+  const int32_t* pool_slice_sys;
+  const int64_t* pool_sys_row0;
+  const int64_t* pool_sys_n;
+  const double* pool_sys_t;
+  const long long* pool_sys_steps;
+  const uint32_t* pool_sys_bits;
+  const struct tc_stimulus* pool_sys_stim;
+  int64_t state_stride;
 } tc_ionic_io;
 
 /* ModifiedMS2v (ionic/ModifiedMS2v.py:4-44).  States: H. */
@@ -273,9 +297,6 @@ int tc_crn_step(const tc_crn_params* p, void* states, const void* Cai_tab,
  * windows and LAT/RT use that t, as in the reference.
  * ---------------------------------------------------------------------- */
 enum tc_model { TC_MODEL_MS2V = 0, TC_MODEL_TTP = 1, TC_MODEL_CRN = 2 };
-typedef struct tc_stimulus {
-  double start, duration, period, count, amp;
-} tc_stimulus;
 typedef struct tc_sim_desc {
   int model;                 /* tc_model */
   const void* params;        /* tc_ms2v_params / tc_ttp_params / tc_crn_params */
@@ -395,6 +416,83 @@ int tc_batch_advance(tc_batch* batch, int64_t nsteps, int32_t* iters_host, void*
 int tc_batch_finish(tc_batch* batch, int64_t* total_iters_host, void* stream);
 /* t, steps done, index 0..1 of desc.x holding the solution. */
 int tc_batch_status(tc_batch* batch, double* t_host, int64_t* steps_host, int* x_index_host);
+
+/* ------------------------------------------------------------------------
+ * Served stream (no counterpart in the reference: independent Monodomain
+ * runs, monodomain.py:141-226, on different meshes, admitted and retired
+ * while the others run).
+ *
+ * A device pool holds every admitted system: a contiguous range of 64-row
+ * slices (rows = nodes padded to whole slices) and of SELL entries in
+ * pool-wide arrays whose column indices are pool rows (block-offset batched
+ * SELL), pool-wide per-row vectors and one ionic state block per model.  All
+ * live systems advance together, one time step per engine step, with their
+ * own PCG scalars, stopping tests, clocks and stimuli.  Admission and
+ * retirement happen between tc_serve_advance calls.
+ * ---------------------------------------------------------------------- */
+typedef struct tc_serve_desc {
+  int dtype, x_f64, w_f64;   /* as tc_sim_desc */
+  int64_t cap_slices;        /* pool rows = 64 * cap_slices */
+  int64_t cap_entries;       /* pool SELL entries */
+  int max_sys;
+  int32_t* slice_off;        /* (cap_slices) first entry of each slice */
+  int32_t* slice_w;          /* (cap_slices) padded width */
+  int32_t* slice_sys;        /* (cap_slices) owning system */
+  int32_t* col;              /* (cap_entries) pool rows */
+  void* Mv;                  /* (cap_entries) dtype: M, K/Chi, A values */
+  void* Kv;
+  void* Av;
+  void* Dinv;                /* (rows) dtype */
+  void* x[2];                /* (rows) iterate (float64 when x_f64) */
+  void* xp[2];               /* (rows) dtype */
+  void* w;                   /* (rows) float64 when w_f64 */
+  void* r;                   /* (rows) dtype */
+  void* p[2];
+  void* Ap;
+  void* AT;                  /* (rows) dtype, or NULL */
+  void* RT;
+  uint32_t* stim_mask;       /* (rows) bits into the row's system stimuli */
+  void* states[3];           /* per tc_model: (S, rows) dtype, NULL if unused */
+  const void* params[3];     /* per tc_model: tc_*_params */
+  const void* tab[3][3];     /* per tc_model: tables */
+  double dt, dt_ionic, Cm, theta, a_tol, r_tol;
+  int max_iter, linear_guess, depth;
+  void* ws;                  /* tc_serve_workspace_size bytes */
+  size_t ws_bytes;
+} tc_serve_desc;
+typedef struct tc_serve_system {
+  int64_t n;                 /* nodes */
+  int64_t n_entries;         /* padded SELL entries (slice_ptr[ceil(n/64)]) */
+  const int32_t* slice_ptr;  /* the system's SELL-64 operator (tc_sell_*) */
+  const int32_t* col;        /* its columns (0 .. n-1) */
+  const void* Mv;            /* SELL values of M, K/Chi, A and Dinv, dtype */
+  const void* Kv;
+  const void* Av;
+  const void* Dinv;
+  int model;                 /* tc_model */
+  int n_states;
+  const void* states;        /* (n_states, n) initial states */
+  const void* V0;            /* (n) initial potential */
+  const uint32_t* stim_mask; /* (n) bit k: stimulus k applies */
+  int n_stim;
+  tc_stimulus stim[TC_POOL_STIM];
+  int lat;                   /* track AT/RT */
+} tc_serve_system;
+typedef struct tc_serve tc_serve;
+int tc_serve_workspace_size(int64_t cap_slices, int max_sys, size_t* bytes_host);
+int tc_serve_create(tc_serve** serve, const tc_serve_desc* desc, void* stream);
+int tc_serve_destroy(tc_serve* serve);
+/* *fits_host = 1 when a system of that size can be admitted now. */
+int tc_serve_fits(tc_serve* serve, int64_t n_slices, int64_t n_entries, int* fits_host);
+/* Copies the system into the pool (synchronizes); *slot_host = its id. */
+int tc_serve_add(tc_serve* serve, const tc_serve_system* sys, int* slot_host, void* stream);
+/* Final LAT/RT test, copies V (and AT, RT) of the system to (n) device
+ * buffers, returns its n_iter sum and steps done, frees its ranges. */
+int tc_serve_remove(tc_serve* serve, int slot, void* V_out, void* AT_out, void* RT_out,
+                    int64_t* total_iters_host, int64_t* steps_host, void* stream);
+/* nsteps engine steps of every live system; iters[k] (host) = the largest
+ * n_iter among them in step k.  Synchronizes at the end. */
+int tc_serve_advance(tc_serve* serve, int64_t nsteps, int32_t* iters_host, void* stream);
 
 /* LAT/RT update alone (monodomain.py:174-176), for the last step of a run. */
 int tc_lat_update(const void* u, void* AT, void* RT, int64_t n, double t, int dtype,

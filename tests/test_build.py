@@ -108,7 +108,8 @@ def test_ctypes_structs_match_c_layout(tmp_path):
     pairs = {"tc_sell": L.Sell, "tc_ionic_io": L.IonicIO, "tc_ms2v_params": L.MS2vParams,
              "tc_ttp_params": L.TTPParams, "tc_crn_params": L.CRNParams,
              "tc_stimulus": L.Stimulus, "tc_sim_desc": L.SimDesc,
-             "tc_batch_group": L.BatchGroup, "tc_batch_desc": L.BatchDesc}
+             "tc_batch_group": L.BatchGroup, "tc_batch_desc": L.BatchDesc,
+             "tc_serve_desc": L.ServeDesc, "tc_serve_system": L.ServeSystem}
     lines = ['#include <stdio.h>', '#include <stddef.h>',
              f'#include "{ROOT / "include" / "torchcor_amd.h"}"', "int main(void) {"]
     for cname, py in pairs.items():

@@ -1,0 +1,85 @@
+"""Served stream (ServeEngine): continuous batching of runs on different
+meshes.
+
+Tolerances:
+  * a run alone in the pool and the same run among others, admitted at other
+    times and pool places (a pool that holds only some of the runs, so later
+    ones wait and reuse freed room): bit-identical V, AT, RT and iteration
+    sums (a run's arithmetic never mixes with another run's).
+  * against the single-run path (Monodomain, same assembly and operators),
+    float64: max |dVm| < 1e-9 mV, LAT equal, n_iter sums equal (only the
+    order of the PCG partial sums differs).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from torchcor_amd import meshes  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+COND = [([0], 0.5272, 0.2076), ([1], 0.9074, 0.3332)]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _requests():
+    from torchcor_amd.serve import Request
+    out = []
+    for k, (N, model, T, extra) in enumerate([(7, "ms2v", 6.0, False), (9, "ttp", 8.0, False),
+                                               (11, "crn", 5.0, False), (8, "ms2v", 7.0, True),
+                                               (10, "ttp", 4.0, True)]):
+        nodes, elems, regions, fib = meshes.cube_workload(N)
+        st = [(meshes.corner_nodes(N, 3), 0.0, 1.0, 100.0)]
+        if extra:
+            st.append((meshes.face_nodes(N, 0, N - 1), 2.0, 0.5, 80.0, 3.0, 2))
+        out.append(Request(nodes, elems, regions, fib, COND, model, st, T, tag=k))
+    return out
+
+
+def _engine(dtype, cap_nodes=20000):
+    from torchcor_amd.serve import ServeEngine
+    return ServeEngine(cap_nodes=cap_nodes, cap_entries=cap_nodes * 16, max_sys=16,
+                       dtype=dtype, device=DEV)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_serve_alone_equals_in_mix(dtype):
+    _say(f"[gpu] serve alone vs mix {dtype}")
+    reqs = _requests()
+    eng = _engine(dtype, cap_nodes=2200)      # holds two or three runs at a time
+    mix = eng.run_until_done([eng.prepare(r) for r in reqs], max_chunk=37)
+    for k, r in enumerate(reqs):
+        alone_eng = _engine(dtype)
+        alone = alone_eng.run_until_done([alone_eng.prepare(r)])[0]
+        assert alone["steps"] == mix[k]["steps"] == int(r.T / 0.01)
+        assert torch.equal(alone["u"], mix[k]["u"]), k
+        assert torch.equal(alone["AT"], mix[k]["AT"]) and torch.equal(alone["RT"], mix[k]["RT"])
+        assert alone["iterations"] == mix[k]["iterations"]
+
+
+def test_serve_matches_single_run_path():
+    from torchcor_amd.simulator import Monodomain
+    from torchcor_amd.serve import MODEL_CLASSES
+    _say("[gpu] serve vs Monodomain f64")
+    reqs = _requests()
+    eng = _engine(torch.float64)
+    res = eng.run_until_done([eng.prepare(r) for r in reqs])
+    for k, r in enumerate(reqs):
+        model = MODEL_CLASSES[r.model](0.01, DEV, torch.float64)
+        sim = Monodomain(model, T=r.T, dt=0.01, device=DEV, dtype=torch.float64)
+        sim.load_arrays(r.nodes, r.elems, r.regions, r.fibres)
+        for c in r.conductivities:
+            sim.add_conductivity(*c)
+        for s in r.stimuli:
+            sim.add_stimulus(s[0], *s[1:])
+        sim.start(1e-5, 1e-5, 100, snapshot_interval=1e9)
+        sim.advance(sim.nt)
+        sim.finish()
+        err = (sim.u - res[k]["u"]).abs().max().item()
+        assert err < 1e-9, (k, err)
+        assert torch.equal(sim.activation_time, res[k]["AT"]), k
+        assert sum(sim.iterations) == res[k]["iterations"], k

@@ -31,12 +31,21 @@ class Sell(C.Structure):
     _fields_ = [("n", i64), ("n_slices", i64), ("slice_ptr", p), ("col", p), ("dcol", p)]
 
 
+class Stimulus(C.Structure):
+    _fields_ = [("start", dbl), ("duration", dbl), ("period", dbl), ("count", dbl), ("amp", dbl)]
+
+
+POOL_STIM = 4
+
+
 class IonicIO(C.Structure):
     _fields_ = [("n", i64), ("dt", dbl), ("dt_rhs", dbl), ("Cm", dbl), ("V", p), ("w", p),
                 ("dV", p), ("stim_mask", p), ("stim_active", C.c_uint32),
                 ("stim_amp", dbl * MAX_STIMULI), ("AT", p), ("RT", p), ("lat_t", dbl),
                 ("gate", p), ("v_f64", i32), ("w_f64", i32), ("lanes", i32),
-                ("lane_stride", i32), ("lane0", i32)]
+                ("lane_stride", i32), ("lane0", i32), ("pool_slices", p), ("pool_slice_sys", p),
+                ("pool_sys_row0", p), ("pool_sys_n", p), ("pool_sys_t", p), ("pool_sys_steps", p),
+                ("pool_sys_bits", p), ("pool_sys_stim", p), ("state_stride", i64)]
 
 
 MS2V_FIELDS = ["tau_in", "tau_out", "tau_open", "tau_close", "u_gate", "u_crit", "vmin", "vmax"]
@@ -65,10 +74,6 @@ class TTPParams(C.Structure):
 
 class CRNParams(C.Structure):
     _fields_ = [(f, dbl) for f in CRN_FIELDS]
-
-
-class Stimulus(C.Structure):
-    _fields_ = [("start", dbl), ("duration", dbl), ("period", dbl), ("count", dbl), ("amp", dbl)]
 
 
 class SimDesc(C.Structure):
@@ -101,6 +106,24 @@ class BatchDesc(C.Structure):
                 ("groups", BatchGroup * BATCH_MAX_GROUPS), ("a_tol", dbl), ("r_tol", dbl),
                 ("max_iter", i32), ("linear_guess", i32), ("depth", i32), ("ws", p),
                 ("ws_bytes", C.c_size_t)]
+
+
+class ServeDesc(C.Structure):
+    _fields_ = [("dtype", i32), ("x_f64", i32), ("w_f64", i32), ("cap_slices", i64),
+                ("cap_entries", i64), ("max_sys", i32), ("slice_off", p), ("slice_w", p),
+                ("slice_sys", p), ("col", p), ("Mv", p), ("Kv", p), ("Av", p), ("Dinv", p),
+                ("x", p * 2), ("xp", p * 2), ("w", p), ("r", p), ("p", p * 2), ("Ap", p),
+                ("AT", p), ("RT", p), ("stim_mask", p), ("states", p * 3), ("params", p * 3),
+                ("tab", (p * 3) * 3), ("dt", dbl), ("dt_ionic", dbl), ("Cm", dbl),
+                ("theta", dbl), ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32),
+                ("linear_guess", i32), ("depth", i32), ("ws", p), ("ws_bytes", C.c_size_t)]
+
+
+class ServeSystem(C.Structure):
+    _fields_ = [("n", i64), ("n_entries", i64), ("slice_ptr", p), ("col", p), ("Mv", p),
+                ("Kv", p), ("Av", p), ("Dinv", p), ("model", i32), ("n_states", i32),
+                ("states", p), ("V0", p), ("stim_mask", p), ("n_stim", i32),
+                ("stim", Stimulus * POOL_STIM), ("lat", i32)]
 
 
 MODEL_MS2V, MODEL_TTP, MODEL_CRN = 0, 1, 2
@@ -161,6 +184,13 @@ FUNCS = {
     "tc_batch_advance": (p, i64, C.POINTER(C.c_int32), p),
     "tc_batch_finish": (p, I64_P, p),
     "tc_batch_status": (p, C.POINTER(dbl), I64_P, INT_P),
+    "tc_serve_workspace_size": (i64, i32, SIZE_P),
+    "tc_serve_create": (C.POINTER(p), C.POINTER(ServeDesc), p),
+    "tc_serve_destroy": (p,),
+    "tc_serve_fits": (p, i64, i64, INT_P),
+    "tc_serve_add": (p, C.POINTER(ServeSystem), INT_P, p),
+    "tc_serve_remove": (p, i32, p, p, p, I64_P, I64_P, p),
+    "tc_serve_advance": (p, i64, C.POINTER(C.c_int32), p),
 }
 for _n, _a in FUNCS.items():
     _proto(_n, *_a)

@@ -115,13 +115,54 @@ __device__ inline double t_log(double x) { return log(x); }
 __device__ inline float t_expm1(float x) { return expm1f(x); }
 __device__ inline double t_expm1(double x) { return expm1(x); }
 
-// Entry of node-instance i in the per-node arrays (V, w, dV, stim_mask,
-// AT, RT): i itself, or in a batch group (lanes > 0) lane lane0 + i % lanes
-// of node i / lanes, whose row holds lane_stride entries.
-__device__ inline int64_t io_index(const tc_ionic_io& io, int64_t i) {
-  if (io.lanes <= 0) return i;
-  const int64_t node = i / io.lanes;
-  return node * io.lane_stride + io.lane0 + (i - node * io.lanes);
+// Where entry i of a launch lives and what its epilogue applies.
+struct NodeCtx {
+  int64_t vi;          // entry in the per-node arrays (V, w, dV, stim_mask, AT, RT)
+  int64_t si, ss;      // state index and state row stride
+  uint32_t bits;       // stimuli on at this step for this entry
+  const double* amp;   // intensity of stimulus k at amp[k * amp_stride]
+  int amp_stride;
+  int lat;             // LAT/RT test on, at lat_t
+  double lat_t;
+  int ok;              // a real node (pool padding rows are skipped)
+};
+
+// Plain: entry i.  Batch group (lanes > 0): lane lane0 + i % lanes of node
+// i / lanes, rows of lane_stride entries.  Pool: row slice*64 + i%64 of the
+// listed slice, with its system's row range, stimulus clock and LAT time.
+__device__ inline NodeCtx node_ctx(const tc_ionic_io& io, int64_t i) {
+  NodeCtx c;
+  if (io.pool_slices) {
+    const int64_t sl = io.pool_slices[i >> 6];
+    const int64_t row = sl * 64 + (i & 63);
+    const int sys = io.pool_slice_sys[sl];
+    c.vi = row;
+    c.si = row;
+    c.ss = io.state_stride;
+    c.ok = row - io.pool_sys_row0[sys] < io.pool_sys_n[sys];
+    const uint32_t on = io.pool_sys_bits[sys];
+    c.bits = (on && io.stim_mask) ? io.stim_mask[row] & on : 0u;
+    c.amp = &io.pool_sys_stim[(int64_t)sys * TC_POOL_STIM].amp;
+    c.amp_stride = (int)(sizeof(tc_stimulus) / sizeof(double));
+    c.lat = io.AT != nullptr && io.pool_sys_steps[sys] > 0;
+    c.lat_t = io.pool_sys_t[sys];
+    return c;
+  }
+  if (io.lanes > 0) {
+    const int64_t node = i / io.lanes;
+    c.vi = node * io.lane_stride + io.lane0 + (i - node * io.lanes);
+  } else {
+    c.vi = i;
+  }
+  c.si = i;
+  c.ss = io.n;
+  c.ok = 1;
+  c.bits = (io.stim_active && io.stim_mask) ? io.stim_mask[c.vi] & io.stim_active : 0u;
+  c.amp = io.stim_amp;
+  c.amp_stride = 1;
+  c.lat = io.AT != nullptr;
+  c.lat_t = io.lat_t;
+  return c;
 }
 
 template <class T>
@@ -129,20 +170,20 @@ __device__ inline T load_v(const tc_ionic_io& io, int64_t vi) {
   return io.v_f64 ? (T)((const double*)io.V)[vi] : ((const T*)io.V)[vi];
 }
 
-// Stimulus, RHS input and LAT/RT epilogue shared by all models; i is the
-// entry in the per-node arrays.
+// Stimulus, RHS input and LAT/RT epilogue shared by all models.
 template <class T>
-__device__ inline void finish_node(const tc_ionic_io& io, int64_t i, T V, T dV) {
+__device__ inline void finish_node(const tc_ionic_io& io, const NodeCtx& c, T V, T dV) {
+  const int64_t i = c.vi;
   if (io.dV) ((T*)io.dV)[i] = dV;
   if (io.w) {
-    const uint32_t bits = (io.stim_active && io.stim_mask) ? io.stim_mask[i] & io.stim_active : 0u;
+    const uint32_t bits = c.bits;
     if (io.w_f64) {
       // float64 right-hand-side input from the model's dV
       double b = (double)V * io.Cm + io.dt_rhs * ((double)dV / 100.0);
       if (bits) {
         double ist = 0.0;
         for (int k = 0; k < TC_MAX_STIMULI; ++k)
-          if (bits & (1u << k)) ist += io.stim_amp[k];
+          if (bits & (1u << k)) ist += c.amp[k * c.amp_stride];
         b = b + io.dt_rhs * (ist / 100.0);
       }
       ((double*)io.w)[i] = b;
@@ -153,21 +194,21 @@ __device__ inline void finish_node(const tc_ionic_io& io, int64_t i, T V, T dV) 
       if (bits) {
         T ist = T(0);
         for (int k = 0; k < TC_MAX_STIMULI; ++k)
-          if (bits & (1u << k)) ist += (T)io.stim_amp[k];
+          if (bits & (1u << k)) ist += (T)c.amp[k * c.amp_stride];
         b = b + (T)io.dt_rhs * (ist / T(100));
       }
       ((T*)io.w)[i] = b;
     }
   }
-  if (io.AT) {
+  if (c.lat) {
     T* AT = (T*)io.AT;
     T* RT = (T*)io.RT;
     T at = AT[i];
     if (V > T(0) && at == T(-1)) {
-      at = (T)io.lat_t;
+      at = (T)c.lat_t;
       AT[i] = at;
     }
-    if (at > T(0) && V < T(-70) && RT[i] == T(-1)) RT[i] = (T)io.lat_t;
+    if (at > T(0) && V < T(-70) && RT[i] == T(-1)) RT[i] = (T)c.lat_t;
   }
 }
 
@@ -179,17 +220,18 @@ k_ms2v(tc_ms2v_params p, T* __restrict__ H, tc_ionic_io io) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= io.n) return;
   if (io.gate && *(volatile const int*)io.gate) return;
-  const int64_t vi = io_index(io, i);
-  const T U = load_v<T>(io, vi);
-  const T h = H[i];
+  const NodeCtx c = node_ctx(io, i);
+  if (!c.ok) return;
+  const T U = load_v<T>(io, c.vi);
+  const T h = H[c.si];
   const T DV = (T)(p.vmax - p.vmin);
   const T Uad = (U - (T)p.vmin) / DV;
   const T J_in = T(-1) * h * Uad * (Uad - (T)p.u_crit) * (T(1) - Uad) / (T)p.tau_in;
   const T J_out = (T(1) - h) * Uad / (T)p.tau_out;
   const T dU = -(DV * (J_in + J_out));
   const T dH = Uad > (T)p.u_gate ? -h / (T)p.tau_close : (T(1) - h) / (T)p.tau_open;
-  H[i] = h + (T)io.dt * dH;
-  finish_node<T>(io, vi, U, dU);
+  H[c.si] = h + (T)io.dt * dH;
+  finish_node<T>(io, c, U, dU);
 }
 
 // ---------------------------------------------------------------- TTP
@@ -337,11 +379,12 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
   const int64_t n = io.n;
   if (i >= n) return;
   if (io.gate && *(volatile const int*)io.gate) return;
-  const int64_t vi = io_index(io, i);
-  const T V = load_v<T>(io, vi);
+  const NodeCtx c = node_ctx(io, i);
+  if (!c.ok) return;
+  const T V = load_v<T>(io, c.vi);
   T s[TS_N];
 #pragma unroll
-  for (int k = 0; k < TS_N; ++k) s[k] = st[k * n + i];
+  for (int k = 0; k < TS_N; ++k) s[k] = st[k * c.ss + c.si];
   const double RTONFd = (p.Rconst * p.T) / p.Fconst;
   const T RTONF = (T)RTONFd;
   const T F_RT = (T)(1. / RTONFd);
@@ -424,8 +467,8 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
   o[TS_Xr2] = vr(TTP_Xr2_A, TTP_VCOLS) + vr(TTP_Xr2_B, TTP_VCOLS) * s[TS_Xr2];
   o[TS_Xs] = vr(TTP_Xs_A, TTP_VCOLS) + vr(TTP_Xs_B, TTP_VCOLS) * s[TS_Xs];
 #pragma unroll
-  for (int k = 0; k < TS_N; ++k) st[k * n + i] = o[k];
-  finish_node<T>(io, vi, V, -Iion);
+  for (int k = 0; k < TS_N; ++k) st[k * c.ss + c.si] = o[k];
+  finish_node<T>(io, c, V, -Iion);
 }
 
 // ---------------------------------------------------------------- CRN
@@ -576,11 +619,12 @@ k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
   const int64_t n = io.n;
   if (i >= n) return;
   if (io.gate && *(volatile const int*)io.gate) return;
-  const int64_t vi = io_index(io, i);
-  const T V = load_v<T>(io, vi);
+  const NodeCtx c = node_ctx(io, i);
+  if (!c.ok) return;
+  const T V = load_v<T>(io, c.vi);
   T s[CS_N];
 #pragma unroll
-  for (int k = 0; k < CS_N; ++k) s[k] = st[k * n + i];
+  for (int k = 0; k < CS_N; ++k) s[k] = st[k * c.ss + c.si];
   const Row<T> vr = row_at<T>(Vtab, CV_NCOLS, V, tb.V);
   const Row<T> car = row_at<T>(Ctab, CA_NCOLS, s[CS_Cai], tb.Cai);
   const T RTF = (T)((p.R * p.T) / p.F);
@@ -635,8 +679,8 @@ k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
   RLU(d) RLU(f) RLU(h) RLU(j) RLU(m) RLU(oa) RLU(oi) RLU(ua) RLU(ui) RLU(w) RLU(xr) RLU(xs)
 #undef RLU
 #pragma unroll
-  for (int k = 0; k < CS_N; ++k) st[k * n + i] = o[k];
-  finish_node<T>(io, vi, V, -Iion);
+  for (int k = 0; k < CS_N; ++k) st[k * c.ss + c.si] = o[k];
+  finish_node<T>(io, c, V, -Iion);
 }
 
 template <class T>

@@ -1,0 +1,1095 @@
+// Served stream: many monodomain runs on different meshes, admitted and
+// retired while the others keep running (continuous batching).
+//
+// Device pool.  Every admitted system owns a contiguous range of 64-row
+// slices (its rows, padded to whole slices) and a contiguous range of SELL
+// entries in pool-wide arrays: a block-offset batched SELL whose column
+// indices are pool rows.  Per-row vectors (iterate, x_prev, w, r, p, Ap,
+// Dinv, AT, RT, stimulus bits) and per-model ionic state blocks are pool-wide
+// too, so a system's state stays resident from admission to retirement.
+// Ranges are handed out first-fit; nothing moves while a system is live.
+//
+// Passes run over a list of the live systems' slices (system by system): one
+// wavefront per slice, one thread per row, the slice's system giving the
+// PCG scalars, the stopping test and the valid row range.  Each wavefront
+// writes its slice's partial sums; a fold kernel with one block per system
+// adds that system's slices in list order (deterministic).  A system that
+// has met the stopping rule is inactive for the rest of the step.  Ionic
+// kernels take per-model slice lists (pool mode of tc_ionic_io).
+//
+// The time loop is the pipelined one of the single-run runtime: ionic ->
+// RHS + PCG start -> K speculative iterations -> check, several steps ahead;
+// the check pass advances every live system's clock and stimulus bits and
+// publishes the step status in pinned host memory; a step that needs more
+// iterations stalls the pipeline until the host has finished it.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <deque>
+#include <map>
+#include <thread>
+#include <vector>
+
+#include "pcg_kernels.h"
+
+namespace tc {
+namespace {
+
+constexpr int kSB = 512;            // threads per block of the passes
+constexpr int kSW = kSB / 64;
+constexpr int kSMaxBlocks = 1024;
+constexpr int kSRing = 64;
+constexpr int kSHistory = 16;
+
+// Per-system device state (arrays of max_sys), plus the pipeline gate.
+struct SvCtl {
+  int stall, iter, pad[2];
+  double a_tol, r_tol;
+  double* rz;
+  double* bnorm;
+  double* alpha;
+  double* beta;
+  int* active;
+  int* n_iter;
+  long long* total;
+  double* t;            // time of the last finished step (t += dt, as the reference)
+  long long* steps;     // steps finished
+  int64_t* row0;        // first pool row
+  int64_t* n;           // rows (nodes)
+  uint32_t* bits;       // stimuli on in the next step
+  int* nstim;
+  tc_stimulus* stim;    // TC_POOL_STIM per system
+};
+
+struct SvStatus {
+  int n_iter, iter, stalled, pad;
+  long long seq, pad2;
+};
+
+// Block -> contiguous range of the slice list; blocks of one XCD adjacent.
+struct LRange {
+  int64_t l0, l1;
+};
+__device__ inline LRange block_list(int64_t nl) {
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  int vb = b;
+  if (G % kXcd == 0) vb = (b % kXcd) * (G / kXcd) + b / kXcd;
+  const int64_t per = (nl + G - 1) / G;
+  LRange r;
+  r.l0 = (int64_t)vb * per;
+  r.l1 = r.l0 + per < nl ? r.l0 + per : nl;
+  return r;
+}
+
+__device__ inline bool sv_stalled(const SvCtl* ctl) {
+  __shared__ int h;
+  if (threadIdx.x == 0) h = ctl->stall;
+  __syncthreads();
+  return h != 0;
+}
+
+struct SvPool {
+  const int32_t* list;      // slices of the live systems, system by system
+  int64_t nl;
+  const int32_t* slice_off; // first entry of each slice
+  const int32_t* slice_w;   // padded width of each slice
+  const int32_t* slice_sys;
+  const int32_t* col;       // pool rows
+};
+
+// Fused RHS + PCG start over the slice list (as k_rhs_init).
+template <class T, class XT, class WT, bool LG>
+__global__ void __launch_bounds__(kSB)
+k_sv_rhs_init(SvPool pl, const T* __restrict__ Mv, const T* __restrict__ Kv,
+              const T* __restrict__ Dinv, const WT* __restrict__ w, const XT* __restrict__ x,
+              const T* __restrict__ xp, XT* __restrict__ xnext, T* __restrict__ xpnext,
+              T* __restrict__ r, double coef, double Cm, double tdt, SvCtl* ctl,
+              double* part0) {
+  if (sv_stalled(ctl)) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const LRange lr = block_list(pl.nl);
+  for (int64_t li = lr.l0 + wv; li < lr.l1; li += kSW) {
+    const int32_t sl = pl.list[li];
+    const int sys = pl.slice_sys[sl];
+    const int64_t i = (int64_t)sl * 64 + lane;
+    const bool ok = i - ctl->row0[sys] < ctl->n[sys];
+    const int32_t b = pl.slice_off[sl], wd = pl.slice_w[sl];
+    const int32_t* c = pl.col + b + lane;
+    const T* m = Mv + b + lane;
+    const T* kk = Kv + b + lane;
+    double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
+    for (int j0 = 0; j0 < wd; j0 += kChunk) {
+      int32_t cj[kChunk];
+      T mv[kChunk], kv[kChunk];
+      XT xj[kChunk];
+      WT wj[kChunk];
+      T pj[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const int jj = j0 + u < wd ? j0 + u : wd - 1;
+        cj[u] = c[jj * 64];
+        mv[u] = m[jj * 64];
+        kv[u] = kk[jj * 64];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        xj[u] = x[cj[u]];
+        wj[u] = w[cj[u]];
+        if (LG) pj[u] = xp[cj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        if (j0 + u >= wd) break;
+        double x0;
+        if (LG) x0 = (double)(XT)(2.0 * (double)xj[u] - (double)pj[u]);
+        else x0 = (double)xj[u];
+        const double mj = (double)mv[u], kj = (double)kv[u];
+        am += mj * (double)wj[u];
+        if (sizeof(WT) == 8) ak += kj * (double)xj[u];
+        else ak += kj * (double)(T)xj[u];
+        mx += mj * x0;
+        kx += kj * x0;
+      }
+    }
+    double red0 = 0.0, red1 = 0.0;
+    if (ok) {
+      if (LG) {
+        xnext[i] = (XT)(2.0 * (double)x[i] - (double)xp[i]);
+        xpnext[i] = (T)x[i];
+      }
+      double bi = am - coef * ak;
+      if (sizeof(WT) != 8) bi = (double)(T)bi;
+      const double d = (double)Dinv[i];
+      const T ri = (T)(bi - (Cm * mx + tdt * kx));
+      r[i] = ri;
+      red0 = (double)ri * (d * (double)ri);
+      const double db = d * bi;
+      red1 = db * db;
+    }
+    red0 = wave_sum(red0);
+    red1 = wave_sum(red1);
+    if (lane == 0) {
+      part0[2 * li] = red0;
+      part0[2 * li + 1] = red1;
+    }
+  }
+}
+
+// spmv pass of iteration k for the active systems (as k_pcg_spmv).
+template <class T>
+__global__ void __launch_bounds__(kSB)
+k_sv_spmv(SvPool pl, const T* __restrict__ Av, const T* __restrict__ Dinv,
+          const T* __restrict__ r, const T* __restrict__ pin, T* __restrict__ pout,
+          T* __restrict__ Ap, int k, SvCtl* ctl, double* part1) {
+  if (sv_stalled(ctl)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->iter = k + 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const LRange lr = block_list(pl.nl);
+  for (int64_t li = lr.l0 + wv; li < lr.l1; li += kSW) {
+    const int32_t sl = pl.list[li];
+    const int sys = pl.slice_sys[sl];
+    if (!ctl->active[sys]) continue;
+    const double beta = k == 0 ? 0.0 : ctl->beta[sys];
+    const int64_t i = (int64_t)sl * 64 + lane;
+    const bool ok = i - ctl->row0[sys] < ctl->n[sys];
+    const int32_t b = pl.slice_off[sl], wd = pl.slice_w[sl];
+    const int32_t* c = pl.col + b + lane;
+    const T* a = Av + b + lane;
+    double acc = 0.0;
+    for (int j0 = 0; j0 < wd; j0 += kChunk) {
+      int32_t cj[kChunk];
+      T av[kChunk], pj[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const int jj = j0 + u < wd ? j0 + u : wd - 1;
+        cj[u] = c[jj * 64];
+        av[u] = a[jj * 64];
+      }
+      if (k == 0) {
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) pj[u] = (T)((double)Dinv[cj[u]] * (double)r[cj[u]]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u)
+          pj[u] = (T)(beta * (double)pin[cj[u]] + (double)Dinv[cj[u]] * (double)r[cj[u]]);
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const double t = acc + (double)av[u] * (double)pj[u];
+        acc = j0 + u < wd ? t : acc;
+      }
+    }
+    double red = 0.0;
+    if (ok) {
+      T pi;
+      if (k == 0) pi = (T)((double)Dinv[i] * (double)r[i]);
+      else pi = (T)(beta * (double)pin[i] + (double)Dinv[i] * (double)r[i]);
+      const T api = (T)acc;
+      pout[i] = pi;
+      Ap[i] = api;
+      red = (double)pi * (double)api;
+    }
+    red = wave_sum(red);
+    if (lane == 0) part1[li] = red;
+  }
+}
+
+// update pass of iteration k for the active systems (as k_pcg_update).
+template <class T, class XT>
+__global__ void __launch_bounds__(kSB)
+k_sv_update(SvPool pl, const T* __restrict__ Dinv, const T* __restrict__ p,
+            const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, SvCtl* ctl,
+            double* part2) {
+  if (sv_stalled(ctl)) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const LRange lr = block_list(pl.nl);
+  for (int64_t li = lr.l0 + wv; li < lr.l1; li += kSW) {
+    const int32_t sl = pl.list[li];
+    const int sys = pl.slice_sys[sl];
+    if (!ctl->active[sys]) continue;
+    const double alpha = ctl->alpha[sys];
+    const int64_t i = (int64_t)sl * 64 + lane;
+    double red0 = 0.0, red1 = 0.0;
+    if (i - ctl->row0[sys] < ctl->n[sys]) {
+      const XT xi = (XT)((double)x[i] + alpha * (double)p[i]);
+      const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
+      x[i] = xi;
+      r[i] = ri;
+      const double z = (double)Dinv[i] * (double)ri;
+      red0 = z * z;
+      red1 = (double)ri * z;
+    }
+    red0 = wave_sum(red0);
+    red1 = wave_sum(red1);
+    if (lane == 0) {
+      part2[2 * li] = red0;
+      part2[2 * li + 1] = red1;
+    }
+  }
+}
+
+// Per-system folds, one block per live system (list ranges l0/l1):
+//   kind 0: start of a step; kind 1: alpha; kind 2: stopping test of
+//   iteration k (solver.py:57-74), else beta and rz.
+__global__ void __launch_bounds__(256)
+k_sv_fold(int kind, int k, const int32_t* __restrict__ sys_list, const int32_t* __restrict__ l0s,
+          const int32_t* __restrict__ l1s, SvCtl* ctl, const double* part) {
+  const int s = sys_list[blockIdx.x];
+  __shared__ int go;
+  if (threadIdx.x == 0) go = ctl->stall == 0 && (kind == 0 || ctl->active[s] != 0);
+  __syncthreads();
+  if (!go) return;
+  const int64_t l0 = l0s[blockIdx.x], l1 = l1s[blockIdx.x];
+  const int nv = kind == 1 ? 1 : 2;
+  double v[2] = {0.0, 0.0};
+  for (int64_t li = l0 + threadIdx.x; li < l1; li += blockDim.x) {
+    if (nv == 1) {
+      v[0] += part[li];
+    } else {
+      v[0] += part[2 * li];
+      v[1] += part[2 * li + 1];
+    }
+  }
+  block_sum<2>(v);
+  if (threadIdx.x != 0) return;
+  if (kind == 0) {
+    ctl->rz[s] = v[0];
+    ctl->bnorm[s] = sqrt(v[1]);
+    ctl->active[s] = 1;
+    ctl->n_iter[s] = 0;
+    ctl->beta[s] = 0.0;
+  } else if (kind == 1) {
+    ctl->alpha[s] = ctl->rz[s] / v[0];
+  } else {
+    const double zn = sqrt(v[0]);
+    if (zn < ctl->a_tol || zn / ctl->bnorm[s] < ctl->r_tol) {
+      ctl->active[s] = 0;
+      ctl->n_iter[s] = k;
+    } else {
+      ctl->beta[s] = v[1] / ctl->rz[s];
+      ctl->rz[s] = v[1];
+      ctl->n_iter[s] = k + 1;
+    }
+  }
+}
+
+// One block: end of a launch chunk after k_end iterations.  Systems still
+// iterating below max_iter stall the pipeline; otherwise the step is final:
+// every live system's n_iter sum, clock (t += dt) and next-step stimulus
+// bits advance, and the status goes to pinned host memory.
+__global__ void __launch_bounds__(256)
+k_sv_check(int k_end, int nsys, const int32_t* __restrict__ sys_list, int max_iter, double dt,
+           SvCtl* ctl, SvStatus* status, long long seq) {
+  __shared__ int stall0, any_stall, nmax;
+  if (threadIdx.x == 0) {
+    stall0 = ctl->stall;
+    any_stall = 0;
+    nmax = 0;
+  }
+  __syncthreads();
+  if (stall0) return;
+  for (int q = threadIdx.x; q < nsys; q += blockDim.x) {
+    const int s = sys_list[q];
+    if (ctl->active[s] && k_end < max_iter) atomicOr(&any_stall, 1);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nsys; q += blockDim.x) {
+    const int s = sys_list[q];
+    if (!any_stall) {
+      ctl->active[s] = 0;
+      ctl->total[s] += ctl->n_iter[s];
+      const double t = ctl->t[s] + dt;
+      ctl->t[s] = t;
+      ctl->steps[s] += 1;
+      ctl->bits[s] = stimulus_bits(ctl->stim + (int64_t)s * TC_POOL_STIM, ctl->nstim[s], t + dt);
+    }
+    atomicMax(&nmax, ctl->n_iter[s]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (any_stall) ctl->stall = 1;
+    if (status) {
+      volatile SvStatus* st = status;
+      st->n_iter = nmax;
+      st->iter = ctl->iter;
+      st->stalled = any_stall;
+      __threadfence_system();
+      st->seq = seq;
+      __threadfence_system();
+    }
+  }
+}
+
+// Copy a system's SELL operator and initial state into its pool ranges.
+template <class T, class XT>
+__global__ void k_sv_place_rows(int64_t row0, int64_t n, int64_t nrows, const T* __restrict__ V0,
+                                const T* __restrict__ Dinv_s, const uint32_t* __restrict__ mask_s,
+                                XT* x0, XT* x1, T* xp0, T* xp1, void* w, int w_f64, T* Dinv,
+                                T* r, T* p0, T* p1, T* Ap, T* AT, T* RT, uint32_t* mask) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nrows) return;
+  const int64_t i = row0 + q;
+  const bool ok = q < n;
+  const T v = ok ? V0[q] : T(0);
+  x0[i] = (XT)v;
+  x1[i] = (XT)v;
+  xp0[i] = v;
+  xp1[i] = v;
+  if (w_f64) ((double*)w)[i] = 0.0;
+  else ((T*)w)[i] = T(0);
+  Dinv[i] = ok ? Dinv_s[q] : T(0);
+  r[i] = p0[i] = p1[i] = Ap[i] = T(0);
+  if (AT) {
+    AT[i] = T(-1);
+    RT[i] = T(-1);
+  }
+  mask[i] = (ok && mask_s) ? mask_s[q] : 0u;
+}
+
+template <class T>
+__global__ void k_sv_place_states(int64_t row0, int64_t n, int64_t nrows, int S,
+                                  const T* __restrict__ st_s, T* st, int64_t ss) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nrows) return;
+  for (int k = 0; k < S; ++k) st[k * ss + row0 + q] = q < n ? st_s[k * n + q] : T(0);
+}
+
+template <class T>
+__global__ void k_sv_place_entries(int64_t ne, int64_t entry0, int64_t row0,
+                                   const int32_t* __restrict__ col_s, const T* __restrict__ M_s,
+                                   const T* __restrict__ K_s, const T* __restrict__ A_s,
+                                   int32_t* col, T* M, T* K, T* A) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  col[entry0 + e] = (int32_t)(row0 + col_s[e]);
+  M[entry0 + e] = M_s[e];
+  K[entry0 + e] = K_s[e];
+  A[entry0 + e] = A_s[e];
+}
+
+__global__ void k_sv_place_slices(int64_t ns, int64_t slice0, int64_t entry0, int sys,
+                                  const int32_t* __restrict__ sptr_s, int32_t* slice_off,
+                                  int32_t* slice_w, int32_t* slice_sys) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  slice_off[slice0 + s] = (int32_t)(entry0 + sptr_s[s]);
+  slice_w[slice0 + s] = (sptr_s[s + 1] - sptr_s[s]) / 64;
+  slice_sys[slice0 + s] = sys;
+}
+
+template <class T, class XT>
+__global__ void k_sv_extract(int64_t row0, int64_t n, const XT* __restrict__ x,
+                             const T* __restrict__ AT, const T* __restrict__ RT, T* V_out,
+                             T* AT_out, T* RT_out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  V_out[q] = (T)x[row0 + q];
+  if (AT_out) {
+    AT_out[q] = AT[row0 + q];
+    RT_out[q] = RT[row0 + q];
+  }
+}
+
+// Final LAT/RT test of a retiring system (its last step's solution).
+template <class T, class XT>
+__global__ void k_sv_lat_final(int64_t row0, int64_t n, const XT* __restrict__ x, T* AT, T* RT,
+                               double t) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int64_t i = row0 + q;
+  const T V = (T)x[i];
+  T at = AT[i];
+  if (V > T(0) && at == T(-1)) {
+    at = (T)t;
+    AT[i] = at;
+  }
+  if (at > T(0) && V < T(-70) && RT[i] == T(-1)) RT[i] = (T)t;
+}
+
+// First-fit allocator of [0, cap).
+struct RangeAlloc {
+  std::map<int64_t, int64_t> free_;  // start -> length
+  void reset(int64_t cap) {
+    free_.clear();
+    if (cap > 0) free_[0] = cap;
+  }
+  int64_t take(int64_t len) {
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= len) {
+        const int64_t s = it->first, rest = it->second - len;
+        free_.erase(it);
+        if (rest > 0) free_[s + len] = rest;
+        return s;
+      }
+    }
+    return -1;
+  }
+  void give(int64_t s, int64_t len) {
+    auto it = free_.emplace(s, len).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  int64_t largest() const {
+    int64_t m = 0;
+    for (auto& kv : free_) m = kv.second > m ? kv.second : m;
+    return m;
+  }
+};
+
+struct SysHost {
+  bool live = false;
+  int model = 0;
+  int64_t slice0 = 0, ns = 0, entry0 = 0, ne = 0, n = 0;
+  bool lat = false;
+};
+
+struct SPlan {
+  int slot, K;
+  long long seq;
+};
+
+}  // namespace
+}  // namespace tc
+
+struct tc_serve {
+  tc_serve_desc d;
+  union {
+    tc_ms2v_params ms2v;
+    tc_ttp_params ttp;
+    tc_crn_params crn;
+  } mp[3];
+  tc::SvCtl hctl;       // host copy of the device pointers
+  tc::SvCtl* ctl;       // device
+  double* part[3];      // per list entry: start (2), spmv (1), update (2)
+  int32_t* list;        // device slice list (all live systems)
+  int32_t* mlist[3];    // device slice lists per model
+  int32_t* sys_list;    // device: live systems
+  int32_t* sys_l0;
+  int32_t* sys_l1;
+  int64_t nl, nml[3];
+  int nsys;
+  std::vector<tc::SysHost> sys;
+  tc::RangeAlloc slices, entries;
+  int xc, pc;
+  tc::SvStatus* host_status;
+  tc::SvStatus* dev_view;
+  long long seq;
+  std::deque<int> recent;
+  int64_t steps;
+};
+
+namespace tc {
+namespace {
+
+struct SvLayout {
+  size_t ctl, arrays, part[3], list, mlist[3], sys_list, sys_l0, sys_l1, total;
+};
+
+SvLayout sv_layout(int64_t cap_slices, int max_sys) {
+  SvLayout L;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += align_up(bytes, 256);
+    return o;
+  };
+  const size_t ms = (size_t)max_sys;
+  L.ctl = take(sizeof(SvCtl));
+  // rz bnorm alpha beta t (double), total steps (ll), row0 n (i64), active n_iter nstim (int),
+  // bits (u32), stim
+  L.arrays = take(ms * (5 * 8 + 2 * 8 + 2 * 8 + 3 * 4 + 4) + ms * TC_POOL_STIM * sizeof(tc_stimulus));
+  L.part[0] = take((size_t)cap_slices * 2 * sizeof(double));
+  L.part[1] = take((size_t)cap_slices * sizeof(double));
+  L.part[2] = take((size_t)cap_slices * 2 * sizeof(double));
+  L.list = take((size_t)cap_slices * 4);
+  for (int m = 0; m < 3; ++m) L.mlist[m] = take((size_t)cap_slices * 4);
+  L.sys_list = take(ms * 4);
+  L.sys_l0 = take(ms * 4);
+  L.sys_l1 = take(ms * 4);
+  L.total = off;
+  return L;
+}
+
+int sv_blocks(int64_t nl) {
+  int64_t g = ceil_div(nl, kSW);
+  if (g > kSMaxBlocks) g = kSMaxBlocks;
+  if (g >= kXcd) g -= g % kXcd;
+  return (int)(g < 1 ? 1 : g);
+}
+
+SvPool sv_pool(const tc_serve* S) {
+  SvPool p;
+  p.list = S->list;
+  p.nl = S->nl;
+  p.slice_off = S->d.slice_off;
+  p.slice_w = S->d.slice_w;
+  p.slice_sys = S->d.slice_sys;
+  p.col = S->d.col;
+  return p;
+}
+
+// Rebuild the slice lists and the live-system list (pipeline drained).
+int sv_rebuild(tc_serve* S, hipStream_t s) {
+  std::vector<int32_t> list, ml[3], sl, l0, l1;
+  for (int q = 0; q < (int)S->sys.size(); ++q) {
+    const SysHost& h = S->sys[q];
+    if (!h.live) continue;
+    sl.push_back(q);
+    l0.push_back((int32_t)list.size());
+    for (int64_t k = 0; k < h.ns; ++k) {
+      list.push_back((int32_t)(h.slice0 + k));
+      ml[h.model].push_back((int32_t)(h.slice0 + k));
+    }
+    l1.push_back((int32_t)list.size());
+  }
+  S->nl = (int64_t)list.size();
+  S->nsys = (int)sl.size();
+  for (int m = 0; m < 3; ++m) S->nml[m] = (int64_t)ml[m].size();
+  auto up = [&](int32_t* dst, const std::vector<int32_t>& v) -> int {
+    if (!v.empty())
+      TC_HIP(hipMemcpyAsync(dst, v.data(), v.size() * 4, hipMemcpyHostToDevice, s));
+    return TC_OK;
+  };
+  int rc;
+  if ((rc = up(S->list, list)) || (rc = up(S->sys_list, sl)) || (rc = up(S->sys_l0, l0)) ||
+      (rc = up(S->sys_l1, l1)))
+    return rc;
+  for (int m = 0; m < 3; ++m)
+    if ((rc = up(S->mlist[m], ml[m]))) return rc;
+  TC_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+  return TC_OK;
+}
+
+int sv_ionic(tc_serve* S, int m, hipStream_t s) {
+  const tc_serve_desc& d = S->d;
+  if (S->nml[m] == 0) return TC_OK;
+  tc_ionic_io io{};
+  io.n = S->nml[m] * 64;
+  io.dt = d.dt_ionic;
+  io.dt_rhs = d.dt;
+  io.Cm = d.Cm;
+  io.V = d.x[S->xc];
+  io.v_f64 = (d.x_f64 && d.dtype == TC_F32) ? 1 : 0;
+  io.w_f64 = (d.w_f64 && d.dtype == TC_F32) ? 1 : 0;
+  io.w = d.w;
+  io.stim_mask = d.stim_mask;
+  io.AT = d.AT;
+  io.RT = d.RT;
+  io.gate = &S->ctl->stall;
+  io.pool_slices = S->mlist[m];
+  io.pool_slice_sys = d.slice_sys;
+  io.pool_sys_row0 = S->hctl.row0;
+  io.pool_sys_n = S->hctl.n;
+  io.pool_sys_t = S->hctl.t;
+  io.pool_sys_steps = S->hctl.steps;
+  io.pool_sys_bits = S->hctl.bits;
+  io.pool_sys_stim = S->hctl.stim;
+  io.state_stride = d.cap_slices * 64;
+  switch (m) {
+    case TC_MODEL_MS2V:
+      return ms2v_step(&S->mp[m].ms2v, d.states[m], &io, d.dtype, s, nullptr, nullptr);
+    case TC_MODEL_TTP:
+      return ttp_step(&S->mp[m].ttp, d.states[m], d.tab[m][0], d.tab[m][1], d.tab[m][2], &io,
+                      d.dtype, s, nullptr, nullptr);
+    default:
+      return crn_step(&S->mp[m].crn, d.states[m], d.tab[m][0], d.tab[m][1], d.tab[m][2], &io,
+                      d.dtype, s, nullptr, nullptr);
+  }
+}
+
+template <class T, class XT>
+int sv_iters(tc_serve* S, int k0, int k1, XT* xs, hipStream_t s) {
+  const tc_serve_desc& d = S->d;
+  const SvPool pl = sv_pool(S);
+  const dim3 g(sv_blocks(S->nl)), blk(kSB);
+  T* pp[2] = {(T*)d.p[0], (T*)d.p[1]};
+  for (int k = k0; k < k1; ++k) {
+    T* pin = pp[k & 1];
+    T* pout = pp[(k + 1) & 1];
+    hipLaunchKernelGGL(k_sv_spmv<T>, g, blk, 0, s, pl, (const T*)d.Av, (const T*)d.Dinv,
+                       (const T*)d.r, (const T*)pin, pout, (T*)d.Ap, k, S->ctl, S->part[1]);
+    TC_LAUNCH();
+    hipLaunchKernelGGL(k_sv_fold, dim3(S->nsys), dim3(256), 0, s, 1, k, S->sys_list, S->sys_l0,
+                       S->sys_l1, S->ctl, (const double*)S->part[1]);
+    TC_LAUNCH();
+    hipLaunchKernelGGL((k_sv_update<T, XT>), g, blk, 0, s, pl, (const T*)d.Dinv,
+                       (const T*)pout, (const T*)d.Ap, xs, (T*)d.r, S->ctl, S->part[2]);
+    TC_LAUNCH();
+    hipLaunchKernelGGL(k_sv_fold, dim3(S->nsys), dim3(256), 0, s, 2, k, S->sys_list, S->sys_l0,
+                       S->sys_l1, S->ctl, (const double*)S->part[2]);
+    TC_LAUNCH();
+  }
+  return TC_OK;
+}
+
+int sv_check(tc_serve* S, int k_end, int slot, long long seq, hipStream_t s) {
+  hipLaunchKernelGGL(k_sv_check, dim3(1), dim3(256), 0, s, k_end, S->nsys, S->sys_list,
+                     S->d.max_iter, S->d.dt, S->ctl, S->dev_view + slot, seq);
+  TC_LAUNCH();
+  return TC_OK;
+}
+
+int sv_wait(tc_serve* S, int slot, long long seq) {
+  volatile SvStatus* st = S->host_status + slot;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (long spins = 0; st->seq != seq; ++spins) {
+    if ((spins & 1023) == 1023) {
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(120)) {
+        set_error("serve runtime: no status for record %lld after 120 s", seq);
+        return TC_ERR_STATE;
+      }
+      std::this_thread::yield();
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return TC_OK;
+}
+
+template <class T, class XT, class WT>
+int sv_enqueue(tc_serve* S, SPlan& plan, hipStream_t s) {
+  const tc_serve_desc& d = S->d;
+  const int64_t step = S->steps + 1;
+  const int slot = (int)(step % kSRing);
+  int K = 1;
+  for (int v : S->recent) K = v > K ? v : K;
+  if (K > d.max_iter) K = d.max_iter;
+  for (int m = 0; m < 3; ++m) {
+    int rc = sv_ionic(S, m, s);
+    if (rc) return rc;
+  }
+  const SvPool pl = sv_pool(S);
+  const dim3 g(sv_blocks(S->nl)), blk(kSB);
+  const int xc = S->xc, pc = S->pc;
+  const double coef = (1.0 - d.theta) * d.dt, tdt = d.theta * d.dt;
+  XT* xs;
+  if (d.linear_guess) {
+    hipLaunchKernelGGL((k_sv_rhs_init<T, XT, WT, true>), g, blk, 0, s, pl, (const T*)d.Mv,
+                       (const T*)d.Kv, (const T*)d.Dinv, (const WT*)d.w, (const XT*)d.x[xc],
+                       (const T*)d.xp[pc], (XT*)d.x[1 - xc], (T*)d.xp[1 - pc], (T*)d.r, coef,
+                       d.Cm, tdt, S->ctl, S->part[0]);
+    xs = (XT*)d.x[1 - xc];
+  } else {
+    hipLaunchKernelGGL((k_sv_rhs_init<T, XT, WT, false>), g, blk, 0, s, pl, (const T*)d.Mv,
+                       (const T*)d.Kv, (const T*)d.Dinv, (const WT*)d.w, (const XT*)d.x[xc],
+                       (const T*)nullptr, (XT*)nullptr, (T*)nullptr, (T*)d.r, coef, d.Cm, tdt,
+                       S->ctl, S->part[0]);
+    xs = (XT*)d.x[xc];
+  }
+  TC_LAUNCH();
+  hipLaunchKernelGGL(k_sv_fold, dim3(S->nsys), dim3(256), 0, s, 0, 0, S->sys_list, S->sys_l0,
+                     S->sys_l1, S->ctl, (const double*)S->part[0]);
+  TC_LAUNCH();
+  int rc = sv_iters<T, XT>(S, 0, K, xs, s);
+  if (rc) return rc;
+  plan.seq = ++S->seq;
+  plan.slot = slot;
+  plan.K = K;
+  rc = sv_check(S, K, slot, plan.seq, s);
+  if (rc) return rc;
+  if (d.linear_guess) {
+    S->xc = 1 - xc;
+    S->pc = 1 - pc;
+  }
+  S->steps = step;
+  return TC_OK;
+}
+
+template <class T, class XT>
+int sv_resolve(tc_serve* S, const SPlan& plan, XT* xs, hipStream_t s) {
+  SvStatus* st = S->host_status + plan.slot;
+  int it = plan.K;
+  int chunk = it > 0 ? it : 1;
+  while (st->stalled) {
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, &S->ctl->stall, 0);
+    TC_LAUNCH();
+    const int c = chunk < S->d.max_iter - it ? chunk : S->d.max_iter - it;
+    int rc = sv_iters<T, XT>(S, it, it + c, xs, s);
+    if (rc) return rc;
+    it += c;
+    const long long seq = ++S->seq;
+    rc = sv_check(S, it, plan.slot, seq, s);
+    if (rc) return rc;
+    rc = sv_wait(S, plan.slot, seq);
+    if (rc) return rc;
+    chunk = chunk * 2 < 64 ? chunk * 2 : 64;
+  }
+  return TC_OK;
+}
+
+struct SvHostPos {
+  int64_t steps;
+  int xc, pc;
+};
+
+template <class T, class XT, class WT>
+int sv_advance(tc_serve* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
+  if (S->nsys == 0) {
+    for (int64_t k = 0; k < nsteps; ++k) iters[k] = 0;
+    return TC_OK;
+  }
+  const int depth = S->d.depth < 1 ? 1 : (S->d.depth > kSRing / 2 ? kSRing / 2 : S->d.depth);
+  struct Inflight {
+    SPlan plan;
+    SvHostPos before;
+  };
+  std::deque<Inflight> inflight;
+  int64_t enq = 0, done = 0;
+  while (done < nsteps) {
+    while ((int)inflight.size() < depth && enq < nsteps) {
+      Inflight f;
+      f.before = {S->steps, S->xc, S->pc};
+      int rc = sv_enqueue<T, XT, WT>(S, f.plan, s);
+      if (rc) return rc;
+      inflight.push_back(f);
+      ++enq;
+    }
+    Inflight f = inflight.front();
+    int rc = sv_wait(S, f.plan.slot, f.plan.seq);
+    if (rc) return rc;
+    SvStatus* st = S->host_status + f.plan.slot;
+    if (st->stalled) {
+      // finish this step, drop the later (gated) ones and resume after it
+      XT* xs = (XT*)S->d.x[S->d.linear_guess ? 1 - f.before.xc : f.before.xc];
+      rc = sv_resolve<T, XT>(S, f.plan, xs, s);
+      if (rc) return rc;
+      enq -= (int64_t)inflight.size() - 1;
+      inflight.clear();
+      S->steps = f.before.steps + 1;
+      S->xc = S->d.linear_guess ? 1 - f.before.xc : f.before.xc;
+      S->pc = S->d.linear_guess ? 1 - f.before.pc : f.before.pc;
+    } else {
+      inflight.pop_front();
+    }
+    iters[done] = st->n_iter;
+    S->recent.push_back(st->iter > 0 ? st->iter : 1);
+    if ((int)S->recent.size() > kSHistory) S->recent.pop_front();
+    ++done;
+  }
+  TC_HIP(hipStreamSynchronize(s));
+  return TC_OK;
+}
+
+}  // namespace
+}  // namespace tc
+
+using namespace tc;
+
+extern "C" int tc_serve_workspace_size(int64_t cap_slices, int max_sys, size_t* bytes) {
+  TC_CHECK(bytes && cap_slices > 0 && max_sys > 0, TC_ERR_ARG,
+           "tc_serve_workspace_size: bad arguments");
+  *bytes = sv_layout(cap_slices, max_sys).total;
+  return TC_OK;
+}
+
+extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* stream) {
+  TC_CHECK(out && d && d->cap_slices > 0 && d->cap_entries > 0 && d->max_sys > 0, TC_ERR_ARG,
+           "tc_serve_create: bad arguments");
+  TC_CHECK(d->dtype == TC_F32 || d->dtype == TC_F64, TC_ERR_ARG, "bad dtype %d", d->dtype);
+  TC_CHECK(!d->w_f64 || d->x_f64 || d->dtype == TC_F64, TC_ERR_ARG, "w_f64 needs x_f64");
+  TC_CHECK(d->cap_slices * 64 < (int64_t)INT32_MAX && d->cap_entries < (int64_t)INT32_MAX,
+           TC_ERR_SIZE, "pool too large for int32 indices");
+  TC_CHECK(d->slice_off && d->slice_w && d->slice_sys && d->col && d->Mv && d->Kv && d->Av &&
+               d->Dinv && d->x[0] && d->x[1] && d->xp[0] && d->xp[1] && d->w && d->r &&
+               d->p[0] && d->p[1] && d->Ap && d->stim_mask,
+           TC_ERR_ARG, "tc_serve_create: missing pool array");
+  size_t need = 0;
+  int rc = tc_serve_workspace_size(d->cap_slices, d->max_sys, &need);
+  if (rc) return rc;
+  TC_CHECK(d->ws && d->ws_bytes >= need, TC_ERR_SIZE, "serve workspace too small (%zu < %zu)",
+           d->ws_bytes, need);
+  tc_serve* S = new tc_serve();
+  S->d = *d;
+  for (int m = 0; m < 3; ++m) {
+    if (!d->params[m]) continue;
+    if (m == TC_MODEL_MS2V) S->mp[m].ms2v = *(const tc_ms2v_params*)d->params[m];
+    if (m == TC_MODEL_TTP) S->mp[m].ttp = *(const tc_ttp_params*)d->params[m];
+    if (m == TC_MODEL_CRN) S->mp[m].crn = *(const tc_crn_params*)d->params[m];
+    S->d.params[m] = nullptr;
+  }
+  const SvLayout L = sv_layout(d->cap_slices, d->max_sys);
+  char* b = (char*)d->ws;
+  S->ctl = (SvCtl*)(b + L.ctl);
+  {
+    char* a = b + L.arrays;
+    const size_t ms = (size_t)d->max_sys;
+    SvCtl& h = S->hctl;
+    h.stall = 0;
+    h.iter = 0;
+    h.pad[0] = h.pad[1] = 0;
+    h.a_tol = d->a_tol;
+    h.r_tol = d->r_tol;
+    h.rz = (double*)a; a += ms * 8;
+    h.bnorm = (double*)a; a += ms * 8;
+    h.alpha = (double*)a; a += ms * 8;
+    h.beta = (double*)a; a += ms * 8;
+    h.t = (double*)a; a += ms * 8;
+    h.total = (long long*)a; a += ms * 8;
+    h.steps = (long long*)a; a += ms * 8;
+    h.row0 = (int64_t*)a; a += ms * 8;
+    h.n = (int64_t*)a; a += ms * 8;
+    h.stim = (tc_stimulus*)a; a += ms * TC_POOL_STIM * sizeof(tc_stimulus);
+    h.active = (int*)a; a += ms * 4;
+    h.n_iter = (int*)a; a += ms * 4;
+    h.nstim = (int*)a; a += ms * 4;
+    h.bits = (uint32_t*)a;
+  }
+  for (int k = 0; k < 3; ++k) S->part[k] = (double*)(b + L.part[k]);
+  S->list = (int32_t*)(b + L.list);
+  for (int m = 0; m < 3; ++m) S->mlist[m] = (int32_t*)(b + L.mlist[m]);
+  S->sys_list = (int32_t*)(b + L.sys_list);
+  S->sys_l0 = (int32_t*)(b + L.sys_l0);
+  S->sys_l1 = (int32_t*)(b + L.sys_l1);
+  S->nl = 0;
+  S->nsys = 0;
+  for (int m = 0; m < 3; ++m) S->nml[m] = 0;
+  S->sys.assign(d->max_sys, SysHost());
+  S->slices.reset(d->cap_slices);
+  S->entries.reset(d->cap_entries);
+  S->xc = S->pc = 0;
+  S->seq = 0;
+  S->steps = 0;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = hipHostMalloc((void**)&S->host_status, sizeof(SvStatus) * kSRing,
+                               hipHostMallocMapped);
+  if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&S->dev_view, S->host_status, 0);
+  if (e == hipSuccess) {
+    for (int k = 0; k < kSRing; ++k) S->host_status[k].seq = -1;
+    e = hipMemcpyAsync(S->ctl, &S->hctl, sizeof(SvCtl), hipMemcpyHostToDevice, s);
+  }
+  if (e == hipSuccess) e = hipMemsetAsync(b + L.arrays, 0, L.part[0] - L.arrays, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    set_error("tc_serve_create: %s", hipGetErrorString(e));
+    tc_serve_destroy(S);
+    return TC_ERR_HIP;
+  }
+  *out = S;
+  return TC_OK;
+}
+
+extern "C" int tc_serve_destroy(tc_serve* S) {
+  if (!S) return TC_OK;
+  if (S->host_status) (void)hipHostFree(S->host_status);
+  delete S;
+  return TC_OK;
+}
+
+extern "C" int tc_serve_fits(tc_serve* S, int64_t n_slices, int64_t n_entries, int* fits) {
+  TC_CHECK(S && fits, TC_ERR_ARG, "tc_serve_fits: bad arguments");
+  bool slot = false;
+  for (auto& h : S->sys) slot = slot || !h.live;
+  *fits = slot && S->slices.largest() >= n_slices && S->entries.largest() >= n_entries;
+  return TC_OK;
+}
+
+extern "C" int tc_serve_add(tc_serve* S, const tc_serve_system* y, int* slot_out, void* stream) {
+  TC_CHECK(S && y && slot_out && y->n > 0 && y->slice_ptr && y->col && y->Mv && y->Kv && y->Av &&
+               y->Dinv && y->V0 && y->states,
+           TC_ERR_ARG, "tc_serve_add: bad arguments");
+  TC_CHECK(y->model >= 0 && y->model < 3 && S->d.states[y->model], TC_ERR_ARG,
+           "tc_serve_add: model %d has no state pool", y->model);
+  TC_CHECK(y->n_stim >= 0 && y->n_stim <= TC_POOL_STIM && (y->n_stim == 0 || y->stim_mask),
+           TC_ERR_ARG, "tc_serve_add: at most %d stimuli, with a mask", TC_POOL_STIM);
+  TC_CHECK(!y->lat || S->d.AT, TC_ERR_ARG, "tc_serve_add: LAT needs AT/RT pool arrays");
+  const tc_serve_desc& d = S->d;
+  hipStream_t s = (hipStream_t)stream;
+  TC_HIP(hipStreamSynchronize(s));
+  int slot = -1;
+  for (int q = 0; q < (int)S->sys.size(); ++q)
+    if (!S->sys[q].live) {
+      slot = q;
+      break;
+    }
+  TC_CHECK(slot >= 0, TC_ERR_SIZE, "tc_serve_add: all %d system slots are in use", d.max_sys);
+  const int64_t ns = ceil_div(y->n, 64);
+  const int64_t s0 = S->slices.take(ns);
+  TC_CHECK(s0 >= 0, TC_ERR_SIZE, "tc_serve_add: no room for %lld slices", (long long)ns);
+  const int64_t e0 = S->entries.take(y->n_entries);
+  if (e0 < 0) {
+    S->slices.give(s0, ns);
+    set_error("tc_serve_add: no room for %lld entries", (long long)y->n_entries);
+    return TC_ERR_SIZE;
+  }
+  SysHost& h = S->sys[slot];
+  h.live = true;
+  h.model = y->model;
+  h.slice0 = s0;
+  h.ns = ns;
+  h.entry0 = e0;
+  h.ne = y->n_entries;
+  h.n = y->n;
+  h.lat = y->lat != 0;
+  const int64_t row0 = s0 * 64, nrows = ns * 64;
+  const int f64 = d.dtype == TC_F64;
+  const int xf = d.x_f64 || f64;
+  hipLaunchKernelGGL(k_sv_place_slices, dim3((int)ceil_div(ns, 256)), dim3(256), 0, s, ns, s0, e0,
+                     slot, y->slice_ptr, (int32_t*)d.slice_off, (int32_t*)d.slice_w,
+                     (int32_t*)d.slice_sys);
+  TC_LAUNCH();
+  const dim3 ge((int)ceil_div(y->n_entries, 256)), gr((int)ceil_div(nrows, 256)), b(256);
+  if (f64) {
+    hipLaunchKernelGGL(k_sv_place_entries<double>, ge, b, 0, s, y->n_entries, e0, row0, y->col,
+                       (const double*)y->Mv, (const double*)y->Kv, (const double*)y->Av,
+                       (int32_t*)d.col, (double*)d.Mv, (double*)d.Kv, (double*)d.Av);
+    hipLaunchKernelGGL((k_sv_place_rows<double, double>), gr, b, 0, s, row0, y->n, nrows,
+                       (const double*)y->V0, (const double*)y->Dinv, y->stim_mask,
+                       (double*)d.x[0], (double*)d.x[1], (double*)d.xp[0], (double*)d.xp[1],
+                       d.w, 0, (double*)d.Dinv, (double*)d.r, (double*)d.p[0],
+                       (double*)d.p[1], (double*)d.Ap, (double*)d.AT, (double*)d.RT,
+                       (uint32_t*)d.stim_mask);
+    hipLaunchKernelGGL(k_sv_place_states<double>, gr, b, 0, s, row0, y->n, nrows, y->n_states,
+                       (const double*)y->states, (double*)d.states[y->model], d.cap_slices * 64);
+  } else {
+    hipLaunchKernelGGL(k_sv_place_entries<float>, ge, b, 0, s, y->n_entries, e0, row0, y->col,
+                       (const float*)y->Mv, (const float*)y->Kv, (const float*)y->Av,
+                       (int32_t*)d.col, (float*)d.Mv, (float*)d.Kv, (float*)d.Av);
+    if (xf)
+      hipLaunchKernelGGL((k_sv_place_rows<float, double>), gr, b, 0, s, row0, y->n, nrows,
+                         (const float*)y->V0, (const float*)y->Dinv, y->stim_mask,
+                         (double*)d.x[0], (double*)d.x[1], (float*)d.xp[0], (float*)d.xp[1],
+                         d.w, d.w_f64, (float*)d.Dinv, (float*)d.r, (float*)d.p[0],
+                         (float*)d.p[1], (float*)d.Ap, (float*)d.AT, (float*)d.RT,
+                         (uint32_t*)d.stim_mask);
+    else
+      hipLaunchKernelGGL((k_sv_place_rows<float, float>), gr, b, 0, s, row0, y->n, nrows,
+                         (const float*)y->V0, (const float*)y->Dinv, y->stim_mask,
+                         (float*)d.x[0], (float*)d.x[1], (float*)d.xp[0], (float*)d.xp[1], d.w,
+                         0, (float*)d.Dinv, (float*)d.r, (float*)d.p[0], (float*)d.p[1],
+                         (float*)d.Ap, (float*)d.AT, (float*)d.RT, (uint32_t*)d.stim_mask);
+    hipLaunchKernelGGL(k_sv_place_states<float>, gr, b, 0, s, row0, y->n, nrows, y->n_states,
+                       (const float*)y->states, (float*)d.states[y->model], d.cap_slices * 64);
+  }
+  TC_LAUNCH();
+  // per-system scalars
+  const SvCtl& hc = S->hctl;
+  const double zero = 0.0;
+  const long long zl = 0;
+  const int zi = 0;
+  const uint32_t bits0 = stimulus_bits(y->stim, y->n_stim, d.dt);
+  tc_stimulus st[TC_POOL_STIM] = {};
+  for (int k = 0; k < y->n_stim; ++k) st[k] = y->stim[k];
+  TC_HIP(hipMemcpyAsync(hc.t + slot, &zero, 8, hipMemcpyHostToDevice, s));
+  TC_HIP(hipMemcpyAsync(hc.steps + slot, &zl, 8, hipMemcpyHostToDevice, s));
+  TC_HIP(hipMemcpyAsync(hc.total + slot, &zl, 8, hipMemcpyHostToDevice, s));
+  TC_HIP(hipMemcpyAsync(hc.row0 + slot, &row0, 8, hipMemcpyHostToDevice, s));
+  TC_HIP(hipMemcpyAsync(hc.n + slot, &y->n, 8, hipMemcpyHostToDevice, s));
+  TC_HIP(hipMemcpyAsync(hc.active + slot, &zi, 4, hipMemcpyHostToDevice, s));
+  TC_HIP(hipMemcpyAsync(hc.n_iter + slot, &zi, 4, hipMemcpyHostToDevice, s));
+  TC_HIP(hipMemcpyAsync(hc.nstim + slot, &y->n_stim, 4, hipMemcpyHostToDevice, s));
+  TC_HIP(hipMemcpyAsync(hc.bits + slot, &bits0, 4, hipMemcpyHostToDevice, s));
+  TC_HIP(hipMemcpyAsync(hc.stim + (int64_t)slot * TC_POOL_STIM, st, sizeof(st),
+                        hipMemcpyHostToDevice, s));
+  // the new system starts in the current x / x_prev buffers
+  int rc = sv_rebuild(S, s);
+  if (rc) return rc;
+  *slot_out = slot;
+  return TC_OK;
+}
+
+extern "C" int tc_serve_remove(tc_serve* S, int slot, void* V_out, void* AT_out, void* RT_out,
+                               int64_t* total_iters, int64_t* steps, void* stream) {
+  TC_CHECK(S && slot >= 0 && slot < (int)S->sys.size() && S->sys[slot].live && V_out,
+           TC_ERR_ARG, "tc_serve_remove: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  const tc_serve_desc& d = S->d;
+  SysHost& h = S->sys[slot];
+  TC_HIP(hipStreamSynchronize(s));
+  long long st_steps = 0, tot = 0;
+  double t = 0.0;
+  TC_HIP(hipMemcpy(&st_steps, S->hctl.steps + slot, 8, hipMemcpyDeviceToHost));
+  TC_HIP(hipMemcpy(&tot, S->hctl.total + slot, 8, hipMemcpyDeviceToHost));
+  TC_HIP(hipMemcpy(&t, S->hctl.t + slot, 8, hipMemcpyDeviceToHost));
+  const int64_t row0 = h.slice0 * 64;
+  const dim3 g((int)ceil_div(h.n, 256)), b(256);
+  const void* x = d.x[S->xc];
+  const bool lat = h.lat && AT_out && RT_out;
+  if (d.dtype == TC_F64) {
+    if (lat && st_steps > 0)
+      hipLaunchKernelGGL((k_sv_lat_final<double, double>), g, b, 0, s, row0, h.n,
+                         (const double*)x, (double*)d.AT, (double*)d.RT, t);
+    hipLaunchKernelGGL((k_sv_extract<double, double>), g, b, 0, s, row0, h.n, (const double*)x,
+                       (const double*)d.AT, (const double*)d.RT, (double*)V_out,
+                       lat ? (double*)AT_out : nullptr, lat ? (double*)RT_out : nullptr);
+  } else if (d.x_f64) {
+    if (lat && st_steps > 0)
+      hipLaunchKernelGGL((k_sv_lat_final<float, double>), g, b, 0, s, row0, h.n,
+                         (const double*)x, (float*)d.AT, (float*)d.RT, t);
+    hipLaunchKernelGGL((k_sv_extract<float, double>), g, b, 0, s, row0, h.n, (const double*)x,
+                       (const float*)d.AT, (const float*)d.RT, (float*)V_out,
+                       lat ? (float*)AT_out : nullptr, lat ? (float*)RT_out : nullptr);
+  } else {
+    if (lat && st_steps > 0)
+      hipLaunchKernelGGL((k_sv_lat_final<float, float>), g, b, 0, s, row0, h.n, (const float*)x,
+                         (float*)d.AT, (float*)d.RT, t);
+    hipLaunchKernelGGL((k_sv_extract<float, float>), g, b, 0, s, row0, h.n, (const float*)x,
+                       (const float*)d.AT, (const float*)d.RT, (float*)V_out,
+                       lat ? (float*)AT_out : nullptr, lat ? (float*)RT_out : nullptr);
+  }
+  TC_LAUNCH();
+  if (total_iters) *total_iters = tot;
+  if (steps) *steps = st_steps;
+  S->slices.give(h.slice0, h.ns);
+  S->entries.give(h.entry0, h.ne);
+  h = SysHost();
+  return sv_rebuild(S, s);
+}
+
+extern "C" int tc_serve_advance(tc_serve* S, int64_t nsteps, int32_t* iters, void* stream) {
+  TC_CHECK(S && iters && nsteps >= 0, TC_ERR_ARG, "tc_serve_advance: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  if (S->d.dtype == TC_F64) return sv_advance<double, double, double>(S, nsteps, iters, s);
+  if (S->d.x_f64 && S->d.w_f64) return sv_advance<float, double, double>(S, nsteps, iters, s);
+  if (S->d.x_f64) return sv_advance<float, double, float>(S, nsteps, iters, s);
+  return sv_advance<float, float, float>(S, nsteps, iters, s);
+}

@@ -83,7 +83,7 @@ __device__ inline void block_sum(double (&v)[NV]) {
 // Bit k set when stimulus k is on at time t (stimulation.py:51-63): on iff
 // t/period <= count and start <= t % period <= start + duration, with
 // Python's % on positive doubles equal to fmod.
-inline uint32_t stimulus_bits(const tc_stimulus* st, int n_stim, double t) {
+__host__ __device__ inline uint32_t stimulus_bits(const tc_stimulus* st, int n_stim, double t) {
   uint32_t bits = 0;
   for (int k = 0; k < n_stim; ++k) {
     const tc_stimulus& s = st[k];
