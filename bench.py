@@ -167,6 +167,84 @@ def batched_config(dev, steps, warmup):
             "pcg_iterations_per_step_max": sum(bat.iterations[warmup:]) / steps + 1}
 
 
+def served_config(dev, n_req, budget_nodes=4_000_000):
+    """Served stream: Poisson arrivals (seeded, mean 20 req/s) of runs on
+    cubes of 11..81 nodes a side (log-uniform), ionic model uniform over
+    {MS2v, TTP ENDO, CRN}, T ~ U[50, 300] ms, conductivity scale U[0.5, 1.5],
+    one seeded boundary patch stimulus (100 uA/cm^2, 0..1 ms), float32; a
+    device pool of `budget_nodes` rows is the HBM budget.  The same requests
+    back to back on the single-run path are estimated from each run's own
+    per-step time over its first steps times its step count, in arrival
+    order, assembly included."""
+    import numpy as np
+    import torch
+    from torchcor_amd import meshes
+    from torchcor_amd.serve import MODEL_CLASSES, Request, ServeEngine
+    from torchcor_amd.simulator import Monodomain
+    f32 = torch.float32
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / 20.0, n_req))
+    reqs = []
+    for k in range(n_req):
+        N = int(round(float(np.exp(rng.uniform(np.log(11), np.log(81))))))
+        model = ("ms2v", "ttp", "crn")[int(rng.integers(3))]
+        T = float(rng.uniform(50.0, 300.0))
+        sc = float(rng.uniform(0.5, 1.5))
+        nodes, elems, regions, fib = meshes.cube_workload(N)
+        surf = meshes.cube_surface_nodes(N)
+        patch = meshes.surface_patch(nodes, surf, min(200, len(surf) // 4), rng)
+        cond = [([0], 0.5272 * sc, 0.2076 * sc), ([1], 0.9074 * sc, 0.3332 * sc)]
+        reqs.append(Request(nodes, elems, regions, fib, cond, model, [(patch, 0.0, 1.0, 100.0)], T,
+                            arrival=float(arrivals[k]), tag=k))
+    eng = ServeEngine(cap_nodes=budget_nodes, cap_entries=16 * budget_nodes, max_sys=128, dtype=f32,
+                      device=dev)
+    # warm-up on a tiny run (kernel and table set-up outside the clock)
+    eng.run_until_done([eng.prepare(Request(*meshes.cube_workload(5), reqs[0].conductivities,
+                                            "ttp", [], 0.5))])
+    print(f"[bench] served: {n_req} requests, {sum(r.nodes.shape[0] for r in reqs)} nodes total",
+          flush=True)
+    records, summary = eng.serve(reqs)
+    eng.close()
+    # back to back on the single-run path (estimated per request)
+    est = []
+    for r in reqs:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim = Monodomain(MODEL_CLASSES[r.model](0.01, dev, f32), T=r.T, dt=0.01, device=dev, dtype=f32)
+        sim.load_arrays(r.nodes, r.elems, r.regions, r.fibres)
+        for c in r.conductivities:
+            sim.add_conductivity(*c)
+        sim.add_stimulus(r.stimuli[0][0], *r.stimuli[0][1:])
+        sim.start(1e-5, 1e-5, 100, snapshot_interval=1e9)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        k = min(sim.nt, 300)
+        t0 = time.perf_counter()
+        sim.advance(k)
+        torch.cuda.synchronize()
+        est.append(t_setup + (time.perf_counter() - t0) / k * sim.nt)
+        del sim
+    done, lat = 0.0, []
+    for r, e in zip(reqs, est):
+        done = max(done, r.arrival) + e
+        lat.append((done - r.arrival) / (r.T / 1000.0))
+    b2b_wall = done
+    b2b = {"sims_per_hour": round(n_req * 3600.0 / b2b_wall, 1),
+           "latency_per_sim_s_p50": round(float(np.percentile(lat, 50)), 3),
+           "latency_per_sim_s_p99": round(float(np.percentile(lat, 99)), 3),
+           "wall_s_estimated": round(b2b_wall, 2)}
+    return {"workload": f"{n_req} requests, Poisson 20/s (seed 5), cubes 11..81 log-uniform, "
+                        f"MS2v/TTP/CRN, T U[50,300] ms, f32, pool budget {budget_nodes} rows",
+            "requests": n_req, "wall_s": round(summary["wall_s"], 2),
+            "assembly_s": round(summary["prepare_s"], 2),
+            "sims_per_hour": round(summary["sims_per_hour"], 1),
+            "latency_per_sim_s_p50": round(summary["latency_per_sim_s_p50"], 3),
+            "latency_per_sim_s_p99": round(summary["latency_per_sim_s_p99"], 3),
+            "node_timesteps": int(sum(r.nodes.shape[0] * int(r.T / 0.01) for r in reqs)),
+            "back_to_back_single_runs": b2b,
+            "speedup_vs_back_to_back": round(summary["sims_per_hour"] / b2b["sims_per_hour"], 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +255,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-warmup", type=int, default=200)
     ap.add_argument("--timing-period", type=int, default=16,
@@ -242,6 +322,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_config(dev, args.batch_steps, args.batch_warmup)
+    served = None
+    if not args.no_served:
+        served = served_config(dev, args.served_requests)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(N, mesh, args.cpu_steps)
@@ -260,6 +343,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
         "setup_s": round(t_setup, 2),
     }
     print(json.dumps(line), flush=True)

@@ -43,7 +43,9 @@ constexpr int kSHistory = 16;
 
 // Per-system device state (arrays of max_sys), plus the pipeline gate.
 struct SvCtl {
-  int stall, iter, pad[2];
+  int stall, iter;
+  int n_active;  // systems still iterating in the current step
+  int pad;
   double a_tol, r_tol;
   double* rz;
   double* bnorm;
@@ -82,9 +84,11 @@ __device__ inline LRange block_list(int64_t nl) {
   return r;
 }
 
-__device__ inline bool sv_stalled(const SvCtl* ctl) {
+// Block gate: the pipeline is stalled, or (need_active) every system of the
+// step has stopped.
+__device__ inline bool sv_halted(const SvCtl* ctl, bool need_active) {
   __shared__ int h;
-  if (threadIdx.x == 0) h = ctl->stall;
+  if (threadIdx.x == 0) h = ctl->stall != 0 || (need_active && ctl->n_active == 0);
   __syncthreads();
   return h != 0;
 }
@@ -106,7 +110,8 @@ k_sv_rhs_init(SvPool pl, const T* __restrict__ Mv, const T* __restrict__ Kv,
               const T* __restrict__ xp, XT* __restrict__ xnext, T* __restrict__ xpnext,
               T* __restrict__ r, double coef, double Cm, double tdt, SvCtl* ctl,
               double* part0) {
-  if (sv_stalled(ctl)) return;
+  if (sv_halted(ctl, false)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->n_active = 0;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const LRange lr = block_list(pl.nl);
   for (int64_t li = lr.l0 + wv; li < lr.l1; li += kSW) {
@@ -182,8 +187,7 @@ __global__ void __launch_bounds__(kSB)
 k_sv_spmv(SvPool pl, const T* __restrict__ Av, const T* __restrict__ Dinv,
           const T* __restrict__ r, const T* __restrict__ pin, T* __restrict__ pout,
           T* __restrict__ Ap, int k, SvCtl* ctl, double* part1) {
-  if (sv_stalled(ctl)) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->iter = k + 1;
+  if (sv_halted(ctl, true)) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const LRange lr = block_list(pl.nl);
   for (int64_t li = lr.l0 + wv; li < lr.l1; li += kSW) {
@@ -241,7 +245,7 @@ __global__ void __launch_bounds__(kSB)
 k_sv_update(SvPool pl, const T* __restrict__ Dinv, const T* __restrict__ p,
             const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, SvCtl* ctl,
             double* part2) {
-  if (sv_stalled(ctl)) return;
+  if (sv_halted(ctl, true)) return;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const LRange lr = block_list(pl.nl);
   for (int64_t li = lr.l0 + wv; li < lr.l1; li += kSW) {
@@ -299,6 +303,7 @@ k_sv_fold(int kind, int k, const int32_t* __restrict__ sys_list, const int32_t* 
     ctl->active[s] = 1;
     ctl->n_iter[s] = 0;
     ctl->beta[s] = 0.0;
+    atomicAdd(&ctl->n_active, 1);
   } else if (kind == 1) {
     ctl->alpha[s] = ctl->rz[s] / v[0];
   } else {
@@ -306,6 +311,7 @@ k_sv_fold(int kind, int k, const int32_t* __restrict__ sys_list, const int32_t* 
     if (zn < ctl->a_tol || zn / ctl->bnorm[s] < ctl->r_tol) {
       ctl->active[s] = 0;
       ctl->n_iter[s] = k;
+      atomicSub(&ctl->n_active, 1);
     } else {
       ctl->beta[s] = v[1] / ctl->rz[s];
       ctl->rz[s] = v[1];
@@ -349,10 +355,12 @@ k_sv_check(int k_end, int nsys, const int32_t* __restrict__ sys_list, int max_it
   __syncthreads();
   if (threadIdx.x == 0) {
     if (any_stall) ctl->stall = 1;
+    else ctl->n_active = 0;
     if (status) {
       volatile SvStatus* st = status;
       st->n_iter = nmax;
-      st->iter = ctl->iter;
+      // iterations the slowest system ran (n_iter + 1 when it converged)
+      st->iter = any_stall ? k_end : (nmax + 1 < k_end ? nmax + 1 : k_end);
       st->stalled = any_stall;
       __threadfence_system();
       st->seq = seq;
@@ -866,7 +874,8 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
     SvCtl& h = S->hctl;
     h.stall = 0;
     h.iter = 0;
-    h.pad[0] = h.pad[1] = 0;
+    h.n_active = 0;
+    h.pad = 0;
     h.a_tol = d->a_tol;
     h.r_tol = d->r_tol;
     h.rz = (double*)a; a += ms * 8;

@@ -119,6 +119,7 @@ class ServeEngine:
         self._desc = d
         self.lat = lat
         self.live = {}
+        self.step_iters = []      # largest n_iter among the live runs, per engine step
 
     # ----------------------------------------------------------- requests
     def prepare(self, req):
@@ -205,7 +206,9 @@ class ServeEngine:
         L.call("tc_serve_advance", self._h, k, iters, L.stream_handle(self.device))
         for pr in self.live.values():
             pr.steps_left -= k
-        return list(iters)
+        out = list(iters)
+        self.step_iters.extend(out)
+        return out
 
     # -------------------------------------------------------------- loop
     def run_until_done(self, prepared, max_chunk=200):
@@ -226,7 +229,7 @@ class ServeEngine:
                 out[slots.pop(s)] = self.remove(s)
         return out
 
-    def serve(self, requests, max_chunk=50, log=None):
+    def serve(self, requests, max_chunk=200, log=None):
         """Real-time serving loop: requests become visible at their arrival
         time (seconds after the start); returns per-request records with
         arrival, admission and completion times and the latency per
@@ -236,6 +239,8 @@ class ServeEngine:
         waiting = collections.deque()
         slots = {}
         records = []
+        prepared = None           # assembled head of the queue, waiting for room
+        t_prepare = 0.0
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
         while pending or waiting or self.live:
@@ -245,11 +250,15 @@ class ServeEngine:
             admitted = True
             while waiting and admitted:
                 admitted = False
-                pr = self.prepare(waiting[0])
-                if self.fits(pr):
+                if prepared is None or prepared.req is not waiting[0]:
+                    ta = clock()
+                    prepared = self.prepare(waiting[0])
+                    t_prepare += clock() - ta
+                if self.fits(prepared):
                     waiting.popleft()
-                    slot = self.add(pr)
-                    slots[slot] = dict(req=pr.req, admitted=clock())
+                    slot = self.add(prepared)
+                    slots[slot] = dict(req=prepared.req, admitted=clock())
+                    prepared = None
                     admitted = True
             if not self.live:
                 if waiting:
@@ -274,7 +283,7 @@ class ServeEngine:
                     log(rec)
         wall = clock()
         lat = np.array([r["latency_per_sim_s"] for r in records])
-        summary = dict(requests=len(records), wall_s=wall,
+        summary = dict(requests=len(records), wall_s=wall, prepare_s=t_prepare,
                        sims_per_hour=len(records) * 3600.0 / wall,
                        latency_per_sim_s_p50=float(np.percentile(lat, 50)),
                        latency_per_sim_s_p99=float(np.percentile(lat, 99)))
