@@ -439,6 +439,7 @@ typedef struct tc_serve_desc {
   int32_t* slice_w;          /* (cap_slices) padded width */
   int32_t* slice_sys;        /* (cap_slices) owning system */
   int32_t* col;              /* (cap_entries) pool rows */
+  int16_t* dcol;             /* (cap_entries) col - row, or NULL (int32 only) */
   void* Mv;                  /* (cap_entries) dtype: M, K/Chi, A values */
   void* Kv;
   void* Av;
@@ -465,6 +466,7 @@ typedef struct tc_serve_system {
   int64_t n_entries;         /* padded SELL entries (slice_ptr[ceil(n/64)]) */
   const int32_t* slice_ptr;  /* the system's SELL-64 operator (tc_sell_*) */
   const int32_t* col;        /* its columns (0 .. n-1) */
+  const int16_t* dcol;       /* its col - row as int16 (tc_sell_deltas), or NULL */
   const void* Mv;            /* SELL values of M, K/Chi, A and Dinv, dtype */
   const void* Kv;
   const void* Av;

@@ -40,10 +40,10 @@ def _requests():
     return out
 
 
-def _engine(dtype, cap_nodes=20000):
+def _engine(dtype, cap_nodes=20000, **kw):
     from torchcor_amd.serve import ServeEngine
     return ServeEngine(cap_nodes=cap_nodes, cap_entries=cap_nodes * 16, max_sys=16,
-                       dtype=dtype, device=DEV)
+                       dtype=dtype, device=DEV, **kw)
 
 
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
@@ -83,3 +83,16 @@ def test_serve_matches_single_run_path():
         assert err < 1e-9, (k, err)
         assert torch.equal(sim.activation_time, res[k]["AT"]), k
         assert sum(sim.iterations) == res[k]["iterations"], k
+
+
+def test_serve_int32_columns_identical():
+    """The pool's int16 column offsets and int32 columns give bit-identical
+    runs."""
+    _say("[gpu] serve int16 vs int32 columns")
+    reqs = _requests()[:3]
+    out = []
+    for flag in (True, False):
+        eng = _engine(torch.float32, int16_columns=flag)
+        out.append(eng.run_until_done([eng.prepare(r) for r in reqs]))
+    for a, b in zip(*out):
+        assert torch.equal(a["u"], b["u"]) and a["iterations"] == b["iterations"]

@@ -79,10 +79,6 @@ constexpr int kBChunk = 4;
 // flight together).
 constexpr int kBSeg = 8;
 
-// Wavefront index in the block, as a wave-uniform (scalar) value, so that
-// with 64 lanes the row index and every pattern / shared-value load of the
-// passes are scalar.
-__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 // Row groups (64/lanes rows each, one per wavefront pass) of this block:
 // one contiguous range per block, blocks of one XCD adjacent.

@@ -133,7 +133,8 @@ struct NodeCtx {
 __device__ inline NodeCtx node_ctx(const tc_ionic_io& io, int64_t i) {
   NodeCtx c;
   if (io.pool_slices) {
-    const int64_t sl = io.pool_slices[i >> 6];
+    // one slice per wavefront: a scalar index, so its descriptors load as scalars
+    const int64_t sl = io.pool_slices[__builtin_amdgcn_readfirstlane((int)(i >> 6))];
     const int64_t row = sl * 64 + (i & 63);
     const int sys = io.pool_slice_sys[sl];
     c.vi = row;

@@ -231,7 +231,7 @@ k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
            T* __restrict__ r, PcgCtl* ctl, double a_tol, double r_tol, double* part0) {
   const int64_t ns = (n + 63) >> 6;
   const Split sp = block_slices(ns);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = wave_id();
   double red[2] = {0.0, 0.0};
   for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
@@ -313,7 +313,7 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
   }
   const int64_t ns = (n + 63) >> 6;
   const Split sp = block_slices(ns);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = wave_id();
   double red[1] = {0.0};
   for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
@@ -365,7 +365,7 @@ k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
   const double alpha = ctl->rz[k & 1] / v[0];
   const int64_t ns = (n + 63) >> 6;
   const Split sp = block_slices(ns);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = wave_id();
   double red[2] = {0.0, 0.0};
   for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
@@ -455,7 +455,7 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
   }
   const int64_t ns = (n + 63) >> 6;
   const Split sp = block_slices(ns);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = wave_id();
   double red[2] = {0.0, 0.0};
   for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;

@@ -37,7 +37,6 @@ namespace {
 
 constexpr int kSB = 512;            // threads per block of the passes
 constexpr int kSW = kSB / 64;
-constexpr int kSMaxBlocks = 1024;
 constexpr int kSRing = 64;
 constexpr int kSHistory = 16;
 
@@ -47,7 +46,7 @@ struct SvCtl {
   int n_active;  // systems still iterating in the current step
   int pad;
   double a_tol, r_tol;
-  double* rz;
+  double* rz2;          // r.z per system, two iterations (2 * max_sys)
   double* bnorm;
   double* alpha;
   double* beta;
@@ -60,6 +59,7 @@ struct SvCtl {
   int64_t* n;           // rows (nodes)
   uint32_t* bits;       // stimuli on in the next step
   int* nstim;
+  int* use16;           // the system's entries have int16 column offsets
   tc_stimulus* stim;    // TC_POOL_STIM per system
 };
 
@@ -84,80 +84,232 @@ __device__ inline LRange block_list(int64_t nl) {
   return r;
 }
 
+
+// Blocks never span two systems: a system of s slices gets
+// min(ceil(s / kSPer), kSBlkMax) blocks of equal slice counts, so its
+// partial sums group the same way whatever else is in the pool (a run gives
+// bit-identical results alone or in a mix), and a fold reads at most
+// kSBlkMax partials (one per lane).
+constexpr int kSPer = 8;
+constexpr int kSBlkMax = 1024;
+
+struct SvPool {
+  const int32_t* list;      // slices of the live systems, system by system
+  int64_t nl;
+  const int32_t* blk_l0;    // list range of block v: [blk_l0[v], blk_l0[v+1])
+  const int32_t* blk_q;     // system index of block v (-1: padding block)
+  const int32_t* sys_list;  // slot of system q
+  const int32_t* sys_vb0;   // first and last block of system q
+  const int32_t* sys_vb1;
+  const int32_t* slice_off; // first entry of each slice
+  const int32_t* slice_w;   // padded width of each slice
+  const int32_t* slice_sys;
+  const int32_t* col;       // pool rows
+  const int16_t* dcol;      // col - row, for systems with use16
+};
+
+// Partial sums are per block (one system each) at part[v * nv + k]; system
+// q's total is the sum over its blocks in block order, which every consumer
+// evaluates itself (one wavefront, lanes striding over the blocks).
+struct BlockView {
+  int vb;
+  int64_t l0, l1;  // list range
+  int q;           // system index, -1 for a padding block
+};
+
+__device__ inline BlockView block_view(const SvPool& pl) {
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  BlockView v;
+  v.vb = b;
+  if (G % kXcd == 0) v.vb = (b % kXcd) * (G / kXcd) + b / kXcd;
+  v.l0 = pl.blk_l0[v.vb];
+  v.l1 = pl.blk_l0[v.vb + 1];
+  v.q = pl.blk_q[v.vb];
+  return v;
+}
+
+// Sum of system q's block partials (nv values each): called by a whole
+// wavefront, result in every lane.
+__device__ inline void fold_sys(const SvPool& pl, const double* __restrict__ P, int nv, int q,
+                                double& a, double& b) {
+  const int lane = threadIdx.x & 63;
+  const int v0 = pl.sys_vb0[q], v1 = pl.sys_vb1[q];
+  double s0 = 0.0, s1 = 0.0;
+  for (int v = v0 + lane; v <= v1; v += 64) {
+    s0 += P[(int64_t)v * nv];
+    if (nv == 2) s1 += P[(int64_t)v * nv + 1];
+  }
+  a = wave_sum(s0);
+  b = wave_sum(s1);
+}
+
+// fold_sys with every thread of the block (same order in every block):
+// result in every thread.
+__device__ inline void fold_sys_block(const SvPool& pl, const double* __restrict__ P, int nv,
+                                      int q, double& a, double& b) {
+  __shared__ double fs[kSW][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int v0 = pl.sys_vb0[q], v1 = pl.sys_vb1[q];
+  double s0 = 0.0, s1 = 0.0;
+  for (int v = v0 + (int)threadIdx.x; v <= v1; v += (int)blockDim.x) {
+    s0 += P[(int64_t)v * nv];
+    if (nv == 2) s1 += P[(int64_t)v * nv + 1];
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  if (lane == 0) {
+    fs[wv][0] = s0;
+    fs[wv][1] = s1;
+  }
+  __syncthreads();
+  a = 0.0;
+  b = 0.0;
+  for (int w = 0; w < kSW; ++w) {
+    a += fs[w][0];
+    b += fs[w][1];
+  }
+  __syncthreads();
+}
+
+// Block partials: each lane's sums over the slices of its wavefront, a
+// wavefront sum, then the wavefronts in order.
+__device__ inline void store_block_partials(const BlockView& v, double r0, double r1, int nv,
+                                            double* P) {
+  __shared__ double ws[kSW][2];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  r0 = wave_sum(r0);
+  if (nv == 2) r1 = wave_sum(r1);
+  if (lane == 0) {
+    ws[wv][0] = r0;
+    ws[wv][1] = r1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && v.q >= 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int w = 0; w < kSW; ++w) {
+      s0 += ws[w][0];
+      s1 += ws[w][1];
+    }
+    P[(int64_t)v.vb * nv] = s0;
+    if (nv == 2) P[(int64_t)v.vb * nv + 1] = s1;
+  }
+}
+
 // Block gate: the pipeline is stalled, or (need_active) every system of the
 // step has stopped.
-__device__ inline bool sv_halted(const SvCtl* ctl, bool need_active) {
+__device__ inline bool sv_gate(const SvCtl* ctl, bool need_active) {
   __shared__ int h;
   if (threadIdx.x == 0) h = ctl->stall != 0 || (need_active && ctl->n_active == 0);
   __syncthreads();
   return h != 0;
 }
 
-struct SvPool {
-  const int32_t* list;      // slices of the live systems, system by system
-  int64_t nl;
-  const int32_t* slice_off; // first entry of each slice
-  const int32_t* slice_w;   // padded width of each slice
-  const int32_t* slice_sys;
-  const int32_t* col;       // pool rows
-};
+// One row of the fused RHS + start pass: M w, K u, M x0, K x0 over the
+// row's SELL entries (columns as int32 pool rows or int16 offsets).
+template <class T, class XT, class WT, bool LG, class CI>
+__device__ inline void rhs_row(const CI* __restrict__ c, const T* __restrict__ m,
+                               const T* __restrict__ kk, int wd, int64_t i,
+                               const XT* __restrict__ x, const WT* __restrict__ w,
+                               const T* __restrict__ xp, double& am, double& ak, double& mx,
+                               double& kx) {
+  for (int j0 = 0; j0 < wd; j0 += kChunk) {
+    int32_t cj[kChunk];
+    T mv[kChunk], kv[kChunk];
+    XT xj[kChunk];
+    WT wj[kChunk];
+    T pj[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const int jj = j0 + u < wd ? j0 + u : wd - 1;
+      cj[u] = col_of(c[jj * 64], i);
+      mv[u] = m[jj * 64];
+      kv[u] = kk[jj * 64];
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      xj[u] = x[cj[u]];
+      wj[u] = w[cj[u]];
+      if (LG) pj[u] = xp[cj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      if (j0 + u >= wd) break;
+      double x0;
+      if (LG) x0 = (double)(XT)(2.0 * (double)xj[u] - (double)pj[u]);
+      else x0 = (double)xj[u];
+      const double mj = (double)mv[u], kj = (double)kv[u];
+      am += mj * (double)wj[u];
+      if (sizeof(WT) == 8) ak += kj * (double)xj[u];
+      else ak += kj * (double)(T)xj[u];
+      mx += mj * x0;
+      kx += kj * x0;
+    }
+  }
+}
 
-// Fused RHS + PCG start over the slice list (as k_rhs_init).
+// One row of the spmv pass: (A p')_i with p' formed on the fly.
+template <class T, class CI>
+__device__ inline double spmv_row(const CI* __restrict__ c, const T* __restrict__ a, int wd,
+                                  int64_t i, int k, double beta, const T* __restrict__ Dinv,
+                                  const T* __restrict__ r, const T* __restrict__ pin) {
+  double acc = 0.0;
+  for (int j0 = 0; j0 < wd; j0 += kChunk) {
+    int32_t cj[kChunk];
+    T av[kChunk], pj[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const int jj = j0 + u < wd ? j0 + u : wd - 1;
+      cj[u] = col_of(c[jj * 64], i);
+      av[u] = a[jj * 64];
+    }
+    if (k == 0) {
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) pj[u] = (T)((double)Dinv[cj[u]] * (double)r[cj[u]]);
+    } else {
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u)
+        pj[u] = (T)(beta * (double)pin[cj[u]] + (double)Dinv[cj[u]] * (double)r[cj[u]]);
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const double tt = acc + (double)av[u] * (double)pj[u];
+      acc = j0 + u < wd ? tt : acc;
+    }
+  }
+  return acc;
+}
+
+// Fused RHS + PCG start over the slice list (as k_rhs_init); block partials
+// of r.z and (Dinv b)^2.
 template <class T, class XT, class WT, bool LG>
 __global__ void __launch_bounds__(kSB)
-k_sv_rhs_init(SvPool pl, const T* __restrict__ Mv, const T* __restrict__ Kv,
+k_sv_rhs_init(SvPool pl, int nsys, const T* __restrict__ Mv, const T* __restrict__ Kv,
               const T* __restrict__ Dinv, const WT* __restrict__ w, const XT* __restrict__ x,
               const T* __restrict__ xp, XT* __restrict__ xnext, T* __restrict__ xpnext,
               T* __restrict__ r, double coef, double Cm, double tdt, SvCtl* ctl,
-              double* part0) {
-  if (sv_halted(ctl, false)) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->n_active = 0;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const LRange lr = block_list(pl.nl);
-  for (int64_t li = lr.l0 + wv; li < lr.l1; li += kSW) {
+              double* P0) {
+  if (sv_gate(ctl, false)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->n_active = nsys;
+  const int lane = threadIdx.x & 63, wv = wave_id();
+  const BlockView bv = block_view(pl);
+  double red0 = 0.0, red1 = 0.0;
+  // one system per block: its fields once
+  const int sys = bv.q >= 0 ? pl.sys_list[bv.q] : 0;
+  const int64_t row_end = bv.q >= 0 ? ctl->row0[sys] + ctl->n[sys] : 0;
+  const bool use16 = bv.q >= 0 && ctl->use16[sys];
+  for (int64_t li = bv.l0 + wv; li < bv.l1; li += kSW) {
     const int32_t sl = pl.list[li];
-    const int sys = pl.slice_sys[sl];
     const int64_t i = (int64_t)sl * 64 + lane;
-    const bool ok = i - ctl->row0[sys] < ctl->n[sys];
+    const bool ok = i < row_end;
     const int32_t b = pl.slice_off[sl], wd = pl.slice_w[sl];
-    const int32_t* c = pl.col + b + lane;
     const T* m = Mv + b + lane;
     const T* kk = Kv + b + lane;
     double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
-    for (int j0 = 0; j0 < wd; j0 += kChunk) {
-      int32_t cj[kChunk];
-      T mv[kChunk], kv[kChunk];
-      XT xj[kChunk];
-      WT wj[kChunk];
-      T pj[kChunk];
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        const int jj = j0 + u < wd ? j0 + u : wd - 1;
-        cj[u] = c[jj * 64];
-        mv[u] = m[jj * 64];
-        kv[u] = kk[jj * 64];
-      }
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        xj[u] = x[cj[u]];
-        wj[u] = w[cj[u]];
-        if (LG) pj[u] = xp[cj[u]];
-      }
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        if (j0 + u >= wd) break;
-        double x0;
-        if (LG) x0 = (double)(XT)(2.0 * (double)xj[u] - (double)pj[u]);
-        else x0 = (double)xj[u];
-        const double mj = (double)mv[u], kj = (double)kv[u];
-        am += mj * (double)wj[u];
-        if (sizeof(WT) == 8) ak += kj * (double)xj[u];
-        else ak += kj * (double)(T)xj[u];
-        mx += mj * x0;
-        kx += kj * x0;
-      }
-    }
-    double red0 = 0.0, red1 = 0.0;
+    if (use16)
+      rhs_row<T, XT, WT, LG, int16_t>(pl.dcol + b + lane, m, kk, wd, i, x, w, xp, am, ak, mx, kx);
+    else
+      rhs_row<T, XT, WT, LG, int32_t>(pl.col + b + lane, m, kk, wd, i, x, w, xp, am, ak, mx, kx);
     if (ok) {
       if (LG) {
         xnext[i] = (XT)(2.0 * (double)x[i] - (double)xp[i]);
@@ -168,63 +320,93 @@ k_sv_rhs_init(SvPool pl, const T* __restrict__ Mv, const T* __restrict__ Kv,
       const double d = (double)Dinv[i];
       const T ri = (T)(bi - (Cm * mx + tdt * kx));
       r[i] = ri;
-      red0 = (double)ri * (d * (double)ri);
+      red0 += (double)ri * (d * (double)ri);
       const double db = d * bi;
-      red1 = db * db;
-    }
-    red0 = wave_sum(red0);
-    red1 = wave_sum(red1);
-    if (lane == 0) {
-      part0[2 * li] = red0;
-      part0[2 * li + 1] = red1;
+      red1 += db * db;
     }
   }
+  store_block_partials(bv, red0, red1, 2, P0);
 }
 
-// spmv pass of iteration k for the active systems (as k_pcg_spmv).
+// spmv pass of iteration k.  First the scalars of the block's systems: k = 0
+// takes r.z and ||Dinv b|| from the start pass; k > 0 applies the stopping
+// test of iteration k-1 (solver.py:57-74) to the last update's sums, then
+// beta.  The first block covering a system records the outcome.  Then
+// p' = beta p + Dinv r on the fly, Ap', block partials of p'.Ap'.
 template <class T>
 __global__ void __launch_bounds__(kSB)
 k_sv_spmv(SvPool pl, const T* __restrict__ Av, const T* __restrict__ Dinv,
           const T* __restrict__ r, const T* __restrict__ pin, T* __restrict__ pout,
-          T* __restrict__ Ap, int k, SvCtl* ctl, double* part1) {
-  if (sv_halted(ctl, true)) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const LRange lr = block_list(pl.nl);
-  for (int64_t li = lr.l0 + wv; li < lr.l1; li += kSW) {
-    const int32_t sl = pl.list[li];
-    const int sys = pl.slice_sys[sl];
-    if (!ctl->active[sys]) continue;
-    const double beta = k == 0 ? 0.0 : ctl->beta[sys];
-    const int64_t i = (int64_t)sl * 64 + lane;
-    const bool ok = i - ctl->row0[sys] < ctl->n[sys];
-    const int32_t b = pl.slice_off[sl], wd = pl.slice_w[sl];
-    const int32_t* c = pl.col + b + lane;
-    const T* a = Av + b + lane;
-    double acc = 0.0;
-    for (int j0 = 0; j0 < wd; j0 += kChunk) {
-      int32_t cj[kChunk];
-      T av[kChunk], pj[kChunk];
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        const int jj = j0 + u < wd ? j0 + u : wd - 1;
-        cj[u] = c[jj * 64];
-        av[u] = a[jj * 64];
+          T* __restrict__ Ap, int k, int max_sys, SvCtl* ctl, const double* __restrict__ P0,
+          const double* __restrict__ P2, double* P1) {
+  if (sv_gate(ctl, k > 0)) return;
+  __shared__ double s_beta;
+  __shared__ int s_act;
+  const int lane = threadIdx.x & 63, wv = wave_id();
+  const BlockView bv = block_view(pl);
+  // one read of the flag for the whole block (another block may clear it
+  // during this pass; the branch below must be block-uniform)
+  __shared__ int s_pre;
+  if (threadIdx.x == 0) s_pre = bv.q >= 0 ? ctl->active[pl.sys_list[bv.q]] : 0;
+  __syncthreads();
+  if (bv.q >= 0) {
+    const int q = bv.q;
+    const int s = pl.sys_list[q];
+    const bool first = pl.sys_vb0[q] == bv.vb;
+    double a, b;
+    if (k == 0) {
+      fold_sys_block(pl, P0, 2, q, a, b);
+      if (threadIdx.x == 0) {
+        s_act = 1;
+        s_beta = 0.0;
+        if (first) {
+          ctl->rz2[s] = a;
+          ctl->bnorm[s] = sqrt(b);
+          ctl->active[s] = 1;
+          ctl->n_iter[s] = 0;
+        }
       }
-      if (k == 0) {
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) pj[u] = (T)((double)Dinv[cj[u]] * (double)r[cj[u]]);
-      } else {
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u)
-          pj[u] = (T)(beta * (double)pin[cj[u]] + (double)Dinv[cj[u]] * (double)r[cj[u]]);
-      }
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        const double t = acc + (double)av[u] * (double)pj[u];
-        acc = j0 + u < wd ? t : acc;
+    } else if (!s_pre) {
+      if (threadIdx.x == 0) s_act = 0;
+    } else {
+      fold_sys_block(pl, P2, 2, q, a, b);
+      if (threadIdx.x == 0) {
+        const double zn = sqrt(a);
+        if (zn < ctl->a_tol || zn / ctl->bnorm[s] < ctl->r_tol) {
+          s_act = 0;
+          if (first) {
+            ctl->active[s] = 0;
+            ctl->n_iter[s] = k - 1;
+            atomicSub(&ctl->n_active, 1);
+          }
+        } else {
+          s_act = 1;
+          s_beta = b / ctl->rz2[((k - 1) & 1) * max_sys + s];
+          if (first) {
+            ctl->rz2[(k & 1) * max_sys + s] = b;
+            ctl->n_iter[s] = k;
+          }
+        }
       }
     }
-    double red = 0.0;
+  }
+  __syncthreads();
+  if (bv.q < 0 || !s_act) return;
+  const double beta = s_beta;
+  double red = 0.0;
+  const int sys = pl.sys_list[bv.q];
+  const int64_t row_end = ctl->row0[sys] + ctl->n[sys];
+  const bool use16 = ctl->use16[sys];
+  for (int64_t li = bv.l0 + wv; li < bv.l1; li += kSW) {
+    const int32_t sl = pl.list[li];
+    const int64_t i = (int64_t)sl * 64 + lane;
+    const bool ok = i < row_end;
+    const int32_t b = pl.slice_off[sl], wd = pl.slice_w[sl];
+    const T* a = Av + b + lane;
+    const double acc =
+        use16
+            ? spmv_row<T, int16_t>(pl.dcol + b + lane, a, wd, i, k, beta, Dinv, r, pin)
+            : spmv_row<T, int32_t>(pl.col + b + lane, a, wd, i, k, beta, Dinv, r, pin);
     if (ok) {
       T pi;
       if (k == 0) pi = (T)((double)Dinv[i] * (double)r[i]);
@@ -232,135 +414,118 @@ k_sv_spmv(SvPool pl, const T* __restrict__ Av, const T* __restrict__ Dinv,
       const T api = (T)acc;
       pout[i] = pi;
       Ap[i] = api;
-      red = (double)pi * (double)api;
+      red += (double)pi * (double)api;
     }
-    red = wave_sum(red);
-    if (lane == 0) part1[li] = red;
   }
+  store_block_partials(bv, red, 0.0, 1, P1);
 }
 
-// update pass of iteration k for the active systems (as k_pcg_update).
+// update pass of iteration k: alpha = rz / p'.Ap' per system, x += alpha p',
+// r -= alpha Ap', block partials of z.z and r.z.
 template <class T, class XT>
 __global__ void __launch_bounds__(kSB)
 k_sv_update(SvPool pl, const T* __restrict__ Dinv, const T* __restrict__ p,
-            const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, SvCtl* ctl,
-            double* part2) {
-  if (sv_halted(ctl, true)) return;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const LRange lr = block_list(pl.nl);
-  for (int64_t li = lr.l0 + wv; li < lr.l1; li += kSW) {
+            const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, int k, int max_sys,
+            SvCtl* ctl, const double* __restrict__ P1, double* P2) {
+  if (sv_gate(ctl, true)) return;
+  __shared__ double s_alpha;
+  __shared__ int s_act;
+  const int lane = threadIdx.x & 63, wv = wave_id();
+  const BlockView bv = block_view(pl);
+  __shared__ int s_pre;
+  if (threadIdx.x == 0) s_pre = bv.q >= 0 ? ctl->active[pl.sys_list[bv.q]] : 0;
+  __syncthreads();
+  if (bv.q >= 0) {
+    const int s = pl.sys_list[bv.q];
+    if (!s_pre) {
+      if (threadIdx.x == 0) s_act = 0;
+    } else {
+      double a, b;
+      fold_sys_block(pl, P1, 1, bv.q, a, b);
+      if (threadIdx.x == 0) {
+        s_act = 1;
+        s_alpha = ctl->rz2[(k & 1) * max_sys + s] / a;
+      }
+    }
+  }
+  __syncthreads();
+  if (bv.q < 0 || !s_act) return;
+  const double alpha = s_alpha;
+  double red0 = 0.0, red1 = 0.0;
+  const int sys = pl.sys_list[bv.q];
+  const int64_t row_end = ctl->row0[sys] + ctl->n[sys];
+  for (int64_t li = bv.l0 + wv; li < bv.l1; li += kSW) {
     const int32_t sl = pl.list[li];
-    const int sys = pl.slice_sys[sl];
-    if (!ctl->active[sys]) continue;
-    const double alpha = ctl->alpha[sys];
     const int64_t i = (int64_t)sl * 64 + lane;
-    double red0 = 0.0, red1 = 0.0;
-    if (i - ctl->row0[sys] < ctl->n[sys]) {
+    if (i < row_end) {
       const XT xi = (XT)((double)x[i] + alpha * (double)p[i]);
       const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
       x[i] = xi;
       r[i] = ri;
       const double z = (double)Dinv[i] * (double)ri;
-      red0 = z * z;
-      red1 = (double)ri * z;
-    }
-    red0 = wave_sum(red0);
-    red1 = wave_sum(red1);
-    if (lane == 0) {
-      part2[2 * li] = red0;
-      part2[2 * li + 1] = red1;
+      red0 += z * z;
+      red1 += (double)ri * z;
     }
   }
+  store_block_partials(bv, red0, red1, 2, P2);
 }
 
-// Per-system folds, one block per live system (list ranges l0/l1):
-//   kind 0: start of a step; kind 1: alpha; kind 2: stopping test of
-//   iteration k (solver.py:57-74), else beta and rz.
-__global__ void __launch_bounds__(256)
-k_sv_fold(int kind, int k, const int32_t* __restrict__ sys_list, const int32_t* __restrict__ l0s,
-          const int32_t* __restrict__ l1s, SvCtl* ctl, const double* part) {
-  const int s = sys_list[blockIdx.x];
-  __shared__ int go;
-  if (threadIdx.x == 0) go = ctl->stall == 0 && (kind == 0 || ctl->active[s] != 0);
-  __syncthreads();
-  if (!go) return;
-  const int64_t l0 = l0s[blockIdx.x], l1 = l1s[blockIdx.x];
-  const int nv = kind == 1 ? 1 : 2;
-  double v[2] = {0.0, 0.0};
-  for (int64_t li = l0 + threadIdx.x; li < l1; li += blockDim.x) {
-    if (nv == 1) {
-      v[0] += part[li];
-    } else {
-      v[0] += part[2 * li];
-      v[1] += part[2 * li + 1];
-    }
-  }
-  block_sum<2>(v);
-  if (threadIdx.x != 0) return;
-  if (kind == 0) {
-    ctl->rz[s] = v[0];
-    ctl->bnorm[s] = sqrt(v[1]);
-    ctl->active[s] = 1;
-    ctl->n_iter[s] = 0;
-    ctl->beta[s] = 0.0;
-    atomicAdd(&ctl->n_active, 1);
-  } else if (kind == 1) {
-    ctl->alpha[s] = ctl->rz[s] / v[0];
-  } else {
-    const double zn = sqrt(v[0]);
-    if (zn < ctl->a_tol || zn / ctl->bnorm[s] < ctl->r_tol) {
-      ctl->active[s] = 0;
-      ctl->n_iter[s] = k;
-      atomicSub(&ctl->n_active, 1);
-    } else {
-      ctl->beta[s] = v[1] / ctl->rz[s];
-      ctl->rz[s] = v[1];
-      ctl->n_iter[s] = k + 1;
-    }
-  }
-}
-
-// One block: end of a launch chunk after k_end iterations.  Systems still
+// One block: end of a launch chunk after k_end iterations.  The stopping
+// test of the last iteration for every system still active; systems still
 // iterating below max_iter stall the pipeline; otherwise the step is final:
 // every live system's n_iter sum, clock (t += dt) and next-step stimulus
 // bits advance, and the status goes to pinned host memory.
-__global__ void __launch_bounds__(256)
-k_sv_check(int k_end, int nsys, const int32_t* __restrict__ sys_list, int max_iter, double dt,
-           SvCtl* ctl, SvStatus* status, long long seq) {
-  __shared__ int stall0, any_stall, nmax;
+__global__ void __launch_bounds__(kSB)
+k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
+           const double* __restrict__ P2, SvStatus* status, long long seq) {
+  __shared__ int stall0, any_stall, nmax, imax;
   if (threadIdx.x == 0) {
     stall0 = ctl->stall;
     any_stall = 0;
     nmax = 0;
+    imax = 0;
   }
   __syncthreads();
   if (stall0) return;
-  for (int q = threadIdx.x; q < nsys; q += blockDim.x) {
-    const int s = sys_list[q];
-    if (ctl->active[s] && k_end < max_iter) atomicOr(&any_stall, 1);
+  const int lane = threadIdx.x & 63, wv = wave_id();
+  for (int q = wv; q < nsys; q += kSW) {
+    const int s = pl.sys_list[q];
+    if (!ctl->active[s]) continue;
+    double a, b;
+    fold_sys(pl, P2, 2, q, a, b);
+    if (lane == 0) {
+      const double zn = sqrt(a);
+      if (zn < ctl->a_tol || zn / ctl->bnorm[s] < ctl->r_tol) {
+        ctl->active[s] = 0;
+        ctl->n_iter[s] = k_end - 1;
+      } else {
+        ctl->n_iter[s] = k_end;
+        if (k_end < max_iter) atomicOr(&any_stall, 1);
+        else ctl->active[s] = 0;  // max_iter reached
+      }
+    }
   }
   __syncthreads();
   for (int q = threadIdx.x; q < nsys; q += blockDim.x) {
-    const int s = sys_list[q];
+    const int s = pl.sys_list[q];
+    const int ni = ctl->n_iter[s];
+    atomicMax(&nmax, ni);
+    atomicMax(&imax, ctl->active[s] ? k_end : (ni + 1 < k_end ? ni + 1 : k_end));
     if (!any_stall) {
-      ctl->active[s] = 0;
-      ctl->total[s] += ctl->n_iter[s];
+      ctl->total[s] += ni;
       const double t = ctl->t[s] + dt;
       ctl->t[s] = t;
       ctl->steps[s] += 1;
       ctl->bits[s] = stimulus_bits(ctl->stim + (int64_t)s * TC_POOL_STIM, ctl->nstim[s], t + dt);
     }
-    atomicMax(&nmax, ctl->n_iter[s]);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     if (any_stall) ctl->stall = 1;
-    else ctl->n_active = 0;
     if (status) {
       volatile SvStatus* st = status;
       st->n_iter = nmax;
-      // iterations the slowest system ran (n_iter + 1 when it converged)
-      st->iter = any_stall ? k_end : (nmax + 1 < k_end ? nmax + 1 : k_end);
+      st->iter = imax;
       st->stalled = any_stall;
       __threadfence_system();
       st->seq = seq;
@@ -405,12 +570,14 @@ __global__ void k_sv_place_states(int64_t row0, int64_t n, int64_t nrows, int S,
 
 template <class T>
 __global__ void k_sv_place_entries(int64_t ne, int64_t entry0, int64_t row0,
-                                   const int32_t* __restrict__ col_s, const T* __restrict__ M_s,
+                                   const int32_t* __restrict__ col_s,
+                                   const int16_t* __restrict__ dcol_s, const T* __restrict__ M_s,
                                    const T* __restrict__ K_s, const T* __restrict__ A_s,
-                                   int32_t* col, T* M, T* K, T* A) {
+                                   int32_t* col, int16_t* dcol, T* M, T* K, T* A) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   col[entry0 + e] = (int32_t)(row0 + col_s[e]);
+  if (dcol && dcol_s) dcol[entry0 + e] = dcol_s[e];
   M[entry0 + e] = M_s[e];
   K[entry0 + e] = K_s[e];
   A[entry0 + e] = A_s[e];
@@ -519,14 +686,16 @@ struct tc_serve {
   } mp[3];
   tc::SvCtl hctl;       // host copy of the device pointers
   tc::SvCtl* ctl;       // device
-  double* part[3];      // per list entry: start (2), spmv (1), update (2)
+  double* part[3];      // block partials: start (2), spmv (1), update (2)
   int32_t* list;        // device slice list (all live systems)
   int32_t* mlist[3];    // device slice lists per model
-  int32_t* sys_list;    // device: live systems
-  int32_t* sys_l0;
-  int32_t* sys_l1;
+  int32_t* sys_list;    // device: slot of live system q
+  int32_t* sys_vb0;     // device: blocks of system q
+  int32_t* sys_vb1;
+  int32_t* blk_l0;      // device: list range of each block (G + 1)
+  int32_t* blk_q;       // device: system of each block
   int64_t nl, nml[3];
-  int nsys;
+  int nsys, G;
   std::vector<tc::SysHost> sys;
   tc::RangeAlloc slices, entries;
   int xc, pc;
@@ -541,7 +710,7 @@ namespace tc {
 namespace {
 
 struct SvLayout {
-  size_t ctl, arrays, part[3], list, mlist[3], sys_list, sys_l0, sys_l1, total;
+  size_t ctl, arrays, part[3], list, mlist[3], sys_list, sys_vb0, sys_vb1, blk_l0, blk_q, total;
 };
 
 SvLayout sv_layout(int64_t cap_slices, int max_sys) {
@@ -554,55 +723,74 @@ SvLayout sv_layout(int64_t cap_slices, int max_sys) {
   };
   const size_t ms = (size_t)max_sys;
   L.ctl = take(sizeof(SvCtl));
-  // rz bnorm alpha beta t (double), total steps (ll), row0 n (i64), active n_iter nstim (int),
-  // bits (u32), stim
-  L.arrays = take(ms * (5 * 8 + 2 * 8 + 2 * 8 + 3 * 4 + 4) + ms * TC_POOL_STIM * sizeof(tc_stimulus));
-  L.part[0] = take((size_t)cap_slices * 2 * sizeof(double));
-  L.part[1] = take((size_t)cap_slices * sizeof(double));
-  L.part[2] = take((size_t)cap_slices * 2 * sizeof(double));
+  // rz2 (2 per system) bnorm alpha beta t (double), total steps (ll), row0 n (i64),
+  // active n_iter nstim (int), bits (u32), stim
+  L.arrays = take(ms * (6 * 8 + 2 * 8 + 2 * 8 + 4 * 4 + 4) + ms * TC_POOL_STIM * sizeof(tc_stimulus));
+  // blocks (and partial slots): sum over systems of ceil(slices / kSPer), padded to 8
+  const size_t slots = (size_t)cap_slices / kSPer + ms + 16;
+  L.part[0] = take(slots * 2 * sizeof(double));
+  L.part[1] = take(slots * sizeof(double));
+  L.part[2] = take(slots * 2 * sizeof(double));
   L.list = take((size_t)cap_slices * 4);
   for (int m = 0; m < 3; ++m) L.mlist[m] = take((size_t)cap_slices * 4);
   L.sys_list = take(ms * 4);
-  L.sys_l0 = take(ms * 4);
-  L.sys_l1 = take(ms * 4);
+  L.sys_vb0 = take(ms * 4);
+  L.sys_vb1 = take(ms * 4);
+  L.blk_l0 = take((slots + 1) * 4);
+  L.blk_q = take(slots * 4);
   L.total = off;
   return L;
-}
-
-int sv_blocks(int64_t nl) {
-  int64_t g = ceil_div(nl, kSW);
-  if (g > kSMaxBlocks) g = kSMaxBlocks;
-  if (g >= kXcd) g -= g % kXcd;
-  return (int)(g < 1 ? 1 : g);
 }
 
 SvPool sv_pool(const tc_serve* S) {
   SvPool p;
   p.list = S->list;
   p.nl = S->nl;
+  p.blk_l0 = S->blk_l0;
+  p.blk_q = S->blk_q;
+  p.sys_list = S->sys_list;
+  p.sys_vb0 = S->sys_vb0;
+  p.sys_vb1 = S->sys_vb1;
   p.slice_off = S->d.slice_off;
   p.slice_w = S->d.slice_w;
   p.slice_sys = S->d.slice_sys;
   p.col = S->d.col;
+  p.dcol = S->d.dcol;
   return p;
 }
 
-// Rebuild the slice lists and the live-system list (pipeline drained).
+// Rebuild the slice lists, the blocks (kSPer slices of one system each) and
+// the per-system block ranges (pipeline drained).
 int sv_rebuild(tc_serve* S, hipStream_t s) {
-  std::vector<int32_t> list, ml[3], sl, l0, l1;
+  std::vector<int32_t> list, ml[3], sl, vb0, vb1, bl0, bq;
   for (int q = 0; q < (int)S->sys.size(); ++q) {
     const SysHost& h = S->sys[q];
     if (!h.live) continue;
+    const int qi = (int)sl.size();
     sl.push_back(q);
-    l0.push_back((int32_t)list.size());
+    vb0.push_back((int32_t)bq.size());
+    int64_t nb = ceil_div(h.ns, kSPer);
+    if (nb > kSBlkMax) nb = kSBlkMax;
+    const int64_t per = ceil_div(h.ns, nb);
     for (int64_t k = 0; k < h.ns; ++k) {
+      if (k % per == 0) {
+        bl0.push_back((int32_t)list.size());
+        bq.push_back(qi);
+      }
       list.push_back((int32_t)(h.slice0 + k));
       ml[h.model].push_back((int32_t)(h.slice0 + k));
     }
-    l1.push_back((int32_t)list.size());
+    vb1.push_back((int32_t)bq.size() - 1);
   }
+  // padding blocks (empty) up to a multiple of the XCD count
+  while (bq.empty() || bq.size() % kXcd != 0) {
+    bl0.push_back((int32_t)list.size());
+    bq.push_back(-1);
+  }
+  bl0.push_back((int32_t)list.size());
   S->nl = (int64_t)list.size();
   S->nsys = (int)sl.size();
+  S->G = (int)bq.size();
   for (int m = 0; m < 3; ++m) S->nml[m] = (int64_t)ml[m].size();
   auto up = [&](int32_t* dst, const std::vector<int32_t>& v) -> int {
     if (!v.empty())
@@ -610,8 +798,8 @@ int sv_rebuild(tc_serve* S, hipStream_t s) {
     return TC_OK;
   };
   int rc;
-  if ((rc = up(S->list, list)) || (rc = up(S->sys_list, sl)) || (rc = up(S->sys_l0, l0)) ||
-      (rc = up(S->sys_l1, l1)))
+  if ((rc = up(S->list, list)) || (rc = up(S->sys_list, sl)) || (rc = up(S->sys_vb0, vb0)) ||
+      (rc = up(S->sys_vb1, vb1)) || (rc = up(S->blk_l0, bl0)) || (rc = up(S->blk_q, bq)))
     return rc;
   for (int m = 0; m < 3; ++m)
     if ((rc = up(S->mlist[m], ml[m]))) return rc;
@@ -660,30 +848,27 @@ template <class T, class XT>
 int sv_iters(tc_serve* S, int k0, int k1, XT* xs, hipStream_t s) {
   const tc_serve_desc& d = S->d;
   const SvPool pl = sv_pool(S);
-  const dim3 g(sv_blocks(S->nl)), blk(kSB);
+  const dim3 g(S->G), blk(kSB);
   T* pp[2] = {(T*)d.p[0], (T*)d.p[1]};
   for (int k = k0; k < k1; ++k) {
     T* pin = pp[k & 1];
     T* pout = pp[(k + 1) & 1];
     hipLaunchKernelGGL(k_sv_spmv<T>, g, blk, 0, s, pl, (const T*)d.Av, (const T*)d.Dinv,
-                       (const T*)d.r, (const T*)pin, pout, (T*)d.Ap, k, S->ctl, S->part[1]);
+                       (const T*)d.r, (const T*)pin, pout, (T*)d.Ap, k, d.max_sys, S->ctl,
+                       (const double*)S->part[0], (const double*)S->part[2], S->part[1]);
     TC_LAUNCH();
-    hipLaunchKernelGGL(k_sv_fold, dim3(S->nsys), dim3(256), 0, s, 1, k, S->sys_list, S->sys_l0,
-                       S->sys_l1, S->ctl, (const double*)S->part[1]);
-    TC_LAUNCH();
-    hipLaunchKernelGGL((k_sv_update<T, XT>), g, blk, 0, s, pl, (const T*)d.Dinv,
-                       (const T*)pout, (const T*)d.Ap, xs, (T*)d.r, S->ctl, S->part[2]);
-    TC_LAUNCH();
-    hipLaunchKernelGGL(k_sv_fold, dim3(S->nsys), dim3(256), 0, s, 2, k, S->sys_list, S->sys_l0,
-                       S->sys_l1, S->ctl, (const double*)S->part[2]);
+    hipLaunchKernelGGL((k_sv_update<T, XT>), g, blk, 0, s, pl, (const T*)d.Dinv, (const T*)pout,
+                       (const T*)d.Ap, xs, (T*)d.r, k, d.max_sys, S->ctl,
+                       (const double*)S->part[1], S->part[2]);
     TC_LAUNCH();
   }
   return TC_OK;
 }
 
 int sv_check(tc_serve* S, int k_end, int slot, long long seq, hipStream_t s) {
-  hipLaunchKernelGGL(k_sv_check, dim3(1), dim3(256), 0, s, k_end, S->nsys, S->sys_list,
-                     S->d.max_iter, S->d.dt, S->ctl, S->dev_view + slot, seq);
+  hipLaunchKernelGGL(k_sv_check, dim3(1), dim3(kSB), 0, s, sv_pool(S), k_end, S->nsys,
+                     S->d.max_iter, S->d.dt, S->ctl, (const double*)S->part[2],
+                     S->dev_view + slot, seq);
   TC_LAUNCH();
   return TC_OK;
 }
@@ -717,26 +902,23 @@ int sv_enqueue(tc_serve* S, SPlan& plan, hipStream_t s) {
     if (rc) return rc;
   }
   const SvPool pl = sv_pool(S);
-  const dim3 g(sv_blocks(S->nl)), blk(kSB);
+  const dim3 g(S->G), blk(kSB);
   const int xc = S->xc, pc = S->pc;
   const double coef = (1.0 - d.theta) * d.dt, tdt = d.theta * d.dt;
   XT* xs;
   if (d.linear_guess) {
-    hipLaunchKernelGGL((k_sv_rhs_init<T, XT, WT, true>), g, blk, 0, s, pl, (const T*)d.Mv,
-                       (const T*)d.Kv, (const T*)d.Dinv, (const WT*)d.w, (const XT*)d.x[xc],
-                       (const T*)d.xp[pc], (XT*)d.x[1 - xc], (T*)d.xp[1 - pc], (T*)d.r, coef,
-                       d.Cm, tdt, S->ctl, S->part[0]);
+    hipLaunchKernelGGL((k_sv_rhs_init<T, XT, WT, true>), g, blk, 0, s, pl, S->nsys,
+                       (const T*)d.Mv, (const T*)d.Kv, (const T*)d.Dinv, (const WT*)d.w,
+                       (const XT*)d.x[xc], (const T*)d.xp[pc], (XT*)d.x[1 - xc],
+                       (T*)d.xp[1 - pc], (T*)d.r, coef, d.Cm, tdt, S->ctl, S->part[0]);
     xs = (XT*)d.x[1 - xc];
   } else {
-    hipLaunchKernelGGL((k_sv_rhs_init<T, XT, WT, false>), g, blk, 0, s, pl, (const T*)d.Mv,
-                       (const T*)d.Kv, (const T*)d.Dinv, (const WT*)d.w, (const XT*)d.x[xc],
-                       (const T*)nullptr, (XT*)nullptr, (T*)nullptr, (T*)d.r, coef, d.Cm, tdt,
-                       S->ctl, S->part[0]);
+    hipLaunchKernelGGL((k_sv_rhs_init<T, XT, WT, false>), g, blk, 0, s, pl, S->nsys,
+                       (const T*)d.Mv, (const T*)d.Kv, (const T*)d.Dinv, (const WT*)d.w,
+                       (const XT*)d.x[xc], (const T*)nullptr, (XT*)nullptr, (T*)nullptr,
+                       (T*)d.r, coef, d.Cm, tdt, S->ctl, S->part[0]);
     xs = (XT*)d.x[xc];
   }
-  TC_LAUNCH();
-  hipLaunchKernelGGL(k_sv_fold, dim3(S->nsys), dim3(256), 0, s, 0, 0, S->sys_list, S->sys_l0,
-                     S->sys_l1, S->ctl, (const double*)S->part[0]);
   TC_LAUNCH();
   int rc = sv_iters<T, XT>(S, 0, K, xs, s);
   if (rc) return rc;
@@ -847,6 +1029,7 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
   TC_CHECK(!d->w_f64 || d->x_f64 || d->dtype == TC_F64, TC_ERR_ARG, "w_f64 needs x_f64");
   TC_CHECK(d->cap_slices * 64 < (int64_t)INT32_MAX && d->cap_entries < (int64_t)INT32_MAX,
            TC_ERR_SIZE, "pool too large for int32 indices");
+  TC_CHECK(d->max_iter >= 1, TC_ERR_ARG, "max_iter must be >= 1");
   TC_CHECK(d->slice_off && d->slice_w && d->slice_sys && d->col && d->Mv && d->Kv && d->Av &&
                d->Dinv && d->x[0] && d->x[1] && d->xp[0] && d->xp[1] && d->w && d->r &&
                d->p[0] && d->p[1] && d->Ap && d->stim_mask,
@@ -878,7 +1061,7 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
     h.pad = 0;
     h.a_tol = d->a_tol;
     h.r_tol = d->r_tol;
-    h.rz = (double*)a; a += ms * 8;
+    h.rz2 = (double*)a; a += 2 * ms * 8;
     h.bnorm = (double*)a; a += ms * 8;
     h.alpha = (double*)a; a += ms * 8;
     h.beta = (double*)a; a += ms * 8;
@@ -891,16 +1074,20 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
     h.active = (int*)a; a += ms * 4;
     h.n_iter = (int*)a; a += ms * 4;
     h.nstim = (int*)a; a += ms * 4;
+    h.use16 = (int*)a; a += ms * 4;
     h.bits = (uint32_t*)a;
   }
   for (int k = 0; k < 3; ++k) S->part[k] = (double*)(b + L.part[k]);
   S->list = (int32_t*)(b + L.list);
   for (int m = 0; m < 3; ++m) S->mlist[m] = (int32_t*)(b + L.mlist[m]);
   S->sys_list = (int32_t*)(b + L.sys_list);
-  S->sys_l0 = (int32_t*)(b + L.sys_l0);
-  S->sys_l1 = (int32_t*)(b + L.sys_l1);
+  S->sys_vb0 = (int32_t*)(b + L.sys_vb0);
+  S->sys_vb1 = (int32_t*)(b + L.sys_vb1);
+  S->blk_l0 = (int32_t*)(b + L.blk_l0);
+  S->blk_q = (int32_t*)(b + L.blk_q);
   S->nl = 0;
   S->nsys = 0;
+  S->G = kXcd;
   for (int m = 0; m < 3; ++m) S->nml[m] = 0;
   S->sys.assign(d->max_sys, SysHost());
   S->slices.reset(d->cap_slices);
@@ -989,8 +1176,8 @@ extern "C" int tc_serve_add(tc_serve* S, const tc_serve_system* y, int* slot_out
   const dim3 ge((int)ceil_div(y->n_entries, 256)), gr((int)ceil_div(nrows, 256)), b(256);
   if (f64) {
     hipLaunchKernelGGL(k_sv_place_entries<double>, ge, b, 0, s, y->n_entries, e0, row0, y->col,
-                       (const double*)y->Mv, (const double*)y->Kv, (const double*)y->Av,
-                       (int32_t*)d.col, (double*)d.Mv, (double*)d.Kv, (double*)d.Av);
+                       y->dcol, (const double*)y->Mv, (const double*)y->Kv, (const double*)y->Av,
+                       (int32_t*)d.col, d.dcol, (double*)d.Mv, (double*)d.Kv, (double*)d.Av);
     hipLaunchKernelGGL((k_sv_place_rows<double, double>), gr, b, 0, s, row0, y->n, nrows,
                        (const double*)y->V0, (const double*)y->Dinv, y->stim_mask,
                        (double*)d.x[0], (double*)d.x[1], (double*)d.xp[0], (double*)d.xp[1],
@@ -1001,8 +1188,8 @@ extern "C" int tc_serve_add(tc_serve* S, const tc_serve_system* y, int* slot_out
                        (const double*)y->states, (double*)d.states[y->model], d.cap_slices * 64);
   } else {
     hipLaunchKernelGGL(k_sv_place_entries<float>, ge, b, 0, s, y->n_entries, e0, row0, y->col,
-                       (const float*)y->Mv, (const float*)y->Kv, (const float*)y->Av,
-                       (int32_t*)d.col, (float*)d.Mv, (float*)d.Kv, (float*)d.Av);
+                       y->dcol, (const float*)y->Mv, (const float*)y->Kv, (const float*)y->Av,
+                       (int32_t*)d.col, d.dcol, (float*)d.Mv, (float*)d.Kv, (float*)d.Av);
     if (xf)
       hipLaunchKernelGGL((k_sv_place_rows<float, double>), gr, b, 0, s, row0, y->n, nrows,
                          (const float*)y->V0, (const float*)y->Dinv, y->stim_mask,
@@ -1036,6 +1223,8 @@ extern "C" int tc_serve_add(tc_serve* S, const tc_serve_system* y, int* slot_out
   TC_HIP(hipMemcpyAsync(hc.active + slot, &zi, 4, hipMemcpyHostToDevice, s));
   TC_HIP(hipMemcpyAsync(hc.n_iter + slot, &zi, 4, hipMemcpyHostToDevice, s));
   TC_HIP(hipMemcpyAsync(hc.nstim + slot, &y->n_stim, 4, hipMemcpyHostToDevice, s));
+  const int use16 = (y->dcol && d.dcol) ? 1 : 0;
+  TC_HIP(hipMemcpyAsync(hc.use16 + slot, &use16, 4, hipMemcpyHostToDevice, s));
   TC_HIP(hipMemcpyAsync(hc.bits + slot, &bits0, 4, hipMemcpyHostToDevice, s));
   TC_HIP(hipMemcpyAsync(hc.stim + (int64_t)slot * TC_POOL_STIM, st, sizeof(st),
                         hipMemcpyHostToDevice, s));

@@ -48,6 +48,11 @@ constexpr int kWave = 64;
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Wavefront index in the block as a wave-uniform (scalar) value, so that
+// per-wavefront indices derived from it (slice, row, list entry) and the
+// loads they address stay scalar.
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
 // Wave64 sum of a double with cross-lane shuffles.
 __device__ inline double wave_sum(double v) {
 #pragma unroll
