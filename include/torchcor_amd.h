@@ -252,7 +252,8 @@ int tc_ms2v_step(const tc_ms2v_params* p, void* states, const tc_ionic_io* io, i
 
 /* TenTusscherPanfilov (ionic/TenTusscherPanfilov.py:6-454).  States:
  * CaSR CaSS Cai D F F2 FCaSS H J Ki M Nai R R_ S Xr1 Xr2 Xs.
- * Tables: V (32000 x 27), CaSS (999999 x 3), VEk (32000 x 1). */
+ * Tables: V (32000 x 28: the 27 columns and a zero, 16-byte rows),
+ * CaSS (999999 x 3), VEk (32000 x 1). */
 typedef struct tc_ttp_params {
   int endo;                 /* 1: ENDO tau_S branch (TenTusscherPanfilov.py:258) */
   double GCaL, GKr, GKs, Gto;

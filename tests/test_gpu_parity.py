@@ -202,6 +202,9 @@ def test_tables_float64(name):
     for ga, oa in pairs:
         a = getattr(g, ga).cpu().numpy()
         b = getattr(o, oa)
+        if ga == "V_tab" and name.startswith("ttp"):
+            assert a.shape[1] == 28 and np.all(a[:, 27] == 0)   # padding column
+            a = a[:, :27]
         assert a.shape == b.shape, ga
         fin = np.isfinite(b) & np.isfinite(a)
         assert (np.isfinite(a) == np.isfinite(b)).mean() > 0.999

@@ -98,6 +98,69 @@ struct Row {
   }
 };
 
+// Both interpolation rows of a table with NS (a multiple of 16 bytes) values
+// per row, loaded into registers with 16-byte vector loads.
+template <class T> struct Vec16;
+template <> struct Vec16<float> { typedef float4 type; static constexpr int n = 4; };
+template <> struct Vec16<double> { typedef double2 type; static constexpr int n = 2; };
+
+template <class T, int NS>
+struct RowV {
+  T lo[NS], hi[NS];
+  T w, w1;
+  __device__ inline T operator()(int c, int) const { return w1 * lo[c] + w * hi[c]; }
+};
+
+// Position of the interpolation rows without loading them; load_group(g)
+// then fetches the 16-byte group g (columns g*E .. g*E+E-1) of both rows.
+template <class T, int NS>
+struct RowG : RowV<T, NS> {
+  const T* base;
+  __device__ inline void at(const T* tab, T X, const TabParam& t) {
+    const Lerp<T> L = lerp_at(X, t);
+    base = tab + L.lo * NS;
+    this->w = L.w;
+    this->w1 = T(1) - L.w;
+  }
+  __device__ inline void load_group(int g) {
+    typedef typename Vec16<T>::type V;
+    constexpr int E = Vec16<T>::n;
+    const V* p = reinterpret_cast<const V*>(base);
+    const V a = p[g], b = p[g + NS / E];
+    const T* x = reinterpret_cast<const T*>(&a);
+    const T* y = reinterpret_cast<const T*>(&b);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      this->lo[g * E + e] = x[e];
+      this->hi[g * E + e] = y[e];
+    }
+  }
+};
+
+template <class T, int NS>
+__device__ inline void rowv_at(const T* tab, T X, const TabParam& t, RowV<T, NS>& R) {
+  typedef typename Vec16<T>::type V;
+  constexpr int E = Vec16<T>::n;
+  static_assert(NS % E == 0, "row stride must be a multiple of 16 bytes");
+  Lerp<T> L = lerp_at(X, t);
+  const V* p = reinterpret_cast<const V*>(tab + L.lo * NS);
+  V a[2 * NS / E];
+#pragma unroll
+  for (int k = 0; k < 2 * NS / E; ++k) a[k] = p[k];
+#pragma unroll
+  for (int k = 0; k < NS / E; ++k) {
+    const T* x = reinterpret_cast<const T*>(&a[k]);
+    const T* y = reinterpret_cast<const T*>(&a[k + NS / E]);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      R.lo[k * E + e] = x[e];
+      R.hi[k * E + e] = y[e];
+    }
+  }
+  R.w = L.w;
+  R.w1 = T(1) - L.w;
+}
+
 template <class T>
 __device__ inline Row<T> row_at(const T* tab, int ncol, T X, const TabParam& t) {
   Lerp<T> L = lerp_at(X, t);
@@ -243,6 +306,8 @@ enum {
   TTP_Xr1_A, TTP_Xr1_B, TTP_Xr2_A, TTP_Xr2_B, TTP_Xs_A, TTP_Xs_B, TTP_a2, TTP_rec_iNaK,
   TTP_rec_ipK, TTP_VCOLS
 };
+// row stride of the V table: the 27 columns and one zero, 16-byte rows
+constexpr int TTP_VSTRIDE = 28;
 enum { TS_CaSR, TS_CaSS, TS_Cai, TS_D, TS_F, TS_F2, TS_FCaSS, TS_H, TS_J, TS_Ki, TS_M, TS_Nai,
        TS_R, TS_R_, TS_S, TS_Xr1, TS_Xr2, TS_Xs, TS_N };
 
@@ -339,9 +404,10 @@ __global__ void k_ttp_vtab(tc_ttp_params p, double dt, TabParam t, void* out, in
   c[TTP_a2] = a2;
   c[TTP_rec_iNaK] = 1. / ((1. + 0.1245 * exp((-0.1 * V) * F_RT)) + 0.0353 * exp(-V * F_RT));
   c[TTP_rec_ipK] = 1. / (1. + exp((25. - V) / 5.98));
-  for (int k = 0; k < TTP_VCOLS; ++k) {
-    if (f64) ((double*)out)[i * TTP_VCOLS + k] = c[k];
-    else ((float*)out)[i * TTP_VCOLS + k] = (float)c[k];
+  for (int k = 0; k < TTP_VSTRIDE; ++k) {
+    const double v = k < TTP_VCOLS ? c[k] : 0.0;
+    if (f64) ((double*)out)[i * TTP_VSTRIDE + k] = v;
+    else ((float*)out)[i * TTP_VSTRIDE + k] = (float)v;
   }
 }
 
@@ -373,7 +439,7 @@ __global__ void k_ttp_vektab(TabParam t, void* out, int f64) {
 
 // One TTP step per node (TenTusscherPanfilov.py:353-454).
 template <class T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
       const T* __restrict__ Ctab, const T* __restrict__ Ktab, TtpTabs tb, tc_ionic_io io) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -396,7 +462,19 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
   const T GKr_sqrtKo = (T)(p.GKr * sqrt(p.Ko / 5.4));
   const T a1c = (T)(p.GCaL * p.Fconst * (1. / RTONFd) * 4.);
 
-  const Row<T> vr = row_at<T>(Vtab, TTP_VCOLS, V, tb.V);
+  // Phase 1: currents and concentrations, with the V-table columns they use
+  // (INaCa_A/B, a2, rec_iNaK, rec_ipK) and the CaSS and VEk tables; the
+  // concentration states, FCaSS and the epilogue are written here.
+  RowG<T, TTP_VSTRIDE> vr;
+  vr.at(Vtab, V, tb.V);
+  constexpr int E = Vec16<T>::n;
+  // 16-byte groups holding the phase-1 columns
+  constexpr unsigned ph1 = (1u << (TTP_INaCa_A / E)) | (1u << (TTP_INaCa_B / E)) |
+                           (1u << (TTP_a2 / E)) | (1u << (TTP_rec_iNaK / E)) |
+                           (1u << (TTP_rec_ipK / E));
+#pragma unroll
+  for (int g = 0; g < TTP_VSTRIDE / E; ++g)
+    if (ph1 >> g & 1u) vr.load_group(g);
   const Row<T> cr = row_at<T>(Ctab, 3, s[TS_CaSS], tb.CaSS);
   const T Cai = s[TS_Cai], CaSS = s[TS_CaSS], CaSR = s[TS_CaSR], Ki = s[TS_Ki], Nai = s[TS_Nai];
   const T Eca = (T)(0.5 * RTONFd) * t_log((T)p.Cao / Cai);
@@ -448,14 +526,22 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
                        (-ICaL * inverseVssF2) * (T)p.CAPACITANCE) /
                       (T(1) + ((T)(p.Bufss * p.Kbufss) / kss) / kss);
   const T dt = (T)io.dt;
+  const int64_t ss = c.ss, si = c.si;
+  st[TS_CaSR * ss + si] = CaSR + diff_CaSR * dt;
+  st[TS_CaSS * ss + si] = CaSS + diff_CaSS * dt;
+  st[TS_Cai * ss + si] = Cai + diff_Cai * dt;
+  st[TS_Ki * ss + si] = Ki + diff_Ki * dt;
+  st[TS_Nai * ss + si] = Nai + diff_Nai * dt;
+  st[TS_R_ * ss + si] = R_ + diff_R_ * dt;
+  st[TS_FCaSS * ss + si] = cr(1, 3) + cr(2, 3) * s[TS_FCaSS];
+  finish_node<T>(io, c, V, -Iion);
+  // Phase 2 (no memory access moves across this point): the remaining
+  // V-table columns and the Rush-Larsen gate updates from the old gates.
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int g = 0; g < TTP_VSTRIDE / E; ++g)
+    if (!(ph1 >> g & 1u)) vr.load_group(g);
   T o[TS_N];
-  o[TS_CaSR] = CaSR + diff_CaSR * dt;
-  o[TS_CaSS] = CaSS + diff_CaSS * dt;
-  o[TS_Cai] = Cai + diff_Cai * dt;
-  o[TS_Ki] = Ki + diff_Ki * dt;
-  o[TS_Nai] = Nai + diff_Nai * dt;
-  o[TS_R_] = R_ + diff_R_ * dt;
-  o[TS_FCaSS] = cr(1, 3) + cr(2, 3) * s[TS_FCaSS];
   o[TS_D] = vr(TTP_D_A, TTP_VCOLS) + vr(TTP_D_B, TTP_VCOLS) * s[TS_D];
   o[TS_F] = vr(TTP_F_A, TTP_VCOLS) + vr(TTP_F_B, TTP_VCOLS) * s[TS_F];
   o[TS_F2] = vr(TTP_F2_A, TTP_VCOLS) + vr(TTP_F2_B, TTP_VCOLS) * s[TS_F2];
@@ -467,9 +553,9 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
   o[TS_Xr1] = vr(TTP_Xr1_A, TTP_VCOLS) + vr(TTP_Xr1_B, TTP_VCOLS) * s[TS_Xr1];
   o[TS_Xr2] = vr(TTP_Xr2_A, TTP_VCOLS) + vr(TTP_Xr2_B, TTP_VCOLS) * s[TS_Xr2];
   o[TS_Xs] = vr(TTP_Xs_A, TTP_VCOLS) + vr(TTP_Xs_B, TTP_VCOLS) * s[TS_Xs];
+  const int gates[11] = {TS_D, TS_F, TS_F2, TS_H, TS_J, TS_M, TS_R, TS_S, TS_Xr1, TS_Xr2, TS_Xs};
 #pragma unroll
-  for (int k = 0; k < TS_N; ++k) st[k * c.ss + c.si] = o[k];
-  finish_node<T>(io, c, V, -Iion);
+  for (int k = 0; k < 11; ++k) st[gates[k] * ss + si] = o[gates[k]];
 }
 
 // ---------------------------------------------------------------- CRN

@@ -69,7 +69,9 @@ class TenTusscherPanfilov(IonicModel):
     INIT = dict(CaSR=1.3, CaSS=0.00007, Cai=0.00007, D=0.0, F=1.0, F2=1.0, FCaSS=1.0, H=0.75,
                 J=0.75, Ki=138.3, M=0.0, Nai=7.67, R=0.0, R_=1.0, S=1.0, Xr1=0.0, Xr2=1.0, Xs=0.0)
     V_init = -86.2
-    V_ROWS, CASS_ROWS, VEK_ROWS = (32000, 27), (999999, 3), (32000, 1)
+    # V table rows: the 27 columns of TenTusscherPanfilov.py:197-318 and a zero
+    # (16-byte rows for vector loads)
+    V_ROWS, CASS_ROWS, VEK_ROWS = (32000, 28), (999999, 3), (32000, 1)
 
     def __init__(self, cell_type, dt, device=None, dtype=torch.float32):
         super().__init__(dt, device, dtype)
