@@ -508,7 +508,6 @@ k_bat_check(int k_end, int B, int max_iter, BatCtl* ctl, BatStatus* status, long
       st->stalled = any_stall ? 1 : 0;
       __threadfence_system();
       st->seq = seq;
-      __threadfence_system();
     }
   }
 }

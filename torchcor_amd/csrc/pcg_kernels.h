@@ -425,7 +425,6 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepSta
     st->stalled = stalled;
     __threadfence_system();
     st->seq = seq;
-    __threadfence_system();
   }
 }
 

@@ -475,7 +475,9 @@ k_sv_update(SvPool pl, const T* __restrict__ Dinv, const T* __restrict__ p,
 // iterating below max_iter stall the pipeline; otherwise the step is final:
 // every live system's n_iter sum, clock (t += dt) and next-step stimulus
 // bits advance, and the status goes to pinned host memory.
-__global__ void __launch_bounds__(kSB)
+constexpr int kSCheck = 1024;  // threads of the check pass (one wavefront per system fold)
+
+__global__ void __launch_bounds__(kSCheck)
 k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
            const double* __restrict__ P2, SvStatus* status, long long seq) {
   __shared__ int stall0, any_stall, nmax, imax;
@@ -488,7 +490,7 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
   __syncthreads();
   if (stall0) return;
   const int lane = threadIdx.x & 63, wv = wave_id();
-  for (int q = wv; q < nsys; q += kSW) {
+  for (int q = wv; q < nsys; q += kSCheck / 64) {
     const int s = pl.sys_list[q];
     if (!ctl->active[s]) continue;
     double a, b;
@@ -529,7 +531,6 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
       st->stalled = any_stall;
       __threadfence_system();
       st->seq = seq;
-      __threadfence_system();
     }
   }
 }
@@ -866,7 +867,7 @@ int sv_iters(tc_serve* S, int k0, int k1, XT* xs, hipStream_t s) {
 }
 
 int sv_check(tc_serve* S, int k_end, int slot, long long seq, hipStream_t s) {
-  hipLaunchKernelGGL(k_sv_check, dim3(1), dim3(kSB), 0, s, sv_pool(S), k_end, S->nsys,
+  hipLaunchKernelGGL(k_sv_check, dim3(1), dim3(kSCheck), 0, s, sv_pool(S), k_end, S->nsys,
                      S->d.max_iter, S->d.dt, S->ctl, (const double*)S->part[2],
                      S->dev_view + slot, seq);
   TC_LAUNCH();
