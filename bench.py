@@ -47,6 +47,21 @@ def alg_bytes(n, nnz, S, ib=4, xb=4, wb=4):
     }
 
 
+# HBM bytes per dispatch of each kernel on the default workload (101^3 TTP
+# cube, float32 storage, float64 iterate and right-hand side, int16 column
+# offsets), from two rocprofv3 --pmc runs of this script (FETCH_SIZE, then
+# WRITE_SIZE, median KiB per dispatch) scaled by what the same counters read
+# for a plain 256 MiB device copy (--copy-probe-mib 256): FETCH_SIZE x2.0,
+# WRITE_SIZE x1.0.  profiles/cube101_ttp_step_hbm_traffic.md holds the
+# counter summary and the commands.
+MEASURED_TRAFFIC_101 = {
+    "ionic": 181_304_695,
+    "rhs_init": 211_451_475,
+    "pcg_spmv": 112_080_940,
+    "pcg_update": 37_291_514,
+}
+
+
 def survey_electric_bytes(n, nnz):
     """Electric-phase algorithmic bytes of the unfused formulation
     (B_rhs, B_init, B_iter): the definition behind 'PCG HBM GB/s'."""
@@ -242,6 +257,18 @@ def served_config(dev, n_req, budget_nodes=4_000_000):
             "speedup_vs_back_to_back": round(summary["sims_per_hour"] / b2b["sims_per_hour"], 2)}
 
 
+def copy_probe(mib):
+    """Five copies of `mib` MiB between two device buffers: each reads and
+    writes exactly mib MiB, which scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    a = torch.ones(mib * 2**18, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": a.numel() * 4}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,11 +285,17 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=200)
     ap.add_argument("--timing-period", type=int, default=16,
                     help="time the kernels of every k-th step with HIP events")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 5 plain device copies of this many MiB (the "
+                         "calibration for HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the simulator runs on a single MI355X).", flush=True)
         sys.exit(2)
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
 
     import torch
     dev = torch.device("cuda:0")
@@ -307,8 +340,10 @@ def main():
     if kernels:
         dom = max((k for k in kernels if k in B), key=lambda k: kernels[k]["total_ms"])
         a = kernels[dom]["GBps"]
+        traffic = MEASURED_TRAFFIC_101.get(dom) if N == 101 else None
         roof = {"kernel": dom, "bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(a / HBM_PEAK_GBPS, 4), "traffic": None}
+                "unit": "GB/s", "frac": round(a / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                "alg_bytes": B[dom]}
         b_rhs, b_init, b_iter = survey_electric_bytes(n, nnz)
         n_it = kernels.get("pcg_spmv", {}).get("launches", 0)
         n_st = kernels.get("rhs_init", {}).get("launches", 0)

@@ -11,11 +11,13 @@
 //
 // Passes run over a list of the live systems' slices (system by system): one
 // wavefront per slice, one thread per row, the slice's system giving the
-// PCG scalars, the stopping test and the valid row range.  Each wavefront
-// writes its slice's partial sums; a fold kernel with one block per system
-// adds that system's slices in list order (deterministic).  A system that
-// has met the stopping rule is inactive for the rest of the step.  Ionic
-// kernels take per-model slice lists (pool mode of tc_ionic_io).
+// PCG scalars, the stopping test and the valid row range.  A block never
+// spans two systems; each block writes its partial sums, and every block of
+// the next pass adds its own system's block partials in block order before
+// it starts (no separate fold launch; the grouping depends only on the
+// system's size, so a run is bit-identical alone or in any mix).  A system
+// that has met the stopping rule is inactive for the rest of the step.
+// Ionic kernels take per-model slice lists (pool mode of tc_ionic_io).
 //
 // The time loop is the pipelined one of the single-run runtime: ionic ->
 // RHS + PCG start -> K speculative iterations -> check, several steps ahead;
