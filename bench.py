@@ -182,6 +182,63 @@ def batched_config(dev, steps, warmup):
             "pcg_iterations_per_step_max": sum(bat.iterations[warmup:]) / steps + 1}
 
 
+def irregular_config(dev, steps, warmup, n_target=640_000):
+    """Irregular volume: the thick ellipsoidal shell (semi-axes 40/30/60 mm,
+    wall 10 mm; jittered-lattice Delaunay, about 640k nodes, randomly
+    permuted node numbers), four conductivity regions, transmural fibres,
+    five stimulus patches (0 and 5 ms onsets), TTP ENDO float32.  Runs the
+    same steps in the mesh's own (scattered) order and in the internal RCM
+    order."""
+    import torch
+    from torchcor_amd import meshes
+    from torchcor_amd.ionic import TenTusscherPanfilov
+    from torchcor_amd.simulator import Monodomain
+    t0 = time.perf_counter()
+    nodes, elems, regions, fib, cond, stims = meshes.shell_workload(n_target, seed=0)
+    t_mesh = time.perf_counter() - t0
+    print(f"[bench] shell: {nodes.shape[0]} nodes, {elems.shape[0]} tets in {t_mesh:.1f} s",
+          flush=True)
+    out = {"workload": f"ellipsoidal shell 40/30/60 mm wall 10 mm, {nodes.shape[0]} nodes, "
+                       f"{elems.shape[0]} Delaunay tets, permuted node order, TTP ENDO f32, "
+                       f"4 regions, 5 patches",
+           "n_nodes": int(nodes.shape[0]), "n_tets": int(elems.shape[0]), "mesh_s": round(t_mesh, 1),
+           "steps_timed": steps, "t_window_ms": [warmup * 0.01, (warmup + steps) * 0.01]}
+    for reorder in (None, "rcm"):
+        sim = Monodomain(TenTusscherPanfilov("ENDO", 0.01, device=dev, dtype=torch.float32),
+                         T=(warmup + steps) * 0.01 + 1.0, dt=0.01, device=dev, dtype=torch.float32,
+                         reorder=reorder)
+        sim.load_arrays(nodes, elems, regions, fib)
+        for c in cond:
+            sim.add_conductivity(*c)
+        for ids, start in stims:
+            sim.add_stimulus(ids, start, 1.0, 100)
+        t0 = time.perf_counter()
+        sim.start(1e-5, 1e-5, 100, snapshot_interval=1e9)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
+        sim.advance(warmup)
+        torch.cuda.synchronize()
+        sim.set_kernel_timing(True, period=16)
+        t0 = time.perf_counter()
+        sim.advance(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        kt = {k: round(ms / cnt * 1e3, 2) for k, (ms, cnt) in sim.kernel_times().items() if cnt}
+        key = "rcm" if reorder else "mesh_order"
+        out[key] = {"ms_per_step": round(el / steps * 1e3, 4),
+                    "node_timesteps_per_s": sim.n_nodes * steps / el,
+                    "setup_s": round(t_setup, 2), "nnz": sim.system.nnz,
+                    "bandwidth": sim.system.bandwidth,
+                    "int16_columns": sim.system.dcol is not None,
+                    "sell_padding": round(sim.system.padded / sim.system.nnz - 1, 4),
+                    "kernel_avg_us": kt,
+                    "pcg_iterations_per_step": sum(sim.iterations[warmup:]) / steps + 1}
+        sim.finish()
+        del sim
+    out["speedup_rcm"] = round(out["mesh_order"]["ms_per_step"] / out["rcm"]["ms_per_step"], 2)
+    return out
+
+
 def served_config(dev, n_req, budget_nodes=4_000_000):
     """Served stream: Poisson arrivals (seeded, mean 20 req/s) of runs on
     cubes of 11..81 nodes a side (log-uniform), ionic model uniform over
@@ -281,6 +338,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--served-requests", type=int, default=32)
+    ap.add_argument("--shell", action="store_true",
+                    help="add the irregular ellipsoidal-shell workload (mesh order vs RCM)")
+    ap.add_argument("--shell-steps", type=int, default=1000)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-warmup", type=int, default=200)
     ap.add_argument("--timing-period", type=int, default=16,
@@ -357,6 +417,9 @@ def main():
     served = None
     if not args.no_served:
         served = served_config(dev, args.served_requests)
+    irregular = None
+    if args.shell:
+        irregular = irregular_config(dev, args.shell_steps, 700)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(N, mesh, args.cpu_steps)
@@ -376,6 +439,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "irregular": irregular,
         "setup_s": round(t_setup, 2),
     }
     print(json.dumps(line), flush=True)

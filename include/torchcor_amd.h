@@ -126,6 +126,20 @@ int tc_sell_values(const int32_t* row_ptr, int64_t n, const int32_t* slice_ptr,
  * (one device int32) receives the largest |col - row|, so dcol is usable
  * when it is <= 32767.  Stream-ordered, does not synchronize. */
 int tc_sell_deltas(const tc_sell* A, int16_t* dcol, int32_t* max_abs_dev, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Node reordering (host).
+ * Replaces torchcor.core.reorder.RCM.calculate_rcm_order
+ * (torchcor/core/reorder.py:15-38, scipy's reverse_cuthill_mckee over the
+ * element pattern): a reverse Cuthill-McKee order of the symmetric pattern
+ * row_ptr/col (HOST arrays, n+1 and nnz, diagonal entries allowed), started
+ * per connected component from a pseudo-peripheral node.  order_host[i] is
+ * the old index of the node placed at position i.  *bw_in_host / *bw_out_host
+ * (either may be NULL) receive max |row - col| of the pattern before and
+ * after.  No GPU work.
+ * ---------------------------------------------------------------------- */
+int tc_rcm_order(int64_t n, const int32_t* row_ptr_host, const int32_t* col_host,
+                 int32_t* order_host, int64_t* bw_in_host, int64_t* bw_out_host);
 /* y = A x over a SELL operator (float64 accumulation), for tests and
  * callers of Monodomain.M/K/A products. */
 int tc_sell_spmv(const tc_sell* A, const void* vals, const void* x, void* y, int dtype,

@@ -157,6 +157,7 @@ FUNCS = {
     "tc_sell_columns": (p, p, i64, p, p, p),
     "tc_sell_values": (p, i64, p, p, p, i32, p),
     "tc_sell_deltas": (SELL_P, p, p, p),
+    "tc_rcm_order": (i64, p, p, p, I64_P, I64_P),
     "tc_sell_spmv": (SELL_P, p, p, p, i32, p),
     "tc_rhs": (SELL_P, p, p, p, p, dbl, p, i32, p),
     "tc_pcg_workspace_size": (i64, i32, SIZE_P),

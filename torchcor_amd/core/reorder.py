@@ -1,0 +1,71 @@
+"""Node reordering (torchcor/core/reorder.py:7-48).
+
+``RCM`` keeps the reference's interface (``calculate_rcm_order``, ``reorder``,
+``map``, ``inverse``); the order itself comes from the library's native
+reverse Cuthill-McKee (``tc_rcm_order``, host code), not from scipy.
+``rcm_order`` is the function the simulator uses on an assembled pattern.
+"""
+import ctypes as C
+from itertools import combinations
+
+import numpy as np
+import torch
+
+from .. import _lib as L
+
+
+def rcm_order(row_ptr, col, n):
+    """Reverse Cuthill-McKee order of a symmetric CSR pattern.
+
+    Returns (order, bandwidth_before, bandwidth_after): ``order[i]`` is the
+    old index of the node placed at position i (int64 numpy array)."""
+    rp = np.ascontiguousarray(torch.as_tensor(row_ptr).cpu().numpy(), dtype=np.int32)
+    cl = np.ascontiguousarray(torch.as_tensor(col).cpu().numpy(), dtype=np.int32)
+    n = int(n)
+    if rp.shape[0] != n + 1:
+        raise ValueError("row_ptr must have n + 1 entries")
+    order = np.empty(n, dtype=np.int32)
+    b0, b1 = C.c_int64(0), C.c_int64(0)
+    L.call("tc_rcm_order", n, rp.ctypes.data, cl.ctypes.data, order.ctypes.data, C.byref(b0),
+           C.byref(b1))
+    return order.astype(np.int64), int(b0.value), int(b1.value)
+
+
+def element_pattern(elems, n):
+    """Symmetric CSR pattern (row_ptr, col) of the node pairs of the elements."""
+    e = np.asarray(elems, dtype=np.int64)
+    k = e.shape[1]
+    pairs = [e[:, [a, b]] for a, b in combinations(range(k), 2)]
+    pr = np.concatenate(pairs + [p[:, ::-1] for p in pairs] + [np.stack([np.arange(n)] * 2, 1)])
+    key = np.unique(pr[:, 0] * n + pr[:, 1])
+    rows, cols = key // n, key % n
+    row_ptr = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(row_ptr, rows + 1, 1)
+    return np.cumsum(row_ptr), cols
+
+
+class RCM:
+    def __init__(self, device, dtype):
+        self.rcm_order = None
+        self.rcm_map = None
+        self.inverse_rcm_order = None
+        self.device = device
+        self.dtype = dtype
+
+    def calculate_rcm_order(self, vertices, triangles):
+        n = int(vertices.shape[0])
+        rp, cl = element_pattern(torch.as_tensor(triangles).cpu().numpy(), n)
+        order, _, _ = rcm_order(rp, cl, n)
+        self.rcm_order = torch.as_tensor(order, dtype=torch.long, device=self.device)
+        self.rcm_map = torch.empty_like(self.rcm_order)
+        self.rcm_map[self.rcm_order] = torch.arange(n, dtype=torch.long, device=self.device)
+        self.inverse_rcm_order = self.rcm_map
+
+    def reorder(self, tensor_any):
+        return tensor_any[self.rcm_order]
+
+    def map(self, tensor_long):
+        return self.rcm_map[tensor_long]
+
+    def inverse(self, tensor_any):
+        return tensor_any[self.inverse_rcm_order]

@@ -176,3 +176,120 @@ def surface_of(nodes, elems):
     remap = -np.ones(nodes.shape[0], dtype=np.int64)
     remap[used] = np.arange(used.shape[0])
     return nodes[used], remap[tri].astype(np.int32)
+
+
+# Thick ellipsoidal shell: outer semi-axes (x, y, z) in mm and wall thickness.
+SHELL_OUTER_MM = (40.0, 30.0, 60.0)
+SHELL_WALL_MM = 10.0
+# conductivities (il, it) of the shell's four regions: the cube's two sets and
+# the same sets scaled by 0.5
+SHELL_CONDUCTIVITIES = (((0,), 0.5272, 0.2076), ((1,), 0.9074, 0.3332),
+                        ((2,), 0.5 * 0.5272, 0.5 * 0.2076), ((3,), 0.5 * 0.9074, 0.5 * 0.3332))
+
+
+def shell_depth(p, outer=SHELL_OUTER_MM, wall=SHELL_WALL_MM, iters=40):
+    """Transmural coordinate s of points p (m,3): p lies on the ellipsoid of
+    semi-axes outer - wall*(1 - s) (s = 0 inner surface, 1 outer), found by
+    bisection on s in [-0.5, 1.5]."""
+    A = np.asarray(outer, dtype=np.float64)
+    lo = np.full(p.shape[0], -0.5)
+    hi = np.full(p.shape[0], 1.5)
+    for _ in range(iters):
+        s = 0.5 * (lo + hi)
+        f = ((p / (A - wall * (1.0 - s)[:, None])) ** 2).sum(1) - 1.0
+        lo = np.where(f > 0, s, lo)   # point outside that ellipsoid: s larger
+        hi = np.where(f > 0, hi, s)
+    return 0.5 * (lo + hi)
+
+
+def shell_fibres(c, outer=SHELL_OUTER_MM, wall=SHELL_WALL_MM, theta0_deg=-60.0, span_deg=120.0):
+    """Unit fibres at points c tangent to the shell's ellipsoid layers,
+    rotating transmurally from theta0 (inner) to theta0 + span (outer)
+    between the circumferential and the apex-base directions."""
+    A = np.asarray(outer, dtype=np.float64)
+    s = np.clip(shell_depth(c, outer, wall), 0.0, 1.0)
+    nrm = c / (A - wall * (1.0 - s)[:, None]) ** 2
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    ec = np.cross(np.array([0.0, 0.0, 1.0]), nrm)
+    small = np.linalg.norm(ec, axis=1) < 1e-6           # at the apex / base pole
+    ec[small] = np.cross(np.array([1.0, 0.0, 0.0]), nrm[small])
+    ec /= np.linalg.norm(ec, axis=1, keepdims=True)
+    el = np.cross(nrm, ec)
+    th = np.deg2rad(theta0_deg + span_deg * s)[:, None]
+    f = np.cos(th) * ec + np.sin(th) * el
+    return f / np.linalg.norm(f, axis=1, keepdims=True)
+
+
+def ellipsoid_shell(n_target=640_000, seed=0, outer=SHELL_OUTER_MM, wall=SHELL_WALL_MM,
+                    jitter=0.3, permute=True):
+    """Irregular volume mesh: a thick ellipsoidal shell (outer semi-axes
+    ``outer`` mm, inner ones ``outer - wall``) from a seeded jittered lattice,
+    Delaunay-tetrahedralised (scipy/Qhull), tets kept when their centroid lies
+    in the shell, zero-volume slivers dropped, unused lattice points removed,
+    nodes randomly renumbered (``permute``).  Lattice spacing is set so the
+    mesh has about ``n_target`` nodes.
+
+    Regions by centroid: 2*(z >= 0) + (x >= 0) (conductivities
+    ``SHELL_CONDUCTIVITIES``); fibres from ``shell_fibres`` at the centroids.
+    Returns (nodes (n,3) mm, elems (E,4) int32, regions (E,) int32,
+    fibres (E,3)).
+    """
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    A = np.asarray(outer, dtype=np.float64)
+    R = A - wall
+    volume = 4.0 / 3.0 * np.pi * (np.prod(A) - np.prod(R))
+    h = (1.10 * volume / n_target) ** (1.0 / 3.0)     # 1.10: nodes of boundary tets
+    ax = [np.arange(-A[i] - h, A[i] + h + 1e-9, h) for i in range(3)]
+    P = np.stack(np.meshgrid(*ax, indexing="ij"), axis=-1).reshape(-1, 3)
+    P = P + rng.uniform(-jitter * h, jitter * h, size=P.shape)
+    keep = (((P / (A + h)) ** 2).sum(1) <= 1.0) & (((P / (R - h)) ** 2).sum(1) >= 1.0)
+    P = P[keep]
+    tets = Delaunay(P).simplices.astype(np.int64)
+    X = P[tets]
+    c = X.mean(axis=1)
+    inside = (((c / A) ** 2).sum(1) <= 1.0) & (((c / R) ** 2).sum(1) >= 1.0)
+    vol = np.abs(np.einsum("ij,ij->i", X[:, 1] - X[:, 0],
+                           np.cross(X[:, 2] - X[:, 0], X[:, 3] - X[:, 0]))) / 6
+    sel = inside & (vol > 1e-6 * h ** 3)
+    tets, c = tets[sel], c[sel]
+    used = np.unique(tets)
+    remap = np.full(P.shape[0], -1, dtype=np.int64)
+    remap[used] = np.arange(used.shape[0])
+    P, tets = P[used], remap[tets]
+    if permute:
+        perm = rng.permutation(P.shape[0])        # new index of old node i
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(P.shape[0])
+        P = P[inv]
+        tets = perm[tets]
+    regions = (2 * (c[:, 2] >= 0) + (c[:, 0] >= 0)).astype(np.int32)
+    return P, tets.astype(np.int32), regions, shell_fibres(c, outer, wall)
+
+
+def shell_stimuli(nodes, seed=0, n_patches=5, radius_mm=3.0, outer=SHELL_OUTER_MM,
+                  wall=SHELL_WALL_MM):
+    """Seeded stimulus patches on the inner (endocardial) surface of the
+    shell, alternately apical (z < -0.6 c) and basal (z > 0.6 c): node sets
+    within ``radius_mm`` of a random inner-surface node; onsets 0 ms for the
+    apical patches and 5 ms for the basal ones.  Returns [(ids, start_ms)]."""
+    rng = np.random.default_rng(seed)
+    s = shell_depth(nodes, outer, wall)
+    inner = s < 0.1
+    zc = outer[2]
+    out = []
+    for k in range(n_patches):
+        apical = k % 2 == 0
+        cand = np.nonzero(inner & ((nodes[:, 2] < -0.6 * zc) if apical else (nodes[:, 2] > 0.6 * zc)))[0]
+        centre = nodes[rng.choice(cand)]
+        ids = np.nonzero(np.linalg.norm(nodes - centre, axis=1) <= radius_mm)[0]
+        out.append((ids, 0.0 if apical else 5.0))
+    return out
+
+
+def shell_workload(n_target=640_000, seed=0):
+    """The irregular-volume workload: shell mesh, its four conductivity sets
+    and five stimulus patches.  Returns (nodes, elems, regions, fibres,
+    conductivities, [(ids, start)])."""
+    nodes, elems, regions, fib = ellipsoid_shell(n_target, seed)
+    return nodes, elems, regions, fib, SHELL_CONDUCTIVITIES, shell_stimuli(nodes, seed + 1)

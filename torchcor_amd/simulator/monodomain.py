@@ -11,6 +11,14 @@ fixed sequence of them:
 
 The solution vector is kept in the working dtype and rotated between three
 buffers (x_prev <- x <- x_next) instead of copied.
+
+Node order.  When the mesh's own numbering scatters neighbours (bandwidth of
+the pattern above n/8 and at least twice what a reverse Cuthill-McKee order
+gives), the operators and every per-node array live in the RCM order on the
+device; the mesh arrays, stimulus node
+ids and every output (u, frames, AT, RT, M, K, A) stay in the caller's
+numbering.  ``reorder`` selects this: "auto" (default), "rcm" (always) or
+None (never).
 """
 import time
 from pathlib import Path
@@ -21,13 +29,14 @@ import torch
 from .. import _lib as L
 from ..core.conductivity import Conductivity
 from ..core.mesh import MeshReader
+from ..core.reorder import rcm_order
 from ..core.stimulation import Stimuli
 from ..core.system import P1System
 from ..ionic.base import default_device
 
 
 class Monodomain:
-    def __init__(self, ionic_model, T, dt, device=None, dtype=None):
+    def __init__(self, ionic_model, T, dt, device=None, dtype=None, reorder="auto"):
         self.device = torch.device(device) if device is not None else default_device()
         self.dtype = dtype if dtype is not None else torch.float32
         self.T = T
@@ -46,6 +55,11 @@ class Monodomain:
         self.result_path = None
         self._x = None
         self._sim = None
+        if reorder not in ("auto", "rcm", None):
+            raise ValueError("reorder must be 'auto', 'rcm' or None")
+        self.reorder = reorder
+        self.node_order = None   # internal position i holds mesh node node_order[i]
+        self.node_map = None     # mesh node j sits at internal position node_map[j]
 
     # ------------------------------------------------------------ set-up
     def load_mesh(self, path="Data/ventricle/Case_1", unit_conversion=1000):
@@ -74,9 +88,42 @@ class Monodomain:
     def add_conductivity(self, region_ids, il, it, el=None, et=None):
         self.conductivity.add(region_ids, il, it, el, et)
 
+    def _choose_order(self, sys_):
+        """RCM order for the pattern of ``sys_`` if the mode asks for it."""
+        self.node_order = self.node_map = None
+        if self.reorder is None:
+            return None
+        if self.reorder == "auto":
+            rows = torch.repeat_interleave(torch.arange(sys_.n, device=sys_.device),
+                                           sys_.row_ptr.diff().to(torch.int64))
+            bw0 = int((sys_.col.to(torch.int64) - rows).abs().max().item()) if sys_.nnz else 0
+            if 8 * bw0 <= sys_.n:
+                return None
+        order, bw0, bw1 = rcm_order(sys_.row_ptr, sys_.col, sys_.n)
+        self.bandwidth_natural, self.bandwidth_rcm = bw0, bw1
+        if self.reorder == "auto" and 2 * bw1 > bw0:
+            return None
+        n = sys_.n
+        dev = self.device
+        self.node_order = torch.as_tensor(order, dtype=torch.long, device=dev)
+        self.node_map = torch.empty_like(self.node_order)
+        self.node_map[self.node_order] = torch.arange(n, dtype=torch.long, device=dev)
+        return self.node_order
+
+    def _to_mesh_order(self, v):
+        """A per-node array (last axis) from the internal order to the mesh's."""
+        return v if self.node_map is None else v[..., self.node_map]
+
+    def _to_internal(self, v):
+        return v if self.node_order is None else v[..., self.node_order]
+
     def assemble(self):
         """Pattern, M, K, A and the Jacobi scaling on the GPU (monodomain.py:83-106)."""
+        self._mask_int = None
         sys_ = P1System(self.nodes, self.elems, self.device)
+        if self._choose_order(sys_) is not None:
+            elems = self.node_map[self.elems.to(torch.long)].to(torch.int32)
+            sys_ = P1System(self.nodes[self.node_order], elems, self.device)
         # The reference forms elements from coordinates and fibres held in the
         # working dtype; the same rounded inputs are used here.
         M, K = sys_.assemble(sigma_l=self.conductivity.sigma_l, sigma_t=self.conductivity.sigma_t,
@@ -85,17 +132,32 @@ class Monodomain:
         sys_.set_operators(M, K, self.Chi, self.Cm, self.theta, self.dt, self.dtype)
         self.system = sys_
 
+    def _csr(self, vals):
+        if self.node_order is None:
+            return self.system.csr(vals)
+        # relabel the internal pattern to mesh numbering: the same pattern and,
+        # entry by entry, the same values as assembling in mesh order
+        sys_, n = self.system, self.system.n
+        rows = torch.repeat_interleave(torch.arange(n, device=sys_.device),
+                                       sys_.row_ptr.diff().to(torch.int64))
+        r = self.node_order[rows]
+        c = self.node_order[sys_.col.to(torch.int64)]
+        perm = torch.argsort(r * n + c)
+        crow = torch.zeros(n + 1, dtype=torch.int64, device=sys_.device)
+        crow[1:] = torch.cumsum(torch.bincount(r, minlength=n), 0)
+        return torch.sparse_csr_tensor(crow, c[perm], vals[perm], size=(n, n))
+
     @property
     def M(self):
-        return self.system.csr(self.system.Mv)
+        return self._csr(self.system.Mv)
 
     @property
     def K(self):
-        return self.system.csr(self.system.Kv)
+        return self._csr(self.system.Kv)
 
     @property
     def A(self):
-        return self.system.csr(self.system.Av)
+        return self._csr(self.system.Av)
 
     # ------------------------------------------------------------ stepping
     def _init_solver(self, u, linear_guess=True):
@@ -107,6 +169,13 @@ class Monodomain:
         self._b = torch.empty_like(u)
         self._linear_guess = linear_guess
 
+    def _stim_mask(self):
+        """The stimulus bit mask in the internal node order (cached per assembly)."""
+        if getattr(self, "_mask_int", None) is None:
+            m = self.stimuli.mask
+            self._mask_int = m if self.node_order is None else m[self.node_order].contiguous()
+        return self._mask_int
+
     def _ionic(self, u, t, lat=None):
         stim = self.stimuli
         bits = stim.active(t) if stim is not None else 0
@@ -115,7 +184,7 @@ class Monodomain:
         if lat is not None:
             AT, RT, lat_t = lat
         io = self.ionic_model._io(u, w=self._w, Cm=self.Cm, dt_rhs=self.dt,
-                                  stim_mask=stim.mask if bits else None, stim_bits=bits,
+                                  stim_mask=self._stim_mask() if bits else None, stim_bits=bits,
                                   stim_amp=stim.amplitudes() if bits else (), AT=AT, RT=RT,
                                   lat_t=lat_t)
         self.ionic_model.step(io)
@@ -130,10 +199,17 @@ class Monodomain:
         return self._x, n_iter
 
     def step(self, u, t, a_tol, r_tol, max_iter, verbose=False, _lat=None):
-        """One time step (monodomain.py:109-139): returns (u, n_iter, ionic_s, electric_s)."""
+        """One time step (monodomain.py:109-139): returns (u, n_iter, ionic_s, electric_s).
+        ``u`` is in mesh numbering; the returned u too."""
+        if self.system is None:
+            self.assemble()
         if self._x is None:
-            self._init_solver(u)
-        if u.data_ptr() != self._x.data_ptr():
+            self._init_solver(self._to_internal(u).contiguous())
+        if self.node_order is not None:
+            if _lat is not None:
+                raise NotImplementedError("LAT arrays in step() need the mesh's own node order")
+            self._x.copy_(self._to_internal(u))
+        elif u.data_ptr() != self._x.data_ptr():
             self._x.copy_(u)
         ionic_time = electric_time = 0.0
         if verbose:
@@ -148,7 +224,7 @@ class Monodomain:
         if verbose:
             torch.cuda.synchronize(self.device)
             electric_time = time.time() - t0
-        return u, n_iter, ionic_time, electric_time
+        return self._to_mesh_order(u), n_iter, ionic_time, electric_time
 
     def solve(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True,
               snapshot_interval=1, verbose=True, result_path=None, nsteps=None):
@@ -213,7 +289,7 @@ class Monodomain:
         d.RT = L.ptr(self.repolarization_time)
         st = self.stimuli.stimulus_list
         d.n_stim = len(st)
-        d.stim_mask = L.ptr(self.stimuli.mask) if st else None
+        d.stim_mask = L.ptr(self._stim_mask()) if st else None
         for k, sc in enumerate(st):
             d.stim[k] = L.Stimulus(float(sc.start), float(sc.duration), float(sc.period),
                                    float(sc.count), float(sc.intensity))
@@ -291,8 +367,11 @@ class Monodomain:
         torch.cuda.synchronize(self.device)
         t, steps, xi, nfr = self.sim_status()
         self.solving_time = time.time() - R["t0"]
-        self.u = self._X[xi].to(self.dtype)
-        self.frames = self._frames[: 1 + nfr]
+        self.u = self._to_mesh_order(self._X[xi].to(self.dtype))
+        self.frames = self._to_mesh_order(self._frames[: 1 + nfr])
+        if self.node_order is not None and self.activation_time is not None:
+            self.activation_time = self._to_mesh_order(self.activation_time)
+            self.repolarization_time = self._to_mesh_order(self.repolarization_time)
         AT, RT = self.activation_time, self.repolarization_time
         if self.result_path is not None:
             if R["calc"]:
