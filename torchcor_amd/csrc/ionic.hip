@@ -179,6 +179,20 @@ __device__ inline float t_expm1(float x) { return expm1f(x); }
 __device__ inline double t_expm1(double x) { return expm1(x); }
 
 // Where entry i of a launch lives and what its epilogue applies.
+// Element idx of an array with a wave-uniform base, addressed by a 32-bit
+// byte offset: loads and stores then use a scalar base register and one
+// offset register per lane.  The host checks that every per-node index of
+// an ionic step stays below 2^29, so idx * 8 fits 32 bits.
+template <class T>
+__device__ inline T& at32(T* base, uint32_t idx) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ inline const T& at32(const T* base, uint32_t idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     idx * (uint32_t)sizeof(T));
+}
+
 struct NodeCtx {
   int64_t vi;          // entry in the per-node arrays (V, w, dV, stim_mask, AT, RT)
   int64_t si, ss;      // state index and state row stride
@@ -221,7 +235,8 @@ __device__ inline NodeCtx node_ctx(const tc_ionic_io& io, int64_t i) {
   c.si = i;
   c.ss = io.n;
   c.ok = 1;
-  c.bits = (io.stim_active && io.stim_mask) ? io.stim_mask[c.vi] & io.stim_active : 0u;
+  c.bits = (io.stim_active && io.stim_mask) ? at32(io.stim_mask, (uint32_t)c.vi) & io.stim_active
+                                             : 0u;
   c.amp = io.stim_amp;
   c.amp_stride = 1;
   c.lat = io.AT != nullptr;
@@ -229,16 +244,35 @@ __device__ inline NodeCtx node_ctx(const tc_ionic_io& io, int64_t i) {
   return c;
 }
 
+// State k of this node: row k of the (S, stride) block.
 template <class T>
+__device__ inline T& state(T* st, const NodeCtx& c, int k) {
+  return at32(st + (int64_t)k * c.ss, (uint32_t)c.si);
+}
+
+// The membrane potential: float64 (VF64, the runtime's float64 iterate) or T.
+template <class T, bool VF64>
 __device__ inline T load_v(const tc_ionic_io& io, int64_t vi) {
-  return io.v_f64 ? (T)((const double*)io.V)[vi] : ((const T*)io.V)[vi];
+  if (VF64) return (T)at32((const double*)io.V, (uint32_t)vi);
+  return at32((const T*)io.V, (uint32_t)vi);
+}
+
+// The pipeline's stall flag (0 without one), read as a plain wave-uniform
+// load at kernel start; kernels test it before their first store, so the
+// flag's latency overlaps the node loads.
+__device__ inline int gate_value(const tc_ionic_io& io) { return io.gate ? *io.gate : 0; }
+
+// AT of this node, loaded with the other node inputs (used by finish_node).
+template <class T>
+__device__ inline T load_at(const tc_ionic_io& io, const NodeCtx& c) {
+  return c.lat ? at32((const T*)io.AT, (uint32_t)c.vi) : T(0);
 }
 
 // Stimulus, RHS input and LAT/RT epilogue shared by all models.
 template <class T>
-__device__ inline void finish_node(const tc_ionic_io& io, const NodeCtx& c, T V, T dV) {
-  const int64_t i = c.vi;
-  if (io.dV) ((T*)io.dV)[i] = dV;
+__device__ inline void finish_node(const tc_ionic_io& io, const NodeCtx& c, T V, T dV, T at) {
+  const uint32_t i = (uint32_t)c.vi;
+  if (io.dV) at32((T*)io.dV, i) = dV;
   if (io.w) {
     const uint32_t bits = c.bits;
     if (io.w_f64) {
@@ -250,7 +284,7 @@ __device__ inline void finish_node(const tc_ionic_io& io, const NodeCtx& c, T V,
           if (bits & (1u << k)) ist += c.amp[k * c.amp_stride];
         b = b + io.dt_rhs * (ist / 100.0);
       }
-      ((double*)io.w)[i] = b;
+      at32((double*)io.w, i) = b;
     } else {
       // the reference's order of operations in the working dtype
       const T du = dV / T(100);
@@ -261,41 +295,42 @@ __device__ inline void finish_node(const tc_ionic_io& io, const NodeCtx& c, T V,
           if (bits & (1u << k)) ist += (T)c.amp[k * c.amp_stride];
         b = b + (T)io.dt_rhs * (ist / T(100));
       }
-      ((T*)io.w)[i] = b;
+      at32((T*)io.w, i) = b;
     }
   }
   if (c.lat) {
     T* AT = (T*)io.AT;
     T* RT = (T*)io.RT;
-    T at = AT[i];
     if (V > T(0) && at == T(-1)) {
       at = (T)c.lat_t;
-      AT[i] = at;
+      at32(AT, i) = at;
     }
-    if (at > T(0) && V < T(-70) && RT[i] == T(-1)) RT[i] = (T)c.lat_t;
+    if (at > T(0) && V < T(-70) && at32(RT, i) == T(-1)) at32(RT, i) = (T)c.lat_t;
   }
 }
 
 // ---------------------------------------------------------------- MS2v
 
-template <class T>
+template <class T, bool VF64>
 __global__ void __launch_bounds__(kBlock)
 k_ms2v(tc_ms2v_params p, T* __restrict__ H, tc_ionic_io io) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= io.n) return;
-  if (io.gate && *(volatile const int*)io.gate) return;
+  const int gated = gate_value(io);
   const NodeCtx c = node_ctx(io, i);
   if (!c.ok) return;
-  const T U = load_v<T>(io, c.vi);
-  const T h = H[c.si];
+  const T U = load_v<T, VF64>(io, c.vi);
+  const T h = state(H, c, 0);
+  const T at = load_at<T>(io, c);
   const T DV = (T)(p.vmax - p.vmin);
   const T Uad = (U - (T)p.vmin) / DV;
   const T J_in = T(-1) * h * Uad * (Uad - (T)p.u_crit) * (T(1) - Uad) / (T)p.tau_in;
   const T J_out = (T(1) - h) * Uad / (T)p.tau_out;
   const T dU = -(DV * (J_in + J_out));
   const T dH = Uad > (T)p.u_gate ? -h / (T)p.tau_close : (T(1) - h) / (T)p.tau_open;
-  H[c.si] = h + (T)io.dt * dH;
-  finish_node<T>(io, c, U, dU);
+  if (gated) return;
+  state(H, c, 0) = h + (T)io.dt * dH;
+  finish_node<T>(io, c, U, dU, at);
 }
 
 // ---------------------------------------------------------------- TTP
@@ -437,30 +472,88 @@ __global__ void k_ttp_vektab(TabParam t, void* out, int f64) {
   else ((float*)out)[i] = (float)v;
 }
 
-// One TTP step per node (TenTusscherPanfilov.py:353-454).
+// The constants of a TTP step in the working dtype.  Each is formed on the
+// host in float64 from the parameters, then rounded once, exactly as the
+// step would form it, so passing them precomputed changes no result; it
+// keeps the kernel's scalar registers free for addresses.
 template <class T>
+struct TtpK {
+  T RTONF, F_RT, hRTONF, inverseVcF2, inverseVssF2, invVcF_Cm, pmf_INaK, GKr_sqrtKo, a1c;
+  T Cao, Ko, Nao, KoNa, pKNa, GpCa, KpCa, GKs, GNa, KmNa, GbCa, GbNa, GpK, Gto, GK1;
+  T Vleak, Vmaxup, Kup2, Vxfer, EC, maxsr, dsr, Kbufc, CAPACITANCE, VsrVc, BufcK;
+  T k4, k2_, k1_, k3, Vrel, Kbufsr, BufsrK, Kbufss, VcVss, VsrVss, BufssK;
+};
+
+template <class T>
+TtpK<T> ttp_constants(const tc_ttp_params& p) {
+  TtpK<T> K;
+  const double RTONFd = (p.Rconst * p.T) / p.Fconst;
+  K.RTONF = (T)RTONFd;
+  K.F_RT = (T)(1. / RTONFd);
+  K.hRTONF = (T)(0.5 * RTONFd);
+  K.inverseVcF2 = (T)(1. / ((2. * p.Vc) * p.Fconst));
+  K.inverseVssF2 = (T)(1. / ((2. * p.Vss) * p.Fconst));
+  K.invVcF_Cm = (T)((1. / (p.Vc * p.Fconst)) * p.CAPACITANCE);
+  K.pmf_INaK = (T)(p.knak * (p.Ko / (p.Ko + p.KmK)));
+  K.GKr_sqrtKo = (T)(p.GKr * sqrt(p.Ko / 5.4));
+  K.a1c = (T)(p.GCaL * p.Fconst * (1. / RTONFd) * 4.);
+  K.Cao = (T)p.Cao;
+  K.Ko = (T)p.Ko;
+  K.Nao = (T)p.Nao;
+  K.KoNa = (T)(p.Ko + p.pKNa * p.Nao);
+  K.pKNa = (T)p.pKNa;
+  K.GpCa = (T)p.GpCa;
+  K.KpCa = (T)p.KpCa;
+  K.GKs = (T)p.GKs;
+  K.GNa = (T)p.GNa;
+  K.KmNa = (T)p.KmNa;
+  K.GbCa = (T)p.GbCa;
+  K.GbNa = (T)p.GbNa;
+  K.GpK = (T)p.GpK;
+  K.Gto = (T)p.Gto;
+  K.GK1 = (T)p.GK1;
+  K.Vleak = (T)p.Vleak;
+  K.Vmaxup = (T)p.Vmaxup;
+  K.Kup2 = (T)(p.Kup * p.Kup);
+  K.Vxfer = (T)p.Vxfer;
+  K.EC = (T)p.EC;
+  K.maxsr = (T)p.maxsr;
+  K.dsr = (T)(p.maxsr - p.minsr);
+  K.Kbufc = (T)p.Kbufc;
+  K.CAPACITANCE = (T)p.CAPACITANCE;
+  K.VsrVc = (T)(p.Vsr / p.Vc);
+  K.BufcK = (T)(p.Bufc * p.Kbufc);
+  K.k4 = (T)p.k4;
+  K.k2_ = (T)p.k2_;
+  K.k1_ = (T)p.k1_;
+  K.k3 = (T)p.k3;
+  K.Vrel = (T)p.Vrel;
+  K.Kbufsr = (T)p.Kbufsr;
+  K.BufsrK = (T)(p.Bufsr * p.Kbufsr);
+  K.Kbufss = (T)p.Kbufss;
+  K.VcVss = (T)(p.Vc / p.Vss);
+  K.VsrVss = (T)(p.Vsr / p.Vss);
+  K.BufssK = (T)(p.Bufss * p.Kbufss);
+  return K;
+}
+
+// One TTP step per node (TenTusscherPanfilov.py:353-454).
+template <class T, bool VF64>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
+k_ttp(TtpK<T> K, T* __restrict__ st, const T* __restrict__ Vtab,
       const T* __restrict__ Ctab, const T* __restrict__ Ktab, TtpTabs tb, tc_ionic_io io) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t n = io.n;
   if (i >= n) return;
-  if (io.gate && *(volatile const int*)io.gate) return;
+  const int gated = gate_value(io);
   const NodeCtx c = node_ctx(io, i);
   if (!c.ok) return;
-  const T V = load_v<T>(io, c.vi);
+  const T V = load_v<T, VF64>(io, c.vi);
   T s[TS_N];
 #pragma unroll
-  for (int k = 0; k < TS_N; ++k) s[k] = st[k * c.ss + c.si];
-  const double RTONFd = (p.Rconst * p.T) / p.Fconst;
-  const T RTONF = (T)RTONFd;
-  const T F_RT = (T)(1. / RTONFd);
-  const T inverseVcF2 = (T)(1. / ((2. * p.Vc) * p.Fconst));
-  const T inverseVssF2 = (T)(1. / ((2. * p.Vss) * p.Fconst));
-  const T invVcF_Cm = (T)((1. / (p.Vc * p.Fconst)) * p.CAPACITANCE);
-  const T pmf_INaK = (T)(p.knak * (p.Ko / (p.Ko + p.KmK)));
-  const T GKr_sqrtKo = (T)(p.GKr * sqrt(p.Ko / 5.4));
-  const T a1c = (T)(p.GCaL * p.Fconst * (1. / RTONFd) * 4.);
+  for (int k = 0; k < TS_N; ++k) s[k] = state(st, c, k);
+  const T at = load_at<T>(io, c);
+  const T RTONF = K.RTONF, F_RT = K.F_RT;
 
   // Phase 1: currents and concentrations, with the V-table columns they use
   // (INaCa_A/B, a2, rec_iNaK, rec_ipK) and the CaSS and VEk tables; the
@@ -477,64 +570,64 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
     if (ph1 >> g & 1u) vr.load_group(g);
   const Row<T> cr = row_at<T>(Ctab, 3, s[TS_CaSS], tb.CaSS);
   const T Cai = s[TS_Cai], CaSS = s[TS_CaSS], CaSR = s[TS_CaSR], Ki = s[TS_Ki], Nai = s[TS_Nai];
-  const T Eca = (T)(0.5 * RTONFd) * t_log((T)p.Cao / Cai);
-  const T Ek = RTONF * t_log((T)p.Ko / Ki);
-  const T Eks = RTONF * t_log((T)(p.Ko + p.pKNa * p.Nao) / (Ki + (T)p.pKNa * Nai));
-  const T Ena = RTONF * t_log((T)p.Nao / Nai);
-  const T IpCa = ((T)p.GpCa * Cai) / ((T)p.KpCa + Cai);
+  const T Eca = K.hRTONF * t_log(K.Cao / Cai);
+  const T Ek = RTONF * t_log(K.Ko / Ki);
+  const T Eks = RTONF * t_log(K.KoNa / (Ki + K.pKNa * Nai));
+  const T Ena = RTONF * t_log(K.Nao / Nai);
+  const T IpCa = (K.GpCa * Cai) / (K.KpCa + Cai);
   const T V15 = V - T(15);
   const T ghk = V == T(15) ? T(0.5) * F_RT : V15 / t_expm1((T(2) * V15) * F_RT);
-  const T a1 = a1c * ghk;
+  const T a1 = K.a1c * ghk;
   const T ICaL_A = a1 * vr(TTP_a2, TTP_VCOLS);
-  const T ICaL_B = a1 * (T)p.Cao;
-  const T IKr = GKr_sqrtKo * s[TS_Xr1] * s[TS_Xr2] * (V - Ek);
-  const T IKs = (T)p.GKs * s[TS_Xs] * s[TS_Xs] * (V - Eks);
+  const T ICaL_B = a1 * K.Cao;
+  const T IKr = K.GKr_sqrtKo * s[TS_Xr1] * s[TS_Xr2] * (V - Ek);
+  const T IKs = K.GKs * s[TS_Xs] * s[TS_Xs] * (V - Eks);
   const T m = s[TS_M];
-  const T INa = (T)p.GNa * m * m * m * s[TS_H] * s[TS_J] * (V - Ena);
-  const T INaK = pmf_INaK * (Nai / (Nai + (T)p.KmNa)) * vr(TTP_rec_iNaK, TTP_VCOLS);
-  const T IbCa = (T)p.GbCa * (V - Eca);
-  const T IbNa = (T)p.GbNa * (V - Ena);
-  const T IpK = (T)p.GpK * vr(TTP_rec_ipK, TTP_VCOLS) * (V - Ek);
-  const T Ito = (T)p.Gto * s[TS_R] * s[TS_S] * (V - Ek);
+  const T INa = K.GNa * m * m * m * s[TS_H] * s[TS_J] * (V - Ena);
+  const T INaK = K.pmf_INaK * (Nai / (Nai + K.KmNa)) * vr(TTP_rec_iNaK, TTP_VCOLS);
+  const T IbCa = K.GbCa * (V - Eca);
+  const T IbNa = K.GbNa * (V - Ena);
+  const T IpK = K.GpK * vr(TTP_rec_ipK, TTP_VCOLS) * (V - Ek);
+  const T Ito = K.Gto * s[TS_R] * s[TS_S] * (V - Ek);
   const Row<T> kr = row_at<T>(Ktab, 1, V - Ek, tb.VEk);
   const T ICaL = s[TS_D] * s[TS_F] * s[TS_F2] * s[TS_FCaSS] * (ICaL_A * CaSS - ICaL_B);
   const T INaCa = vr(TTP_INaCa_A, TTP_VCOLS) * Nai * Nai * Nai - vr(TTP_INaCa_B, TTP_VCOLS) * Cai;
-  const T IK1 = (T)p.GK1 * kr(0, 1) * (V - Ek);
+  const T IK1 = K.GK1 * kr(0, 1) * (V - Ek);
   const T Iion = IKr + IKs + IK1 + Ito + INa + IbNa + ICaL + IbCa + INaK + INaCa + IpCa + IpK;
 
-  const T Ileak = (T)p.Vleak * (CaSR - Cai);
-  const T Iup = (T)p.Vmaxup / (T(1) + (T)(p.Kup * p.Kup) / (Cai * Cai));
-  const T Ixfer = (T)p.Vxfer * (CaSS - Cai);
-  const T diff_Ki = -((IK1 + Ito + IKr + IKs - T(2) * INaK) + IpK) * invVcF_Cm;
-  const T diff_Nai = -((INa + IbNa + T(3) * INaK) + T(3) * INaCa) * invVcF_Cm;
-  const T ecs = (T)p.EC / CaSR;
-  const T kCaSR = (T)p.maxsr - (T)(p.maxsr - p.minsr) / (T(1) + ecs * ecs);
-  const T kbc = (T)p.Kbufc + Cai;
-  const T diff_Cai = ((Ixfer - ((IbCa + IpCa - T(2) * INaCa) * inverseVcF2) * (T)p.CAPACITANCE) -
-                      (Iup - Ileak) * (T)(p.Vsr / p.Vc)) /
-                     (T(1) + ((T)(p.Bufc * p.Kbufc) / kbc) / kbc);
+  const T Ileak = K.Vleak * (CaSR - Cai);
+  const T Iup = K.Vmaxup / (T(1) + K.Kup2 / (Cai * Cai));
+  const T Ixfer = K.Vxfer * (CaSS - Cai);
+  const T diff_Ki = -((IK1 + Ito + IKr + IKs - T(2) * INaK) + IpK) * K.invVcF_Cm;
+  const T diff_Nai = -((INa + IbNa + T(3) * INaK) + T(3) * INaCa) * K.invVcF_Cm;
+  const T ecs = K.EC / CaSR;
+  const T kCaSR = K.maxsr - K.dsr / (T(1) + ecs * ecs);
+  const T kbc = K.Kbufc + Cai;
+  const T diff_Cai = ((Ixfer - ((IbCa + IpCa - T(2) * INaCa) * K.inverseVcF2) * K.CAPACITANCE) -
+                      (Iup - Ileak) * K.VsrVc) /
+                     (T(1) + (K.BufcK / kbc) / kbc);
   const T R_ = s[TS_R_];
-  const T diff_R_ = (T)p.k4 * (T(1) - R_) - (T)p.k2_ * kCaSR * CaSS * R_;
-  const T k1 = (T)p.k1_ / kCaSR;
+  const T diff_R_ = K.k4 * (T(1) - R_) - K.k2_ * kCaSR * CaSS * R_;
+  const T k1 = K.k1_ / kCaSR;
   const T c2 = cr(0, 3);
-  const T O = (k1 * c2 * R_) / ((T)p.k3 + k1 * c2);
-  const T Irel = (T)p.Vrel * O * (CaSR - CaSS);
-  const T ksr = CaSR + (T)p.Kbufsr;
-  const T diff_CaSR = (Iup - Irel - Ileak) / (T(1) + ((T)(p.Bufsr * p.Kbufsr) / ksr) / ksr);
-  const T kss = CaSS + (T)p.Kbufss;
-  const T diff_CaSS = ((-Ixfer * (T)(p.Vc / p.Vss) + Irel * (T)(p.Vsr / p.Vss)) +
-                       (-ICaL * inverseVssF2) * (T)p.CAPACITANCE) /
-                      (T(1) + ((T)(p.Bufss * p.Kbufss) / kss) / kss);
+  const T O = (k1 * c2 * R_) / (K.k3 + k1 * c2);
+  const T Irel = K.Vrel * O * (CaSR - CaSS);
+  const T ksr = CaSR + K.Kbufsr;
+  const T diff_CaSR = (Iup - Irel - Ileak) / (T(1) + (K.BufsrK / ksr) / ksr);
+  const T kss = CaSS + K.Kbufss;
+  const T diff_CaSS = ((-Ixfer * K.VcVss + Irel * K.VsrVss) +
+                       (-ICaL * K.inverseVssF2) * K.CAPACITANCE) /
+                      (T(1) + (K.BufssK / kss) / kss);
   const T dt = (T)io.dt;
-  const int64_t ss = c.ss, si = c.si;
-  st[TS_CaSR * ss + si] = CaSR + diff_CaSR * dt;
-  st[TS_CaSS * ss + si] = CaSS + diff_CaSS * dt;
-  st[TS_Cai * ss + si] = Cai + diff_Cai * dt;
-  st[TS_Ki * ss + si] = Ki + diff_Ki * dt;
-  st[TS_Nai * ss + si] = Nai + diff_Nai * dt;
-  st[TS_R_ * ss + si] = R_ + diff_R_ * dt;
-  st[TS_FCaSS * ss + si] = cr(1, 3) + cr(2, 3) * s[TS_FCaSS];
-  finish_node<T>(io, c, V, -Iion);
+  if (gated) return;
+  state(st, c, TS_CaSR) = CaSR + diff_CaSR * dt;
+  state(st, c, TS_CaSS) = CaSS + diff_CaSS * dt;
+  state(st, c, TS_Cai) = Cai + diff_Cai * dt;
+  state(st, c, TS_Ki) = Ki + diff_Ki * dt;
+  state(st, c, TS_Nai) = Nai + diff_Nai * dt;
+  state(st, c, TS_R_) = R_ + diff_R_ * dt;
+  state(st, c, TS_FCaSS) = cr(1, 3) + cr(2, 3) * s[TS_FCaSS];
+  finish_node<T>(io, c, V, -Iion, at);
   // Phase 2 (no memory access moves across this point): the remaining
   // V-table columns and the Rush-Larsen gate updates from the old gates.
   asm volatile("" ::: "memory");
@@ -555,7 +648,7 @@ k_ttp(tc_ttp_params p, T* __restrict__ st, const T* __restrict__ Vtab,
   o[TS_Xs] = vr(TTP_Xs_A, TTP_VCOLS) + vr(TTP_Xs_B, TTP_VCOLS) * s[TS_Xs];
   const int gates[11] = {TS_D, TS_F, TS_F2, TS_H, TS_J, TS_M, TS_R, TS_S, TS_Xr1, TS_Xr2, TS_Xs};
 #pragma unroll
-  for (int k = 0; k < 11; ++k) st[gates[k] * ss + si] = o[gates[k]];
+  for (int k = 0; k < 11; ++k) state(st, c, gates[k]) = o[gates[k]];
 }
 
 // ---------------------------------------------------------------- CRN
@@ -698,20 +791,21 @@ __global__ void k_crn_fntab(tc_crn_params p, double dt, TabParam t, void* out, i
 }
 
 // One CRN step per node (CourtemancheRamirezNattel.py:358-444).
-template <class T>
+template <class T, bool VF64>
 __global__ void __launch_bounds__(kBlock)
 k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
       const T* __restrict__ Vtab, const T* __restrict__ Ftab, CrnTabs tb, tc_ionic_io io) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t n = io.n;
   if (i >= n) return;
-  if (io.gate && *(volatile const int*)io.gate) return;
+  const int gated = gate_value(io);
   const NodeCtx c = node_ctx(io, i);
   if (!c.ok) return;
-  const T V = load_v<T>(io, c.vi);
+  const T V = load_v<T, VF64>(io, c.vi);
   T s[CS_N];
 #pragma unroll
-  for (int k = 0; k < CS_N; ++k) s[k] = st[k * c.ss + c.si];
+  for (int k = 0; k < CS_N; ++k) s[k] = state(st, c, k);
+  const T at = load_at<T>(io, c);
   const Row<T> vr = row_at<T>(Vtab, CV_NCOLS, V, tb.V);
   const Row<T> car = row_at<T>(Ctab, CA_NCOLS, s[CS_Cai], tb.Cai);
   const T RTF = (T)((p.R * p.T) / p.F);
@@ -765,9 +859,10 @@ k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
 #define RLU(G) o[CS_##G] = vr(CV_##G##_A, CV_NCOLS) + vr(CV_##G##_B, CV_NCOLS) * s[CS_##G];
   RLU(d) RLU(f) RLU(h) RLU(j) RLU(m) RLU(oa) RLU(oi) RLU(ua) RLU(ui) RLU(w) RLU(xr) RLU(xs)
 #undef RLU
+  if (gated) return;
 #pragma unroll
-  for (int k = 0; k < CS_N; ++k) st[k * c.ss + c.si] = o[k];
-  finish_node<T>(io, c, V, -Iion);
+  for (int k = 0; k < CS_N; ++k) state(st, c, k) = o[k];
+  finish_node<T>(io, c, V, -Iion, at);
 }
 
 template <class T>
@@ -788,8 +883,15 @@ int check_io(const tc_ionic_io* io, int dtype) {
   TC_CHECK(io && io->V && io->n > 0, TC_ERR_ARG, "ionic step: bad io");
   TC_CHECK(dtype == TC_F32 || dtype == TC_F64, TC_ERR_ARG, "bad dtype %d", dtype);
   TC_CHECK(!io->AT || io->RT, TC_ERR_ARG, "ionic step: AT without RT");
+  // per-node arrays are addressed with 32-bit byte offsets (at32)
+  int64_t top = io->n;
+  if (io->pool_slices) top = io->state_stride;
+  else if (io->lanes > 0) top = (io->n - 1) / io->lanes * io->lane_stride + io->lane0 + io->lanes;
+  TC_CHECK(top < ((int64_t)1 << 29), TC_ERR_SIZE,
+           "ionic step: %lld per-node entries exceed the 2^29 limit", (long long)top);
   return TC_OK;
 }
+
 
 int check_tab(const TabParam& t) {
   TC_CHECK(t.mx_idx >= 1 && t.mx_idx <= t.rows - 1, TC_ERR_ARG,
@@ -808,10 +910,14 @@ int tc::ms2v_step(const tc_ms2v_params* p, void* states, const tc_ionic_io* io, 
   if (rc) return rc;
   TC_CHECK(p && states, TC_ERR_ARG, "tc_ms2v_step: bad arguments");
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
-  if (dtype == TC_F32)
-    launch(k_ms2v<float>, g, b, s, e0, e1, *p, (float*)states, *io);
+  // float32 kernels read V as float64 when io->v_f64 is set (the runtime's
+  // float64 iterate); float64 kernels read double either way
+  if (dtype == TC_F32 && io->v_f64)
+    launch(k_ms2v<float, true>, g, b, s, e0, e1, *p, (float*)states, *io);
+  else if (dtype == TC_F32)
+    launch(k_ms2v<float, false>, g, b, s, e0, e1, *p, (float*)states, *io);
   else
-    launch(k_ms2v<double>, g, b, s, e0, e1, *p, (double*)states, *io);
+    launch(k_ms2v<double, false>, g, b, s, e0, e1, *p, (double*)states, *io);
   TC_LAUNCH();
   return TC_OK;
 }
@@ -849,11 +955,17 @@ int tc::ttp_step(const tc_ttp_params* p, void* states, const void* V_tab, const 
   TtpTabs tb = ttp_tabs();
   if ((rc = check_tab(tb.V)) || (rc = check_tab(tb.CaSS)) || (rc = check_tab(tb.VEk))) return rc;
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
-  if (dtype == TC_F32)
-    launch(k_ttp<float>, g, b, s, e0, e1, *p, (float*)states, (const float*)V_tab,
+  if (dtype == TC_F32 && io->v_f64)
+    launch(k_ttp<float, true>, g, b, s, e0, e1, ttp_constants<float>(*p), (float*)states,
+           (const float*)V_tab,
+           (const float*)CaSS_tab, (const float*)VEk_tab, tb, *io);
+  else if (dtype == TC_F32)
+    launch(k_ttp<float, false>, g, b, s, e0, e1, ttp_constants<float>(*p), (float*)states,
+           (const float*)V_tab,
            (const float*)CaSS_tab, (const float*)VEk_tab, tb, *io);
   else
-    launch(k_ttp<double>, g, b, s, e0, e1, *p, (double*)states, (const double*)V_tab,
+    launch(k_ttp<double, false>, g, b, s, e0, e1, ttp_constants<double>(*p), (double*)states,
+           (const double*)V_tab,
            (const double*)CaSS_tab, (const double*)VEk_tab, tb, *io);
   TC_LAUNCH();
   return TC_OK;
@@ -894,11 +1006,14 @@ int tc::crn_step(const tc_crn_params* p, void* states, const void* Cai_tab, cons
   CrnTabs tb = crn_tabs();
   if ((rc = check_tab(tb.Cai)) || (rc = check_tab(tb.V)) || (rc = check_tab(tb.fn))) return rc;
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
-  if (dtype == TC_F32)
-    launch(k_crn<float>, g, b, s, e0, e1, *p, (float*)states, (const float*)Cai_tab,
+  if (dtype == TC_F32 && io->v_f64)
+    launch(k_crn<float, true>, g, b, s, e0, e1, *p, (float*)states, (const float*)Cai_tab,
+           (const float*)V_tab, (const float*)fn_tab, tb, *io);
+  else if (dtype == TC_F32)
+    launch(k_crn<float, false>, g, b, s, e0, e1, *p, (float*)states, (const float*)Cai_tab,
            (const float*)V_tab, (const float*)fn_tab, tb, *io);
   else
-    launch(k_crn<double>, g, b, s, e0, e1, *p, (double*)states, (const double*)Cai_tab,
+    launch(k_crn<double, false>, g, b, s, e0, e1, *p, (double*)states, (const double*)Cai_tab,
            (const double*)V_tab, (const double*)fn_tab, tb, *io);
   TC_LAUNCH();
   return TC_OK;
