@@ -790,10 +790,50 @@ __global__ void k_crn_fntab(tc_crn_params p, double dt, TabParam t, void* out, i
   put(out, i * 3 + CF_v_B, exp(-dt / tau_v), f64);
 }
 
-// One CRN step per node (CourtemancheRamirezNattel.py:358-444).
+// The constants of a CRN step in the working dtype, formed on the host in
+// float64 from the parameters and the step length, then rounded once (as the
+// step would form them).
+template <class T>
+struct CrnK {
+  T RTF, Ko, gach, Gto, factorGKur, GKr, GKs, GK1, factorGtr, tau_tr, factorGrel, k_rel,
+      maxIup_Caup, KmCsqn, C_dCa_rel, FVoli, C_dCaup, C_B1d, C_B1e, C_Fn1, C_Fn2, e_fCa, e_u;
+};
+
+template <class T>
+CrnK<T> crn_constants(const tc_crn_params& p, double dt) {
+  CrnK<T> K;
+  K.RTF = (T)((p.R * p.T) / p.F);
+  K.Ko = (T)p.Ko;
+  K.gach = (T)(p.GACh * (10.0 / (1.0 + 9.13652 / pow(p.ACh, 0.477811))));
+  K.Gto = (T)p.Gto;
+  K.factorGKur = (T)p.factorGKur;
+  K.GKr = (T)p.GKr;
+  K.GKs = (T)p.GKs;
+  K.GK1 = (T)p.GK1;
+  K.factorGtr = (T)p.factorGtr;
+  K.tau_tr = (T)p.tau_tr;
+  K.factorGrel = (T)p.factorGrel;
+  K.k_rel = (T)p.k_rel;
+  K.maxIup_Caup = (T)(p.maxIup / p.maxCaup);
+  K.KmCsqn = (T)p.KmCsqn;
+  K.C_dCa_rel = (T)p.C_dCa_rel;
+  K.FVoli = (T)(p.F * p.Voli);
+  K.C_dCaup = (T)p.C_dCaup;
+  K.C_B1d = (T)p.C_B1d;
+  K.C_B1e = (T)p.C_B1e;
+  K.C_Fn1 = (T)p.C_Fn1;
+  K.C_Fn2 = (T)p.C_Fn2;
+  K.e_fCa = (T)exp(-dt / p.tau_f_Ca);
+  K.e_u = (T)exp(-dt / p.tau_u);
+  return K;
+}
+
+// One CRN step per node (CourtemancheRamirezNattel.py:358-444), in two
+// phases like the TTP step: currents and concentrations first (with the
+// V-table groups they read), then the Rush-Larsen gates.
 template <class T, bool VF64>
 __global__ void __launch_bounds__(kBlock)
-k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
+k_crn(CrnK<T> K, T* __restrict__ st, const T* __restrict__ Ctab,
       const T* __restrict__ Vtab, const T* __restrict__ Ftab, CrnTabs tb, tc_ionic_io io) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int64_t n = io.n;
@@ -806,22 +846,30 @@ k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
 #pragma unroll
   for (int k = 0; k < CS_N; ++k) s[k] = state(st, c, k);
   const T at = load_at<T>(io, c);
-  const Row<T> vr = row_at<T>(Vtab, CV_NCOLS, V, tb.V);
+  RowG<T, CV_NCOLS> vr;
+  vr.at(Vtab, V, tb.V);
+  constexpr int E = Vec16<T>::n;
+  // 16-byte groups holding the phase-1 columns
+  constexpr unsigned ph1 = (1u << (CV_GKur / E)) | (1u << (CV_INaK / E)) | (1u << (CV_IbNa / E)) |
+                           (1u << (CV_vrow_29 / E)) | (1u << (CV_vrow_31 / E)) |
+                           (1u << (CV_vrow_32 / E)) | (1u << (CV_vrow_36 / E)) |
+                           (1u << (CV_vrow_7 / E));
+#pragma unroll
+  for (int g = 0; g < CV_NCOLS / E; ++g)
+    if (ph1 >> g & 1u) vr.load_group(g);
   const Row<T> car = row_at<T>(Ctab, CA_NCOLS, s[CS_Cai], tb.Cai);
-  const T RTF = (T)((p.R * p.T) / p.F);
-  const T E_K = RTF * t_log((T)p.Ko / s[CS_Ki]);
+  const T E_K = K.RTF * t_log(K.Ko / s[CS_Ki]);
   const T VmEK = V - E_K;
   const T ICaL = vr(CV_vrow_29, CV_NCOLS) * s[CS_d] * s[CS_f] * s[CS_f_Ca];
-  const T gach = (T)(p.GACh * (10.0 / (1.0 + 9.13652 / pow(p.ACh, 0.477811))));
-  const T IKACh = gach * (T(0.0517) + T(0.4516) / (T(1) + t_exp((V + T(59.53)) / T(17.18)))) * VmEK;
+  const T IKACh = K.gach * (T(0.0517) + T(0.4516) / (T(1) + t_exp((V + T(59.53)) / T(17.18)))) * VmEK;
   const T m = s[CS_m];
   const T INa = vr(CV_vrow_7, CV_NCOLS) * m * m * m * s[CS_h] * s[CS_j];
   const T INaCa = vr(CV_vrow_31, CV_NCOLS) - s[CS_Cai] * vr(CV_vrow_32, CV_NCOLS);
-  const T vrow_13 = (T)p.Gto * VmEK;
-  const T vrow_18 = ((T)p.factorGKur * vr(CV_GKur, CV_NCOLS)) * VmEK;
-  const T vrow_21 = ((T)p.GKr * VmEK) / (T(1) + t_exp((V + T(15)) / T(22.4)));
-  const T vrow_24 = (T)p.GKs * VmEK;
-  const T IK1 = ((T)p.GK1 * VmEK) / (T(1) + t_exp(T(0.07) * (V + T(80))));
+  const T vrow_13 = K.Gto * VmEK;
+  const T vrow_18 = (K.factorGKur * vr(CV_GKur, CV_NCOLS)) * VmEK;
+  const T vrow_21 = (K.GKr * VmEK) / (T(1) + t_exp((V + T(15)) / T(22.4)));
+  const T vrow_24 = K.GKs * VmEK;
+  const T IK1 = (K.GK1 * VmEK) / (T(1) + t_exp(T(0.07) * (V + T(80))));
   const T IKr = vrow_21 * s[CS_xr];
   const T IKs = vrow_24 * s[CS_xs] * s[CS_xs];
   const T ua = s[CS_ua];
@@ -834,35 +882,38 @@ k_crn(tc_crn_params p, T* __restrict__ st, const T* __restrict__ Ctab,
                  vr(CV_IbNa, CV_NCOLS) + INaK + IKACh;
 
   const T Ca_rel = s[CS_Ca_rel], Ca_up = s[CS_Ca_up];
-  const T Itr = ((T)p.factorGtr * (Ca_up - Ca_rel)) / (T)p.tau_tr;
+  const T Itr = (K.factorGtr * (Ca_up - Ca_rel)) / K.tau_tr;
   const T u = s[CS_u];
-  const T Irel = (T)p.factorGrel * u * u * s[CS_v] * (T)p.k_rel * s[CS_w] *
+  const T Irel = K.factorGrel * u * u * s[CS_v] * K.k_rel * s[CS_w] *
                  (Ca_rel - car(CA_conCa, CA_NCOLS));
-  const T dIups = car(CA_carow_1, CA_NCOLS) - (T)(p.maxIup / p.maxCaup) * Ca_up;
-  const T kc = Ca_rel + (T)p.KmCsqn;
-  const T diff_Ca_rel = (Itr - Irel) / (T(1) + ((T)p.C_dCa_rel / kc) / kc);
-  const T diff_Ki = -((Ito + IKr + IKur + IKs + IK1 + IKACh) - T(2) * INaK) / (T)(p.F * p.Voli);
-  const T diff_Ca_up = dIups - Itr * (T)p.C_dCaup;
-  const T diff_Cai = (((T)p.C_B1d * ((INaCa + INaCa - IpCa) - ICaL) - (T)p.C_B1e * dIups) + Irel) /
+  const T dIups = car(CA_carow_1, CA_NCOLS) - K.maxIup_Caup * Ca_up;
+  const T kc = Ca_rel + K.KmCsqn;
+  const T diff_Ca_rel = (Itr - Irel) / (T(1) + (K.C_dCa_rel / kc) / kc);
+  const T diff_Ki = -((Ito + IKr + IKur + IKs + IK1 + IKACh) - T(2) * INaK) / K.FVoli;
+  const T diff_Ca_up = dIups - Itr * K.C_dCaup;
+  const T diff_Cai = ((K.C_B1d * ((INaCa + INaCa - IpCa) - ICaL) - K.C_B1e * dIups) + Irel) /
                      car(CA_carow_2, CA_NCOLS);
   const T dt = (T)io.dt;
-  T o[CS_N];
-  o[CS_Ca_rel] = Ca_rel + diff_Ca_rel * dt;
-  o[CS_Ca_up] = Ca_up + diff_Ca_up * dt;
-  o[CS_Cai] = s[CS_Cai] + diff_Cai * dt;
-  o[CS_Ki] = s[CS_Ki] + diff_Ki * dt;
-  const T fn = (T)p.C_Fn1 * Irel - (T)p.C_Fn2 * (ICaL - T(0.4) * INaCa);
+  const T fn = K.C_Fn1 * Irel - K.C_Fn2 * (ICaL - T(0.4) * INaCa);
   const Row<T> fr = row_at<T>(Ftab, CF_NCOLS, fn, tb.fn);
-  o[CS_f_Ca] = car(CA_f_Ca_A, CA_NCOLS) + (T)exp(-io.dt / p.tau_f_Ca) * s[CS_f_Ca];
-  o[CS_u] = fr(CF_u_A, CF_NCOLS) + (T)exp(-io.dt / p.tau_u) * u;
-  o[CS_v] = fr(CF_v_A, CF_NCOLS) + fr(CF_v_B, CF_NCOLS) * s[CS_v];
-#define RLU(G) o[CS_##G] = vr(CV_##G##_A, CV_NCOLS) + vr(CV_##G##_B, CV_NCOLS) * s[CS_##G];
+  if (gated) return;
+  state(st, c, CS_Ca_rel) = Ca_rel + diff_Ca_rel * dt;
+  state(st, c, CS_Ca_up) = Ca_up + diff_Ca_up * dt;
+  state(st, c, CS_Cai) = s[CS_Cai] + diff_Cai * dt;
+  state(st, c, CS_Ki) = s[CS_Ki] + diff_Ki * dt;
+  state(st, c, CS_f_Ca) = car(CA_f_Ca_A, CA_NCOLS) + K.e_fCa * s[CS_f_Ca];
+  state(st, c, CS_u) = fr(CF_u_A, CF_NCOLS) + K.e_u * u;
+  state(st, c, CS_v) = fr(CF_v_A, CF_NCOLS) + fr(CF_v_B, CF_NCOLS) * s[CS_v];
+  finish_node<T>(io, c, V, -Iion, at);
+  // Phase 2 (no memory access moves across this point): the gate columns
+  // of the V table and the Rush-Larsen updates from the old gates.
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int g = 0; g < CV_NCOLS / E; ++g) vr.load_group(g);
+#define RLU(G) \
+  state(st, c, CS_##G) = vr(CV_##G##_A, CV_NCOLS) + vr(CV_##G##_B, CV_NCOLS) * s[CS_##G];
   RLU(d) RLU(f) RLU(h) RLU(j) RLU(m) RLU(oa) RLU(oi) RLU(ua) RLU(ui) RLU(w) RLU(xr) RLU(xs)
 #undef RLU
-  if (gated) return;
-#pragma unroll
-  for (int k = 0; k < CS_N; ++k) state(st, c, k) = o[k];
-  finish_node<T>(io, c, V, -Iion, at);
 }
 
 template <class T>
@@ -1007,13 +1058,16 @@ int tc::crn_step(const tc_crn_params* p, void* states, const void* Cai_tab, cons
   if ((rc = check_tab(tb.Cai)) || (rc = check_tab(tb.V)) || (rc = check_tab(tb.fn))) return rc;
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
   if (dtype == TC_F32 && io->v_f64)
-    launch(k_crn<float, true>, g, b, s, e0, e1, *p, (float*)states, (const float*)Cai_tab,
+    launch(k_crn<float, true>, g, b, s, e0, e1, crn_constants<float>(*p, io->dt), (float*)states,
+           (const float*)Cai_tab,
            (const float*)V_tab, (const float*)fn_tab, tb, *io);
   else if (dtype == TC_F32)
-    launch(k_crn<float, false>, g, b, s, e0, e1, *p, (float*)states, (const float*)Cai_tab,
+    launch(k_crn<float, false>, g, b, s, e0, e1, crn_constants<float>(*p, io->dt), (float*)states,
+           (const float*)Cai_tab,
            (const float*)V_tab, (const float*)fn_tab, tb, *io);
   else
-    launch(k_crn<double, false>, g, b, s, e0, e1, *p, (double*)states, (const double*)Cai_tab,
+    launch(k_crn<double, false>, g, b, s, e0, e1, crn_constants<double>(*p, io->dt), (double*)states,
+           (const double*)Cai_tab,
            (const double*)V_tab, (const double*)fn_tab, tb, *io);
   TC_LAUNCH();
   return TC_OK;
