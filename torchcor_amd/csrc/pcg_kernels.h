@@ -181,14 +181,29 @@ __device__ inline void store_partials(double (&v)[NV], double* partials) {
 
 // Sum of the previous pass's G partials, in a fixed order, returned to
 // every thread of the block.
+// Blocks of kPcgBlock threads and G <= kMaxBlocks: each thread's loads are
+// all issued before its (in-order) additions.
 template <int NV>
 __device__ inline void fold_partials(const double* __restrict__ partials, int G,
                                      double (&out)[NV]) {
   __shared__ double bc[NV];
+  constexpr int R = kMaxBlocks / kPcgBlock;
+  double v[NV][R];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      // unconditional loads (a clamped index, then a select) keep the
+      // loads of one thread in flight together
+      const int b = (int)threadIdx.x + q * kPcgBlock;
+      const double t = partials[k * G + (b < G ? b : G - 1)];
+      v[k][q] = b < G ? t : 0.0;
+    }
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < G; b += blockDim.x) s += partials[k * G + b];
+#pragma unroll
+    for (int q = 0; q < R; ++q) s += v[k][q];
     out[k] = s;
   }
   block_sum<NV>(out);
@@ -275,8 +290,41 @@ k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
   }
 }
 
+// One chunk of a row of the spmv pass: columns and A values, then the
+// gathered Dinv, r (and the previous p) of those columns; add() forms
+// p' = beta p + Dinv r for them and accumulates A p' in order.
+template <class T, class CI>
+struct SpmvChunk {
+  int32_t cj[kChunk];
+  T av[kChunk], dj[kChunk], rj[kChunk], qj[kChunk];
+  __device__ inline void fetch(const CI* c, const T* a, int j0, int wd, int64_t i,
+                               const T* __restrict__ Dinv, const T* __restrict__ r,
+                               const T* __restrict__ pin, int k) {
+    load_chunk(c, a, j0, wd, i, cj, av);
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      dj[u] = Dinv[cj[u]];
+      rj[u] = r[cj[u]];
+      qj[u] = k == 0 ? T(0) : pin[cj[u]];
+    }
+  }
+  __device__ inline double add(double acc, int j0, int wd, int k, double beta) const {
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      T pj;
+      if (k == 0) pj = (T)((double)dj[u] * (double)rj[u]);
+      else pj = (T)(beta * (double)qj[u] + (double)dj[u] * (double)rj[u]);
+      const double t = acc + (double)av[u] * (double)pj;
+      acc = j0 + u < wd ? t : acc;
+    }
+    return acc;
+  }
+};
+
 // spmv pass of iteration k.  pin is the previous p (unused when k == 0),
-// pout receives p'.  Iteration k-1's stopping test happens here first.
+// pout receives p'.  Iteration k-1's stopping test happens here first.  The
+// first chunk of each wave's first slice is fetched before the block reads
+// the flags and folds the partial sums, so those latencies overlap.
 template <class T, class CI>
 __global__ void __launch_bounds__(kPcgBlock)
 k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ scol,
@@ -284,17 +332,30 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
            const T* __restrict__ pin, T* __restrict__ pout, T* __restrict__ Ap, int k,
            PcgCtl* ctl, const double* __restrict__ part0, const double* __restrict__ part2,
            double* part1) {
-  if (block_halted(ctl)) return;
+  const int64_t ns = (n + 63) >> 6;
+  const Split sp = block_slices(ns);
+  const int lane = threadIdx.x & 63, wv = wave_id();
+  int64_t s = sp.s0 + wv;
+  SpmvChunk<T, CI> ch;
+  int32_t b = 0, wd = 0;
+  if (s < sp.s1) {
+    b = sptr[s];
+    wd = (sptr[s + 1] - b) >> 6;
+    if (wd > 0) ch.fetch(scol + b + lane, Av + b + lane, 0, wd, s * 64 + lane, Dinv, r, pin, k);
+  }
+  // the flags and the partial sums are read before the first wait
+  __shared__ int halted;
+  if (threadIdx.x == 0) halted = ctl->converged | ctl->stall;
   const int G = gridDim.x;
+  const double rz_prev = ctl->rz[(k - 1) & 1];
+  double v[2];
+  fold_partials<2>(k == 0 ? part0 : part2, G, v);
+  if (halted) return;
   double beta = 0.0, rz;
   if (k == 0) {
-    double v[2];
-    fold_partials<2>(part0, G, v);
     rz = v[0];
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->bnorm = sqrt(v[1]);
   } else {
-    double v[2];
-    fold_partials<2>(part2, G, v);
     if (stop_rule(sqrt(v[0]), ctl)) {
       if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->converged = 1;
@@ -304,41 +365,29 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
       return;
     }
     rz = v[1];
-    beta = rz / ctl->rz[(k - 1) & 1];
+    beta = rz / rz_prev;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->rz[k & 1] = rz;
     ctl->n_iter = k;
     ctl->iter = k;
   }
-  const int64_t ns = (n + 63) >> 6;
-  const Split sp = block_slices(ns);
-  const int lane = threadIdx.x & 63, wv = wave_id();
   double red[1] = {0.0};
-  for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
+  bool first = true;
+  for (; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
-    const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
+    if (!first) {
+      b = sptr[s];
+      wd = (sptr[s + 1] - b) >> 6;
+    }
     const CI* c = scol + b + lane;
     const T* a = Av + b + lane;
     double acc = 0.0;
     for (int j0 = 0; j0 < wd; j0 += kChunk) {
-      int32_t cj[kChunk];
-      T av[kChunk], pj[kChunk];
-      load_chunk(c, a, j0, wd, i, cj, av);
-      if (k == 0) {
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u) pj[u] = (T)((double)Dinv[cj[u]] * (double)r[cj[u]]);
-      } else {
-#pragma unroll
-        for (int u = 0; u < kChunk; ++u)
-          pj[u] = (T)(beta * (double)pin[cj[u]] + (double)Dinv[cj[u]] * (double)r[cj[u]]);
-      }
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        const double t = acc + (double)av[u] * (double)pj[u];
-        acc = j0 + u < wd ? t : acc;
-      }
+      if (!(first && j0 == 0)) ch.fetch(c, a, j0, wd, i, Dinv, r, pin, k);
+      acc = ch.add(acc, j0, wd, k, beta);
     }
+    first = false;
     if (i < n) {
       T pi;
       if (k == 0) pi = (T)((double)Dinv[i] * (double)r[i]);
@@ -359,24 +408,49 @@ __global__ void __launch_bounds__(kPcgBlock)
 k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
              const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, int k, PcgCtl* ctl,
              const double* __restrict__ part1, double* part2) {
-  if (block_halted(ctl)) return;
-  double v[1];
-  fold_partials<1>(part1, gridDim.x, v);
-  const double alpha = ctl->rz[k & 1] / v[0];
   const int64_t ns = (n + 63) >> 6;
   const Split sp = block_slices(ns);
   const int lane = threadIdx.x & 63, wv = wave_id();
+  // the first slice's operands load while the block reads the flags and
+  // folds the partial sums
+  int64_t s = sp.s0 + wv;
+  int64_t i = s * 64 + lane;
+  XT xv = XT(0);
+  T pv = T(0), av = T(0), rv = T(0), dv = T(0);
+  if (s < sp.s1 && i < n) {
+    xv = x[i];
+    pv = p[i];
+    av = Ap[i];
+    rv = r[i];
+    dv = Dinv[i];
+  }
+  // the flags, r.z and the partial sums are all read before the first wait
+  __shared__ int halted;
+  if (threadIdx.x == 0) halted = ctl->converged | ctl->stall;
+  const double rzk = ctl->rz[k & 1];
+  double v[1];
+  fold_partials<1>(part1, gridDim.x, v);
+  if (halted) return;
+  const double alpha = rzk / v[0];
   double red[2] = {0.0, 0.0};
-  for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
-    const int64_t i = s * 64 + lane;
+  while (s < sp.s1) {
     if (i < n) {
-      const XT xi = (XT)((double)x[i] + alpha * (double)p[i]);
-      const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
+      const XT xi = (XT)((double)xv + alpha * (double)pv);
+      const T ri = (T)((double)rv - alpha * (double)av);
       x[i] = xi;
       r[i] = ri;
-      const double z = (double)Dinv[i] * (double)ri;
+      const double z = (double)dv * (double)ri;
       red[0] += z * z;
       red[1] += (double)ri * z;
+    }
+    s += kWaves;
+    i = s * 64 + lane;
+    if (s < sp.s1 && i < n) {
+      xv = x[i];
+      pv = p[i];
+      av = Ap[i];
+      rv = r[i];
+      dv = Dinv[i];
     }
   }
   store_partials<2>(red, part2);
@@ -395,6 +469,10 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepSta
     flags[0] = ctl->stall;
     flags[1] = ctl->converged;
   }
+  // the fold reads the partials while the flags arrive (its result is used
+  // only when the solve is still running)
+  double v[2] = {0.0, 0.0};
+  if (k >= 0) fold_partials<2>(part2, G, v);
   __syncthreads();
   if (flags[0]) return;
   int n_iter, iter, conv, stalled = 0;
@@ -403,8 +481,6 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepSta
     iter = k < 0 ? 0 : ctl->iter;
     conv = k < 0 ? 0 : 1;
   } else {
-    double v[2];
-    fold_partials<2>(part2, G, v);
     const bool stop = stop_rule(sqrt(v[0]), ctl);
     conv = stop ? 1 : 0;
     n_iter = stop ? k : k + 1;
@@ -440,58 +516,82 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepSta
 // WT is the type of w; with a float64 w (precise RHS) b is not rounded and
 // K u uses the float64 iterate itself.
 template <class T, class XT, class WT, bool LG, class CI>
+struct RhsChunk {
+  int32_t cj[kChunk];
+  T mv[kChunk], kv[kChunk];
+  XT xj[kChunk];
+  WT wj[kChunk];
+  T pj[kChunk];
+  __device__ inline void fetch(const CI* c, const T* m, const T* kk, int j0, int wd, int64_t i,
+                               const XT* __restrict__ x, const WT* __restrict__ w,
+                               const T* __restrict__ xp) {
+    load_chunk(c, m, j0, wd, i, cj, mv);
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) kv[u] = kk[(j0 + u < wd ? j0 + u : wd - 1) * 64];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      xj[u] = x[cj[u]];
+      wj[u] = w[cj[u]];
+      if (LG) pj[u] = xp[cj[u]];
+    }
+  }
+};
+
+template <class T, class XT, class WT, bool LG, class CI>
 __global__ void __launch_bounds__(kPcgBlock)
 k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ scol,
            const T* __restrict__ Mv, const T* __restrict__ Kv, const T* __restrict__ Dinv,
            const WT* __restrict__ w, const XT* __restrict__ x, const T* __restrict__ xp,
            XT* __restrict__ xnext, T* __restrict__ xpnext, T* __restrict__ r, double coef,
            double Cm, double tdt, PcgCtl* ctl, double a_tol, double r_tol, double* part0) {
+  const int64_t ns = (n + 63) >> 6;
+  const Split sp = block_slices(ns);
+  const int lane = threadIdx.x & 63, wv = wave_id();
+  // the first chunk of each wave's first slice loads while the block reads
+  // the stall flag
+  int64_t s = sp.s0 + wv;
+  RhsChunk<T, XT, WT, LG, CI> ch;
+  int32_t b = 0, wd = 0;
+  if (s < sp.s1) {
+    b = sptr[s];
+    wd = (sptr[s + 1] - b) >> 6;
+    if (wd > 0) ch.fetch(scol + b + lane, Mv + b + lane, Kv + b + lane, 0, wd, s * 64 + lane, x, w, xp);
+  }
   {
     __shared__ int st;
     if (threadIdx.x == 0) st = ctl->stall;
     __syncthreads();
     if (st) return;
   }
-  const int64_t ns = (n + 63) >> 6;
-  const Split sp = block_slices(ns);
-  const int lane = threadIdx.x & 63, wv = wave_id();
   double red[2] = {0.0, 0.0};
-  for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
+  bool first = true;
+  for (; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
-    const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
+    if (!first) {
+      b = sptr[s];
+      wd = (sptr[s + 1] - b) >> 6;
+    }
     const CI* c = scol + b + lane;
     const T* m = Mv + b + lane;
     const T* kk = Kv + b + lane;
     double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
     for (int j0 = 0; j0 < wd; j0 += kChunk) {
-      int32_t cj[kChunk];
-      T mv[kChunk], kv[kChunk];
-      XT xj[kChunk];
-      WT wj[kChunk];
-      T pj[kChunk];
-      load_chunk(c, m, j0, wd, i, cj, mv);
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u) kv[u] = kk[(j0 + u < wd ? j0 + u : wd - 1) * 64];
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        xj[u] = x[cj[u]];
-        wj[u] = w[cj[u]];
-        if (LG) pj[u] = xp[cj[u]];
-      }
+      if (!(first && j0 == 0)) ch.fetch(c, m, kk, j0, wd, i, x, w, xp);
 #pragma unroll
       for (int u = 0; u < kChunk; ++u) {
         if (j0 + u >= wd) break;
         double x0;
-        if (LG) x0 = (double)(XT)(2.0 * (double)xj[u] - (double)pj[u]);
-        else x0 = (double)xj[u];
-        const double mj = (double)mv[u], kj = (double)kv[u];
-        am += mj * (double)wj[u];
-        if (sizeof(WT) == 8) ak += kj * (double)xj[u];
-        else ak += kj * (double)(T)xj[u];
+        if (LG) x0 = (double)(XT)(2.0 * (double)ch.xj[u] - (double)ch.pj[u]);
+        else x0 = (double)ch.xj[u];
+        const double mj = (double)ch.mv[u], kj = (double)ch.kv[u];
+        am += mj * (double)ch.wj[u];
+        if (sizeof(WT) == 8) ak += kj * (double)ch.xj[u];
+        else ak += kj * (double)(T)ch.xj[u];
         mx += mj * x0;
         kx += kj * x0;
       }
     }
+    first = false;
     if (i < n) {
       if (LG) {
         xnext[i] = (XT)(2.0 * (double)x[i] - (double)xp[i]);
