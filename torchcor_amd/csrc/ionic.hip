@@ -312,9 +312,8 @@ __device__ inline void finish_node(const tc_ionic_io& io, const NodeCtx& c, T V,
 // ---------------------------------------------------------------- MS2v
 
 template <class T, bool VF64>
-__global__ void __launch_bounds__(kBlock)
-k_ms2v(tc_ms2v_params p, T* __restrict__ H, tc_ionic_io io) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void ms2v_node(const tc_ms2v_params& p, T* __restrict__ H,
+                                          const tc_ionic_io& io, int64_t i) {
   if (i >= io.n) return;
   const int gated = gate_value(io);
   const NodeCtx c = node_ctx(io, i);
@@ -331,6 +330,12 @@ k_ms2v(tc_ms2v_params p, T* __restrict__ H, tc_ionic_io io) {
   if (gated) return;
   state(H, c, 0) = h + (T)io.dt * dH;
   finish_node<T>(io, c, U, dU, at);
+}
+
+template <class T, bool VF64>
+__global__ void __launch_bounds__(kBlock)
+k_ms2v(tc_ms2v_params p, T* __restrict__ H, tc_ionic_io io) {
+  ms2v_node<T, VF64>(p, H, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- TTP
@@ -539,10 +544,10 @@ TtpK<T> ttp_constants(const tc_ttp_params& p) {
 
 // One TTP step per node (TenTusscherPanfilov.py:353-454).
 template <class T, bool VF64>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_ttp(TtpK<T> K, T* __restrict__ st, const T* __restrict__ Vtab,
-      const T* __restrict__ Ctab, const T* __restrict__ Ktab, TtpTabs tb, tc_ionic_io io) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
+                                         const T* __restrict__ Vtab, const T* __restrict__ Ctab,
+                                         const T* __restrict__ Ktab, const TtpTabs& tb,
+                                         const tc_ionic_io& io, int64_t i) {
   const int64_t n = io.n;
   if (i >= n) return;
   const int gated = gate_value(io);
@@ -649,6 +654,13 @@ k_ttp(TtpK<T> K, T* __restrict__ st, const T* __restrict__ Vtab,
   const int gates[11] = {TS_D, TS_F, TS_F2, TS_H, TS_J, TS_M, TS_R, TS_S, TS_Xr1, TS_Xr2, TS_Xs};
 #pragma unroll
   for (int k = 0; k < 11; ++k) state(st, c, gates[k]) = o[gates[k]];
+}
+
+template <class T, bool VF64>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_ttp(TtpK<T> K, T* __restrict__ st, const T* __restrict__ Vtab,
+      const T* __restrict__ Ctab, const T* __restrict__ Ktab, TtpTabs tb, tc_ionic_io io) {
+  ttp_node<T, VF64>(K, st, Vtab, Ctab, Ktab, tb, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- CRN
@@ -832,10 +844,10 @@ CrnK<T> crn_constants(const tc_crn_params& p, double dt) {
 // phases like the TTP step: currents and concentrations first (with the
 // V-table groups they read), then the Rush-Larsen gates.
 template <class T, bool VF64>
-__global__ void __launch_bounds__(kBlock)
-k_crn(CrnK<T> K, T* __restrict__ st, const T* __restrict__ Ctab,
-      const T* __restrict__ Vtab, const T* __restrict__ Ftab, CrnTabs tb, tc_ionic_io io) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void crn_node(const CrnK<T>& K, T* __restrict__ st,
+                                         const T* __restrict__ Ctab, const T* __restrict__ Vtab,
+                                         const T* __restrict__ Ftab, const CrnTabs& tb,
+                                         const tc_ionic_io& io, int64_t i) {
   const int64_t n = io.n;
   if (i >= n) return;
   const int gated = gate_value(io);
@@ -914,6 +926,50 @@ k_crn(CrnK<T> K, T* __restrict__ st, const T* __restrict__ Ctab,
   state(st, c, CS_##G) = vr(CV_##G##_A, CV_NCOLS) + vr(CV_##G##_B, CV_NCOLS) * s[CS_##G];
   RLU(d) RLU(f) RLU(h) RLU(j) RLU(m) RLU(oa) RLU(oi) RLU(ua) RLU(ui) RLU(w) RLU(xr) RLU(xs)
 #undef RLU
+}
+
+template <class T, bool VF64>
+__global__ void __launch_bounds__(kBlock)
+k_crn(CrnK<T> K, T* __restrict__ st, const T* __restrict__ Ctab,
+      const T* __restrict__ Vtab, const T* __restrict__ Ftab, CrnTabs tb, tc_ionic_io io) {
+  crn_node<T, VF64>(K, st, Ctab, Vtab, Ftab, tb, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
+}
+
+// The three models of a pool in one launch: blocks [0, b1) step the MS2v
+// entries, [b1, b2) the TTP entries, the rest the CRN entries (each model's
+// io lists its own slices).  One launch instead of three removes two kernel
+// boundaries from every pooled time step.  PoolIonic gathers the host side
+// of the arguments.
+template <class T>
+struct PoolIonic {
+  tc_ms2v_params mp;
+  TtpK<T> tk;
+  CrnK<T> ck;
+  T* st[3];
+  const T* ttab[3];
+  const T* ctab[3];
+  TtpTabs ttb;
+  CrnTabs ctb;
+  tc_ionic_io io[3];
+  int b1, b2;
+};
+
+// (separate by-value arguments: the kernel argument block is read in place)
+template <class T, bool VF64>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_pool_ionic(tc_ms2v_params mp, TtpK<T> tk, CrnK<T> ck, T* __restrict__ st0,
+             T* __restrict__ st1, T* __restrict__ st2, const T* __restrict__ tt0,
+             const T* __restrict__ tt1, const T* __restrict__ tt2, const T* __restrict__ ct0,
+             const T* __restrict__ ct1, const T* __restrict__ ct2, TtpTabs ttb, CrnTabs ctb,
+             tc_ionic_io io0, tc_ionic_io io1, tc_ionic_io io2, int b1, int b2) {
+  const int b = blockIdx.x;
+  if (b < b1) {
+    ms2v_node<T, VF64>(mp, st0, io0, (int64_t)b * kBlock + threadIdx.x);
+  } else if (b < b2) {
+    ttp_node<T, VF64>(tk, st1, tt0, tt1, tt2, ttb, io1, (int64_t)(b - b1) * kBlock + threadIdx.x);
+  } else {
+    crn_node<T, VF64>(ck, st2, ct0, ct1, ct2, ctb, io2, (int64_t)(b - b2) * kBlock + threadIdx.x);
+  }
 }
 
 template <class T>
@@ -1078,6 +1134,73 @@ extern "C" int tc_crn_step(const tc_crn_params* p, void* states, const void* Cai
                            int dtype, void* stream) {
   return crn_step(p, states, Cai_tab, V_tab, fn_tab, io, dtype, (hipStream_t)stream, nullptr,
                   nullptr);
+}
+
+template <class T>
+int pool_launch(const PoolIonic<T>& a, bool vf64, int blocks, hipStream_t s) {
+  auto k = vf64 ? k_pool_ionic<T, true> : k_pool_ionic<T, false>;
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), 0, s, a.mp, a.tk, a.ck, a.st[0], a.st[1],
+                     a.st[2], a.ttab[0], a.ttab[1], a.ttab[2], a.ctab[0], a.ctab[1], a.ctab[2],
+                     a.ttb, a.ctb, a.io[0], a.io[1], a.io[2], a.b1, a.b2);
+  TC_LAUNCH();
+  return TC_OK;
+}
+
+int tc::pool_ionic_step(const tc_ms2v_params* mp, const tc_ttp_params* tp,
+                        const tc_crn_params* cp, void* const states[3],
+                        const void* const ttp_tab[3], const void* const crn_tab[3],
+                        const tc_ionic_io io[3], int dtype, hipStream_t s) {
+  int64_t nb[3];
+  for (int m = 0; m < 3; ++m) {
+    nb[m] = io[m].n > 0 ? ceil_div(io[m].n, kBlock) : 0;
+    if (nb[m] == 0) continue;
+    int rc = check_io(&io[m], dtype);
+    if (rc) return rc;
+    TC_CHECK(states[m] && (m != 0 || mp) && (m != 1 || (tp && ttp_tab[0] && ttp_tab[1] && ttp_tab[2])) &&
+                 (m != 2 || (cp && crn_tab[0] && crn_tab[1] && crn_tab[2])),
+             TC_ERR_ARG, "pool ionic step: model %d lacks parameters, states or tables", m);
+  }
+  int vf = -1;
+  for (int m = 0; m < 3; ++m) {
+    if (nb[m] == 0) continue;
+    TC_CHECK(vf < 0 || vf == io[m].v_f64, TC_ERR_ARG, "pool ionic step: mixed V types");
+    vf = io[m].v_f64;
+  }
+  const int64_t blocks = nb[0] + nb[1] + nb[2];
+  if (blocks == 0) return TC_OK;
+  TC_CHECK(blocks < (1LL << 31), TC_ERR_SIZE, "pool ionic step: too many entries");
+  const bool vf64 = dtype == TC_F32 && vf == 1;
+  TtpTabs ttb = ttp_tabs();
+  CrnTabs ctb = crn_tabs();
+  int rc;
+  if ((rc = check_tab(ttb.V)) || (rc = check_tab(ttb.CaSS)) || (rc = check_tab(ttb.VEk)) ||
+      (rc = check_tab(ctb.Cai)) || (rc = check_tab(ctb.V)) || (rc = check_tab(ctb.fn)))
+    return rc;
+  auto fill = [&](auto& a, auto tag) -> void {
+    using T = decltype(tag);
+    a.mp = mp ? *mp : tc_ms2v_params{};
+    if (tp) a.tk = ttp_constants<T>(*tp);
+    if (cp) a.ck = crn_constants<T>(*cp, io[2].dt);
+    for (int m = 0; m < 3; ++m) {
+      a.st[m] = (T*)states[m];
+      a.io[m] = io[m];
+      if (nb[m] == 0) a.io[m].n = 0;
+      a.ttab[m] = (const T*)ttp_tab[m];
+      a.ctab[m] = (const T*)crn_tab[m];
+    }
+    a.ttb = ttb;
+    a.ctb = ctb;
+    a.b1 = (int)nb[0];
+    a.b2 = (int)(nb[0] + nb[1]);
+  };
+  if (dtype == TC_F32) {
+    PoolIonic<float> a{};
+    fill(a, 0.0f);
+    return pool_launch(a, vf64, (int)blocks, s);
+  }
+  PoolIonic<double> a{};
+  fill(a, 0.0);
+  return pool_launch(a, false, (int)blocks, s);
 }
 
 extern "C" int tc_lat_update(const void* u, void* AT, void* RT, int64_t n, double t, int dtype,

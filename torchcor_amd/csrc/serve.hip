@@ -95,11 +95,28 @@ __device__ inline LRange block_list(int64_t nl) {
 constexpr int kSPer = 8;
 constexpr int kSBlkMax = 1024;
 
+// One block's work, formed on the host when the lists are rebuilt: a
+// contiguous range of the slice list, all of one system, with the system
+// fields the passes need (one scalar load instead of a chain of lookups).
+struct SvBlk {
+  int32_t l0, l1;       // list range
+  int32_t q, sys;       // live-system index (-1: padding block) and slot
+  int64_t row_end;      // first pool row past the system
+  int32_t use16;        // the system's entries have int16 column offsets
+  int32_t vb0, vb1;     // the system's block range
+  int32_t pad;
+};
+
+// One entry of the slice list: the slice and its SELL position and width.
+struct SvEnt {
+  int32_t sl, off, w, pad;
+};
+
 struct SvPool {
+  const SvBlk* blk;         // per block
+  const SvEnt* ent;         // per list entry
   const int32_t* list;      // slices of the live systems, system by system
   int64_t nl;
-  const int32_t* blk_l0;    // list range of block v: [blk_l0[v], blk_l0[v+1])
-  const int32_t* blk_q;     // system index of block v (-1: padding block)
   const int32_t* sys_list;  // slot of system q
   const int32_t* sys_vb0;   // first and last block of system q
   const int32_t* sys_vb1;
@@ -119,17 +136,6 @@ struct BlockView {
   int q;           // system index, -1 for a padding block
 };
 
-__device__ inline BlockView block_view(const SvPool& pl) {
-  const int G = gridDim.x;
-  const int b = blockIdx.x;
-  BlockView v;
-  v.vb = b;
-  if (G % kXcd == 0) v.vb = (b % kXcd) * (G / kXcd) + b / kXcd;
-  v.l0 = pl.blk_l0[v.vb];
-  v.l1 = pl.blk_l0[v.vb + 1];
-  v.q = pl.blk_q[v.vb];
-  return v;
-}
 
 // Sum of system q's block partials (nv values each): called by a whole
 // wavefront, result in every lane.
@@ -146,17 +152,30 @@ __device__ inline void fold_sys(const SvPool& pl, const double* __restrict__ P, 
   b = wave_sum(s1);
 }
 
-// fold_sys with every thread of the block (same order in every block):
-// result in every thread.
-__device__ inline void fold_sys_block(const SvPool& pl, const double* __restrict__ P, int nv,
-                                      int q, double& a, double& b) {
+// Sum of the block partials [v0, v1] (at most kSBlkMax blocks, nv values
+// each) with every thread of the block, in the same order in every block:
+// each thread issues its loads (clamped index, then a select) before adding
+// them in order.  Result in every thread.
+__device__ inline void fold_range_block(const double* __restrict__ P, int nv, int v0, int v1,
+                                        double& a, double& b) {
   __shared__ double fs[kSW][2];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int v0 = pl.sys_vb0[q], v1 = pl.sys_vb1[q];
+  constexpr int R = kSBlkMax / kSB;
+  double x0[R], x1[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int v = v0 + (int)threadIdx.x + q * kSB;
+    const int vc = v <= v1 ? v : v1;
+    const double t0 = P[(int64_t)vc * nv];
+    const double t1 = nv == 2 ? P[(int64_t)vc * nv + 1] : 0.0;
+    x0[q] = v <= v1 ? t0 : 0.0;
+    x1[q] = v <= v1 ? t1 : 0.0;
+  }
   double s0 = 0.0, s1 = 0.0;
-  for (int v = v0 + (int)threadIdx.x; v <= v1; v += (int)blockDim.x) {
-    s0 += P[(int64_t)v * nv];
-    if (nv == 2) s1 += P[(int64_t)v * nv + 1];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    s0 += x0[q];
+    s1 += x1[q];
   }
   s0 = wave_sum(s0);
   s1 = wave_sum(s1);
@@ -172,6 +191,12 @@ __device__ inline void fold_sys_block(const SvPool& pl, const double* __restrict
     b += fs[w][1];
   }
   __syncthreads();
+}
+
+// Position of this block in the block order (blocks of one XCD adjacent).
+__device__ inline int sv_vblock() {
+  const int G = gridDim.x, b = blockIdx.x;
+  return G % kXcd == 0 ? (b % kXcd) * (G / kXcd) + b / kXcd : b;
 }
 
 // Block partials: each lane's sums over the slices of its wavefront, a
@@ -198,121 +223,56 @@ __device__ inline void store_block_partials(const BlockView& v, double r0, doubl
   }
 }
 
-// Block gate: the pipeline is stalled, or (need_active) every system of the
-// step has stopped.
-__device__ inline bool sv_gate(const SvCtl* ctl, bool need_active) {
-  __shared__ int h;
-  if (threadIdx.x == 0) h = ctl->stall != 0 || (need_active && ctl->n_active == 0);
-  __syncthreads();
-  return h != 0;
-}
-
-// One row of the fused RHS + start pass: M w, K u, M x0, K x0 over the
-// row's SELL entries (columns as int32 pool rows or int16 offsets).
+// Rows of one block of the fused RHS + PCG start pass.  The first chunk of
+// each wavefront's first slice is fetched before the barrier that publishes
+// the stall flag (*s_stop), so the flag's latency overlaps those loads.
 template <class T, class XT, class WT, bool LG, class CI>
-__device__ inline void rhs_row(const CI* __restrict__ c, const T* __restrict__ m,
-                               const T* __restrict__ kk, int wd, int64_t i,
-                               const XT* __restrict__ x, const WT* __restrict__ w,
-                               const T* __restrict__ xp, double& am, double& ak, double& mx,
-                               double& kx) {
-  for (int j0 = 0; j0 < wd; j0 += kChunk) {
-    int32_t cj[kChunk];
-    T mv[kChunk], kv[kChunk];
-    XT xj[kChunk];
-    WT wj[kChunk];
-    T pj[kChunk];
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-      const int jj = j0 + u < wd ? j0 + u : wd - 1;
-      cj[u] = col_of(c[jj * 64], i);
-      mv[u] = m[jj * 64];
-      kv[u] = kk[jj * 64];
-    }
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-      xj[u] = x[cj[u]];
-      wj[u] = w[cj[u]];
-      if (LG) pj[u] = xp[cj[u]];
-    }
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-      if (j0 + u >= wd) break;
-      double x0;
-      if (LG) x0 = (double)(XT)(2.0 * (double)xj[u] - (double)pj[u]);
-      else x0 = (double)xj[u];
-      const double mj = (double)mv[u], kj = (double)kv[u];
-      am += mj * (double)wj[u];
-      if (sizeof(WT) == 8) ak += kj * (double)xj[u];
-      else ak += kj * (double)(T)xj[u];
-      mx += mj * x0;
-      kx += kj * x0;
-    }
-  }
-}
-
-// One row of the spmv pass: (A p')_i with p' formed on the fly.
-template <class T, class CI>
-__device__ inline double spmv_row(const CI* __restrict__ c, const T* __restrict__ a, int wd,
-                                  int64_t i, int k, double beta, const T* __restrict__ Dinv,
-                                  const T* __restrict__ r, const T* __restrict__ pin) {
-  double acc = 0.0;
-  for (int j0 = 0; j0 < wd; j0 += kChunk) {
-    int32_t cj[kChunk];
-    T av[kChunk], pj[kChunk];
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-      const int jj = j0 + u < wd ? j0 + u : wd - 1;
-      cj[u] = col_of(c[jj * 64], i);
-      av[u] = a[jj * 64];
-    }
-    if (k == 0) {
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u) pj[u] = (T)((double)Dinv[cj[u]] * (double)r[cj[u]]);
-    } else {
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u)
-        pj[u] = (T)(beta * (double)pin[cj[u]] + (double)Dinv[cj[u]] * (double)r[cj[u]]);
-    }
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-      const double tt = acc + (double)av[u] * (double)pj[u];
-      acc = j0 + u < wd ? tt : acc;
-    }
-  }
-  return acc;
-}
-
-// Fused RHS + PCG start over the slice list (as k_rhs_init); block partials
-// of r.z and (Dinv b)^2.
-template <class T, class XT, class WT, bool LG>
-__global__ void __launch_bounds__(kSB)
-k_sv_rhs_init(SvPool pl, int nsys, const T* __restrict__ Mv, const T* __restrict__ Kv,
-              const T* __restrict__ Dinv, const WT* __restrict__ w, const XT* __restrict__ x,
-              const T* __restrict__ xp, XT* __restrict__ xnext, T* __restrict__ xpnext,
-              T* __restrict__ r, double coef, double Cm, double tdt, SvCtl* ctl,
-              double* P0) {
-  if (sv_gate(ctl, false)) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->n_active = nsys;
+__device__ inline void sv_rhs_rows(const SvBlk& bk, const SvEnt* __restrict__ ent,
+                                   const CI* __restrict__ cols, const T* __restrict__ Mv,
+                                   const T* __restrict__ Kv, const T* __restrict__ Dinv,
+                                   const WT* __restrict__ w, const XT* __restrict__ x,
+                                   const T* __restrict__ xp, XT* __restrict__ xnext,
+                                   T* __restrict__ xpnext, T* __restrict__ r, double coef,
+                                   double Cm, double tdt, const int* s_stop, double& red0,
+                                   double& red1) {
   const int lane = threadIdx.x & 63, wv = wave_id();
-  const BlockView bv = block_view(pl);
-  double red0 = 0.0, red1 = 0.0;
-  // one system per block: its fields once
-  const int sys = bv.q >= 0 ? pl.sys_list[bv.q] : 0;
-  const int64_t row_end = bv.q >= 0 ? ctl->row0[sys] + ctl->n[sys] : 0;
-  const bool use16 = bv.q >= 0 && ctl->use16[sys];
-  for (int64_t li = bv.l0 + wv; li < bv.l1; li += kSW) {
-    const int32_t sl = pl.list[li];
-    const int64_t i = (int64_t)sl * 64 + lane;
-    const bool ok = i < row_end;
-    const int32_t b = pl.slice_off[sl], wd = pl.slice_w[sl];
-    const T* m = Mv + b + lane;
-    const T* kk = Kv + b + lane;
+  int64_t li = bk.l0 + wv;
+  RhsChunk<T, XT, WT, LG, CI> ch;
+  SvEnt e{0, 0, 0, 0};
+  if (li < bk.l1) {
+    e = ent[li];
+    if (e.w > 0)
+      ch.fetch(cols + e.off + lane, Mv + e.off + lane, Kv + e.off + lane, 0, e.w,
+               (int64_t)e.sl * 64 + lane, x, w, xp);
+  }
+  __syncthreads();
+  if (*s_stop) return;
+  bool first = true;
+  for (; li < bk.l1; li += kSW) {
+    if (!first) e = ent[li];
+    const int64_t i = (int64_t)e.sl * 64 + lane;
+    const CI* c = cols + e.off + lane;
+    const T* m = Mv + e.off + lane;
+    const T* kk = Kv + e.off + lane;
     double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
-    if (use16)
-      rhs_row<T, XT, WT, LG, int16_t>(pl.dcol + b + lane, m, kk, wd, i, x, w, xp, am, ak, mx, kx);
-    else
-      rhs_row<T, XT, WT, LG, int32_t>(pl.col + b + lane, m, kk, wd, i, x, w, xp, am, ak, mx, kx);
-    if (ok) {
+    for (int j0 = 0; j0 < e.w; j0 += kChunk) {
+      if (!(first && j0 == 0)) ch.fetch(c, m, kk, j0, e.w, i, x, w, xp);
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        if (j0 + u >= e.w) break;
+        double x0;
+        if (LG) x0 = (double)(XT)(2.0 * (double)ch.xj[u] - (double)ch.pj[u]);
+        else x0 = (double)ch.xj[u];
+        const double mj = (double)ch.mv[u], kj = (double)ch.kv[u];
+        am += mj * (double)ch.wj[u];
+        if (sizeof(WT) == 8) ak += kj * (double)ch.xj[u];
+        else ak += kj * (double)(T)ch.xj[u];
+        mx += mj * x0;
+        kx += kj * x0;
+      }
+    }
+    first = false;
+    if (i < bk.row_end) {
       if (LG) {
         xnext[i] = (XT)(2.0 * (double)x[i] - (double)xp[i]);
         xpnext[i] = (T)x[i];
@@ -327,89 +287,125 @@ k_sv_rhs_init(SvPool pl, int nsys, const T* __restrict__ Mv, const T* __restrict
       red1 += db * db;
     }
   }
-  store_block_partials(bv, red0, red1, 2, P0);
 }
 
-// spmv pass of iteration k.  First the scalars of the block's systems: k = 0
-// takes r.z and ||Dinv b|| from the start pass; k > 0 applies the stopping
-// test of iteration k-1 (solver.py:57-74) to the last update's sums, then
-// beta.  The first block covering a system records the outcome.  Then
-// p' = beta p + Dinv r on the fly, Ap', block partials of p'.Ap'.
-template <class T>
+// Fused RHS + PCG start over the slice list (as k_rhs_init); block partials
+// of r.z and (Dinv b)^2.
+template <class T, class XT, class WT, bool LG>
 __global__ void __launch_bounds__(kSB)
-k_sv_spmv(SvPool pl, const T* __restrict__ Av, const T* __restrict__ Dinv,
-          const T* __restrict__ r, const T* __restrict__ pin, T* __restrict__ pout,
-          T* __restrict__ Ap, int k, int max_sys, SvCtl* ctl, const double* __restrict__ P0,
-          const double* __restrict__ P2, double* P1) {
-  if (sv_gate(ctl, k > 0)) return;
+k_sv_rhs_init(SvPool pl, int nsys, const T* __restrict__ Mv, const T* __restrict__ Kv,
+              const T* __restrict__ Dinv, const WT* __restrict__ w, const XT* __restrict__ x,
+              const T* __restrict__ xp, XT* __restrict__ xnext, T* __restrict__ xpnext,
+              T* __restrict__ r, double coef, double Cm, double tdt, SvCtl* ctl,
+              double* P0) {
+  __shared__ int s_stop;
+  const int vb = sv_vblock();
+  const SvBlk bk = pl.blk[vb];
+  if (threadIdx.x == 0) s_stop = ctl->stall;
+  double red0 = 0.0, red1 = 0.0;
+  if (bk.use16)
+    sv_rhs_rows<T, XT, WT, LG, int16_t>(bk, pl.ent, pl.dcol, Mv, Kv, Dinv, w, x, xp, xnext,
+                                        xpnext, r, coef, Cm, tdt, &s_stop, red0, red1);
+  else
+    sv_rhs_rows<T, XT, WT, LG, int32_t>(bk, pl.ent, pl.col, Mv, Kv, Dinv, w, x, xp, xnext,
+                                        xpnext, r, coef, Cm, tdt, &s_stop, red0, red1);
+  if (s_stop) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->n_active = nsys;
+  store_block_partials(BlockView{vb, bk.l0, bk.l1, bk.q}, red0, red1, 2, P0);
+}
+
+// Rows of one block of the spmv pass of iteration k.  First the scalars of
+// the block's system: k = 0 takes r.z and ||Dinv b|| from the start pass;
+// k > 0 applies the stopping test of iteration k-1 (solver.py:57-74) to the
+// last update's sums, then beta.  The system's first block records the
+// outcome.  Then p' = beta p + Dinv r on the fly, Ap', block partials of
+// p'.Ap'.  The flags, the partial sums and the first chunk of each
+// wavefront's first slice are all loaded before the first wait.
+template <class T, class CI>
+__device__ inline void sv_spmv_block(const SvPool& pl, const SvBlk& bk, int vb,
+                                     const CI* __restrict__ cols, const T* __restrict__ Av,
+                                     const T* __restrict__ Dinv, const T* __restrict__ r,
+                                     const T* __restrict__ pin, T* __restrict__ pout,
+                                     T* __restrict__ Ap, int k, int max_sys, SvCtl* ctl,
+                                     const double* __restrict__ P0,
+                                     const double* __restrict__ P2, double* P1) {
+  __shared__ int s_stop, s_pre, s_act;
   __shared__ double s_beta;
-  __shared__ int s_act;
   const int lane = threadIdx.x & 63, wv = wave_id();
-  const BlockView bv = block_view(pl);
-  // one read of the flag for the whole block (another block may clear it
-  // during this pass; the branch below must be block-uniform)
-  __shared__ int s_pre;
-  if (threadIdx.x == 0) s_pre = bv.q >= 0 ? ctl->active[pl.sys_list[bv.q]] : 0;
-  __syncthreads();
-  if (bv.q >= 0) {
-    const int q = bv.q;
-    const int s = pl.sys_list[q];
-    const bool first = pl.sys_vb0[q] == bv.vb;
-    double a, b;
+  const int sys = bk.sys;
+  // one read of the flags for the whole block (another block may clear the
+  // system's flag during this pass; the branches below must be uniform)
+  double bn = 1.0, rzp = 1.0;
+  if (threadIdx.x == 0) {
+    s_stop = ctl->stall != 0 || (k > 0 && ctl->n_active == 0);
+    s_pre = bk.q >= 0 ? ctl->active[sys] : 0;
+    if (k > 0 && bk.q >= 0) {
+      bn = ctl->bnorm[sys];
+      rzp = ctl->rz2[((k - 1) & 1) * max_sys + sys];
+    }
+  }
+  int64_t li = bk.l0 + wv;
+  SpmvChunk<T, CI> ch;
+  SvEnt e{0, 0, 0, 0};
+  if (li < bk.l1) {
+    e = pl.ent[li];
+    if (e.w > 0)
+      ch.fetch(cols + e.off + lane, Av + e.off + lane, 0, e.w, (int64_t)e.sl * 64 + lane, Dinv, r,
+               pin, k);
+  }
+  double a = 0.0, b = 0.0;
+  if (bk.q >= 0) fold_range_block(k == 0 ? P0 : P2, 2, bk.vb0, bk.vb1, a, b);
+  else __syncthreads();
+  if (s_stop || bk.q < 0) return;
+  const bool firstblk = bk.vb0 == vb;
+  if (threadIdx.x == 0) {
     if (k == 0) {
-      fold_sys_block(pl, P0, 2, q, a, b);
-      if (threadIdx.x == 0) {
-        s_act = 1;
-        s_beta = 0.0;
-        if (first) {
-          ctl->rz2[s] = a;
-          ctl->bnorm[s] = sqrt(b);
-          ctl->active[s] = 1;
-          ctl->n_iter[s] = 0;
-        }
+      s_act = 1;
+      s_beta = 0.0;
+      if (firstblk) {
+        ctl->rz2[sys] = a;
+        ctl->bnorm[sys] = sqrt(b);
+        ctl->active[sys] = 1;
+        ctl->n_iter[sys] = 0;
       }
     } else if (!s_pre) {
-      if (threadIdx.x == 0) s_act = 0;
+      s_act = 0;
     } else {
-      fold_sys_block(pl, P2, 2, q, a, b);
-      if (threadIdx.x == 0) {
-        const double zn = sqrt(a);
-        if (zn < ctl->a_tol || zn / ctl->bnorm[s] < ctl->r_tol) {
-          s_act = 0;
-          if (first) {
-            ctl->active[s] = 0;
-            ctl->n_iter[s] = k - 1;
-            atomicSub(&ctl->n_active, 1);
-          }
-        } else {
-          s_act = 1;
-          s_beta = b / ctl->rz2[((k - 1) & 1) * max_sys + s];
-          if (first) {
-            ctl->rz2[(k & 1) * max_sys + s] = b;
-            ctl->n_iter[s] = k;
-          }
+      const double zn = sqrt(a);
+      if (zn < ctl->a_tol || zn / bn < ctl->r_tol) {
+        s_act = 0;
+        if (firstblk) {
+          ctl->active[sys] = 0;
+          ctl->n_iter[sys] = k - 1;
+          atomicSub(&ctl->n_active, 1);
+        }
+      } else {
+        s_act = 1;
+        s_beta = b / rzp;
+        if (firstblk) {
+          ctl->rz2[(k & 1) * max_sys + sys] = b;
+          ctl->n_iter[sys] = k;
         }
       }
     }
   }
   __syncthreads();
-  if (bv.q < 0 || !s_act) return;
+  if (!s_act) return;
   const double beta = s_beta;
   double red = 0.0;
-  const int sys = pl.sys_list[bv.q];
-  const int64_t row_end = ctl->row0[sys] + ctl->n[sys];
-  const bool use16 = ctl->use16[sys];
-  for (int64_t li = bv.l0 + wv; li < bv.l1; li += kSW) {
-    const int32_t sl = pl.list[li];
-    const int64_t i = (int64_t)sl * 64 + lane;
-    const bool ok = i < row_end;
-    const int32_t b = pl.slice_off[sl], wd = pl.slice_w[sl];
-    const T* a = Av + b + lane;
-    const double acc =
-        use16
-            ? spmv_row<T, int16_t>(pl.dcol + b + lane, a, wd, i, k, beta, Dinv, r, pin)
-            : spmv_row<T, int32_t>(pl.col + b + lane, a, wd, i, k, beta, Dinv, r, pin);
-    if (ok) {
+  bool first = true;
+  for (; li < bk.l1; li += kSW) {
+    if (!first) e = pl.ent[li];
+    const int64_t i = (int64_t)e.sl * 64 + lane;
+    const CI* c = cols + e.off + lane;
+    const T* av = Av + e.off + lane;
+    double acc = 0.0;
+    for (int j0 = 0; j0 < e.w; j0 += kChunk) {
+      if (!(first && j0 == 0)) ch.fetch(c, av, j0, e.w, i, Dinv, r, pin, k);
+      acc = ch.add(acc, j0, e.w, k, beta);
+    }
+    first = false;
+    if (i < bk.row_end) {
       T pi;
       if (k == 0) pi = (T)((double)Dinv[i] * (double)r[i]);
       else pi = (T)(beta * (double)pin[i] + (double)Dinv[i] * (double)r[i]);
@@ -419,57 +415,95 @@ k_sv_spmv(SvPool pl, const T* __restrict__ Av, const T* __restrict__ Dinv,
       red += (double)pi * (double)api;
     }
   }
-  store_block_partials(bv, red, 0.0, 1, P1);
+  store_block_partials(BlockView{vb, bk.l0, bk.l1, bk.q}, red, 0.0, 1, P1);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kSB)
+k_sv_spmv(SvPool pl, const T* __restrict__ Av, const T* __restrict__ Dinv,
+          const T* __restrict__ r, const T* __restrict__ pin, T* __restrict__ pout,
+          T* __restrict__ Ap, int k, int max_sys, SvCtl* ctl, const double* __restrict__ P0,
+          const double* __restrict__ P2, double* P1) {
+  const int vb = sv_vblock();
+  const SvBlk bk = pl.blk[vb];
+  if (bk.use16)
+    sv_spmv_block<T, int16_t>(pl, bk, vb, pl.dcol, Av, Dinv, r, pin, pout, Ap, k, max_sys, ctl,
+                              P0, P2, P1);
+  else
+    sv_spmv_block<T, int32_t>(pl, bk, vb, pl.col, Av, Dinv, r, pin, pout, Ap, k, max_sys, ctl,
+                              P0, P2, P1);
 }
 
 // update pass of iteration k: alpha = rz / p'.Ap' per system, x += alpha p',
-// r -= alpha Ap', block partials of z.z and r.z.
+// r -= alpha Ap', block partials of z.z and r.z.  The first slice's operands,
+// the flags, r.z and the partial sums load before the first wait.
 template <class T, class XT>
 __global__ void __launch_bounds__(kSB)
 k_sv_update(SvPool pl, const T* __restrict__ Dinv, const T* __restrict__ p,
             const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, int k, int max_sys,
             SvCtl* ctl, const double* __restrict__ P1, double* P2) {
-  if (sv_gate(ctl, true)) return;
-  __shared__ double s_alpha;
-  __shared__ int s_act;
+  __shared__ int s_stop, s_pre;
   const int lane = threadIdx.x & 63, wv = wave_id();
-  const BlockView bv = block_view(pl);
-  __shared__ int s_pre;
-  if (threadIdx.x == 0) s_pre = bv.q >= 0 ? ctl->active[pl.sys_list[bv.q]] : 0;
-  __syncthreads();
-  if (bv.q >= 0) {
-    const int s = pl.sys_list[bv.q];
-    if (!s_pre) {
-      if (threadIdx.x == 0) s_act = 0;
-    } else {
-      double a, b;
-      fold_sys_block(pl, P1, 1, bv.q, a, b);
-      if (threadIdx.x == 0) {
-        s_act = 1;
-        s_alpha = ctl->rz2[(k & 1) * max_sys + s] / a;
-      }
+  const int vb = sv_vblock();
+  const SvBlk bk = pl.blk[vb];
+  if (threadIdx.x == 0) {
+    s_stop = ctl->stall != 0 || ctl->n_active == 0;
+    s_pre = bk.q >= 0 ? ctl->active[bk.sys] : 0;
+  }
+  const double rzk = bk.q >= 0 ? ctl->rz2[(k & 1) * max_sys + bk.sys] : 1.0;
+  int64_t li = bk.l0 + wv;
+  int64_t i = 0;
+  XT xv = XT(0);
+  T pv = T(0), av = T(0), rv = T(0), dv = T(0);
+  if (li < bk.l1) {
+    i = (int64_t)pl.ent[li].sl * 64 + lane;
+    if (i < bk.row_end) {
+      xv = x[i];
+      pv = p[i];
+      av = Ap[i];
+      rv = r[i];
+      dv = Dinv[i];
     }
   }
-  __syncthreads();
-  if (bv.q < 0 || !s_act) return;
-  const double alpha = s_alpha;
+  double a = 0.0, b = 0.0;
+  if (bk.q >= 0) fold_range_block(P1, 1, bk.vb0, bk.vb1, a, b);
+  else __syncthreads();
+  if (s_stop || bk.q < 0 || !s_pre) return;
+  const double alpha = rzk / a;
   double red0 = 0.0, red1 = 0.0;
-  const int sys = pl.sys_list[bv.q];
-  const int64_t row_end = ctl->row0[sys] + ctl->n[sys];
-  for (int64_t li = bv.l0 + wv; li < bv.l1; li += kSW) {
-    const int32_t sl = pl.list[li];
-    const int64_t i = (int64_t)sl * 64 + lane;
-    if (i < row_end) {
-      const XT xi = (XT)((double)x[i] + alpha * (double)p[i]);
-      const T ri = (T)((double)r[i] - alpha * (double)Ap[i]);
+  while (li < bk.l1) {
+    if (i < bk.row_end) {
+      const XT xi = (XT)((double)xv + alpha * (double)pv);
+      const T ri = (T)((double)rv - alpha * (double)av);
       x[i] = xi;
       r[i] = ri;
-      const double z = (double)Dinv[i] * (double)ri;
+      const double z = (double)dv * (double)ri;
       red0 += z * z;
       red1 += (double)ri * z;
     }
+    li += kSW;
+    if (li < bk.l1) {
+      i = (int64_t)pl.ent[li].sl * 64 + lane;
+      if (i < bk.row_end) {
+        xv = x[i];
+        pv = p[i];
+        av = Ap[i];
+        rv = r[i];
+        dv = Dinv[i];
+      }
+    }
   }
-  store_block_partials(bv, red0, red1, 2, P2);
+  store_block_partials(BlockView{vb, bk.l0, bk.l1, bk.q}, red0, red1, 2, P2);
+}
+
+// The list entries' slice positions (after the lists change).
+__global__ void k_sv_ent(int64_t nl, const int32_t* __restrict__ list,
+                         const int32_t* __restrict__ slice_off,
+                         const int32_t* __restrict__ slice_w, SvEnt* ent) {
+  const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nl) return;
+  const int32_t sl = list[li];
+  ent[li] = SvEnt{sl, slice_off[sl], slice_w[sl], 0};
 }
 
 // One block: end of a launch chunk after k_end iterations.  The stopping
@@ -670,6 +704,7 @@ struct SysHost {
   int model = 0;
   int64_t slice0 = 0, ns = 0, entry0 = 0, ne = 0, n = 0;
   bool lat = false;
+  bool use16 = false;
 };
 
 struct SPlan {
@@ -695,8 +730,8 @@ struct tc_serve {
   int32_t* sys_list;    // device: slot of live system q
   int32_t* sys_vb0;     // device: blocks of system q
   int32_t* sys_vb1;
-  int32_t* blk_l0;      // device: list range of each block (G + 1)
-  int32_t* blk_q;       // device: system of each block
+  tc::SvBlk* blk_desc;  // device: per block
+  tc::SvEnt* ent;       // device: per list entry
   int64_t nl, nml[3];
   int nsys, G;
   std::vector<tc::SysHost> sys;
@@ -713,7 +748,7 @@ namespace tc {
 namespace {
 
 struct SvLayout {
-  size_t ctl, arrays, part[3], list, mlist[3], sys_list, sys_vb0, sys_vb1, blk_l0, blk_q, total;
+  size_t ctl, arrays, part[3], list, mlist[3], sys_list, sys_vb0, sys_vb1, blk_desc, ent, total;
 };
 
 SvLayout sv_layout(int64_t cap_slices, int max_sys) {
@@ -739,18 +774,18 @@ SvLayout sv_layout(int64_t cap_slices, int max_sys) {
   L.sys_list = take(ms * 4);
   L.sys_vb0 = take(ms * 4);
   L.sys_vb1 = take(ms * 4);
-  L.blk_l0 = take((slots + 1) * 4);
-  L.blk_q = take(slots * 4);
+  L.blk_desc = take(slots * sizeof(SvBlk));
+  L.ent = take((size_t)cap_slices * sizeof(SvEnt));
   L.total = off;
   return L;
 }
 
 SvPool sv_pool(const tc_serve* S) {
   SvPool p;
+  p.blk = S->blk_desc;
+  p.ent = S->ent;
   p.list = S->list;
   p.nl = S->nl;
-  p.blk_l0 = S->blk_l0;
-  p.blk_q = S->blk_q;
   p.sys_list = S->sys_list;
   p.sys_vb0 = S->sys_vb0;
   p.sys_vb1 = S->sys_vb1;
@@ -800,19 +835,39 @@ int sv_rebuild(tc_serve* S, hipStream_t s) {
       TC_HIP(hipMemcpyAsync(dst, v.data(), v.size() * 4, hipMemcpyHostToDevice, s));
     return TC_OK;
   };
+  std::vector<SvBlk> bd(bq.size());
+  for (size_t v = 0; v < bq.size(); ++v) {
+    SvBlk& k = bd[v];
+    k.l0 = bl0[v];
+    k.l1 = bl0[v + 1];
+    k.q = bq[v];
+    k.sys = k.q >= 0 ? sl[k.q] : 0;
+    const SysHost* h = k.q >= 0 ? &S->sys[k.sys] : nullptr;
+    k.row_end = h ? h->slice0 * 64 + h->n : 0;
+    k.use16 = h && h->use16 ? 1 : 0;
+    k.vb0 = k.q >= 0 ? vb0[k.q] : 0;
+    k.vb1 = k.q >= 0 ? vb1[k.q] : -1;
+    k.pad = 0;
+  }
   int rc;
   if ((rc = up(S->list, list)) || (rc = up(S->sys_list, sl)) || (rc = up(S->sys_vb0, vb0)) ||
-      (rc = up(S->sys_vb1, vb1)) || (rc = up(S->blk_l0, bl0)) || (rc = up(S->blk_q, bq)))
+      (rc = up(S->sys_vb1, vb1)))
     return rc;
   for (int m = 0; m < 3; ++m)
     if ((rc = up(S->mlist[m], ml[m]))) return rc;
+  TC_HIP(hipMemcpyAsync(S->blk_desc, bd.data(), bd.size() * sizeof(SvBlk), hipMemcpyHostToDevice,
+                        s));
+  if (S->nl > 0) {
+    hipLaunchKernelGGL(k_sv_ent, dim3((int)ceil_div(S->nl, 256)), dim3(256), 0, s, S->nl, S->list,
+                       S->d.slice_off, S->d.slice_w, S->ent);
+    TC_LAUNCH();
+  }
   TC_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
   return TC_OK;
 }
 
-int sv_ionic(tc_serve* S, int m, hipStream_t s) {
+tc_ionic_io sv_ionic_io(tc_serve* S, int m) {
   const tc_serve_desc& d = S->d;
-  if (S->nml[m] == 0) return TC_OK;
   tc_ionic_io io{};
   io.n = S->nml[m] * 64;
   io.dt = d.dt_ionic;
@@ -835,16 +890,25 @@ int sv_ionic(tc_serve* S, int m, hipStream_t s) {
   io.pool_sys_bits = S->hctl.bits;
   io.pool_sys_stim = S->hctl.stim;
   io.state_stride = d.cap_slices * 64;
-  switch (m) {
-    case TC_MODEL_MS2V:
-      return ms2v_step(&S->mp[m].ms2v, d.states[m], &io, d.dtype, s, nullptr, nullptr);
-    case TC_MODEL_TTP:
-      return ttp_step(&S->mp[m].ttp, d.states[m], d.tab[m][0], d.tab[m][1], d.tab[m][2], &io,
-                      d.dtype, s, nullptr, nullptr);
-    default:
-      return crn_step(&S->mp[m].crn, d.states[m], d.tab[m][0], d.tab[m][1], d.tab[m][2], &io,
-                      d.dtype, s, nullptr, nullptr);
+  return io;
+}
+
+// The ionic step of every live system: the three models' slice lists in one
+// launch.
+int sv_ionic(tc_serve* S, hipStream_t s) {
+  const tc_serve_desc& d = S->d;
+  tc_ionic_io io[3];
+  void* st[3];
+  for (int m = 0; m < 3; ++m) {
+    io[m] = sv_ionic_io(S, m);
+    st[m] = d.states[m];
   }
+  const void* ttab[3] = {d.tab[TC_MODEL_TTP][0], d.tab[TC_MODEL_TTP][1], d.tab[TC_MODEL_TTP][2]};
+  const void* ctab[3] = {d.tab[TC_MODEL_CRN][0], d.tab[TC_MODEL_CRN][1], d.tab[TC_MODEL_CRN][2]};
+  return pool_ionic_step(S->nml[TC_MODEL_MS2V] ? &S->mp[TC_MODEL_MS2V].ms2v : nullptr,
+                         S->nml[TC_MODEL_TTP] ? &S->mp[TC_MODEL_TTP].ttp : nullptr,
+                         S->nml[TC_MODEL_CRN] ? &S->mp[TC_MODEL_CRN].crn : nullptr, st, ttab,
+                         ctab, io, d.dtype, s);
 }
 
 template <class T, class XT>
@@ -900,8 +964,8 @@ int sv_enqueue(tc_serve* S, SPlan& plan, hipStream_t s) {
   int K = 1;
   for (int v : S->recent) K = v > K ? v : K;
   if (K > d.max_iter) K = d.max_iter;
-  for (int m = 0; m < 3; ++m) {
-    int rc = sv_ionic(S, m, s);
+  {
+    int rc = sv_ionic(S, s);
     if (rc) return rc;
   }
   const SvPool pl = sv_pool(S);
@@ -1086,8 +1150,8 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
   S->sys_list = (int32_t*)(b + L.sys_list);
   S->sys_vb0 = (int32_t*)(b + L.sys_vb0);
   S->sys_vb1 = (int32_t*)(b + L.sys_vb1);
-  S->blk_l0 = (int32_t*)(b + L.blk_l0);
-  S->blk_q = (int32_t*)(b + L.blk_q);
+  S->blk_desc = (SvBlk*)(b + L.blk_desc);
+  S->ent = (SvEnt*)(b + L.ent);
   S->nl = 0;
   S->nsys = 0;
   S->G = kXcd;
@@ -1227,6 +1291,7 @@ extern "C" int tc_serve_add(tc_serve* S, const tc_serve_system* y, int* slot_out
   TC_HIP(hipMemcpyAsync(hc.n_iter + slot, &zi, 4, hipMemcpyHostToDevice, s));
   TC_HIP(hipMemcpyAsync(hc.nstim + slot, &y->n_stim, 4, hipMemcpyHostToDevice, s));
   const int use16 = (y->dcol && d.dcol) ? 1 : 0;
+  h.use16 = use16 != 0;
   TC_HIP(hipMemcpyAsync(hc.use16 + slot, &use16, 4, hipMemcpyHostToDevice, s));
   TC_HIP(hipMemcpyAsync(hc.bits + slot, &bits0, 4, hipMemcpyHostToDevice, s));
   TC_HIP(hipMemcpyAsync(hc.stim + (int64_t)slot * TC_POOL_STIM, st, sizeof(st),

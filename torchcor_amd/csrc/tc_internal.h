@@ -118,5 +118,11 @@ int ttp_step(const tc_ttp_params* p, void* states, const void* V_tab, const void
 int crn_step(const tc_crn_params* p, void* states, const void* Cai_tab, const void* V_tab,
              const void* fn_tab, const tc_ionic_io* io, int dtype, hipStream_t s, hipEvent_t e0,
              hipEvent_t e1);
+// The three models of a pool (MS2v, TTP, CRN; io[m].n == 0 skips a model) in
+// one launch.  Every io must read V the same way (v_f64).
+int pool_ionic_step(const tc_ms2v_params* mp, const tc_ttp_params* tp, const tc_crn_params* cp,
+                    void* const states[3], const void* const ttp_tab[3],
+                    const void* const crn_tab[3], const tc_ionic_io io[3], int dtype,
+                    hipStream_t s);
 
 }  // namespace tc
