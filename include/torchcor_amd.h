@@ -449,7 +449,7 @@ typedef struct tc_serve_desc {
   int dtype, x_f64, w_f64;   /* as tc_sim_desc */
   int64_t cap_slices;        /* pool rows = 64 * cap_slices */
   int64_t cap_entries;       /* pool SELL entries */
-  int max_sys;
+  int max_sys;               /* live systems at most (1..1024) */
   int32_t* slice_off;        /* (cap_slices) first entry of each slice */
   int32_t* slice_w;          /* (cap_slices) padded width */
   int32_t* slice_sys;        /* (cap_slices) owning system */

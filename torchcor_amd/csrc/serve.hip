@@ -112,9 +112,15 @@ struct SvEnt {
   int32_t sl, off, w, pad;
 };
 
+// Live system q: its slot and block range.
+struct SvSys {
+  int32_t slot, vb0, vb1, pad;
+};
+
 struct SvPool {
   const SvBlk* blk;         // per block
   const SvEnt* ent;         // per list entry
+  const SvSys* sysd;        // per live system
   const int32_t* list;      // slices of the live systems, system by system
   int64_t nl;
   const int32_t* sys_list;  // slot of system q
@@ -512,49 +518,80 @@ __global__ void k_sv_ent(int64_t nl, const int32_t* __restrict__ list,
 // every live system's n_iter sum, clock (t += dt) and next-step stimulus
 // bits advance, and the status goes to pinned host memory.
 constexpr int kSCheck = 1024;  // threads of the check pass (one wavefront per system fold)
+constexpr int kSCheckSys = 1024;  // systems of one check pass (max_sys is checked against it)
 
 __global__ void __launch_bounds__(kSCheck)
 k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
            const double* __restrict__ P2, SvStatus* status, long long seq) {
   __shared__ int stall0, any_stall, nmax, imax;
+  __shared__ int s_slot[kSCheckSys], s_act[kSCheckSys], s_ni[kSCheckSys];
   if (threadIdx.x == 0) {
     stall0 = ctl->stall;
     any_stall = 0;
     nmax = 0;
     imax = 0;
   }
-  __syncthreads();
-  if (stall0) return;
   const int lane = threadIdx.x & 63, wv = wave_id();
+  // phase 1, one wavefront per system: the stopping test of the last
+  // iteration.  Every load of a system (its descriptor, then its flag, norm
+  // and block partials, each lane's at once) is issued before the first wait.
   for (int q = wv; q < nsys; q += kSCheck / 64) {
-    const int s = pl.sys_list[q];
-    if (!ctl->active[s]) continue;
-    double a, b;
-    fold_sys(pl, P2, 2, q, a, b);
+    const SvSys y = pl.sysd[q];
+    const int act = ctl->active[y.slot];
+    const double bn = ctl->bnorm[y.slot];
+    constexpr int R = kSBlkMax / 64;
+    double x0[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+      const int v = y.vb0 + lane + u * 64;
+      const double t = P2[(int64_t)(v <= y.vb1 ? v : y.vb1) * 2];
+      x0[u] = v <= y.vb1 ? t : 0.0;
+    }
+    double a = 0.0;
+#pragma unroll
+    for (int u = 0; u < R; ++u) a += x0[u];
+    a = wave_sum(a);
     if (lane == 0) {
-      const double zn = sqrt(a);
-      if (zn < ctl->a_tol || zn / ctl->bnorm[s] < ctl->r_tol) {
-        ctl->active[s] = 0;
-        ctl->n_iter[s] = k_end - 1;
-      } else {
-        ctl->n_iter[s] = k_end;
-        if (k_end < max_iter) atomicOr(&any_stall, 1);
-        else ctl->active[s] = 0;  // max_iter reached
+      int nact = act, ni = -1;
+      if (act) {
+        const double zn = sqrt(a);
+        if (zn < ctl->a_tol || zn / bn < ctl->r_tol) {
+          nact = 0;
+          ni = k_end - 1;
+        } else {
+          ni = k_end;
+          if (k_end < max_iter) atomicOr(&any_stall, 1);
+          else nact = 0;  // max_iter reached
+        }
       }
+      s_slot[q] = y.slot;
+      s_act[q] = nact | (act << 1);
+      s_ni[q] = ni;
     }
   }
   __syncthreads();
+  if (stall0) return;
+  // phase 2, one thread per system: flags, iteration counts and (when no
+  // system stalls) the step's clock and next-step stimulus bits
   for (int q = threadIdx.x; q < nsys; q += blockDim.x) {
-    const int s = pl.sys_list[q];
-    const int ni = ctl->n_iter[s];
+    const int s = s_slot[q];
+    const int was = s_act[q] >> 1, now = s_act[q] & 1;
+    const int ni = was ? s_ni[q] : ctl->n_iter[s];
+    const long long tot = ctl->total[s];
+    const double t = ctl->t[s] + dt;
+    const long long stp = ctl->steps[s];
+    const int nst = ctl->nstim[s];
+    if (was) {
+      ctl->active[s] = now;
+      ctl->n_iter[s] = ni;
+    }
     atomicMax(&nmax, ni);
-    atomicMax(&imax, ctl->active[s] ? k_end : (ni + 1 < k_end ? ni + 1 : k_end));
+    atomicMax(&imax, now ? k_end : (ni + 1 < k_end ? ni + 1 : k_end));
     if (!any_stall) {
-      ctl->total[s] += ni;
-      const double t = ctl->t[s] + dt;
+      ctl->total[s] = tot + ni;
       ctl->t[s] = t;
-      ctl->steps[s] += 1;
-      ctl->bits[s] = stimulus_bits(ctl->stim + (int64_t)s * TC_POOL_STIM, ctl->nstim[s], t + dt);
+      ctl->steps[s] = stp + 1;
+      ctl->bits[s] = stimulus_bits(ctl->stim + (int64_t)s * TC_POOL_STIM, nst, t + dt);
     }
   }
   __syncthreads();
@@ -732,6 +769,7 @@ struct tc_serve {
   int32_t* sys_vb1;
   tc::SvBlk* blk_desc;  // device: per block
   tc::SvEnt* ent;       // device: per list entry
+  tc::SvSys* sysd;      // device: per live system
   int64_t nl, nml[3];
   int nsys, G;
   std::vector<tc::SysHost> sys;
@@ -748,7 +786,8 @@ namespace tc {
 namespace {
 
 struct SvLayout {
-  size_t ctl, arrays, part[3], list, mlist[3], sys_list, sys_vb0, sys_vb1, blk_desc, ent, total;
+  size_t ctl, arrays, part[3], list, mlist[3], sys_list, sys_vb0, sys_vb1, blk_desc, ent, sysd,
+      total;
 };
 
 SvLayout sv_layout(int64_t cap_slices, int max_sys) {
@@ -776,6 +815,7 @@ SvLayout sv_layout(int64_t cap_slices, int max_sys) {
   L.sys_vb1 = take(ms * 4);
   L.blk_desc = take(slots * sizeof(SvBlk));
   L.ent = take((size_t)cap_slices * sizeof(SvEnt));
+  L.sysd = take(ms * sizeof(SvSys));
   L.total = off;
   return L;
 }
@@ -784,6 +824,7 @@ SvPool sv_pool(const tc_serve* S) {
   SvPool p;
   p.blk = S->blk_desc;
   p.ent = S->ent;
+  p.sysd = S->sysd;
   p.list = S->list;
   p.nl = S->nl;
   p.sys_list = S->sys_list;
@@ -857,6 +898,11 @@ int sv_rebuild(tc_serve* S, hipStream_t s) {
     if ((rc = up(S->mlist[m], ml[m]))) return rc;
   TC_HIP(hipMemcpyAsync(S->blk_desc, bd.data(), bd.size() * sizeof(SvBlk), hipMemcpyHostToDevice,
                         s));
+  std::vector<SvSys> sd(sl.size());
+  for (size_t q = 0; q < sl.size(); ++q) sd[q] = SvSys{sl[q], vb0[q], vb1[q], 0};
+  if (!sd.empty())
+    TC_HIP(hipMemcpyAsync(S->sysd, sd.data(), sd.size() * sizeof(SvSys), hipMemcpyHostToDevice,
+                          s));
   if (S->nl > 0) {
     hipLaunchKernelGGL(k_sv_ent, dim3((int)ceil_div(S->nl, 256)), dim3(256), 0, s, S->nl, S->list,
                        S->d.slice_off, S->d.slice_w, S->ent);
@@ -1083,15 +1129,16 @@ int sv_advance(tc_serve* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
 using namespace tc;
 
 extern "C" int tc_serve_workspace_size(int64_t cap_slices, int max_sys, size_t* bytes) {
-  TC_CHECK(bytes && cap_slices > 0 && max_sys > 0, TC_ERR_ARG,
-           "tc_serve_workspace_size: bad arguments");
+  TC_CHECK(bytes && cap_slices > 0 && max_sys > 0 && max_sys <= kSCheckSys, TC_ERR_ARG,
+           "tc_serve_workspace_size: bad arguments (max_sys 1..%d)", kSCheckSys);
   *bytes = sv_layout(cap_slices, max_sys).total;
   return TC_OK;
 }
 
 extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* stream) {
-  TC_CHECK(out && d && d->cap_slices > 0 && d->cap_entries > 0 && d->max_sys > 0, TC_ERR_ARG,
-           "tc_serve_create: bad arguments");
+  TC_CHECK(out && d && d->cap_slices > 0 && d->cap_entries > 0 && d->max_sys > 0 &&
+               d->max_sys <= kSCheckSys,
+           TC_ERR_ARG, "tc_serve_create: bad arguments (max_sys 1..%d)", kSCheckSys);
   TC_CHECK(d->dtype == TC_F32 || d->dtype == TC_F64, TC_ERR_ARG, "bad dtype %d", d->dtype);
   TC_CHECK(!d->w_f64 || d->x_f64 || d->dtype == TC_F64, TC_ERR_ARG, "w_f64 needs x_f64");
   TC_CHECK(d->cap_slices * 64 < (int64_t)INT32_MAX && d->cap_entries < (int64_t)INT32_MAX,
@@ -1152,6 +1199,7 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
   S->sys_vb1 = (int32_t*)(b + L.sys_vb1);
   S->blk_desc = (SvBlk*)(b + L.blk_desc);
   S->ent = (SvEnt*)(b + L.ent);
+  S->sysd = (SvSys*)(b + L.sysd);
   S->nl = 0;
   S->nsys = 0;
   S->G = kXcd;
