@@ -96,3 +96,26 @@ def test_serve_int32_columns_identical():
         out.append(eng.run_until_done([eng.prepare(r) for r in reqs]))
     for a, b in zip(*out):
         assert torch.equal(a["u"], b["u"]) and a["iterations"] == b["iterations"]
+
+
+def test_serve_loop_with_arrivals_matches_batch_admission():
+    """The real-time loop (arrivals, assembly on a worker thread and its own
+    stream, admission as room frees up) gives each run bit-identical
+    results to admitting everything up front."""
+    _say("[gpu] serve loop with arrivals")
+    reqs = _requests()
+    for k, r in enumerate(reqs):
+        r.arrival = 0.02 * k
+    eng = _engine(torch.float32, cap_nodes=2200)
+    records, summary = eng.serve(reqs, max_chunk=40, admit_chunk=7)
+    assert summary["requests"] == len(reqs)
+    ref_eng = _engine(torch.float32)
+    ref = ref_eng.run_until_done([ref_eng.prepare(r) for r in reqs])
+    got = {rec["tag"]: rec["result"] for rec in records}
+    for k in range(len(reqs)):
+        assert got[k]["steps"] == ref[k]["steps"]
+        assert torch.equal(got[k]["u"], ref[k]["u"]), k
+        assert torch.equal(got[k]["AT"], ref[k]["AT"]) and torch.equal(got[k]["RT"], ref[k]["RT"])
+        assert got[k]["iterations"] == ref[k]["iterations"]
+    for rec in records:
+        assert rec["admitted"] >= rec["arrival"] and rec["done"] > rec["admitted"]

@@ -16,6 +16,7 @@ operator values and Jacobi scaling, the same time loop semantics; its
 result does not depend on which other runs share the pool.
 """
 import collections
+import concurrent.futures
 import ctypes as C
 import time
 
@@ -231,61 +232,82 @@ class ServeEngine:
                 out[slots.pop(s)] = self.remove(s)
         return out
 
-    def serve(self, requests, max_chunk=200, log=None):
+    def serve(self, requests, max_chunk=200, log=None, admit_chunk=25):
         """Real-time serving loop: requests become visible at their arrival
         time (seconds after the start); returns per-request records with
         arrival, admission and completion times and the latency per
-        simulated second, plus summary statistics."""
+        simulated second, plus summary statistics.
+
+        Visible requests are assembled by one worker thread on its own
+        stream while the pool keeps stepping (the library call that steps
+        the pool releases the interpreter lock); they are admitted in
+        arrival order as their assembly finishes and the pool has room.
+        While a request waits for admission the pool steps in chunks of at
+        most ``admit_chunk`` steps, else ``max_chunk``."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         pending = collections.deque(reqs)
-        waiting = collections.deque()
+        waiting = collections.deque()       # (request, future of its Prepared)
         slots = {}
         records = []
-        prepared = None           # assembled head of the queue, waiting for room
-        t_prepare = 0.0
+        spent = {"prepare": 0.0}
+        side = torch.cuda.Stream(device=self.device)
+
+        def work(req):
+            torch.cuda.set_device(self.device)
+            ta = time.perf_counter()
+            with torch.cuda.stream(side):
+                pr = self.prepare(req)
+            side.synchronize()
+            spent["prepare"] += time.perf_counter() - ta
+            return pr
+
+        worker = concurrent.futures.ThreadPoolExecutor(max_workers=1)
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
-        while pending or waiting or self.live:
-            now = clock()
-            while pending and pending[0].arrival <= now:
-                waiting.append(pending.popleft())
-            admitted = True
-            while waiting and admitted:
-                admitted = False
-                if prepared is None or prepared.req is not waiting[0]:
-                    ta = clock()
-                    prepared = self.prepare(waiting[0])
-                    t_prepare += clock() - ta
-                if self.fits(prepared):
+        try:
+            while pending or waiting or self.live:
+                now = clock()
+                while pending and pending[0].arrival <= now:
+                    req = pending.popleft()
+                    waiting.append((req, worker.submit(work, req)))
+                while waiting:
+                    req, fut = waiting[0]
+                    if self.live and not fut.done():
+                        break
+                    pr = fut.result()          # waits only when nothing is stepping
+                    if not self.fits(pr):
+                        if not self.live:
+                            raise RuntimeError("a request does not fit the empty pool")
+                        break
                     waiting.popleft()
-                    slot = self.add(prepared)
-                    slots[slot] = dict(req=prepared.req, admitted=clock())
-                    prepared = None
-                    admitted = True
-            if not self.live:
+                    slot = self.add(pr)
+                    slots[slot] = dict(req=req, admitted=clock())
+                if not self.live:
+                    if pending:
+                        time.sleep(max(0.0, pending[0].arrival - clock()))
+                    continue
+                k = min(max_chunk, min(pr.steps_left for pr in self.live.values()))
                 if waiting:
-                    raise RuntimeError("a request does not fit the empty pool")
-                if pending:
-                    time.sleep(max(0.0, pending[0].arrival - clock()))
-                continue
-            k = min(max_chunk, min(pr.steps_left for pr in self.live.values()))
-            self.advance(k)
-            for s in [s for s, pr in self.live.items() if pr.steps_left <= 0]:
-                res = self.remove(s)
-                info = slots.pop(s)
-                done = clock()
-                req = info["req"]
-                rec = dict(tag=req.tag, n=int(np.asarray(req.nodes).shape[0]), T=req.T,
-                           model=req.model, arrival=req.arrival, admitted=info["admitted"],
-                           done=done, latency=done - req.arrival,
-                           latency_per_sim_s=(done - req.arrival) / (req.T / 1000.0),
-                           iterations=res["iterations"], result=res)
-                records.append(rec)
-                if log:
-                    log(rec)
+                    k = min(k, admit_chunk)
+                self.advance(k)
+                for s in [s for s, pr in self.live.items() if pr.steps_left <= 0]:
+                    res = self.remove(s)
+                    info = slots.pop(s)
+                    done = clock()
+                    req = info["req"]
+                    rec = dict(tag=req.tag, n=int(np.asarray(req.nodes).shape[0]), T=req.T,
+                               model=req.model, arrival=req.arrival, admitted=info["admitted"],
+                               done=done, latency=done - req.arrival,
+                               latency_per_sim_s=(done - req.arrival) / (req.T / 1000.0),
+                               iterations=res["iterations"], result=res)
+                    records.append(rec)
+                    if log:
+                        log(rec)
+        finally:
+            worker.shutdown(wait=True)
         wall = clock()
         lat = np.array([r["latency_per_sim_s"] for r in records])
-        summary = dict(requests=len(records), wall_s=wall, prepare_s=t_prepare,
+        summary = dict(requests=len(records), wall_s=wall, prepare_s=spent["prepare"],
                        sims_per_hour=len(records) * 3600.0 / wall,
                        latency_per_sim_s_p50=float(np.percentile(lat, 50)),
                        latency_per_sim_s_p99=float(np.percentile(lat, 99)))
