@@ -657,7 +657,7 @@ __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
 }
 
 template <class T, bool VF64>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_ttp(TtpK<T> K, T* __restrict__ st, const T* __restrict__ Vtab,
       const T* __restrict__ Ctab, const T* __restrict__ Ktab, TtpTabs tb, tc_ionic_io io) {
   ttp_node<T, VF64>(K, st, Vtab, Ctab, Ktab, tb, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
@@ -956,7 +956,7 @@ struct PoolIonic {
 
 // (separate by-value arguments: the kernel argument block is read in place)
 template <class T, bool VF64>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_pool_ionic(tc_ms2v_params mp, TtpK<T> tk, CrnK<T> ck, T* __restrict__ st0,
              T* __restrict__ st1, T* __restrict__ st2, const T* __restrict__ tt0,
              const T* __restrict__ tt1, const T* __restrict__ tt2, const T* __restrict__ ct0,
