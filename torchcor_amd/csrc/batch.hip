@@ -449,13 +449,26 @@ k_bat_fold(int kind, int k, int G, int lanes, int B, BatCtl* ctl, const double* 
   if (b >= B) return;
   __shared__ int go;
   if (threadIdx.x == 0) go = ctl->stall == 0 && (kind == 0 || ctl->active[b] != 0);
+  // the partials load (clamped index, then a select) while the flags arrive
+  const int64_t GL = (int64_t)G * lanes;
+  constexpr int R = (kBMaxBlocks + kBB - 1) / kBB;
+  double x0[R], x1[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int g = (int)threadIdx.x + q * kBB;
+    const int gc = g < G ? g : G - 1;
+    const double t0 = part[(int64_t)gc * lanes + b];
+    const double t1 = kind != 1 ? part[GL + (int64_t)gc * lanes + b] : 0.0;
+    x0[q] = g < G ? t0 : 0.0;
+    x1[q] = g < G ? t1 : 0.0;
+  }
   __syncthreads();
   if (!go) return;
-  const int64_t GL = (int64_t)G * lanes;
   double v[2] = {0.0, 0.0};
-  for (int g = threadIdx.x; g < G; g += blockDim.x) {
-    v[0] += part[(int64_t)g * lanes + b];
-    if (kind != 1) v[1] += part[GL + (int64_t)g * lanes + b];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    v[0] += x0[q];
+    v[1] += x1[q];
   }
   block_sum<2>(v);
   if (threadIdx.x != 0) return;
