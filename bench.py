@@ -55,10 +55,10 @@ def alg_bytes(n, nnz, S, ib=4, xb=4, wb=4):
 # WRITE_SIZE x1.0.  profiles/cube101_ttp_step_hbm_traffic.md holds the
 # counter summary and the commands.
 MEASURED_TRAFFIC_101 = {
-    "ionic": 181_304_695,
-    "rhs_init": 211_451_475,
-    "pcg_spmv": 112_080_940,
-    "pcg_update": 37_291_514,
+    "ionic": 183_906_422,
+    "rhs_init": 204_057_276,
+    "pcg_spmv": 112_142_918,
+    "pcg_update": 37_280_251,
 }
 
 
