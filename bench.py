@@ -306,6 +306,7 @@ def served_config(dev, n_req, budget_nodes=4_000_000):
                         f"MS2v/TTP/CRN, T U[50,300] ms, f32, pool budget {budget_nodes} rows",
             "requests": n_req, "wall_s": round(summary["wall_s"], 2),
             "assembly_s": round(summary["prepare_s"], 2),
+            "engine_steps": summary["steps"], "stalls": summary["stalls"],
             "sims_per_hour": round(summary["sims_per_hour"], 1),
             "latency_per_sim_s_p50": round(summary["latency_per_sim_s_p50"], 3),
             "latency_per_sim_s_p99": round(summary["latency_per_sim_s_p99"], 3),

@@ -510,6 +510,9 @@ int tc_serve_remove(tc_serve* serve, int slot, void* V_out, void* AT_out, void* 
 /* nsteps engine steps of every live system; iters[k] (host) = the largest
  * n_iter among them in step k.  Synchronizes at the end. */
 int tc_serve_advance(tc_serve* serve, int64_t nsteps, int32_t* iters_host, void* stream);
+/* Engine steps since creation, and how many of them needed more PCG
+ * iterations than the pipeline had launched (each drains the pipeline). */
+int tc_serve_stats(const tc_serve* serve, int64_t* steps_host, int64_t* stalls_host);
 
 /* LAT/RT update alone (monodomain.py:174-176), for the last step of a run. */
 int tc_lat_update(const void* u, void* AT, void* RT, int64_t n, double t, int dtype,

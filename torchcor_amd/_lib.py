@@ -192,6 +192,7 @@ FUNCS = {
     "tc_serve_add": (p, C.POINTER(ServeSystem), INT_P, p),
     "tc_serve_remove": (p, i32, p, p, p, I64_P, I64_P, p),
     "tc_serve_advance": (p, i64, C.POINTER(C.c_int32), p),
+    "tc_serve_stats": (p, I64_P, I64_P),
 }
 for _n, _a in FUNCS.items():
     _proto(_n, *_a)

@@ -780,6 +780,7 @@ struct tc_serve {
   long long seq;
   std::deque<int> recent;
   int64_t steps;
+  int64_t stalls = 0;    // steps that needed more iterations than were launched
 };
 
 namespace tc {
@@ -1102,6 +1103,7 @@ int sv_advance(tc_serve* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     if (rc) return rc;
     SvStatus* st = S->host_status + f.plan.slot;
     if (st->stalled) {
+      ++S->stalls;
       // finish this step, drop the later (gated) ones and resume after it
       XT* xs = (XT*)S->d.x[S->d.linear_guess ? 1 - f.before.xc : f.before.xc];
       rc = sv_resolve<T, XT>(S, f.plan, xs, s);
@@ -1397,6 +1399,13 @@ extern "C" int tc_serve_remove(tc_serve* S, int slot, void* V_out, void* AT_out,
   S->entries.give(h.entry0, h.ne);
   h = SysHost();
   return sv_rebuild(S, s);
+}
+
+extern "C" int tc_serve_stats(const tc_serve* S, int64_t* steps, int64_t* stalls) {
+  TC_CHECK(S && steps && stalls, TC_ERR_ARG, "tc_serve_stats: bad arguments");
+  *steps = S->steps;
+  *stalls = S->stalls;
+  return TC_OK;
 }
 
 extern "C" int tc_serve_advance(tc_serve* S, int64_t nsteps, int32_t* iters, void* stream) {

@@ -213,6 +213,13 @@ class ServeEngine:
         self.step_iters.extend(out)
         return out
 
+    def stats(self):
+        """Engine steps so far and how many needed more PCG iterations than
+        the pipeline had launched (each such step drains the pipeline)."""
+        steps, stalls = C.c_int64(0), C.c_int64(0)
+        L.call("tc_serve_stats", self._h, C.byref(steps), C.byref(stalls))
+        return dict(steps=int(steps.value), stalls=int(stalls.value))
+
     # -------------------------------------------------------------- loop
     def run_until_done(self, prepared, max_chunk=200):
         """Admit all prepared runs (room permitting) and run them to the end;
@@ -232,7 +239,7 @@ class ServeEngine:
                 out[slots.pop(s)] = self.remove(s)
         return out
 
-    def serve(self, requests, max_chunk=200, log=None, admit_chunk=25):
+    def serve(self, requests, max_chunk=1000, log=None, admit_chunk=25):
         """Real-time serving loop: requests become visible at their arrival
         time (seconds after the start); returns per-request records with
         arrival, admission and completion times and the latency per
@@ -243,7 +250,9 @@ class ServeEngine:
         the pool releases the interpreter lock); they are admitted in
         arrival order as their assembly finishes and the pool has room.
         While a request waits for admission the pool steps in chunks of at
-        most ``admit_chunk`` steps, else ``max_chunk``."""
+        most ``admit_chunk`` steps; otherwise a chunk ends at the next
+        retirement, at ``max_chunk`` steps, or near the next arrival (from
+        the measured time per step), whichever comes first."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         pending = collections.deque(reqs)
         waiting = collections.deque()       # (request, future of its Prepared)
@@ -262,6 +271,8 @@ class ServeEngine:
             return pr
 
         worker = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        s_per_step = None                   # measured, smoothed
+        st0 = self.stats()
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
         try:
@@ -289,7 +300,12 @@ class ServeEngine:
                 k = min(max_chunk, min(pr.steps_left for pr in self.live.values()))
                 if waiting:
                     k = min(k, admit_chunk)
+                elif pending and s_per_step:
+                    k = min(k, max(admit_chunk, int((pending[0].arrival - clock()) / s_per_step) + 1))
+                ta = time.perf_counter()
                 self.advance(k)
+                dt_step = (time.perf_counter() - ta) / k
+                s_per_step = dt_step if s_per_step is None else 0.7 * s_per_step + 0.3 * dt_step
                 for s in [s for s, pr in self.live.items() if pr.steps_left <= 0]:
                     res = self.remove(s)
                     info = slots.pop(s)
@@ -307,7 +323,9 @@ class ServeEngine:
             worker.shutdown(wait=True)
         wall = clock()
         lat = np.array([r["latency_per_sim_s"] for r in records])
+        st1 = self.stats()
         summary = dict(requests=len(records), wall_s=wall, prepare_s=spent["prepare"],
+                       steps=st1["steps"] - st0["steps"], stalls=st1["stalls"] - st0["stalls"],
                        sims_per_hour=len(records) * 3600.0 / wall,
                        latency_per_sim_s_p50=float(np.percentile(lat, 50)),
                        latency_per_sim_s_p99=float(np.percentile(lat, 99)))
