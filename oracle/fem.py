@@ -37,6 +37,22 @@ def conductivity_fields(regions, calls):
     return sl, st
 
 
+def bidomain_fields(regions, calls):
+    """Per-element (il, it, el, et) from ``add_conductivity`` calls, or None
+    when any call leaves el/et out (conductivity.py:18-47 then drops the
+    intra/extracellular fields)."""
+    regions = np.asarray(regions)
+    f = [np.zeros(regions.shape[0]) for _ in range(4)]
+    for (ids, il, it, el, et) in calls:
+        if el is None and et is None:
+            return None
+        for rid in ids:
+            m = regions == rid
+            for arr, v in zip(f, (il, it, el, et)):
+                arr[m] = v
+    return tuple(f)
+
+
 def sigma_tensor(sl, st, fibres):
     """sigma = st*I + (sl-st) f f^T per element (conductivity.py:64-67)."""
     f = np.asarray(fibres, dtype=np.float64)

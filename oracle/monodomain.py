@@ -82,6 +82,35 @@ class Monodomain:
     def spmv(self, vals, x):
         return fem.spmv(self.row_ptr, self.col, vals, x)
 
+    def phie_recovery(self, Vm, a_tol=1e-5, r_tol=1e-5, max_iter=100):
+        """phi_e of every Vm frame (monodomain.py:254-286), float64 run:
+        b = -K_i V; K_ie phi = b by one PCG object carried over the frames
+        with the linear guess; phi -= mean(phi) applied to the solver's own
+        iterate (in a float64 run the reference's returned solution is that
+        iterate).  Returns (frames (F, n), n_iter per frame)."""
+        fields = fem.bidomain_fields(self.regions, self.cond_calls)
+        if fields is None:
+            raise ValueError("phie_recovery needs el and et")
+        il, it, el, et = fields
+        si = fem.sigma_tensor(il, it, self.fibres)
+        se = fem.sigma_tensor(el, et, self.fibres)
+        coords = self.nodes[self.elems]
+        local = fem.tri_local if self.elems.shape[1] == 3 else fem.tet_local
+        _, k_ie = local(coords, si + se)
+        _, k_i = local(coords, si)
+        rp, col, kie = fem.assemble_csr(self.elems, k_ie, self.n, round_f32=self.round_f32)
+        _, _, ki = fem.assemble_csr(self.elems, k_i, self.n, round_f32=self.round_f32)
+        cg = pcg.PCG(rp, col, kie, pcg.jacobi(rp, col, kie))
+        cg.initialize(np.zeros(self.n), linear_guess=True)
+        out, iters = [], []
+        for V in np.asarray(Vm, dtype=np.float64):
+            b = -fem.spmv(rp, col, ki, V)
+            x, it_ = cg.solve(b, a_tol, r_tol, max_iter)
+            cg.x = x - x.mean()
+            out.append(cg.x.copy())
+            iters.append(it_)
+        return np.stack(out), np.array(iters)
+
     def solve(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True,
               snapshot_interval=1, nsteps=None):
         self.assemble()

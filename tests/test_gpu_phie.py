@@ -1,0 +1,113 @@
+"""phi_e recovery (Monodomain.phie_recovery, monodomain.py:254-286) on the
+GPU against the float64 oracle (run with -m gpu).
+
+Tolerances:
+  * float64, oracle's tolerances and frames: same n_iter per frame, max
+    |dphi| <= 1e-9 relative to max |phi| (only summation orders differ).
+  * sigma_e = sigma_i (K_ie = 2 K_i): phi = -(V - mean V) / 2 to 1e-8
+    relative in float64 and 1e-3 in float32 (K_i V is formed in float32
+    there, as the reference does, and K_ie^-1 amplifies that rounding by up
+    to its condition number).
+  * RCM order vs the mesh's order on a permuted shell: 1e-4 relative (both
+    solve to 1e-10; a wrong mapping of the node order would be O(1)).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import ionic as oion  # noqa: E402
+from oracle import monodomain as omono  # noqa: E402
+from torchcor_amd import meshes  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _case(N=9, seed=0, el=(2.5, 0.6), frames=5):
+    nodes, elems = meshes.kuhn_cube(N)
+    E = len(elems)
+    rng = np.random.default_rng(seed)
+    f = rng.normal(size=(E, 3))
+    f /= np.linalg.norm(f, axis=1, keepdims=True)
+    regions = (np.arange(E) % 2).astype(np.int32)
+    cond = [([0], 0.17, 0.05, 0.17 * el[0], 0.05 * el[1]),
+            ([1], 0.3, 0.1, 0.3 * el[0], 0.1 * el[1])]
+    # smooth frames: a travelling front, as snapshots of a run would look
+    x = nodes[:, 0] / nodes[:, 0].max()
+    Vm = np.stack([-80 + 100 / (1 + np.exp((x - 0.2 * k) * 20)) for k in range(frames)])
+    return nodes, elems, regions, f, cond, Vm
+
+
+def _gpu(nodes, elems, regions, f, cond, dtype, reorder="auto"):
+    from torchcor_amd.ionic import ModifiedMS2v
+    from torchcor_amd.simulator import Monodomain
+    sim = Monodomain(ModifiedMS2v(0.01, device=DEV, dtype=dtype), T=1.0, dt=0.01, device=DEV,
+                     dtype=dtype, reorder=reorder)
+    sim.load_arrays(nodes, elems, regions, f)
+    for c in cond:
+        sim.add_conductivity(*c)
+    return sim
+
+
+def test_phie_float64_matches_oracle():
+    _say("[gpu] phie f64 vs oracle")
+    nodes, elems, regions, f, cond, Vm = _case()
+    om = omono.Monodomain(oion.ModifiedMS2v(0.01), T=1.0, dt=0.01)
+    om.load_arrays(nodes, elems, regions, f)
+    for c in cond:
+        om.add_conductivity(*c)
+    ref, its = om.phie_recovery(Vm, 1e-5, 1e-5, 100)
+    sim = _gpu(nodes, elems, regions, f, cond, torch.float64)
+    got = sim.phie_recovery(1e-5, 1e-5, 100, Vm=torch.from_numpy(Vm), save=False).cpu().numpy()
+    assert sim.phie_iterations == list(its)
+    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-8), (torch.float32, 1e-3)])
+def test_phie_closed_form(dtype, tol):
+    _say(f"[gpu] phie closed form {dtype}")
+    nodes, elems, regions, f, cond, Vm = _case(N=8, el=(1.0, 1.0))
+    sim = _gpu(nodes, elems, regions, f, cond, dtype)
+    got = sim.phie_recovery(1e-12, 1e-12, 1000, Vm=torch.from_numpy(Vm), save=False)
+    got = got.double().cpu().numpy()
+    want = -(Vm - Vm.mean(axis=1, keepdims=True)) / 2
+    assert np.abs(got - want).max() <= tol * np.abs(want).max()
+
+
+def test_phie_rcm_order_matches_mesh_order():
+    _say("[gpu] phie rcm vs mesh order")
+    nodes, elems, regions, f = meshes.ellipsoid_shell(6000, 3, outer=(10.0, 7.5, 15.0), wall=2.5)
+    cond = [([r], 0.17, 0.05, 0.4, 0.03) for r in range(4)]
+    x = nodes[:, 2] / np.abs(nodes[:, 2]).max()
+    Vm = np.stack([-80 + 100 / (1 + np.exp((x + 1 - 0.4 * k) * 8)) for k in range(4)])
+    a = _gpu(nodes, elems, regions, f, cond, torch.float64, reorder="rcm")
+    b = _gpu(nodes, elems, regions, f, cond, torch.float64, reorder=None)
+    pa = a.phie_recovery(1e-10, 1e-10, 3000, Vm=torch.from_numpy(Vm), save=False)
+    pb = b.phie_recovery(1e-10, 1e-10, 3000, Vm=torch.from_numpy(Vm), save=False)
+    assert a.node_order is not None and b.node_order is None
+    assert (pa - pb).abs().max().item() <= 1e-4 * pb.abs().max().item()
+
+
+def test_phie_from_result_files(tmp_path):
+    """solve() writes Vm.pt; phie_recovery() reads it and writes Phi_e.pt."""
+    _say("[gpu] phie from files")
+    nodes, elems, regions, f, cond, _ = _case(N=7)
+    from torchcor_amd.ionic import ModifiedMS2v
+    from torchcor_amd.simulator import Monodomain
+    sim = Monodomain(ModifiedMS2v(0.01, device=DEV, dtype=torch.float32), T=3.0, dt=0.01,
+                     device=DEV, dtype=torch.float32)
+    sim.load_arrays(nodes, elems, regions, f)
+    for c in cond:
+        sim.add_conductivity(*c)
+    sim.add_stimulus(meshes.face_nodes(7, 0, 0), 0.0, 1.0, 100)
+    sim.solve(1e-5, 1e-5, 100, snapshot_interval=1.0, verbose=False, result_path=tmp_path)
+    phi = sim.phie_recovery()
+    saved = torch.load(tmp_path / "Phi_e.pt", weights_only=True)
+    assert saved.shape == (4, len(nodes)) and torch.equal(saved, phi.cpu())
+    assert torch.isfinite(phi).all() and phi.abs().max().item() > 0
+    assert phi.double().mean(dim=1).abs().max().item() < 1e-3

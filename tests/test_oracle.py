@@ -271,3 +271,66 @@ def test_monodomain_planar_wave():
     assert (np.diff(mean_x) > 0).all()                    # activation moves along +x
     spread = np.array([AT[i].max() - AT[i].min() for i in range(N // 2)])
     assert spread.max() < 1.0                             # nearly planar (ms)
+
+
+def _phie_case(el_scale, seed=0, N=7):
+    nodes, elems = meshes.kuhn_cube(N)
+    E = len(elems)
+    rng = np.random.default_rng(seed)
+    f = rng.normal(size=(E, 3))
+    f /= np.linalg.norm(f, axis=1, keepdims=True)
+    m = monodomain.Monodomain(ionic.ModifiedMS2v(0.01), T=1.0, dt=0.01)
+    m.load_arrays(nodes, elems, np.zeros(E, int), f)
+    m.add_conductivity([0], 0.17, 0.05, 0.17 * el_scale[0], 0.05 * el_scale[1])
+    Vm = rng.normal(size=(4, len(nodes))) * 30.0 - 50.0
+    return m, Vm
+
+
+def test_phie_closed_form_equal_anisotropy():
+    """sigma_e = sigma_i: K_ie = 2 K_i exactly (also after the float32
+    rounding of contributions), so phi = -(V - mean V) / 2."""
+    m, Vm = _phie_case((1.0, 1.0))
+    phi, its = monodomain.Monodomain.phie_recovery(m, Vm, 1e-12, 1e-12, 500)
+    want = -(Vm - Vm.mean(axis=1, keepdims=True)) / 2
+    assert np.abs(phi - want).max() < 1e-8
+    assert np.all(its < 500)
+
+
+def test_phie_general_solves_the_system():
+    """Unequal anisotropy: K_ie phi = -K_i V (to the tolerance), zero mean,
+    and the linear guess across frames only changes the starting point."""
+    m, Vm = _phie_case((2.5, 0.6), seed=1)
+    phi, its = m.phie_recovery(Vm, 1e-10, 1e-10, 1000)
+    fields = fem.bidomain_fields(m.regions, m.cond_calls)
+    il, it, el, et = fields
+    si = fem.sigma_tensor(il, it, m.fibres)
+    se = fem.sigma_tensor(el, et, m.fibres)
+    coords = m.nodes[m.elems]
+    _, k_ie = fem.tet_local(coords, si + se)
+    _, k_i = fem.tet_local(coords, si)
+    rp, col, kie = fem.assemble_csr(m.elems, k_ie, m.n)
+    _, _, ki = fem.assemble_csr(m.elems, k_i, m.n)
+    A = sp.csr_matrix((kie, col, rp), shape=(m.n, m.n))
+    B = sp.csr_matrix((ki, col, rp), shape=(m.n, m.n))
+    for k in range(Vm.shape[0]):
+        res = A @ phi[k] + B @ Vm[k]
+        assert np.abs(res).max() < 1e-6 * np.abs(B @ Vm[k]).max()
+        assert abs(phi[k].mean()) < 1e-10
+    # against a direct solve of the grounded system (node 0 fixed), mean removed
+    Ag = A.tolil()
+    Ag[0, :] = 0
+    Ag[0, 0] = 1
+    rhs = -(B @ Vm[2])
+    rhs[0] = 0
+    ref = spla.spsolve(Ag.tocsr(), rhs)
+    ref -= ref.mean()
+    assert np.abs(phi[2] - ref).max() < 1e-5 * np.abs(ref).max()
+
+
+def test_phie_needs_extracellular_conductivity():
+    nodes, elems = meshes.kuhn_cube(4)
+    m = monodomain.Monodomain(ionic.ModifiedMS2v(0.01), T=1.0, dt=0.01)
+    m.load_arrays(nodes, elems, np.zeros(len(elems), int), np.tile([1.0, 0, 0], (len(elems), 1)))
+    m.add_conductivity([0], 0.1, 0.1)
+    with pytest.raises(ValueError):
+        m.phie_recovery(np.zeros((1, len(nodes))))

@@ -361,6 +361,80 @@ class Monodomain:
         names = ("ionic", "rhs_init", "pcg_spmv", "pcg_update", "pcg_check")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}
 
+    # ------------------------------------------------------ phi_e recovery
+    def phie_recovery(self, a_tol=1e-5, r_tol=1e-5, max_iter=100, Vm=None, save=True):
+        """Extracellular potential of every Vm snapshot (monodomain.py:254-286).
+
+        For each frame V: b = -K_i V, then K_ie phi = b by the Jacobi-PCG
+        (float64, warm-started from the previous frames' solutions by the
+        linear guess, as one ConjugateGradient object carried over the
+        frames), and phi minus its mean.  K_i and K_ie are the stiffness
+        matrices of sigma_i and sigma_i + sigma_e (conductivities given with
+        el/et), assembled like the monodomain operators.  As in the
+        reference, a float64 run subtracts the mean from the solver's own
+        iterate (the returned solution is that iterate), a float32 run from
+        a float32 copy.  ``Vm`` (frames, n) defaults to ``result_path /
+        'Vm.pt'``; the result is saved as ``Phi_e.pt`` there when ``save``
+        and returned, (frames, n) in the working dtype, mesh numbering.
+        """
+        import ctypes as C
+        dev, dt_ = self.device, self.dtype
+        if Vm is None:
+            if self.result_path is None:
+                raise ValueError("phie_recovery needs Vm frames or a result_path holding Vm.pt")
+            Vm = torch.load(self.result_path / "Vm.pt", weights_only=True)
+        Vm = torch.as_tensor(Vm).to(device=dev, dtype=dt_)
+        if Vm.dim() != 2 or Vm.shape[1] != self.n_nodes:
+            raise ValueError(f"Vm must be (frames, {self.n_nodes})")
+        si, se, _ = self.conductivity.calculate_sigma(self.fibres)
+        if si is None or se is None:
+            raise ValueError("phie_recovery needs intra- and extracellular conductivities "
+                             "(add_conductivity with el and et)")
+        if self.system is None:
+            self.assemble()
+        order = self.node_order
+        if order is not None:
+            elems = self.node_map[self.elems.to(torch.long)].to(torch.int32)
+            P = P1System(self.nodes[order], elems, dev)
+            Vm = Vm[:, order]
+        else:
+            P = P1System(self.nodes, self.elems, dev)
+        _, K_ie = P.assemble(sigma=si + se)
+        _, K_i = P.assemble(sigma=si)
+        P.release_assembly_maps()
+        f64 = torch.float64
+        # the reference's CG converts K_ie (held in the working dtype) to
+        # float64 and keeps its Jacobi scaling in the working dtype
+        K_ie = K_ie.to(dt_).to(f64)
+        P.set_operators(K_ie, torch.zeros_like(K_ie), 1.0, 1.0, 0.0, 0.0, f64)
+        P.Dinv.copy_(P.Dinv.to(dt_).to(f64))
+        sK_i = P.to_sell(K_i.to(dt_))
+        n = P.n
+        x = torch.zeros(n, dtype=f64, device=dev)
+        x_prev, x_next = torch.zeros_like(x), torch.empty_like(x)
+        b = torch.empty(n, dtype=dt_, device=dev)
+        out = torch.empty_like(Vm)
+        self.phie_iterations = []
+        for k in range(Vm.shape[0]):
+            V = Vm[k].contiguous()
+            L.call("tc_sell_spmv", C.byref(P.sell), L.ptr(sK_i), L.ptr(V), L.ptr(b),
+                   L.dtype_code(dt_), L.stream_handle(dev))
+            rhs = (-b).to(f64)
+            it = P.pcg_solve(rhs, x, x_prev, x_next, a_tol, r_tol, max_iter, True)
+            x_prev, x, x_next = x, x_next, x_prev
+            self.phie_iterations.append(it)
+            if dt_ == f64:
+                x -= x.mean()
+                out[k].copy_(x)
+            else:
+                phi = x.to(dt_)
+                out[k].copy_(phi - phi.mean())
+        if order is not None:
+            out = out[:, self.node_map]
+        if save and self.result_path is not None:
+            torch.save(out.cpu(), self.result_path / "Phi_e.pt")
+        return out
+
     def finish(self):
         R = self._run
         L.call("tc_sim_finish", self._sim, L.stream_handle(self.device))
