@@ -140,6 +140,23 @@ int tc_sell_deltas(const tc_sell* A, int16_t* dcol, int32_t* max_abs_dev, void* 
  * ---------------------------------------------------------------------- */
 int tc_rcm_order(int64_t n, const int32_t* row_ptr_host, const int32_t* col_host,
                  int32_t* order_host, int64_t* bw_in_host, int64_t* bw_out_host);
+
+/* ------------------------------------------------------------------------
+ * Mesh and stimulus files (host).
+ * Replace the np.loadtxt parsing of MeshReader.read_nodes / read_elems /
+ * read_fibres (torchcor/core/mesh.py:26-57) and load_stimulus_region
+ * (torchcor/core/stimulation.py:27-36): a whitespace-separated numeric
+ * table after `skip_lines` header lines (blank lines and '#' comments
+ * ignored, every row as wide as the first), parsed by several host threads.
+ * tc_table_shape gives its rows and columns; tc_table_read_* parse columns
+ * [col0, col0 + ncols) of all `rows` rows into a row-major HOST array (rows
+ * must match the file's count, else TC_ERR_ARG).  No GPU work.
+ * ---------------------------------------------------------------------- */
+int tc_table_shape(const char* path, int skip_lines, int64_t* rows_host, int32_t* cols_host);
+int tc_table_read_f64(const char* path, int skip_lines, int col0, int ncols, int64_t rows,
+                      double* out_host);
+int tc_table_read_i64(const char* path, int skip_lines, int col0, int ncols, int64_t rows,
+                      int64_t* out_host);
 /* y = A x over a SELL operator (float64 accumulation), for tests and
  * callers of Monodomain.M/K/A products. */
 int tc_sell_spmv(const tc_sell* A, const void* vals, const void* x, void* y, int dtype,

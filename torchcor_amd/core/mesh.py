@@ -4,11 +4,36 @@
 ``.elem``: count line, then ``<tag> n0 n1 ... region`` per element (the tag is
 dropped, the last column is the region); ``.lon``: a header line, then one
 fibre vector per element.  A count line that disagrees with the data raises
-``ValueError`` as in the reference.
+``ValueError`` as in the reference.  The tables are parsed by the library's
+multi-threaded reader (``tc_table_*``) instead of np.loadtxt.
 """
+import ctypes as C
 from pathlib import Path
 
 import numpy as np
+
+from .. import _lib as L
+
+
+def read_table(path, skip, col0=0, ncols=None, dtype=np.float64):
+    """Rows of the numeric table in ``path`` after ``skip`` header lines, as
+    a (rows, ncols) array of columns col0.. (np.loadtxt semantics: blank
+    lines and '#' comments ignored, rows of equal width)."""
+    p = str(path).encode()
+    rows, cols = C.c_int64(0), C.c_int32(0)
+    L.call("tc_table_shape", p, int(skip), C.byref(rows), C.byref(cols))
+    if ncols is None:
+        ncols = cols.value - col0
+    out = np.empty((rows.value, max(ncols, 0)), dtype=dtype)
+    if rows.value and ncols > 0:
+        fn = "tc_table_read_f64" if dtype == np.float64 else "tc_table_read_i64"
+        L.call(fn, p, int(skip), int(col0), int(ncols), rows.value, out.ctypes.data)
+    return out
+
+
+def header_count(path):
+    with Path(path).open("r") as f:
+        return int(f.readline().strip())
 
 
 class MeshReader:
@@ -25,27 +50,23 @@ class MeshReader:
         self.nodes = self.elems = self.regions = self.fibres = None
 
     def read_nodes(self, unit_conversion=1000):
-        with self.node_file.open("r") as f:
-            expected = int(f.readline().strip())
-        nodes = np.loadtxt(self.node_file, dtype=float, skiprows=1, ndmin=2)
+        expected = header_count(self.node_file)
+        nodes = read_table(self.node_file, 1)
         if nodes.shape[0] != expected:
             raise ValueError(f"Mismatch in number of nodes: expected {expected}, "
                              f"but found {nodes.shape[0]}")
         self.nodes = nodes / unit_conversion
 
     def read_elems(self):
-        with self.elem_file.open("r") as f:
-            expected = int(f.readline().strip())
-            first = f.readline().strip().split()
-        usecols = list(range(1, len(first)))
-        data = np.loadtxt(self.elem_file, dtype=int, skiprows=1, usecols=usecols, ndmin=2)
+        expected = header_count(self.elem_file)
+        data = read_table(self.elem_file, 1, col0=1, dtype=np.int64)
         if data.shape[0] != expected:
             raise ValueError(f"Mismatch: expected {expected}, but found {data.shape[0]}")
         self.elems = data[:, :-1]
         self.regions = data[:, -1]
 
     def read_fibres(self):
-        self.fibres = np.loadtxt(self.fibre_file, dtype=np.float64, skiprows=1, ndmin=2)
+        self.fibres = read_table(self.fibre_file, 1)
 
     def read(self, unit_conversion=1000):
         self.read_nodes(unit_conversion)

@@ -27,9 +27,10 @@ class StimConfig:
 
 def load_stimulus_region(vtx_filepath, device=None):
     """Node ids of a ``.vtx`` file: count line, a tag line, one id per line."""
+    from .mesh import read_table
     with Path(vtx_filepath).open("r") as f:
         size = int(f.readline().strip())
-    region = np.loadtxt(vtx_filepath, dtype=int, skiprows=2, ndmin=1)
+    region = read_table(vtx_filepath, 2, dtype=np.int64).reshape(-1)
     if len(region) != size:
         raise Exception(f"Error loading {vtx_filepath}")
     return torch.from_numpy(region).to(dtype=torch.long, device=device)
