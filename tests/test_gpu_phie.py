@@ -94,7 +94,8 @@ def test_phie_rcm_order_matches_mesh_order():
 
 
 def test_phie_from_result_files(tmp_path):
-    """solve() writes Vm.pt; phie_recovery() reads it and writes Phi_e.pt."""
+    """solve() writes Vm.pt; phie_recovery() reads it and writes Phi_e.pt;
+    pt_to_vtk() writes the frames and AT/RT as VTK files."""
     _say("[gpu] phie from files")
     nodes, elems, regions, f, cond, _ = _case(N=7)
     from torchcor_amd.ionic import ModifiedMS2v
@@ -111,3 +112,6 @@ def test_phie_from_result_files(tmp_path):
     assert saved.shape == (4, len(nodes)) and torch.equal(saved, phi.cpu())
     assert torch.isfinite(phi).all() and phi.abs().max().item() > 0
     assert phi.double().mean(dim=1).abs().max().item() < 1e-3
+    sim.pt_to_vtk()
+    assert (tmp_path / "Vm_vtk" / "frame_3.vtk").exists()
+    assert (tmp_path / "ATs.vtk").exists() and (tmp_path / "RTs.vtk").exists()

@@ -361,6 +361,21 @@ class Monodomain:
         names = ("ionic", "rhs_init", "pcg_spmv", "pcg_update", "pcg_check")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}
 
+    # ------------------------------------------------------------ output
+    def pt_to_vtk(self):
+        """Vm.pt frames, ATs.pt and RTs.pt of ``result_path`` as legacy VTK
+        files (monodomain.py:229-251): Vm_vtk/frame_{i}.vtk, ATs.vtk, RTs.vtk."""
+        from ..core.vtk import VTK3D, VTK3DSurface
+        t0 = time.time()
+        viz = (VTK3DSurface if self.elems.shape[1] == 3 else VTK3D)(self.nodes, self.elems)
+        frames = torch.load(self.result_path / "Vm.pt", weights_only=True)
+        for i in range(frames.shape[0]):
+            viz.save_frame(frames[i], self.result_path / f"Vm_vtk/frame_{i}.vtk")
+        for name in ("ATs", "RTs"):
+            viz.save_frame(torch.load(self.result_path / f"{name}.pt", weights_only=True),
+                           self.result_path / f"{name}.vtk")
+        print(f"Saved vtk files in {round(time.time() - t0, 2)}", flush=True)
+
     # ------------------------------------------------------ phi_e recovery
     def phie_recovery(self, a_tol=1e-5, r_tol=1e-5, max_iter=100, Vm=None, save=True):
         """Extracellular potential of every Vm snapshot (monodomain.py:254-286).
