@@ -137,30 +137,6 @@ struct RowG : RowV<T, NS> {
   }
 };
 
-template <class T, int NS>
-__device__ inline void rowv_at(const T* tab, T X, const TabParam& t, RowV<T, NS>& R) {
-  typedef typename Vec16<T>::type V;
-  constexpr int E = Vec16<T>::n;
-  static_assert(NS % E == 0, "row stride must be a multiple of 16 bytes");
-  Lerp<T> L = lerp_at(X, t);
-  const V* p = reinterpret_cast<const V*>(tab + L.lo * NS);
-  V a[2 * NS / E];
-#pragma unroll
-  for (int k = 0; k < 2 * NS / E; ++k) a[k] = p[k];
-#pragma unroll
-  for (int k = 0; k < NS / E; ++k) {
-    const T* x = reinterpret_cast<const T*>(&a[k]);
-    const T* y = reinterpret_cast<const T*>(&a[k + NS / E]);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      R.lo[k * E + e] = x[e];
-      R.hi[k * E + e] = y[e];
-    }
-  }
-  R.w = L.w;
-  R.w1 = T(1) - L.w;
-}
-
 template <class T>
 __device__ inline Row<T> row_at(const T* tab, int ncol, T X, const TabParam& t) {
   Lerp<T> L = lerp_at(X, t);

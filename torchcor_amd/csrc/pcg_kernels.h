@@ -216,22 +216,6 @@ __device__ inline void fold_partials(const double* __restrict__ partials, int G,
   for (int k = 0; k < NV; ++k) out[k] = bc[k];
 }
 
-// Block-uniform read of a flag (one load, shared by all waves).
-__device__ inline bool block_flag(const int* f) {
-  __shared__ int v;
-  if (threadIdx.x == 0) v = f ? *(volatile const int*)f : 0;
-  __syncthreads();
-  return v != 0;
-}
-
-// Block-uniform: this solve has stopped, or the pipeline is stalled.
-__device__ inline bool block_halted(const PcgCtl* ctl) {
-  __shared__ int v;
-  if (threadIdx.x == 0) v = ctl->converged | ctl->stall;
-  __syncthreads();
-  return v != 0;
-}
-
 __device__ inline bool stop_rule(double zn, const PcgCtl* ctl) {
   return zn < ctl->a_tol || zn / ctl->bnorm < ctl->r_tol;
 }

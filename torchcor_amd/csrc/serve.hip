@@ -123,9 +123,6 @@ struct SvPool {
   const SvSys* sysd;        // per live system
   const int32_t* list;      // slices of the live systems, system by system
   int64_t nl;
-  const int32_t* sys_list;  // slot of system q
-  const int32_t* sys_vb0;   // first and last block of system q
-  const int32_t* sys_vb1;
   const int32_t* slice_off; // first entry of each slice
   const int32_t* slice_w;   // padded width of each slice
   const int32_t* slice_sys;
@@ -142,21 +139,6 @@ struct BlockView {
   int q;           // system index, -1 for a padding block
 };
 
-
-// Sum of system q's block partials (nv values each): called by a whole
-// wavefront, result in every lane.
-__device__ inline void fold_sys(const SvPool& pl, const double* __restrict__ P, int nv, int q,
-                                double& a, double& b) {
-  const int lane = threadIdx.x & 63;
-  const int v0 = pl.sys_vb0[q], v1 = pl.sys_vb1[q];
-  double s0 = 0.0, s1 = 0.0;
-  for (int v = v0 + lane; v <= v1; v += 64) {
-    s0 += P[(int64_t)v * nv];
-    if (nv == 2) s1 += P[(int64_t)v * nv + 1];
-  }
-  a = wave_sum(s0);
-  b = wave_sum(s1);
-}
 
 // Sum of the block partials [v0, v1] (at most kSBlkMax blocks, nv values
 // each) with every thread of the block, in the same order in every block:
@@ -764,9 +746,6 @@ struct tc_serve {
   double* part[3];      // block partials: start (2), spmv (1), update (2)
   int32_t* list;        // device slice list (all live systems)
   int32_t* mlist[3];    // device slice lists per model
-  int32_t* sys_list;    // device: slot of live system q
-  int32_t* sys_vb0;     // device: blocks of system q
-  int32_t* sys_vb1;
   tc::SvBlk* blk_desc;  // device: per block
   tc::SvEnt* ent;       // device: per list entry
   tc::SvSys* sysd;      // device: per live system
@@ -787,8 +766,7 @@ namespace tc {
 namespace {
 
 struct SvLayout {
-  size_t ctl, arrays, part[3], list, mlist[3], sys_list, sys_vb0, sys_vb1, blk_desc, ent, sysd,
-      total;
+  size_t ctl, arrays, part[3], list, mlist[3], blk_desc, ent, sysd, total;
 };
 
 SvLayout sv_layout(int64_t cap_slices, int max_sys) {
@@ -811,9 +789,6 @@ SvLayout sv_layout(int64_t cap_slices, int max_sys) {
   L.part[2] = take(slots * 2 * sizeof(double));
   L.list = take((size_t)cap_slices * 4);
   for (int m = 0; m < 3; ++m) L.mlist[m] = take((size_t)cap_slices * 4);
-  L.sys_list = take(ms * 4);
-  L.sys_vb0 = take(ms * 4);
-  L.sys_vb1 = take(ms * 4);
   L.blk_desc = take(slots * sizeof(SvBlk));
   L.ent = take((size_t)cap_slices * sizeof(SvEnt));
   L.sysd = take(ms * sizeof(SvSys));
@@ -828,9 +803,6 @@ SvPool sv_pool(const tc_serve* S) {
   p.sysd = S->sysd;
   p.list = S->list;
   p.nl = S->nl;
-  p.sys_list = S->sys_list;
-  p.sys_vb0 = S->sys_vb0;
-  p.sys_vb1 = S->sys_vb1;
   p.slice_off = S->d.slice_off;
   p.slice_w = S->d.slice_w;
   p.slice_sys = S->d.slice_sys;
@@ -892,9 +864,7 @@ int sv_rebuild(tc_serve* S, hipStream_t s) {
     k.pad = 0;
   }
   int rc;
-  if ((rc = up(S->list, list)) || (rc = up(S->sys_list, sl)) || (rc = up(S->sys_vb0, vb0)) ||
-      (rc = up(S->sys_vb1, vb1)))
-    return rc;
+  if ((rc = up(S->list, list))) return rc;
   for (int m = 0; m < 3; ++m)
     if ((rc = up(S->mlist[m], ml[m]))) return rc;
   TC_HIP(hipMemcpyAsync(S->blk_desc, bd.data(), bd.size() * sizeof(SvBlk), hipMemcpyHostToDevice,
@@ -1196,9 +1166,6 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
   for (int k = 0; k < 3; ++k) S->part[k] = (double*)(b + L.part[k]);
   S->list = (int32_t*)(b + L.list);
   for (int m = 0; m < 3; ++m) S->mlist[m] = (int32_t*)(b + L.mlist[m]);
-  S->sys_list = (int32_t*)(b + L.sys_list);
-  S->sys_vb0 = (int32_t*)(b + L.sys_vb0);
-  S->sys_vb1 = (int32_t*)(b + L.sys_vb1);
   S->blk_desc = (SvBlk*)(b + L.blk_desc);
   S->ent = (SvEnt*)(b + L.ent);
   S->sysd = (SvSys*)(b + L.sysd);
