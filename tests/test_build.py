@@ -52,6 +52,38 @@ def test_alias_package_surface():
     assert ms.u_gate == 0.1 and ms.DV == 100.0
 
 
+def test_alias_module_paths():
+    """Every module path a reference user script imports from resolves
+    (torchcor/__init__.py, torchcor/core/__init__.py:1-9,
+    torchcor/ionic/__init__.py) and names the same objects."""
+    import importlib
+    import torchcor_amd
+    from torchcor_amd import core, ionic
+    paths = {
+        "torchcor.device": ["get_device", "set_device"],
+        "torchcor.core.assemble": ["Matrices3D", "Matrices3DSurface"],
+        "torchcor.core.solver": ["ConjugateGradient"],
+        "torchcor.core.preconditioner": ["Preconditioner"],
+        "torchcor.core.mesh": ["MeshReader"],
+        "torchcor.core.stimulation": ["Stimuli", "load_stimulus_region"],
+        "torchcor.core.conductivity": ["Conductivity"],
+        "torchcor.core.reorder": ["RCM"],
+        "torchcor.core.visualize": ["VTK3D", "VTK3DSurface"],
+        "torchcor.ionic.TenTusscherPanfilov": ["TenTusscherPanfilov"],
+        "torchcor.ionic.ModifiedMS2v": ["ModifiedMS2v"],
+        "torchcor.ionic.CourtemancheRamirezNattel": ["CourtemancheRamirezNattel"],
+    }
+    for path, names in paths.items():
+        mod = importlib.import_module(path)
+        for name in names:
+            ours = getattr(torchcor_amd, name, None) or getattr(core, name, None) \
+                or getattr(ionic, name, None)
+            if ours is None:
+                from torchcor_amd.core import vtk
+                ours = getattr(vtk, name)
+            assert getattr(mod, name) is ours, (path, name)
+
+
 def test_mesh_files_round_trip(tmp_path):
     from torchcor_amd import meshes
     from torchcor_amd.core.mesh import MeshReader

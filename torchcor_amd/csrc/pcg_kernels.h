@@ -187,7 +187,7 @@ template <int NV>
 __device__ inline void fold_partials(const double* __restrict__ partials, int G,
                                      double (&out)[NV]) {
   __shared__ double bc[NV];
-  constexpr int R = kMaxBlocks / kPcgBlock;
+  constexpr int R = (kMaxBlocks + kPcgBlock - 1) / kPcgBlock;
   double v[NV][R];
 #pragma unroll
   for (int k = 0; k < NV; ++k)
