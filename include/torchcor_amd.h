@@ -365,6 +365,17 @@ int tc_sim_destroy(tc_sim* sim);
  * `stream` before returning. */
 int tc_sim_advance(tc_sim* sim, int64_t nsteps, int32_t* iters_host, void* frames,
                    int64_t ts_per_frame, int64_t max_frames, void* stream);
+/* As tc_sim_advance, with the frames in host memory: frames_host (pinned,
+ * max_frames x n in dtype) receives frame k through device slot
+ * k % ring_slots of `ring` (ring_slots x n in dtype, 1..16 slots), copied by
+ * DMA on a copy stream of the runtime while the next steps run; a slot is
+ * rewritten only after its previous copy ended.  Both streams are
+ * synchronized before returning.  Replaces the reference's list of device
+ * snapshots (torchcor/simulator/monodomain.py:162,186-187) when the frames
+ * of a run should not stay in device memory. */
+int tc_sim_advance_ring(tc_sim* sim, int64_t nsteps, int32_t* iters_host, void* frames_host,
+                        void* ring, int ring_slots, int64_t ts_per_frame, int64_t max_frames,
+                        void* stream);
 /* Apply the LAT/RT test to the last step's solution (it is otherwise folded
  * into the next step's ionic pass).  Idempotent. */
 int tc_sim_finish(tc_sim* sim, void* stream);

@@ -178,6 +178,7 @@ FUNCS = {
     "tc_sim_create": (C.POINTER(p), C.POINTER(SimDesc)),
     "tc_sim_destroy": (p,),
     "tc_sim_advance": (p, i64, C.POINTER(C.c_int32), p, i64, i64, p),
+    "tc_sim_advance_ring": (p, i64, C.POINTER(C.c_int32), p, p, i32, i64, i64, p),
     "tc_sim_finish": (p, p),
     "tc_sim_status": (p, C.POINTER(dbl), I64_P, INT_P, I64_P),
     "tc_sim_set_timing": (p, i32),

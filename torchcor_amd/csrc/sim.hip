@@ -8,7 +8,9 @@
 //      largest iteration count among the last steps,
 //   4. a one-block stopping check that stores the step status and, when the
 //      solve is not finished, raises ctl->stall,
-//   5. an optional snapshot copy.
+//   5. an optional snapshot copy: into a device frame buffer, or into one
+//      slot of a small device ring that a copy stream drains into pinned
+//      host memory while the next steps run (tc_sim_advance_ring).
 // The check kernel writes the step status straight into pinned host memory
 // (with a sequence number written last), so no per-step event or copy is
 // enqueued; the host polls that record.
@@ -32,6 +34,7 @@ namespace {
 constexpr int kRing = 64;
 constexpr int kTimers = 5;  // ionic, rhs+start, spmv, update, check
 constexpr int kHistory = 16;
+constexpr int kMaxRingSlots = 16;
 
 struct HostState {
   int64_t steps = 0;
@@ -72,9 +75,17 @@ struct tc_sim {
   std::vector<hipEvent_t> tev[tc::kRing];
   double acc_ms[tc::kTimers];
   int64_t acc_n[tc::kTimers];
-  // frame buffer of the current advance call
+  // frame buffer of the current advance call (host memory in ring mode)
   void* frames;
   int64_t tpf, max_frames;
+  // ring mode: frame k goes to device slot k % ring_slots, then by DMA on
+  // the copy stream cs to frames + k*n.  written[j] orders the copy after
+  // the slot's write; copied[j] orders the slot's next write after the copy.
+  void* ring;
+  int ring_slots;
+  hipStream_t cs;
+  hipEvent_t written[tc::kMaxRingSlots], copied[tc::kMaxRingSlots];
+  bool slot_busy[tc::kMaxRingSlots];
 };
 
 namespace tc {
@@ -168,10 +179,28 @@ int wait_status(tc_sim* S, int slot, long long seq) {
 template <class T, class XT>
 int copy_frame(tc_sim* S, const void* src, int64_t frame, const int* gate, hipStream_t s) {
   const int64_t n = S->d.n;
-  T* dst = (T*)S->frames + frame * n;
-  hipLaunchKernelGGL((k_copy_gated<T, XT>), dim3((int)ceil_div(n, 256)), dim3(256), 0, s, gate,
-                     (const XT*)src, dst, n);
+  const dim3 g((int)ceil_div(n, 256)), b(256);
+  if (!S->ring) {
+    T* dst = (T*)S->frames + frame * n;
+    hipLaunchKernelGGL((k_copy_gated<T, XT>), g, b, 0, s, gate, (const XT*)src, dst, n);
+    TC_LAUNCH();
+    return TC_OK;
+  }
+  // A gated copy (its step stalled) leaves the slot as it was and the DMA
+  // below then carries stale values; the stalled step's redo and the
+  // re-issued later steps copy the same frames again, later on the same copy
+  // stream, so the host frame ends up with the right values.
+  const int j = (int)(frame % S->ring_slots);
+  if (S->slot_busy[j]) TC_HIP(hipStreamWaitEvent(s, S->copied[j], 0));
+  T* slot = (T*)S->ring + (int64_t)j * n;
+  hipLaunchKernelGGL((k_copy_gated<T, XT>), g, b, 0, s, gate, (const XT*)src, slot, n);
   TC_LAUNCH();
+  TC_HIP(hipEventRecord(S->written[j], s));
+  TC_HIP(hipStreamWaitEvent(S->cs, S->written[j], 0));
+  TC_HIP(hipMemcpyAsync((T*)S->frames + frame * n, slot, (size_t)n * sizeof(T),
+                        hipMemcpyDeviceToHost, S->cs));
+  TC_HIP(hipEventRecord(S->copied[j], S->cs));
+  S->slot_busy[j] = true;
   return TC_OK;
 }
 
@@ -384,7 +413,30 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     ++done;
   }
   TC_HIP(hipStreamSynchronize(s));
+  if (S->ring) {
+    TC_HIP(hipStreamSynchronize(S->cs));
+    for (int j = 0; j < kMaxRingSlots; ++j) S->slot_busy[j] = false;
+  }
   return TC_OK;
+}
+
+// Copy stream and events of ring mode, made on first use.
+int ring_setup(tc_sim* S) {
+  if (S->cs) return TC_OK;
+  TC_HIP(hipStreamCreateWithFlags(&S->cs, hipStreamNonBlocking));
+  for (int j = 0; j < kMaxRingSlots; ++j) {
+    TC_HIP(hipEventCreateWithFlags(&S->written[j], hipEventDisableTiming));
+    TC_HIP(hipEventCreateWithFlags(&S->copied[j], hipEventDisableTiming));
+    S->slot_busy[j] = false;
+  }
+  return TC_OK;
+}
+
+int run(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
+  if (S->d.dtype == TC_F64) return advance<double, double, double>(S, nsteps, iters, s);
+  if (S->d.x_f64 && S->d.w_f64) return advance<float, double, double>(S, nsteps, iters, s);
+  if (S->d.x_f64) return advance<float, double, float>(S, nsteps, iters, s);
+  return advance<float, float, float>(S, nsteps, iters, s);
 }
 
 }  // namespace
@@ -446,6 +498,14 @@ extern "C" int tc_sim_destroy(tc_sim* S) {
   if (!S) return TC_OK;
   for (int k = 0; k < kRing; ++k)
     for (hipEvent_t e : S->tev[k]) (void)hipEventDestroy(e);
+  if (S->cs) {
+    (void)hipStreamSynchronize(S->cs);
+    for (int j = 0; j < kMaxRingSlots; ++j) {
+      if (S->written[j]) (void)hipEventDestroy(S->written[j]);
+      if (S->copied[j]) (void)hipEventDestroy(S->copied[j]);
+    }
+    (void)hipStreamDestroy(S->cs);
+  }
   if (S->host_status) (void)hipHostFree(S->host_status);
   delete S;
   return TC_OK;
@@ -455,13 +515,30 @@ extern "C" int tc_sim_advance(tc_sim* S, int64_t nsteps, int32_t* iters, void* f
                               int64_t tpf, int64_t max_frames, void* stream) {
   TC_CHECK(S && iters && nsteps >= 0, TC_ERR_ARG, "tc_sim_advance: bad arguments");
   S->frames = frames;
+  S->ring = nullptr;
   S->tpf = tpf;
   S->max_frames = frames ? max_frames : 0;
-  hipStream_t s = (hipStream_t)stream;
-  if (S->d.dtype == TC_F64) return advance<double, double, double>(S, nsteps, iters, s);
-  if (S->d.x_f64 && S->d.w_f64) return advance<float, double, double>(S, nsteps, iters, s);
-  if (S->d.x_f64) return advance<float, double, float>(S, nsteps, iters, s);
-  return advance<float, float, float>(S, nsteps, iters, s);
+  return run(S, nsteps, iters, (hipStream_t)stream);
+}
+
+extern "C" int tc_sim_advance_ring(tc_sim* S, int64_t nsteps, int32_t* iters, void* frames_host,
+                                   void* ring, int ring_slots, int64_t tpf, int64_t max_frames,
+                                   void* stream) {
+  TC_CHECK(S && iters && nsteps >= 0 && frames_host && ring, TC_ERR_ARG,
+           "tc_sim_advance_ring: bad arguments");
+  TC_CHECK(ring_slots >= 1 && ring_slots <= kMaxRingSlots, TC_ERR_ARG,
+           "tc_sim_advance_ring: ring_slots %d outside 1..%d", ring_slots, kMaxRingSlots);
+  int rc = ring_setup(S);
+  if (rc) return rc;
+  S->frames = frames_host;
+  S->ring = ring;
+  S->ring_slots = ring_slots;
+  S->tpf = tpf;
+  S->max_frames = max_frames;
+  rc = run(S, nsteps, iters, (hipStream_t)stream);
+  if (rc) (void)hipStreamSynchronize(S->cs);  // no DMA left in flight into frames_host
+  S->ring = nullptr;
+  return rc;
 }
 
 extern "C" int tc_sim_finish(tc_sim* S, void* stream) {

@@ -34,6 +34,11 @@ from ..core.stimulation import Stimuli
 from ..core.system import P1System
 from ..ionic.base import default_device
 
+# Vm snapshots up to this size stay in device memory; larger runs stream them
+# to pinned host memory through a ring of RING_SLOTS device frames.
+FRAMES_DEVICE_BYTES = 16 << 30
+RING_SLOTS = 4
+
 
 class Monodomain:
     def __init__(self, ionic_model, T, dt, device=None, dtype=None, reorder="auto"):
@@ -112,7 +117,7 @@ class Monodomain:
 
     def _to_mesh_order(self, v):
         """A per-node array (last axis) from the internal order to the mesh's."""
-        return v if self.node_map is None else v[..., self.node_map]
+        return v if self.node_map is None else v[..., self.node_map.to(v.device)]
 
     def _to_internal(self, v):
         return v if self.node_order is None else v[..., self.node_order]
@@ -227,14 +232,16 @@ class Monodomain:
         return self._to_mesh_order(u), n_iter, ionic_time, electric_time
 
     def solve(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True,
-              snapshot_interval=1, verbose=True, result_path=None, nsteps=None):
+              snapshot_interval=1, verbose=True, result_path=None, nsteps=None, frames="auto"):
         """Time loop (monodomain.py:141-226).  Writes ATs.pt, RTs.pt and Vm.pt
         like the reference and returns the total PCG iteration count.
-        ``nsteps`` (extension) stops after that many steps."""
+        Extensions: ``nsteps`` stops after that many steps; ``frames`` places
+        the Vm snapshots (see ``start``)."""
         if result_path is not None:
             self.result_path = Path(result_path)
             self.result_path.mkdir(parents=True, exist_ok=True)
-        self.start(a_tol, r_tol, max_iter, calculate_AT_RT, linear_guess, snapshot_interval, verbose)
+        self.start(a_tol, r_tol, max_iter, calculate_AT_RT, linear_guess, snapshot_interval, verbose,
+                   frames=frames)
         nt = self.nt if nsteps is None else min(int(nsteps), self.nt)
         self.advance(nt)
         return self.finish()
@@ -244,12 +251,18 @@ class Monodomain:
     # loop itself runs in the native runtime (tc_sim_*): steps are pipelined
     # on the device and no host round trip is needed per step.
     def start(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True,
-              snapshot_interval=1, verbose=False, depth=4, x_f64=True, precise_rhs=True):
+              snapshot_interval=1, verbose=False, depth=4, x_f64=True, precise_rhs=True,
+              frames="auto"):
         """Set up a run.  ``x_f64`` keeps the PCG iterate in float64 in float32
         runs, as the reference's ConjugateGradient does (solver.py:24-26);
         x_prev stays in the working dtype.  ``precise_rhs`` also keeps the
         right-hand side in float64 (w stored in float64, b never rounded);
-        with it off a float32 run rounds w and b as the reference does."""
+        with it off a float32 run rounds w and b as the reference does.
+        ``frames`` places the Vm snapshots: "device" keeps them in device
+        memory, as the reference's list of snapshots does
+        (monodomain.py:162,186-187); "host" streams each one through a
+        device ring into pinned host memory while the run goes on; "auto"
+        is "device" up to FRAMES_DEVICE_BYTES of frames, else "host"."""
         import ctypes as C
         self.assemble()
         ion = self.ionic_model
@@ -261,7 +274,20 @@ class Monodomain:
         self._w = torch.empty_like(u, dtype=torch.float64 if w_f64 else self.dtype)
         tpf = int(snapshot_interval / self.dt)
         nframes = self.nt // tpf if tpf > 0 else 0
-        self._frames = torch.empty((nframes + 1, self.n_nodes), dtype=self.dtype, device=self.device)
+        nbytes = (nframes + 1) * self.n_nodes * u.element_size()
+        if frames == "auto":
+            frames = "host" if nbytes > FRAMES_DEVICE_BYTES else "device"
+        if frames == "device":
+            self._ring = None
+            self._frames = torch.empty((nframes + 1, self.n_nodes), dtype=self.dtype,
+                                       device=self.device)
+        elif frames == "host":
+            self._ring = torch.empty((RING_SLOTS, self.n_nodes), dtype=self.dtype,
+                                     device=self.device)
+            self._frames = torch.empty((nframes + 1, self.n_nodes), dtype=self.dtype,
+                                       pin_memory=True)
+        else:
+            raise ValueError(f"frames must be 'auto', 'device' or 'host', not {frames!r}")
         self._frames[0].copy_(u)
         self.activation_time = self.repolarization_time = None
         if calculate_AT_RT:
@@ -334,9 +360,14 @@ class Monodomain:
                 chunk = min(k, R["tpf"] - steps % R["tpf"])
             iters = (C.c_int32 * chunk)()
             frames = self._frames[1:] if R["nframes"] > 0 else None
-            L.call("tc_sim_advance", self._sim, chunk, iters,
-                   L.ptr(frames) if frames is not None else None, R["tpf"], R["nframes"],
-                   L.stream_handle(self.device))
+            if frames is not None and self._ring is not None:
+                L.call("tc_sim_advance_ring", self._sim, chunk, iters, frames.data_ptr(),
+                       L.ptr(self._ring), RING_SLOTS, R["tpf"], R["nframes"],
+                       L.stream_handle(self.device))
+            else:
+                L.call("tc_sim_advance", self._sim, chunk, iters,
+                       L.ptr(frames) if frames is not None else None, R["tpf"], R["nframes"],
+                       L.stream_handle(self.device))
             self.iterations.extend(iters)
             R["n_total_iter"] += sum(iters)
             k -= chunk
