@@ -376,6 +376,30 @@ def test_monodomain_unstructured_permuted():
     assert np.array_equal(np.array(gm.iterations), ro["iters"])
 
 
+@pytest.mark.parametrize("model,dtype", [("ms2v", torch.float64), ("ttp", torch.float32)])
+def test_monodomain_surface_triangles(model, dtype):
+    """Monodomain on a triangle surface (the reference's Matrices3DSurface
+    branch, monodomain.py:86-89): the closed surface of the 21^3 cube,
+    anisotropic conductivity with fibres along x (normal to two of the six
+    faces), stimulus on the x = 0 face."""
+    _say(f"[gpu] monodomain surface triangles {model} {dtype}")
+    nodes, tris = meshes.surface_of(*meshes.kuhn_cube(21))
+    stim = np.flatnonzero(nodes[:, 0] == 0.0)
+    ro, gm, Vg, it = _run_pair(None, model, dtype, 15.0, stim, cond=(0.3, 0.1),
+                               mesh=(nodes, tris))
+    assert gm.system.k == 3
+    if dtype == torch.float64:
+        assert np.abs(Vg - ro["Vm"]).max() < 1e-6
+        assert np.array_equal(np.array(gm.iterations), ro["iters"])
+        assert np.array_equal(gm.activation_time.cpu().numpy(), ro["AT"])
+    else:
+        assert np.abs(Vg - ro["Vm"]).max() <= 0.5
+        ATg = gm.activation_time.cpu().numpy()
+        assert np.array_equal(ATg > 0, ro["AT"] > 0)
+        assert np.abs(ATg - ro["AT"]).max() <= 0.01 + 1e-6
+    assert (ro["AT"] > 0).mean() > 0.5
+
+
 def test_monodomain_files_api(tmp_path):
     """load_mesh/add_stimulus from files, outputs ATs.pt/RTs.pt/Vm.pt."""
     _say("[gpu] monodomain files api")

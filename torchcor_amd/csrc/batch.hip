@@ -51,11 +51,6 @@ struct BatCtl {
   long long total[kLanes];
 };
 
-struct BatStatus {
-  int n_iter, iter, stalled, pad;
-  long long seq, pad2;
-};
-
 template <class T>
 struct BatOp {
   int64_t n;
@@ -497,9 +492,9 @@ k_bat_fold(int kind, int k, int G, int lanes, int B, BatCtl* ctl, const double* 
 // One wavefront: end of a launch chunk after k_end iterations.  Instances
 // still iterating below max_iter stall the pipeline; otherwise the step is
 // final: per-instance n_iter sums advance and the status (largest n_iter)
-// goes to pinned host memory with the sequence number written last.
+// goes to pinned host memory as a StatusRec.
 __global__ void __launch_bounds__(64)
-k_bat_check(int k_end, int B, int max_iter, BatCtl* ctl, BatStatus* status, long long seq) {
+k_bat_check(int k_end, int B, int max_iter, BatCtl* ctl, StatusRec* status, long long seq) {
   if (ctl->stall) return;
   const int b = threadIdx.x;
   const int act = b < B ? ctl->active[b] : 0;
@@ -514,14 +509,8 @@ k_bat_check(int k_end, int B, int max_iter, BatCtl* ctl, BatStatus* status, long
   if (!any_stall && b < B) ctl->total[b] += ctl->n_iter[b];
   if (b == 0) {
     if (any_stall) ctl->stall = 1;
-    if (status) {
-      volatile BatStatus* st = status;
-      st->n_iter = ni;
-      st->iter = ctl->iter;
-      st->stalled = any_stall ? 1 : 0;
-      __threadfence_system();
-      st->seq = seq;
-    }
+    if (status)
+      put_status(status, (uint32_t)ni, (uint32_t)ctl->iter | (any_stall ? kStStalled : 0u), seq);
   }
 }
 
@@ -587,8 +576,9 @@ struct tc_batch {
   void* r;
   void* p[2];
   void* Ap;
-  tc::BatStatus* host_status;
-  tc::BatStatus* dev_view;
+  tc::StatusRec* host_status;  // pinned, written by the check pass
+  tc::StatusRec* dev_view;     // device address of host_status
+  tc::StepStatus status[tc::kBRing];  // records as read by bat_wait
   long long seq;
   tc::BHost vs;
   std::deque<int> recent;
@@ -695,9 +685,8 @@ int bat_check(tc_batch* S, int k_end, int slot, long long seq, hipStream_t s) {
 }
 
 int bat_wait(tc_batch* S, int slot, long long seq) {
-  volatile BatStatus* st = S->host_status + slot;
   const auto t0 = std::chrono::steady_clock::now();
-  for (long spins = 0; st->seq != seq; ++spins) {
+  for (long spins = 0; !status_take(S->host_status + slot, seq, S->status + slot); ++spins) {
     if ((spins & 1023) == 1023) {
       if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(120)) {
         set_error("batch runtime: no status for record %lld after 120 s", seq);
@@ -806,7 +795,7 @@ int bat_enqueue(tc_batch* S, BHost& ps, BPlan& plan, hipStream_t s) {
 template <class T, class XT, int NK>
 int bat_resolve(tc_batch* S, const BPlan& plan, hipStream_t s) {
   const tc_batch_desc& d = S->d;
-  BatStatus* st = S->host_status + plan.slot;
+  StepStatus* st = S->status + plan.slot;
   XT* xs = (XT*)d.x[plan.after.xc];
   int it = plan.K;
   int chunk = it > 0 ? it : 1;
@@ -844,7 +833,7 @@ int bat_advance(tc_batch* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     BPlan plan = inflight.front();
     int rc = bat_wait(S, plan.slot, plan.seq);
     if (rc) return rc;
-    BatStatus* st = S->host_status + plan.slot;
+    StepStatus* st = S->status + plan.slot;
     if (st->stalled) {
       rc = bat_resolve<T, XT, NK>(S, plan, s);
       if (rc) return rc;
@@ -924,7 +913,8 @@ extern "C" int tc_batch_create(tc_batch** out, const tc_batch_desc* d, void* str
   TC_CHECK(d->n_stim >= 0 && d->n_stim <= TC_MAX_STIMULI, TC_ERR_ARG, "bad stimulus count");
   TC_CHECK(d->n_stim == 0 || d->stim_mask, TC_ERR_ARG, "stimuli need a mask");
   TC_CHECK(!d->AT || d->RT, TC_ERR_ARG, "AT without RT");
-  TC_CHECK(d->max_iter >= 0, TC_ERR_ARG, "max_iter < 0");
+  TC_CHECK(d->max_iter >= 0 && d->max_iter <= kMaxIterLimit, TC_ERR_ARG,
+           "max_iter %d outside 0..%d", d->max_iter, kMaxIterLimit);
   TC_CHECK(d->n_groups >= 1 && d->n_groups <= TC_BATCH_MAX_GROUPS, TC_ERR_ARG,
            "bad group count %d", d->n_groups);
   TC_CHECK(d->n * d->lanes < (int64_t)INT32_MAX * 4, TC_ERR_SIZE, "batch too large");
@@ -976,7 +966,7 @@ extern "C" int tc_batch_create(tc_batch** out, const tc_batch_desc* d, void* str
   S->p[0] = base + L.p0;
   S->p[1] = base + L.p1;
   S->Ap = base + L.Ap;
-  hipError_t e = hipHostMalloc((void**)&S->host_status, sizeof(BatStatus) * kBRing,
+  hipError_t e = hipHostMalloc((void**)&S->host_status, sizeof(StatusRec) * kBRing,
                                hipHostMallocMapped);
   if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&S->dev_view, S->host_status, 0);
   if (e != hipSuccess) {
@@ -984,7 +974,7 @@ extern "C" int tc_batch_create(tc_batch** out, const tc_batch_desc* d, void* str
     tc_batch_destroy(S);
     return TC_ERR_HIP;
   }
-  for (int k = 0; k < kBRing; ++k) S->host_status[k].seq = -1;
+  status_clear(S->host_status, kBRing);
   S->seq = 0;
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_bat_init_ctl, dim3(1), dim3(kLanes), 0, s, S->ctl, d->a_tol, d->r_tol);

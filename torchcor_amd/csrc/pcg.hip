@@ -66,7 +66,7 @@ int pcg_solve(tc_pcg* P, const CI* sc, const T* b, T* x, const T* xp, T* xn, dou
     }
     it += c;
     hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kPcgBlock), 0, s, it - 1, P->G, P->ctl, P->part[2],
-                       (StepStatus*)nullptr, 0, max_iter, 0LL);
+                       (StatusRec*)nullptr, 0, max_iter, 0LL);
     TC_LAUNCH();
     TC_HIP(hipMemcpyAsync(P->host_ctl, P->ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, s));
     TC_HIP(hipStreamSynchronize(s));

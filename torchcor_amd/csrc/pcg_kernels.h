@@ -41,14 +41,50 @@ struct PcgCtl {
                   // every later gated kernel then returns at once
 };
 
-// Per-step result written by the check pass; `seq` (the step number) is
-// written last, after a system-scope fence, so a host polling pinned memory
-// sees a complete record.
+// Step status in pinned host memory: two 8-byte words, each a 32-bit
+// payload beside the low 32 bits of the record's sequence number.  A
+// naturally aligned 8-byte store reaches host memory whole and an aligned
+// 8-byte host load reads it whole, so a word that carries the awaited
+// sequence number carries that record's payload: no fence is needed between
+// payload and flag (a system-scope fence would also write back the L2 of
+// the writer's XCD, microseconds after a pass that wrote megabytes).
+struct StatusRec {
+  unsigned long long w[2];
+};
+
+// Payload of the time-step runtimes' records: p0 = n_iter, p1 = iter with
+// bit 30 = converged and bit 31 = stalled (iteration counts stay below 2^30).
+constexpr uint32_t kStConverged = 1u << 30, kStStalled = 1u << 31, kStIterMask = kStConverged - 1;
+constexpr int kMaxIterLimit = (int)kStIterMask;
+
+__device__ inline void put_status(StatusRec* r, uint32_t p0, uint32_t p1, long long seq) {
+  const unsigned long long q = (unsigned long long)(uint32_t)seq << 32;
+  __hip_atomic_store(&r->w[0], q | p0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&r->w[1], q | p1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The status of record `seq` as the host reads it.
 struct StepStatus {
   int n_iter, iter, converged, stalled;
-  long long seq;
-  long long pad;
 };
+
+inline void status_clear(StatusRec* r, int count) {
+  for (int k = 0; k < count; ++k) r[k].w[0] = r[k].w[1] = ~0ull;
+}
+
+// True (and *out filled) once both words of `r` carry sequence number seq.
+inline bool status_take(const StatusRec* r, long long seq, StepStatus* out) {
+  const volatile unsigned long long* w = r->w;
+  const unsigned long long a = w[0], b = w[1];
+  const uint32_t q = (uint32_t)seq;
+  if ((uint32_t)(a >> 32) != q || (uint32_t)(b >> 32) != q) return false;
+  const uint32_t p1 = (uint32_t)b;
+  out->n_iter = (int)(uint32_t)a;
+  out->iter = (int)(p1 & kStIterMask);
+  out->converged = (p1 & kStConverged) ? 1 : 0;
+  out->stalled = (p1 & kStStalled) ? 1 : 0;
+  return true;
+}
 
 }  // namespace tc
 
@@ -446,7 +482,7 @@ k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
 // result below max_iter sets ctl->stall, which gates every later launch of
 // a pipelined run until the host has launched more iterations.
 __global__ void __launch_bounds__(kPcgBlock)
-k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepStatus* status,
+k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StatusRec* status,
             int stall_open, int max_iter, long long seq) {
   __shared__ int flags[2];
   if (threadIdx.x == 0) {
@@ -477,15 +513,9 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StepSta
       if (stalled) ctl->stall = 1;
     }
   }
-  if (threadIdx.x == 0 && status) {
-    volatile StepStatus* st = status;
-    st->n_iter = n_iter;
-    st->iter = iter;
-    st->converged = conv;
-    st->stalled = stalled;
-    __threadfence_system();
-    st->seq = seq;
-  }
+  if (threadIdx.x == 0 && status)
+    put_status(status, (uint32_t)n_iter,
+               (uint32_t)iter | (conv ? kStConverged : 0u) | (stalled ? kStStalled : 0u), seq);
 }
 
 // Fused right-hand side and PCG start of one time step:

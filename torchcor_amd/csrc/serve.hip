@@ -65,11 +65,6 @@ struct SvCtl {
   tc_stimulus* stim;    // TC_POOL_STIM per system
 };
 
-struct SvStatus {
-  int n_iter, iter, stalled, pad;
-  long long seq, pad2;
-};
-
 // Block -> contiguous range of the slice list; blocks of one XCD adjacent.
 struct LRange {
   int64_t l0, l1;
@@ -504,23 +499,38 @@ constexpr int kSCheckSys = 1024;  // systems of one check pass (max_sys is check
 
 __global__ void __launch_bounds__(kSCheck)
 k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
-           const double* __restrict__ P2, SvStatus* status, long long seq) {
+           const double* __restrict__ P2, StatusRec* status, long long seq) {
+  static_assert((TC_POOL_STIM & (TC_POOL_STIM - 1)) == 0 && TC_POOL_STIM <= 64, "stimulus lanes");
   __shared__ int stall0, any_stall, nmax, imax;
   __shared__ int s_slot[kSCheckSys], s_act[kSCheckSys], s_ni[kSCheckSys];
+  __shared__ uint32_t s_bits[kSCheckSys];
+  __shared__ long long s_tot[kSCheckSys], s_stp[kSCheckSys];
+  __shared__ double s_t[kSCheckSys];
   if (threadIdx.x == 0) {
     stall0 = ctl->stall;
     any_stall = 0;
     nmax = 0;
     imax = 0;
   }
+  __syncthreads();
   const int lane = threadIdx.x & 63, wv = wave_id();
   // phase 1, one wavefront per system: the stopping test of the last
-  // iteration.  Every load of a system (its descriptor, then its flag, norm
-  // and block partials, each lane's at once) is issued before the first wait.
+  // iteration and the system's clock and next-step stimulus bits (lane k
+  // tests stimulus k).  Once the system's descriptor is in, every other load
+  // of it (flags, norms, clock, stimulus table and block partials; each
+  // lane's at once, at clamped in-range addresses) is issued before the
+  // first wait.
   for (int q = wv; q < nsys; q += kSCheck / 64) {
     const SvSys y = pl.sysd[q];
-    const int act = ctl->active[y.slot];
-    const double bn = ctl->bnorm[y.slot];
+    const int s = y.slot;
+    const int act = ctl->active[s];
+    const double bn = ctl->bnorm[s];
+    const int ni_old = ctl->n_iter[s];
+    const long long tot = ctl->total[s];
+    const double t_old = ctl->t[s];
+    const long long stp = ctl->steps[s];
+    const int nst = ctl->nstim[s];
+    const tc_stimulus sk = ctl->stim[(int64_t)s * TC_POOL_STIM + (lane & (TC_POOL_STIM - 1))];
     constexpr int R = kSBlkMax / 64;
     double x0[R];
 #pragma unroll
@@ -529,12 +539,23 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
       const double t = P2[(int64_t)(v <= y.vb1 ? v : y.vb1) * 2];
       x0[u] = v <= y.vb1 ? t : 0.0;
     }
+    // the clock after this step (t += dt, as the reference) and the
+    // stimuli on in the next one (the test of stimulus_bits, per lane)
+    const double t = t_old + dt;
+    const double tn = t + dt;
+    bool on = false;
+    if (lane < nst) {
+      const double p_t = fmod(tn, sk.period);
+      const double t_t = tn / sk.period;
+      on = t_t <= sk.count && p_t >= sk.start && p_t <= sk.start + sk.duration;
+    }
+    const uint32_t bits = (uint32_t)__ballot(on);
     double a = 0.0;
 #pragma unroll
     for (int u = 0; u < R; ++u) a += x0[u];
     a = wave_sum(a);
     if (lane == 0) {
-      int nact = act, ni = -1;
+      int nact = act, ni = ni_old;
       if (act) {
         const double zn = sqrt(a);
         if (zn < ctl->a_tol || zn / bn < ctl->r_tol) {
@@ -546,46 +567,38 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
           else nact = 0;  // max_iter reached
         }
       }
-      s_slot[q] = y.slot;
+      s_slot[q] = s;
       s_act[q] = nact | (act << 1);
       s_ni[q] = ni;
+      s_bits[q] = bits;
+      s_tot[q] = tot + ni;
+      s_t[q] = t;
+      s_stp[q] = stp + 1;
+      atomicMax(&nmax, ni);
+      atomicMax(&imax, nact ? k_end : (ni + 1 < k_end ? ni + 1 : k_end));
     }
   }
   __syncthreads();
   if (stall0) return;
-  // phase 2, one thread per system: flags, iteration counts and (when no
-  // system stalls) the step's clock and next-step stimulus bits
-  for (int q = threadIdx.x; q < nsys; q += blockDim.x) {
-    const int s = s_slot[q];
-    const int was = s_act[q] >> 1, now = s_act[q] & 1;
-    const int ni = was ? s_ni[q] : ctl->n_iter[s];
-    const long long tot = ctl->total[s];
-    const double t = ctl->t[s] + dt;
-    const long long stp = ctl->steps[s];
-    const int nst = ctl->nstim[s];
-    if (was) {
-      ctl->active[s] = now;
-      ctl->n_iter[s] = ni;
-    }
-    atomicMax(&nmax, ni);
-    atomicMax(&imax, now ? k_end : (ni + 1 < k_end ? ni + 1 : k_end));
-    if (!any_stall) {
-      ctl->total[s] = tot + ni;
-      ctl->t[s] = t;
-      ctl->steps[s] = stp + 1;
-      ctl->bits[s] = stimulus_bits(ctl->stim + (int64_t)s * TC_POOL_STIM, nst, t + dt);
-    }
-  }
-  __syncthreads();
+  // phase 2: the step status (the next kernels and host copies are ordered
+  // after this kernel on the stream) and, one thread per system, the flags,
+  // iteration counts and, when no system stalls, the clock and bits
   if (threadIdx.x == 0) {
     if (any_stall) ctl->stall = 1;
-    if (status) {
-      volatile SvStatus* st = status;
-      st->n_iter = nmax;
-      st->iter = imax;
-      st->stalled = any_stall;
-      __threadfence_system();
-      st->seq = seq;
+    if (status)
+      put_status(status, (uint32_t)nmax, (uint32_t)imax | (any_stall ? kStStalled : 0u), seq);
+  }
+  for (int q = threadIdx.x; q < nsys; q += blockDim.x) {
+    const int s = s_slot[q];
+    if (s_act[q] >> 1) {
+      ctl->active[s] = s_act[q] & 1;
+      ctl->n_iter[s] = s_ni[q];
+    }
+    if (!any_stall) {
+      ctl->total[s] = s_tot[q];
+      ctl->t[s] = s_t[q];
+      ctl->steps[s] = s_stp[q];
+      ctl->bits[s] = s_bits[q];
     }
   }
 }
@@ -754,8 +767,9 @@ struct tc_serve {
   std::vector<tc::SysHost> sys;
   tc::RangeAlloc slices, entries;
   int xc, pc;
-  tc::SvStatus* host_status;
-  tc::SvStatus* dev_view;
+  tc::StatusRec* host_status;  // pinned, written by the check pass
+  tc::StatusRec* dev_view;     // device address of host_status
+  tc::StepStatus status[tc::kSRing];  // records as read by sv_wait
   long long seq;
   std::deque<int> recent;
   int64_t steps;
@@ -958,9 +972,8 @@ int sv_check(tc_serve* S, int k_end, int slot, long long seq, hipStream_t s) {
 }
 
 int sv_wait(tc_serve* S, int slot, long long seq) {
-  volatile SvStatus* st = S->host_status + slot;
   const auto t0 = std::chrono::steady_clock::now();
-  for (long spins = 0; st->seq != seq; ++spins) {
+  for (long spins = 0; !status_take(S->host_status + slot, seq, S->status + slot); ++spins) {
     if ((spins & 1023) == 1023) {
       if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(120)) {
         set_error("serve runtime: no status for record %lld after 120 s", seq);
@@ -1021,7 +1034,7 @@ int sv_enqueue(tc_serve* S, SPlan& plan, hipStream_t s) {
 
 template <class T, class XT>
 int sv_resolve(tc_serve* S, const SPlan& plan, XT* xs, hipStream_t s) {
-  SvStatus* st = S->host_status + plan.slot;
+  StepStatus* st = S->status + plan.slot;
   int it = plan.K;
   int chunk = it > 0 ? it : 1;
   while (st->stalled) {
@@ -1071,7 +1084,7 @@ int sv_advance(tc_serve* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     Inflight f = inflight.front();
     int rc = sv_wait(S, f.plan.slot, f.plan.seq);
     if (rc) return rc;
-    SvStatus* st = S->host_status + f.plan.slot;
+    StepStatus* st = S->status + f.plan.slot;
     if (st->stalled) {
       ++S->stalls;
       // finish this step, drop the later (gated) ones and resume after it
@@ -1115,7 +1128,8 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
   TC_CHECK(!d->w_f64 || d->x_f64 || d->dtype == TC_F64, TC_ERR_ARG, "w_f64 needs x_f64");
   TC_CHECK(d->cap_slices * 64 < (int64_t)INT32_MAX && d->cap_entries < (int64_t)INT32_MAX,
            TC_ERR_SIZE, "pool too large for int32 indices");
-  TC_CHECK(d->max_iter >= 1, TC_ERR_ARG, "max_iter must be >= 1");
+  TC_CHECK(d->max_iter >= 1 && d->max_iter <= kMaxIterLimit, TC_ERR_ARG,
+           "max_iter %d outside 1..%d", d->max_iter, kMaxIterLimit);
   TC_CHECK(d->slice_off && d->slice_w && d->slice_sys && d->col && d->Mv && d->Kv && d->Av &&
                d->Dinv && d->x[0] && d->x[1] && d->xp[0] && d->xp[1] && d->w && d->r &&
                d->p[0] && d->p[1] && d->Ap && d->stim_mask,
@@ -1180,11 +1194,11 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
   S->seq = 0;
   S->steps = 0;
   hipStream_t s = (hipStream_t)stream;
-  hipError_t e = hipHostMalloc((void**)&S->host_status, sizeof(SvStatus) * kSRing,
+  hipError_t e = hipHostMalloc((void**)&S->host_status, sizeof(StatusRec) * kSRing,
                                hipHostMallocMapped);
   if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&S->dev_view, S->host_status, 0);
   if (e == hipSuccess) {
-    for (int k = 0; k < kSRing; ++k) S->host_status[k].seq = -1;
+    status_clear(S->host_status, kSRing);
     e = hipMemcpyAsync(S->ctl, &S->hctl, sizeof(SvCtl), hipMemcpyHostToDevice, s);
   }
   if (e == hipSuccess) e = hipMemsetAsync(b + L.arrays, 0, L.part[0] - L.arrays, s);

@@ -12,8 +12,8 @@
 //      slot of a small device ring that a copy stream drains into pinned
 //      host memory while the next steps run (tc_sim_advance_ring).
 // The check kernel writes the step status straight into pinned host memory
-// (with a sequence number written last), so no per-step event or copy is
-// enqueued; the host polls that record.
+// (two 8-byte words, each tagged with the record's sequence number), so no
+// per-step event, copy or fence is needed; the host polls that record.
 // The host keeps up to `depth` steps in flight and validates the oldest.
 // While ctl->stall is set every later kernel returns at once, so a step that
 // needed more than K iterations leaves the device exactly where the plain
@@ -66,8 +66,9 @@ struct tc_sim {
     tc_ttp_params ttp;
     tc_crn_params crn;
   } params;
-  tc::StepStatus* host_status;  // pinned, written by the device
-  tc::StepStatus* dev_view;     // device address of host_status
+  tc::StatusRec* host_status;   // pinned, written by the device
+  tc::StatusRec* dev_view;      // device address of host_status
+  tc::StepStatus status[tc::kRing];  // records as read by wait_status
   long long seq;
   tc::HostState vs;  // validated
   std::deque<int> recent;
@@ -161,9 +162,8 @@ int check_launch(tc_sim* S, int k, int slot, long long seq, hipStream_t s,
 
 // Wait until the check pass has published record `seq` in slot `slot`.
 int wait_status(tc_sim* S, int slot, long long seq) {
-  volatile StepStatus* st = S->host_status + slot;
   const auto t0 = std::chrono::steady_clock::now();
-  for (long spins = 0; st->seq != seq; ++spins) {
+  for (long spins = 0; !status_take(S->host_status + slot, seq, S->status + slot); ++spins) {
     if ((spins & 1023) == 1023) {
       if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(120)) {
         set_error("time-step runtime: no status for record %lld after 120 s", seq);
@@ -327,7 +327,7 @@ template <class T, class XT>
 int resolve_stall(tc_sim* S, const Plan& plan, hipStream_t s) {
   const tc_sim_desc& d = S->d;
   tc_pcg* P = d.pcg;
-  StepStatus* st = S->host_status + plan.slot;
+  StepStatus* st = S->status + plan.slot;
   XT* xs = (XT*)d.x[plan.after.xc];
   int it = st->iter;
   int chunk = it > 0 ? it : 1;
@@ -395,7 +395,7 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     Plan plan = inflight.front();
     int rc = wait_status(S, plan.slot, plan.seq);
     if (rc) return rc;
-    StepStatus* st = S->host_status + plan.slot;
+    StepStatus* st = S->status + plan.slot;
     if (plan.timed) accumulate_timing(S, plan, st->iter);
     if (st->stalled) {
       int rc = resolve_stall<T, XT>(S, plan, s);
@@ -452,7 +452,8 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
            "tc_sim_create: need two x buffers (and two x_prev buffers with the linear guess)");
   TC_CHECK(d->n_stim >= 0 && d->n_stim <= TC_MAX_STIMULI, TC_ERR_ARG, "bad stimulus count");
   TC_CHECK(d->n_stim == 0 || d->stim_mask, TC_ERR_ARG, "stimuli need a mask");
-  TC_CHECK(d->max_iter >= 0, TC_ERR_ARG, "max_iter < 0");
+  TC_CHECK(d->max_iter >= 0 && d->max_iter <= kMaxIterLimit, TC_ERR_ARG,
+           "max_iter %d outside 0..%d", d->max_iter, kMaxIterLimit);
   TC_CHECK(!d->w_f64 || d->x_f64 || d->dtype == TC_F64, TC_ERR_ARG, "w_f64 needs x_f64");
   TC_CHECK(!d->AT || d->RT, TC_ERR_ARG, "AT without RT");
   TC_CHECK(d->pcg->n == d->n && d->pcg->dtype == d->dtype, TC_ERR_ARG,
@@ -469,7 +470,7 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
       return TC_ERR_ARG;
   }
   S->d.params = nullptr;
-  hipError_t e = hipHostMalloc((void**)&S->host_status, sizeof(StepStatus) * kRing,
+  hipError_t e = hipHostMalloc((void**)&S->host_status, sizeof(StatusRec) * kRing,
                                hipHostMallocMapped);
   if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&S->dev_view, S->host_status, 0);
   if (e != hipSuccess) {
@@ -477,7 +478,7 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
     tc_sim_destroy(S);
     return TC_ERR_HIP;
   }
-  for (int k = 0; k < kRing; ++k) S->host_status[k].seq = -1;
+  status_clear(S->host_status, kRing);
   S->seq = 0;
   // The stall flag lives in the shared PCG control block; start clear.
   e = hipMemset(&d->pcg->ctl->stall, 0, sizeof(int));
