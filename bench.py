@@ -17,6 +17,7 @@ kernels run on, the PCG algorithmic HBM rate, the roofline of the dominant
 kernel, and the CPU oracle timed on a bounded sample of the same workload.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -412,12 +413,22 @@ def main():
         ele_s = sum(kernels[k]["total_ms"] for k in ("rhs_init", "pcg_spmv", "pcg_update",
                                                      "pcg_check") if k in kernels) / 1e3
         pcg_rate = round(ele_bytes / ele_s / 1e9, 1)
+    # release the single-run workload's device memory before the next parts,
+    # so that each part runs on a device holding only its own data
+    n_elems = int(sim.elems.shape[0])
+    sim._free_sim()
+    del sim
+    torch.cuda.empty_cache()
     batched = None
     if not args.no_batched:
         batched = batched_config(dev, args.batch_steps, args.batch_warmup)
+        gc.collect()
+        torch.cuda.empty_cache()
     served = None
     if not args.no_served:
         served = served_config(dev, args.served_requests)
+        gc.collect()
+        torch.cuda.empty_cache()
     irregular = None
     if args.shell:
         irregular = irregular_config(dev, args.shell_steps, 700)
@@ -429,7 +440,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"cube{N}_ttp_endo_f32: {n} nodes, {sim.elems.shape[0]} P1 tets, "
+        "config": {"workload": f"cube{N}_ttp_endo_f32: {n} nodes, {n_elems} P1 tets, "
                                f"nnz {nnz}, dt 0.01 ms, Jacobi-PCG tol 1e-5",
                    "n_nodes": n, "nnz": nnz, "t_window_ms": [args.warmup * 0.01,
                                                             (args.warmup + args.steps) * 0.01],

@@ -272,6 +272,12 @@ typedef struct tc_ionic_io {
   const uint32_t* pool_sys_bits;
   const struct tc_stimulus* pool_sys_stim;
   int64_t state_stride;
+  /* Load and store the states non-temporally: they then pass through the
+   * caches without displacing what is reused across steps.  Worth it when
+   * the states share the last-level cache with large operators (the
+   * time-step runtime, the served pool); a batch on a small shared mesh is
+   * faster without. */
+  int32_t stream_states;
 } tc_ionic_io;
 
 /* ModifiedMS2v (ionic/ModifiedMS2v.py:4-44).  States: H. */

@@ -45,7 +45,8 @@ class IonicIO(C.Structure):
                 ("gate", p), ("v_f64", i32), ("w_f64", i32), ("lanes", i32),
                 ("lane_stride", i32), ("lane0", i32), ("pool_slices", p), ("pool_slice_sys", p),
                 ("pool_sys_row0", p), ("pool_sys_n", p), ("pool_sys_t", p), ("pool_sys_steps", p),
-                ("pool_sys_bits", p), ("pool_sys_stim", p), ("state_stride", i64)]
+                ("pool_sys_bits", p), ("pool_sys_stim", p), ("state_stride", i64),
+                ("stream_states", i32)]
 
 
 MS2V_FIELDS = ["tau_in", "tau_out", "tau_open", "tau_close", "u_gate", "u_crit", "vmin", "vmax"]

@@ -226,6 +226,25 @@ __device__ inline T& state(T* st, const NodeCtx& c, int k) {
   return at32(st + (int64_t)k * c.ss, (uint32_t)c.si);
 }
 
+// State k of this node, with non-temporal loads and stores when NT (the
+// io's stream_states): the states stream through once per step (76 MB each
+// way for TTP at 1 M nodes) and so no longer displace the operator values
+// and columns, which the RHS and PCG passes then find in the last-level
+// cache step after step.
+template <bool NT, class T>
+__device__ inline T ld_state(const T* st, const NodeCtx& c, int k) {
+  const T* q = &at32(st + (int64_t)k * c.ss, (uint32_t)c.si);
+  if constexpr (NT) return __builtin_nontemporal_load(q);
+  return *q;
+}
+
+template <bool NT, class T>
+__device__ inline void st_state(T* st, const NodeCtx& c, int k, T v) {
+  T* q = &at32(st + (int64_t)k * c.ss, (uint32_t)c.si);
+  if constexpr (NT) __builtin_nontemporal_store(v, q);
+  else *q = v;
+}
+
 // The membrane potential: float64 (VF64, the runtime's float64 iterate) or T.
 template <class T, bool VF64>
 __device__ inline T load_v(const tc_ionic_io& io, int64_t vi) {
@@ -287,7 +306,7 @@ __device__ inline void finish_node(const tc_ionic_io& io, const NodeCtx& c, T V,
 
 // ---------------------------------------------------------------- MS2v
 
-template <class T, bool VF64>
+template <class T, bool VF64, bool NT>
 __device__ __forceinline__ void ms2v_node(const tc_ms2v_params& p, T* __restrict__ H,
                                           const tc_ionic_io& io, int64_t i) {
   if (i >= io.n) return;
@@ -295,7 +314,7 @@ __device__ __forceinline__ void ms2v_node(const tc_ms2v_params& p, T* __restrict
   const NodeCtx c = node_ctx(io, i);
   if (!c.ok) return;
   const T U = load_v<T, VF64>(io, c.vi);
-  const T h = state(H, c, 0);
+  const T h = ld_state<NT>(H, c, 0);
   const T at = load_at<T>(io, c);
   const T DV = (T)(p.vmax - p.vmin);
   const T Uad = (U - (T)p.vmin) / DV;
@@ -304,14 +323,14 @@ __device__ __forceinline__ void ms2v_node(const tc_ms2v_params& p, T* __restrict
   const T dU = -(DV * (J_in + J_out));
   const T dH = Uad > (T)p.u_gate ? -h / (T)p.tau_close : (T(1) - h) / (T)p.tau_open;
   if (gated) return;
-  state(H, c, 0) = h + (T)io.dt * dH;
+  st_state<NT>(H, c, 0, T(h + (T)io.dt * dH));
   finish_node<T>(io, c, U, dU, at);
 }
 
-template <class T, bool VF64>
+template <class T, bool VF64, bool NT>
 __global__ void __launch_bounds__(kBlock)
 k_ms2v(tc_ms2v_params p, T* __restrict__ H, tc_ionic_io io) {
-  ms2v_node<T, VF64>(p, H, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
+  ms2v_node<T, VF64, NT>(p, H, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- TTP
@@ -519,7 +538,7 @@ TtpK<T> ttp_constants(const tc_ttp_params& p) {
 }
 
 // One TTP step per node (TenTusscherPanfilov.py:353-454).
-template <class T, bool VF64>
+template <class T, bool VF64, bool NT>
 __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
                                          const T* __restrict__ Vtab, const T* __restrict__ Ctab,
                                          const T* __restrict__ Ktab, const TtpTabs& tb,
@@ -532,7 +551,7 @@ __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
   const T V = load_v<T, VF64>(io, c.vi);
   T s[TS_N];
 #pragma unroll
-  for (int k = 0; k < TS_N; ++k) s[k] = state(st, c, k);
+  for (int k = 0; k < TS_N; ++k) s[k] = ld_state<NT>(st, c, k);
   const T at = load_at<T>(io, c);
   const T RTONF = K.RTONF, F_RT = K.F_RT;
 
@@ -601,13 +620,13 @@ __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
                       (T(1) + (K.BufssK / kss) / kss);
   const T dt = (T)io.dt;
   if (gated) return;
-  state(st, c, TS_CaSR) = CaSR + diff_CaSR * dt;
-  state(st, c, TS_CaSS) = CaSS + diff_CaSS * dt;
-  state(st, c, TS_Cai) = Cai + diff_Cai * dt;
-  state(st, c, TS_Ki) = Ki + diff_Ki * dt;
-  state(st, c, TS_Nai) = Nai + diff_Nai * dt;
-  state(st, c, TS_R_) = R_ + diff_R_ * dt;
-  state(st, c, TS_FCaSS) = cr(1, 3) + cr(2, 3) * s[TS_FCaSS];
+  st_state<NT>(st, c, TS_CaSR, T(CaSR + diff_CaSR * dt));
+  st_state<NT>(st, c, TS_CaSS, T(CaSS + diff_CaSS * dt));
+  st_state<NT>(st, c, TS_Cai, T(Cai + diff_Cai * dt));
+  st_state<NT>(st, c, TS_Ki, T(Ki + diff_Ki * dt));
+  st_state<NT>(st, c, TS_Nai, T(Nai + diff_Nai * dt));
+  st_state<NT>(st, c, TS_R_, T(R_ + diff_R_ * dt));
+  st_state<NT>(st, c, TS_FCaSS, T(cr(1, 3) + cr(2, 3) * s[TS_FCaSS]));
   finish_node<T>(io, c, V, -Iion, at);
   // Phase 2 (no memory access moves across this point): the remaining
   // V-table columns and the Rush-Larsen gate updates from the old gates.
@@ -629,14 +648,14 @@ __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
   o[TS_Xs] = vr(TTP_Xs_A, TTP_VCOLS) + vr(TTP_Xs_B, TTP_VCOLS) * s[TS_Xs];
   const int gates[11] = {TS_D, TS_F, TS_F2, TS_H, TS_J, TS_M, TS_R, TS_S, TS_Xr1, TS_Xr2, TS_Xs};
 #pragma unroll
-  for (int k = 0; k < 11; ++k) state(st, c, gates[k]) = o[gates[k]];
+  for (int k = 0; k < 11; ++k) st_state<NT>(st, c, gates[k], T(o[gates[k]]));
 }
 
-template <class T, bool VF64>
+template <class T, bool VF64, bool NT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_ttp(TtpK<T> K, T* __restrict__ st, const T* __restrict__ Vtab,
       const T* __restrict__ Ctab, const T* __restrict__ Ktab, TtpTabs tb, tc_ionic_io io) {
-  ttp_node<T, VF64>(K, st, Vtab, Ctab, Ktab, tb, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
+  ttp_node<T, VF64, NT>(K, st, Vtab, Ctab, Ktab, tb, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- CRN
@@ -819,7 +838,7 @@ CrnK<T> crn_constants(const tc_crn_params& p, double dt) {
 // One CRN step per node (CourtemancheRamirezNattel.py:358-444), in two
 // phases like the TTP step: currents and concentrations first (with the
 // V-table groups they read), then the Rush-Larsen gates.
-template <class T, bool VF64>
+template <class T, bool VF64, bool NT>
 __device__ __forceinline__ void crn_node(const CrnK<T>& K, T* __restrict__ st,
                                          const T* __restrict__ Ctab, const T* __restrict__ Vtab,
                                          const T* __restrict__ Ftab, const CrnTabs& tb,
@@ -832,7 +851,7 @@ __device__ __forceinline__ void crn_node(const CrnK<T>& K, T* __restrict__ st,
   const T V = load_v<T, VF64>(io, c.vi);
   T s[CS_N];
 #pragma unroll
-  for (int k = 0; k < CS_N; ++k) s[k] = state(st, c, k);
+  for (int k = 0; k < CS_N; ++k) s[k] = ld_state<NT>(st, c, k);
   const T at = load_at<T>(io, c);
   RowG<T, CV_NCOLS> vr;
   vr.at(Vtab, V, tb.V);
@@ -885,13 +904,13 @@ __device__ __forceinline__ void crn_node(const CrnK<T>& K, T* __restrict__ st,
   const T fn = K.C_Fn1 * Irel - K.C_Fn2 * (ICaL - T(0.4) * INaCa);
   const Row<T> fr = row_at<T>(Ftab, CF_NCOLS, fn, tb.fn);
   if (gated) return;
-  state(st, c, CS_Ca_rel) = Ca_rel + diff_Ca_rel * dt;
-  state(st, c, CS_Ca_up) = Ca_up + diff_Ca_up * dt;
-  state(st, c, CS_Cai) = s[CS_Cai] + diff_Cai * dt;
-  state(st, c, CS_Ki) = s[CS_Ki] + diff_Ki * dt;
-  state(st, c, CS_f_Ca) = car(CA_f_Ca_A, CA_NCOLS) + K.e_fCa * s[CS_f_Ca];
-  state(st, c, CS_u) = fr(CF_u_A, CF_NCOLS) + K.e_u * u;
-  state(st, c, CS_v) = fr(CF_v_A, CF_NCOLS) + fr(CF_v_B, CF_NCOLS) * s[CS_v];
+  st_state<NT>(st, c, CS_Ca_rel, T(Ca_rel + diff_Ca_rel * dt));
+  st_state<NT>(st, c, CS_Ca_up, T(Ca_up + diff_Ca_up * dt));
+  st_state<NT>(st, c, CS_Cai, T(s[CS_Cai] + diff_Cai * dt));
+  st_state<NT>(st, c, CS_Ki, T(s[CS_Ki] + diff_Ki * dt));
+  st_state<NT>(st, c, CS_f_Ca, T(car(CA_f_Ca_A, CA_NCOLS) + K.e_fCa * s[CS_f_Ca]));
+  st_state<NT>(st, c, CS_u, T(fr(CF_u_A, CF_NCOLS) + K.e_u * u));
+  st_state<NT>(st, c, CS_v, T(fr(CF_v_A, CF_NCOLS) + fr(CF_v_B, CF_NCOLS) * s[CS_v]));
   finish_node<T>(io, c, V, -Iion, at);
   // Phase 2 (no memory access moves across this point): the gate columns
   // of the V table and the Rush-Larsen updates from the old gates.
@@ -899,16 +918,16 @@ __device__ __forceinline__ void crn_node(const CrnK<T>& K, T* __restrict__ st,
 #pragma unroll
   for (int g = 0; g < CV_NCOLS / E; ++g) vr.load_group(g);
 #define RLU(G) \
-  state(st, c, CS_##G) = vr(CV_##G##_A, CV_NCOLS) + vr(CV_##G##_B, CV_NCOLS) * s[CS_##G];
+  st_state<NT>(st, c, CS_##G, T(vr(CV_##G##_A, CV_NCOLS) + vr(CV_##G##_B, CV_NCOLS) * s[CS_##G]));
   RLU(d) RLU(f) RLU(h) RLU(j) RLU(m) RLU(oa) RLU(oi) RLU(ua) RLU(ui) RLU(w) RLU(xr) RLU(xs)
 #undef RLU
 }
 
-template <class T, bool VF64>
+template <class T, bool VF64, bool NT>
 __global__ void __launch_bounds__(kBlock)
 k_crn(CrnK<T> K, T* __restrict__ st, const T* __restrict__ Ctab,
       const T* __restrict__ Vtab, const T* __restrict__ Ftab, CrnTabs tb, tc_ionic_io io) {
-  crn_node<T, VF64>(K, st, Ctab, Vtab, Ftab, tb, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
+  crn_node<T, VF64, NT>(K, st, Ctab, Vtab, Ftab, tb, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
 }
 
 // The three models of a pool in one launch: blocks [0, b1) step the MS2v
@@ -931,7 +950,7 @@ struct PoolIonic {
 };
 
 // (separate by-value arguments: the kernel argument block is read in place)
-template <class T, bool VF64>
+template <class T, bool VF64, bool NT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_pool_ionic(tc_ms2v_params mp, TtpK<T> tk, CrnK<T> ck, T* __restrict__ st0,
              T* __restrict__ st1, T* __restrict__ st2, const T* __restrict__ tt0,
@@ -940,11 +959,11 @@ k_pool_ionic(tc_ms2v_params mp, TtpK<T> tk, CrnK<T> ck, T* __restrict__ st0,
              tc_ionic_io io0, tc_ionic_io io1, tc_ionic_io io2, int b1, int b2) {
   const int b = blockIdx.x;
   if (b < b1) {
-    ms2v_node<T, VF64>(mp, st0, io0, (int64_t)b * kBlock + threadIdx.x);
+    ms2v_node<T, VF64, NT>(mp, st0, io0, (int64_t)b * kBlock + threadIdx.x);
   } else if (b < b2) {
-    ttp_node<T, VF64>(tk, st1, tt0, tt1, tt2, ttb, io1, (int64_t)(b - b1) * kBlock + threadIdx.x);
+    ttp_node<T, VF64, NT>(tk, st1, tt0, tt1, tt2, ttb, io1, (int64_t)(b - b1) * kBlock + threadIdx.x);
   } else {
-    crn_node<T, VF64>(ck, st2, ct0, ct1, ct2, ctb, io2, (int64_t)(b - b2) * kBlock + threadIdx.x);
+    crn_node<T, VF64, NT>(ck, st2, ct0, ct1, ct2, ctb, io2, (int64_t)(b - b2) * kBlock + threadIdx.x);
   }
 }
 
@@ -995,12 +1014,16 @@ int tc::ms2v_step(const tc_ms2v_params* p, void* states, const tc_ionic_io* io, 
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
   // float32 kernels read V as float64 when io->v_f64 is set (the runtime's
   // float64 iterate); float64 kernels read double either way
+  const bool nt = io->stream_states != 0;
   if (dtype == TC_F32 && io->v_f64)
-    launch(k_ms2v<float, true>, g, b, s, e0, e1, *p, (float*)states, *io);
+    launch(nt ? k_ms2v<float, true, true> : k_ms2v<float, true, false>, g, b, s, e0, e1, *p,
+           (float*)states, *io);
   else if (dtype == TC_F32)
-    launch(k_ms2v<float, false>, g, b, s, e0, e1, *p, (float*)states, *io);
+    launch(nt ? k_ms2v<float, false, true> : k_ms2v<float, false, false>, g, b, s, e0, e1, *p,
+           (float*)states, *io);
   else
-    launch(k_ms2v<double, false>, g, b, s, e0, e1, *p, (double*)states, *io);
+    launch(nt ? k_ms2v<double, false, true> : k_ms2v<double, false, false>, g, b, s, e0, e1, *p,
+           (double*)states, *io);
   TC_LAUNCH();
   return TC_OK;
 }
@@ -1038,17 +1061,18 @@ int tc::ttp_step(const tc_ttp_params* p, void* states, const void* V_tab, const 
   TtpTabs tb = ttp_tabs();
   if ((rc = check_tab(tb.V)) || (rc = check_tab(tb.CaSS)) || (rc = check_tab(tb.VEk))) return rc;
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
+  const bool nt = io->stream_states != 0;
   if (dtype == TC_F32 && io->v_f64)
-    launch(k_ttp<float, true>, g, b, s, e0, e1, ttp_constants<float>(*p), (float*)states,
-           (const float*)V_tab,
+    launch(nt ? k_ttp<float, true, true> : k_ttp<float, true, false>, g, b, s, e0, e1,
+           ttp_constants<float>(*p), (float*)states, (const float*)V_tab,
            (const float*)CaSS_tab, (const float*)VEk_tab, tb, *io);
   else if (dtype == TC_F32)
-    launch(k_ttp<float, false>, g, b, s, e0, e1, ttp_constants<float>(*p), (float*)states,
-           (const float*)V_tab,
+    launch(nt ? k_ttp<float, false, true> : k_ttp<float, false, false>, g, b, s, e0, e1,
+           ttp_constants<float>(*p), (float*)states, (const float*)V_tab,
            (const float*)CaSS_tab, (const float*)VEk_tab, tb, *io);
   else
-    launch(k_ttp<double, false>, g, b, s, e0, e1, ttp_constants<double>(*p), (double*)states,
-           (const double*)V_tab,
+    launch(nt ? k_ttp<double, false, true> : k_ttp<double, false, false>, g, b, s, e0, e1,
+           ttp_constants<double>(*p), (double*)states, (const double*)V_tab,
            (const double*)CaSS_tab, (const double*)VEk_tab, tb, *io);
   TC_LAUNCH();
   return TC_OK;
@@ -1089,17 +1113,18 @@ int tc::crn_step(const tc_crn_params* p, void* states, const void* Cai_tab, cons
   CrnTabs tb = crn_tabs();
   if ((rc = check_tab(tb.Cai)) || (rc = check_tab(tb.V)) || (rc = check_tab(tb.fn))) return rc;
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
+  const bool nt = io->stream_states != 0;
   if (dtype == TC_F32 && io->v_f64)
-    launch(k_crn<float, true>, g, b, s, e0, e1, crn_constants<float>(*p, io->dt), (float*)states,
-           (const float*)Cai_tab,
+    launch(nt ? k_crn<float, true, true> : k_crn<float, true, false>, g, b, s, e0, e1,
+           crn_constants<float>(*p, io->dt), (float*)states, (const float*)Cai_tab,
            (const float*)V_tab, (const float*)fn_tab, tb, *io);
   else if (dtype == TC_F32)
-    launch(k_crn<float, false>, g, b, s, e0, e1, crn_constants<float>(*p, io->dt), (float*)states,
-           (const float*)Cai_tab,
+    launch(nt ? k_crn<float, false, true> : k_crn<float, false, false>, g, b, s, e0, e1,
+           crn_constants<float>(*p, io->dt), (float*)states, (const float*)Cai_tab,
            (const float*)V_tab, (const float*)fn_tab, tb, *io);
   else
-    launch(k_crn<double, false>, g, b, s, e0, e1, crn_constants<double>(*p, io->dt), (double*)states,
-           (const double*)Cai_tab,
+    launch(nt ? k_crn<double, false, true> : k_crn<double, false, false>, g, b, s, e0, e1,
+           crn_constants<double>(*p, io->dt), (double*)states, (const double*)Cai_tab,
            (const double*)V_tab, (const double*)fn_tab, tb, *io);
   TC_LAUNCH();
   return TC_OK;
@@ -1113,8 +1138,9 @@ extern "C" int tc_crn_step(const tc_crn_params* p, void* states, const void* Cai
 }
 
 template <class T>
-int pool_launch(const PoolIonic<T>& a, bool vf64, int blocks, hipStream_t s) {
-  auto k = vf64 ? k_pool_ionic<T, true> : k_pool_ionic<T, false>;
+int pool_launch(const PoolIonic<T>& a, bool vf64, bool nt, int blocks, hipStream_t s) {
+  auto k = vf64 ? (nt ? k_pool_ionic<T, true, true> : k_pool_ionic<T, true, false>)
+                : (nt ? k_pool_ionic<T, false, true> : k_pool_ionic<T, false, false>);
   hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), 0, s, a.mp, a.tk, a.ck, a.st[0], a.st[1],
                      a.st[2], a.ttab[0], a.ttab[1], a.ttab[2], a.ctab[0], a.ctab[1], a.ctab[2],
                      a.ttb, a.ctb, a.io[0], a.io[1], a.io[2], a.b1, a.b2);
@@ -1136,11 +1162,14 @@ int tc::pool_ionic_step(const tc_ms2v_params* mp, const tc_ttp_params* tp,
                  (m != 2 || (cp && crn_tab[0] && crn_tab[1] && crn_tab[2])),
              TC_ERR_ARG, "pool ionic step: model %d lacks parameters, states or tables", m);
   }
-  int vf = -1;
+  int vf = -1, ns = -1;
   for (int m = 0; m < 3; ++m) {
     if (nb[m] == 0) continue;
     TC_CHECK(vf < 0 || vf == io[m].v_f64, TC_ERR_ARG, "pool ionic step: mixed V types");
+    TC_CHECK(ns < 0 || ns == (io[m].stream_states != 0), TC_ERR_ARG,
+             "pool ionic step: mixed stream_states");
     vf = io[m].v_f64;
+    ns = io[m].stream_states != 0;
   }
   const int64_t blocks = nb[0] + nb[1] + nb[2];
   if (blocks == 0) return TC_OK;
@@ -1172,11 +1201,11 @@ int tc::pool_ionic_step(const tc_ms2v_params* mp, const tc_ttp_params* tp,
   if (dtype == TC_F32) {
     PoolIonic<float> a{};
     fill(a, 0.0f);
-    return pool_launch(a, vf64, (int)blocks, s);
+    return pool_launch(a, vf64, ns == 1, (int)blocks, s);
   }
   PoolIonic<double> a{};
   fill(a, 0.0);
-  return pool_launch(a, false, (int)blocks, s);
+  return pool_launch(a, false, ns == 1, (int)blocks, s);
 }
 
 extern "C" int tc_lat_update(const void* u, void* AT, void* RT, int64_t n, double t, int dtype,

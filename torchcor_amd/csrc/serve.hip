@@ -912,6 +912,7 @@ tc_ionic_io sv_ionic_io(tc_serve* S, int m) {
   io.AT = d.AT;
   io.RT = d.RT;
   io.gate = &S->ctl->stall;
+  io.stream_states = 1;
   io.pool_slices = S->mlist[m];
   io.pool_slice_sys = d.slice_sys;
   io.pool_sys_row0 = S->hctl.row0;

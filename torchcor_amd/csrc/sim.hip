@@ -261,6 +261,7 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
     io.lat_t = ps.lat_t;
   }
   io.gate = &P->ctl->stall;
+  io.stream_states = 1;
   int rc = ionic_launch(S, &io, s, ev ? (*ev)[0] : nullptr, ev ? (*ev)[1] : nullptr);
   if (rc) return rc;
   // 2. RHS + PCG start
