@@ -48,18 +48,20 @@ def alg_bytes(n, nnz, S, ib=4, xb=4, wb=4):
     }
 
 
-# HBM bytes per dispatch of each kernel on the default workload (101^3 TTP
+# Memory bytes per dispatch of each kernel on the default workload (101^3 TTP
 # cube, float32 storage, float64 iterate and right-hand side, int16 column
-# offsets), from two rocprofv3 --pmc runs of this script (FETCH_SIZE, then
+# offsets): what the L2 caches fetch from and write to the memory fabric
+# (HBM, or the last-level cache in front of it), from two rocprofv3 --pmc
+# runs of this script (FETCH_SIZE, then
 # WRITE_SIZE, median KiB per dispatch) scaled by what the same counters read
 # for a plain 256 MiB device copy (--copy-probe-mib 256): FETCH_SIZE x2.0,
 # WRITE_SIZE x1.0.  profiles/cube101_ttp_step_hbm_traffic.md holds the
 # counter summary and the commands.
 MEASURED_TRAFFIC_101 = {
-    "ionic": 183_906_422,
-    "rhs_init": 204_057_276,
-    "pcg_spmv": 112_142_918,
-    "pcg_update": 37_280_251,
+    "ionic": 192_103_728,
+    "rhs_init": 204_164_883,
+    "pcg_spmv": 112_219_328,
+    "pcg_update": 37_284_347,
 }
 
 
