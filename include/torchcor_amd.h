@@ -140,6 +140,14 @@ int tc_sell_deltas(const tc_sell* A, int16_t* dcol, int32_t* max_abs_dev, void* 
  * ---------------------------------------------------------------------- */
 int tc_rcm_order(int64_t n, const int32_t* row_ptr_host, const int32_t* col_host,
                  int32_t* order_host, int64_t* bw_in_host, int64_t* bw_out_host);
+/* Locality order for the SELL passes (no counterpart in the reference; the
+ * simulator applies it inside, undoing it at the API): the Cuthill-McKee
+ * level structure of tc_rcm_order, components and levels in order, each
+ * level's nodes along a Z-order curve of xyz_host (n x 3 doubles).  Same
+ * arguments and outputs as tc_rcm_order otherwise. */
+int tc_level_order(int64_t n, const int32_t* row_ptr_host, const int32_t* col_host,
+                   const double* xyz_host, int32_t* order_host, int64_t* bw_in_host,
+                   int64_t* bw_out_host);
 
 /* ------------------------------------------------------------------------
  * Mesh and stimulus files (host).

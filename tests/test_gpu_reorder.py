@@ -1,12 +1,12 @@
-"""Internal RCM node order on the irregular shell (run with -m gpu).
+"""Internal node orders on the irregular shell (run with -m gpu).
 
 The shell mesh comes with randomly permuted node numbers.  With
-reorder="rcm" the simulator runs in the RCM order and maps every input and
-output through the permutation; with reorder=None it runs in the mesh's own
-order.  Tolerances:
+reorder="levels" (what "auto" picks here) or "rcm" the simulator runs in
+that order and maps every input and output through the permutation; with
+reorder=None it runs in the mesh's own order.  Tolerances:
   * M, K, A returned in mesh numbering: bit-identical between the two orders
     (same pattern; each entry sums the same contributions in the same order).
-  * float64 runs, RCM vs mesh order (only the summation order inside the
+  * float64 runs, reordered vs mesh order (only the summation order inside the
     products and reductions differs): TTP, whose right-hand side is
     continuous in V, max |dVm| < 1e-8 mV over the first millisecond and
     < 1e-6 mV over the run, the same per-step PCG iteration counts; MS2v, whose gate switches at
@@ -14,7 +14,7 @@ order.  Tolerances:
     at a node crossing it (0.16 mV after 10 ms on a coarser shell), so it is
     held to max |dVm| <= 0.5 mV.  Both:
     |dLAT|, |dRT| <= dt where both runs record one.
-  * float32 with RCM vs the float64 oracle in mesh order: max |dVm| <= 0.5 mV,
+  * float32 in the "auto" order vs the float64 oracle in mesh order: max |dVm| <= 0.5 mV,
     |dLAT| <= dt.
 """
 import numpy as np
@@ -66,16 +66,17 @@ def _sim(mesh, model, dtype, T, reorder):
     return sim
 
 
-def test_shell_operators_identical_in_mesh_numbering():
-    _say("[gpu] shell M/K/A rcm vs mesh order")
+@pytest.mark.parametrize("order", ["levels", "rcm"])
+def test_shell_operators_identical_in_mesh_numbering(order):
+    _say(f"[gpu] shell M/K/A {order} vs mesh order")
     mesh = _shell()
-    a = _sim(mesh, "ms2v", torch.float64, 1.0, "rcm")
+    a = _sim(mesh, "ms2v", torch.float64, 1.0, order)
     b = _sim(mesh, "ms2v", torch.float64, 1.0, None)
     a.assemble()
     b.assemble()
     assert a.node_order is not None and b.node_order is None
     assert a.system.bandwidth < 0.1 * mesh[0].shape[0] < b.system.bandwidth
-    assert a.system.bandwidth == a.bandwidth_rcm
+    assert a.system.bandwidth == a.bandwidth_reordered
     for name in ("M", "K", "A"):
         x, y = getattr(a, name), getattr(b, name)
         assert torch.equal(x.crow_indices(), y.crow_indices())
@@ -89,7 +90,11 @@ def test_auto_reorder_choice():
     mesh = _shell()
     s = _sim(mesh, "ms2v", torch.float64, 1.0, "auto")
     s.assemble()
-    assert s.node_order is not None and s.bandwidth_rcm * 2 < s.bandwidth_natural
+    assert s.node_order is not None and s.bandwidth_reordered * 2 < s.bandwidth_natural
+    from torchcor_amd.core.reorder import element_pattern, level_order
+    rp, col = element_pattern(mesh[1], mesh[0].shape[0])
+    lo, _, _ = level_order(rp, col, mesh[0].shape[0], np.asarray(mesh[0], dtype=np.float64))
+    assert np.array_equal(s.node_order.cpu().numpy(), lo)   # "auto" applies the level order
     nodes, elems, regions, fib = meshes.cube_workload(21)
     c = Monodomain(_gpu_model("ms2v", torch.float64), T=1.0, dt=0.01, device=DEV,
                    dtype=torch.float64)
@@ -100,13 +105,14 @@ def test_auto_reorder_choice():
     assert c.node_order is None            # the cube's own numbering is already banded
 
 
-@pytest.mark.parametrize("model,tol", [("ttp", 1e-6), ("ms2v", 0.5)])
-def test_shell_rcm_matches_mesh_order_float64(model, tol):
-    _say(f"[gpu] shell f64 {model} rcm vs mesh order")
+@pytest.mark.parametrize("model,tol,order", [("ttp", 1e-6, "levels"), ("ms2v", 0.5, "levels"),
+                                             ("ttp", 1e-6, "rcm")])
+def test_shell_reordered_matches_mesh_order_float64(model, tol, order):
+    _say(f"[gpu] shell f64 {model} {order} vs mesh order")
     mesh = _shell()
     T = 10.0
     runs = []
-    for reorder in ("rcm", None):
+    for reorder in (order, None):
         s = _sim(mesh, model, torch.float64, T, reorder)
         s.solve(1e-5, 1e-5, 100, snapshot_interval=1.0, verbose=False)
         runs.append(s)
@@ -126,12 +132,12 @@ def test_shell_rcm_matches_mesh_order_float64(model, tol):
 
 
 def test_shell_step_api_in_mesh_numbering():
-    """step() takes and returns u in mesh numbering under the RCM order."""
-    _say("[gpu] shell step() rcm")
+    """step() takes and returns u in mesh numbering under the internal order."""
+    _say("[gpu] shell step() levels")
     mesh = _shell(5000, seed=5)
-    a = _sim(mesh, "ms2v", torch.float64, 0.5, "rcm")
+    a = _sim(mesh, "ms2v", torch.float64, 0.5, "levels")
     a.solve(1e-5, 1e-5, 100, snapshot_interval=1.0, verbose=False)
-    b = _sim(mesh, "ms2v", torch.float64, 0.5, "rcm")
+    b = _sim(mesh, "ms2v", torch.float64, 0.5, "levels")
     b.assemble()
     assert b.node_order is not None
     u = b.ionic_model.initialize(b.n_nodes)
@@ -144,8 +150,8 @@ def test_shell_step_api_in_mesh_numbering():
     assert (u - a.u).abs().max().item() < 1e-10
 
 
-def test_shell_float32_rcm_vs_float64_oracle():
-    _say("[gpu] shell f32 ttp rcm vs f64 oracle")
+def test_shell_float32_reordered_vs_float64_oracle():
+    _say("[gpu] shell f32 ttp auto order vs f64 oracle")
     mesh = _shell()
     nodes, elems, regions, fib, stims = mesh
     T = 12.0
@@ -156,7 +162,7 @@ def test_shell_float32_rcm_vs_float64_oracle():
     for ids, start in stims:
         om.add_stimulus(ids, start, 1.0, 100)
     ro = om.solve(1e-5, 1e-5, 100, snapshot_interval=1.0)
-    g = _sim(mesh, "ttp", torch.float32, T, "rcm")
+    g = _sim(mesh, "ttp", torch.float32, T, "auto")
     g.solve(1e-5, 1e-5, 100, snapshot_interval=1.0, verbose=False)
     assert g.node_order is not None
     Vg = g.frames.cpu().numpy()

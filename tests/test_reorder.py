@@ -7,6 +7,12 @@ grid with bandwidth at most its short side + 1, disjoint components stay
 contiguous, and on the shell mesh the bandwidth and profile are no worse
 than scipy's reverse_cuthill_mckee (the routine the reference's RCM class
 calls, torchcor/core/reorder.py:32).
+
+The level order (Cuthill-McKee levels, each along a Z-order curve of the
+coordinates) is checked against its definition computed independently:
+breadth-first levels from its first node (scipy), positions sorted by level,
+and inside a level by a Morton key formed here by plain bit loops; its
+bandwidth is then at most two level widths.
 """
 import numpy as np
 import pytest
@@ -16,7 +22,7 @@ from scipy.sparse.csgraph import reverse_cuthill_mckee
 torch = pytest.importorskip("torch")
 
 from torchcor_amd import meshes  # noqa: E402
-from torchcor_amd.core.reorder import RCM, element_pattern, rcm_order  # noqa: E402
+from torchcor_amd.core.reorder import RCM, element_pattern, level_order, rcm_order  # noqa: E402
 
 
 def _csr(n, edges):
@@ -162,3 +168,62 @@ def test_shell_mesh_properties():
         z = nodes[ids, 2]
         assert np.all(z < -20) if st == 0.0 else np.all(z > 20)
     assert len(cond) == 4
+
+
+def _morton_loop(xyz):
+    """30-bit Z-order keys on a 1024^3 grid over the bounding box, one bit at a time."""
+    lo = xyz.min(0)
+    ext = (xyz.max(0) - lo).max()
+    q = np.clip(((xyz - lo) * (1023.0 / ext)).astype(np.int64), 0, 1023)
+    key = np.zeros(len(xyz), dtype=np.int64)
+    for b in range(10):
+        for a in range(3):
+            key |= ((q[:, a] >> b) & 1) << (3 * b + a)
+    return key
+
+
+def test_level_order_definition_on_mesh():
+    from scipy.sparse.csgraph import breadth_first_order
+    nodes, elems = meshes.jittered_delaunay(9, seed=2)
+    n = nodes.shape[0]
+    rp, col = element_pattern(elems, n)
+    order, bw0, bw1 = level_order(rp, col, n, nodes)
+    assert sorted(order.tolist()) == list(range(n))
+    A = sp.csr_matrix((np.ones(col.shape[0]), col, rp), shape=(n, n))
+    bfs, pred = breadth_first_order(A, order[0], directed=False, return_predecessors=True)
+    assert len(bfs) == n                       # one component
+    lev = np.zeros(n, dtype=np.int64)
+    for v in bfs[1:]:
+        lev[v] = lev[pred[v]] + 1
+    key = _morton_loop(np.asarray(nodes, dtype=np.float64))
+    expect = np.lexsort((np.arange(n), key, lev))
+    assert np.array_equal(order, expect)
+    width = np.bincount(lev).max()
+    b, _ = _band_profile(rp, col, order)
+    assert b == bw1 <= 2 * width
+    assert bw0 == _band_profile(rp, col, np.arange(n))[0]
+
+
+def test_level_order_path_and_components():
+    rng = np.random.default_rng(4)
+    n = 120
+    lab = rng.permutation(n)
+    edges = [(lab[i], lab[i + 1]) for i in range(59)] + [(lab[i], lab[i + 1]) for i in range(60, 119)]
+    rp, col = _csr(n, edges)
+    xyz = np.zeros((n, 3))
+    xyz[lab, 0] = np.arange(n)
+    order, _, bw1 = level_order(rp, col, n, xyz)
+    assert bw1 == 1
+    first = set(lab[:60].tolist())
+    k = sum(1 for v in order[:60] if v in first)
+    assert k in (0, 60)                         # each component contiguous
+
+
+def test_level_order_rejects_bad_coordinates():
+    rp, col = _csr(3, [(0, 1), (1, 2)])
+    with pytest.raises(ValueError):
+        level_order(rp, col, 3, np.zeros((3, 2)))
+    bad = np.zeros((3, 3))
+    bad[1, 2] = np.nan
+    with pytest.raises(Exception):
+        level_order(rp, col, 3, bad)

@@ -159,6 +159,7 @@ FUNCS = {
     "tc_sell_values": (p, i64, p, p, p, i32, p),
     "tc_sell_deltas": (SELL_P, p, p, p),
     "tc_rcm_order": (i64, p, p, p, I64_P, I64_P),
+    "tc_level_order": (i64, p, p, p, p, I64_P, I64_P),
     "tc_table_shape": (C.c_char_p, i32, I64_P, C.POINTER(C.c_int32)),
     "tc_table_read_f64": (C.c_char_p, i32, i32, i32, i64, p),
     "tc_table_read_i64": (C.c_char_p, i32, i32, i32, i64, p),

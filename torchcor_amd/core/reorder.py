@@ -3,7 +3,10 @@
 ``RCM`` keeps the reference's interface (``calculate_rcm_order``, ``reorder``,
 ``map``, ``inverse``); the order itself comes from the library's native
 reverse Cuthill-McKee (``tc_rcm_order``, host code), not from scipy.
-``rcm_order`` is the function the simulator uses on an assembled pattern.
+``rcm_order`` orders an assembled pattern by reverse Cuthill-McKee;
+``level_order`` (what the simulator applies) keeps the Cuthill-McKee level
+structure and orders each level along a Z-order curve of the node
+coordinates (``tc_level_order``).
 """
 import ctypes as C
 from itertools import combinations
@@ -28,6 +31,25 @@ def rcm_order(row_ptr, col, n):
     b0, b1 = C.c_int64(0), C.c_int64(0)
     L.call("tc_rcm_order", n, rp.ctypes.data, cl.ctypes.data, order.ctypes.data, C.byref(b0),
            C.byref(b1))
+    return order.astype(np.int64), int(b0.value), int(b1.value)
+
+
+def level_order(row_ptr, col, n, xyz):
+    """Cuthill-McKee levels, each along a Z-order curve of ``xyz`` (n, 3).
+
+    Returns (order, bandwidth_before, bandwidth_after) as ``rcm_order``."""
+    rp = np.ascontiguousarray(torch.as_tensor(row_ptr).cpu().numpy(), dtype=np.int32)
+    cl = np.ascontiguousarray(torch.as_tensor(col).cpu().numpy(), dtype=np.int32)
+    p = np.ascontiguousarray(torch.as_tensor(xyz).cpu().numpy(), dtype=np.float64)
+    n = int(n)
+    if rp.shape[0] != n + 1:
+        raise ValueError("row_ptr must have n + 1 entries")
+    if p.shape != (n, 3):
+        raise ValueError("xyz must be (n, 3)")
+    order = np.empty(n, dtype=np.int32)
+    b0, b1 = C.c_int64(0), C.c_int64(0)
+    L.call("tc_level_order", n, rp.ctypes.data, cl.ctypes.data, p.ctypes.data, order.ctypes.data,
+           C.byref(b0), C.byref(b1))
     return order.astype(np.int64), int(b0.value), int(b1.value)
 
 

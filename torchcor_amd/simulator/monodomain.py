@@ -13,12 +13,13 @@ The solution vector is kept in the working dtype and rotated between three
 buffers (x_prev <- x <- x_next) instead of copied.
 
 Node order.  When the mesh's own numbering scatters neighbours (bandwidth of
-the pattern above n/8 and at least twice what a reverse Cuthill-McKee order
-gives), the operators and every per-node array live in the RCM order on the
-device; the mesh arrays, stimulus node
-ids and every output (u, frames, AT, RT, M, K, A) stay in the caller's
-numbering.  ``reorder`` selects this: "auto" (default), "rcm" (always) or
-None (never).
+the pattern above n/8 and at least twice what the new order gives), the
+operators and every per-node array live in a locality order on the device:
+the Cuthill-McKee level structure with each level along a Z-order curve of
+the node coordinates (``level_order``).  The mesh arrays, stimulus node ids
+and every output (u, frames, AT, RT, M, K, A) stay in the caller's
+numbering.  ``reorder`` selects this: "auto" (default), "levels" (always),
+"rcm" (always, plain reverse Cuthill-McKee) or None (never).
 """
 import time
 from pathlib import Path
@@ -29,7 +30,7 @@ import torch
 from .. import _lib as L
 from ..core.conductivity import Conductivity
 from ..core.mesh import MeshReader
-from ..core.reorder import rcm_order
+from ..core.reorder import level_order, rcm_order
 from ..core.stimulation import Stimuli
 from ..core.system import P1System
 from ..ionic.base import default_device
@@ -60,8 +61,8 @@ class Monodomain:
         self.result_path = None
         self._x = None
         self._sim = None
-        if reorder not in ("auto", "rcm", None):
-            raise ValueError("reorder must be 'auto', 'rcm' or None")
+        if reorder not in ("auto", "levels", "rcm", None):
+            raise ValueError("reorder must be 'auto', 'levels', 'rcm' or None")
         self.reorder = reorder
         self.node_order = None   # internal position i holds mesh node node_order[i]
         self.node_map = None     # mesh node j sits at internal position node_map[j]
@@ -94,7 +95,7 @@ class Monodomain:
         self.conductivity.add(region_ids, il, it, el, et)
 
     def _choose_order(self, sys_):
-        """RCM order for the pattern of ``sys_`` if the mode asks for it."""
+        """Node order for the pattern of ``sys_`` if the mode asks for one."""
         self.node_order = self.node_map = None
         if self.reorder is None:
             return None
@@ -104,8 +105,11 @@ class Monodomain:
             bw0 = int((sys_.col.to(torch.int64) - rows).abs().max().item()) if sys_.nnz else 0
             if 8 * bw0 <= sys_.n:
                 return None
-        order, bw0, bw1 = rcm_order(sys_.row_ptr, sys_.col, sys_.n)
-        self.bandwidth_natural, self.bandwidth_rcm = bw0, bw1
+        if self.reorder == "rcm":
+            order, bw0, bw1 = rcm_order(sys_.row_ptr, sys_.col, sys_.n)
+        else:
+            order, bw0, bw1 = level_order(sys_.row_ptr, sys_.col, sys_.n, self.nodes)
+        self.bandwidth_natural, self.bandwidth_reordered = bw0, bw1
         if self.reorder == "auto" and 2 * bw1 > bw0:
             return None
         n = sys_.n
