@@ -279,6 +279,9 @@ typedef struct tc_ionic_io {
   const long long* pool_sys_steps;
   const uint32_t* pool_sys_bits;
   const struct tc_stimulus* pool_sys_stim;
+  /* Row stride of the (S, stride) state block: the pool's, or in the plain
+   * layout a padded one (0 = n); rows starting on 256-byte boundaries keep
+   * every wavefront's state access to whole cache lines. */
   int64_t state_stride;
   /* Load and store the states non-temporally: they then pass through the
    * caches without displacing what is reused across steps.  Worth it when
@@ -368,6 +371,7 @@ typedef struct tc_sim_desc {
   double a_tol, r_tol;
   int max_iter, linear_guess;
   int depth;                 /* steps enqueued ahead of the status read, >= 1 */
+  int64_t state_stride;      /* row stride of the state block (0 = n) */
 } tc_sim_desc;
 typedef struct tc_sim tc_sim;
 int tc_sim_create(tc_sim** sim, const tc_sim_desc* desc);
@@ -429,9 +433,10 @@ int tc_sim_get_timing(tc_sim* sim, double* ms_host, int64_t* count_host);
 typedef struct tc_batch_group {
   int model;                 /* tc_model */
   const void* params;        /* tc_*_params */
-  void* states;              /* (S, n * count) */
+  void* states;              /* (S, state_stride) block of n * count columns */
   const void* tab[3];        /* model tables */
   int b0, count;
+  int64_t state_stride;      /* row stride of the state block (0 = n * count) */
 } tc_batch_group;
 typedef struct tc_batch_desc {
   int64_t n, nnz;

@@ -85,7 +85,7 @@ class SimDesc(C.Structure):
                 ("w", p), ("AT", p), ("RT", p),
                 ("stim_mask", p), ("n_stim", i32), ("stim", Stimulus * MAX_STIMULI),
                 ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32), ("linear_guess", i32),
-                ("depth", i32)]
+                ("depth", i32), ("state_stride", i64)]
 
 
 BATCH_MAX_K = 5
@@ -94,7 +94,7 @@ BATCH_MAX_GROUPS = 8
 
 class BatchGroup(C.Structure):
     _fields_ = [("model", i32), ("params", p), ("states", p), ("tab", p * 3), ("b0", i32),
-                ("count", i32)]
+                ("count", i32), ("state_stride", i64)]
 
 
 class BatchDesc(C.Structure):

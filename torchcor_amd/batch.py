@@ -269,6 +269,7 @@ class SharedMeshBatch:
             d.groups[g].model = model.MODEL
             d.groups[g].params = C.cast(C.pointer(cp), C.c_void_p)
             d.groups[g].states = L.ptr(model.states)
+            d.groups[g].state_stride = model.states.stride(0)
             for k, tab in enumerate(tabs):
                 d.groups[g].tab[k] = L.ptr(tab)
             d.groups[g].b0, d.groups[g].count = gb0, cnt

@@ -751,6 +751,7 @@ int bat_enqueue(tc_batch* S, BHost& ps, BPlan& plan, hipStream_t s) {
     io.lanes = G.count;
     io.lane_stride = d.lanes;
     io.lane0 = G.b0;
+    io.state_stride = G.state_stride;
     int rc = group_ionic(S, g, &io, s);
     if (rc) return rc;
   }

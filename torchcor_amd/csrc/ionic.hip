@@ -209,7 +209,7 @@ __device__ inline NodeCtx node_ctx(const tc_ionic_io& io, int64_t i) {
     c.vi = i;
   }
   c.si = i;
-  c.ss = io.n;
+  c.ss = io.state_stride > 0 ? io.state_stride : io.n;
   c.ok = 1;
   c.bits = (io.stim_active && io.stim_mask) ? at32(io.stim_mask, (uint32_t)c.vi) & io.stim_active
                                              : 0u;
@@ -1011,6 +1011,9 @@ int check_io(const tc_ionic_io* io, int dtype) {
   int64_t top = io->n;
   if (io->pool_slices) top = io->state_stride;
   else if (io->lanes > 0) top = (io->n - 1) / io->lanes * io->lane_stride + io->lane0 + io->lanes;
+  TC_CHECK(io->pool_slices || io->state_stride == 0 || io->state_stride >= io->n, TC_ERR_ARG,
+           "ionic step: state stride %lld below n %lld", (long long)io->state_stride,
+           (long long)io->n);
   TC_CHECK(top < ((int64_t)1 << 29), TC_ERR_SIZE,
            "ionic step: %lld per-node entries exceed the 2^29 limit", (long long)top);
   return TC_OK;

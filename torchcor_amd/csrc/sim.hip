@@ -262,6 +262,7 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   }
   io.gate = &P->ctl->stall;
   io.stream_states = 1;
+  io.state_stride = d.state_stride;
   int rc = ionic_launch(S, &io, s, ev ? (*ev)[0] : nullptr, ev ? (*ev)[1] : nullptr);
   if (rc) return rc;
   // 2. RHS + PCG start
@@ -453,6 +454,8 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
            "tc_sim_create: need two x buffers (and two x_prev buffers with the linear guess)");
   TC_CHECK(d->n_stim >= 0 && d->n_stim <= TC_MAX_STIMULI, TC_ERR_ARG, "bad stimulus count");
   TC_CHECK(d->n_stim == 0 || d->stim_mask, TC_ERR_ARG, "stimuli need a mask");
+  TC_CHECK(d->state_stride == 0 || d->state_stride >= d->n, TC_ERR_ARG,
+           "tc_sim_create: state stride below n");
   TC_CHECK(d->max_iter >= 0 && d->max_iter <= kMaxIterLimit, TC_ERR_ARG,
            "max_iter %d outside 0..%d", d->max_iter, kMaxIterLimit);
   TC_CHECK(!d->w_f64 || d->x_f64 || d->dtype == TC_F64, TC_ERR_ARG, "w_f64 needs x_f64");

@@ -53,17 +53,23 @@ class IonicModel:
     def _param_values(self):
         return {f: float(getattr(self, f)) for f in self.PARAMS}
 
-    def _alloc_states(self, n):
+    def _alloc_states(self, n, stride=None):
+        """(S, n) states; with ``stride`` the rows lie ``stride`` entries apart
+        in one (S, stride) block and ``states`` is the (S, n) view."""
         if self.device.type != "cuda":
             raise L.TorchcorError(f"{self.name} runs on the GPU only (no CPU path)")
-        st = torch.empty((len(self.STATES), n), dtype=self.dtype, device=self.device)
+        stride = n if stride is None else int(stride)
+        buf = torch.empty((len(self.STATES), stride), dtype=self.dtype, device=self.device)
+        st = buf[:, :n]
         for k, s in enumerate(self.STATES):
             st[k].fill_(self.INIT[s])
         object.__setattr__(self, "_states", st)
         self.n_nodes = n
 
     def initialize(self, n_nodes):
-        self._alloc_states(n_nodes)
+        # state rows padded to whole 64-entry groups (256 bytes in float32),
+        # so every row starts on a cache-line boundary
+        self._alloc_states(n_nodes, stride=-(-n_nodes // 64) * 64)
         self.construct_tables()
         return torch.full((n_nodes,), self.V_init, dtype=self.dtype, device=self.device)
 
@@ -87,6 +93,7 @@ class IonicModel:
         io.AT = L.ptr(AT)
         io.RT = L.ptr(RT)
         io.lat_t = float(lat_t)
+        io.state_stride = self._states.stride(0) if self._states is not None else 0
         return io
 
     def step(self, io):

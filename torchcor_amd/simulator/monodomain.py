@@ -302,6 +302,7 @@ class Monodomain:
         self._cparams = ion.c_params()
         d.params = C.cast(C.pointer(self._cparams), C.c_void_p)
         d.states = L.ptr(ion.states)
+        d.state_stride = ion.states.stride(0)
         for k, tab in enumerate(ion.c_tables()):
             d.tab[k] = L.ptr(tab)
         d.pcg = self.system._pcg
