@@ -538,7 +538,7 @@ TtpK<T> ttp_constants(const tc_ttp_params& p) {
 }
 
 // One TTP step per node (TenTusscherPanfilov.py:353-454).
-template <class T, bool VF64, bool NT, bool UP>
+template <class T, bool VF64, bool NT>
 __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
                                          const T* __restrict__ Vtab, const T* __restrict__ Ctab,
                                          const T* __restrict__ Ktab, const TtpTabs& tb,
@@ -555,24 +555,19 @@ __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
   const T at = load_at<T>(io, c);
   const T RTONF = K.RTONF, F_RT = K.F_RT;
 
-  // Phase 1: currents and concentrations, with the CaSS and VEk tables;
-  // the concentration states, FCaSS and the epilogue are written here.  With
-  // UP (float32 k_ttp_stream) all V-table groups, the gates' columns too,
-  // are fetched here with the CaSS row, so their latency overlaps the
-  // currents' chain instead of following it (134 VGPRs, 3 waves per SIMD);
-  // otherwise the gate groups wait for phase 2, which keeps the live ranges
-  // short (4 waves per SIMD in float32).
+  // Phase 1: currents and concentrations, with the V-table columns they use
+  // (INaCa_A/B, a2, rec_iNaK, rec_ipK) and the CaSS and VEk tables; the
+  // concentration states, FCaSS and the epilogue are written here.
   RowG<T, TTP_VSTRIDE> vr;
   vr.at(Vtab, V, tb.V);
   constexpr int E = Vec16<T>::n;
-  constexpr bool upfront = UP;
-  // 16-byte groups holding the phase-1 columns (INaCa_A/B, a2, rec_iNaK, rec_ipK)
+  // 16-byte groups holding the phase-1 columns
   constexpr unsigned ph1 = (1u << (TTP_INaCa_A / E)) | (1u << (TTP_INaCa_B / E)) |
                            (1u << (TTP_a2 / E)) | (1u << (TTP_rec_iNaK / E)) |
                            (1u << (TTP_rec_ipK / E));
 #pragma unroll
   for (int g = 0; g < TTP_VSTRIDE / E; ++g)
-    if (upfront || (ph1 >> g & 1u)) vr.load_group(g);
+    if (ph1 >> g & 1u) vr.load_group(g);
   const Row<T> cr = row_at<T>(Ctab, 3, s[TS_CaSS], tb.CaSS);
   const T Cai = s[TS_Cai], CaSS = s[TS_CaSS], CaSR = s[TS_CaSR], Ki = s[TS_Ki], Nai = s[TS_Nai];
   const T Eca = K.hRTONF * t_log(K.Cao / Cai);
@@ -633,14 +628,12 @@ __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
   st_state<NT>(st, c, TS_R_, T(R_ + diff_R_ * dt));
   st_state<NT>(st, c, TS_FCaSS, T(cr(1, 3) + cr(2, 3) * s[TS_FCaSS]));
   finish_node<T>(io, c, V, -Iion, at);
-  // Phase 2 (no memory access moves across this point): the Rush-Larsen
-  // gate updates from the old gates.
+  // Phase 2 (no memory access moves across this point): the remaining
+  // V-table columns and the Rush-Larsen gate updates from the old gates.
   asm volatile("" ::: "memory");
-  if constexpr (!upfront) {
 #pragma unroll
-    for (int g = 0; g < TTP_VSTRIDE / E; ++g)
-      if (!(ph1 >> g & 1u)) vr.load_group(g);
-  }
+  for (int g = 0; g < TTP_VSTRIDE / E; ++g)
+    if (!(ph1 >> g & 1u)) vr.load_group(g);
   T o[TS_N];
   o[TS_D] = vr(TTP_D_A, TTP_VCOLS) + vr(TTP_D_B, TTP_VCOLS) * s[TS_D];
   o[TS_F] = vr(TTP_F_A, TTP_VCOLS) + vr(TTP_F_B, TTP_VCOLS) * s[TS_F];
@@ -658,26 +651,11 @@ __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
   for (int k = 0; k < 11; ++k) st_state<NT>(st, c, gates[k], T(o[gates[k]]));
 }
 
-template <class T, bool VF64>
+template <class T, bool VF64, bool NT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_ttp(TtpK<T> K, T* __restrict__ st, const T* __restrict__ Vtab,
       const T* __restrict__ Ctab, const T* __restrict__ Ktab, TtpTabs tb, tc_ionic_io io) {
-  ttp_node<T, VF64, false, false>(K, st, Vtab, Ctab, Ktab, tb, io,
-                                  (int64_t)blockIdx.x * kBlock + threadIdx.x);
-}
-
-// The same step with streamed states (stream_states): the states then come
-// from HBM rather than the last-level cache, and fetching every V-table
-// group in phase 1 (float32) hides more of that latency, at 3 waves per
-// SIMD (measured on the 101^3 cube: 48.2 -> 47.3 us; with cached states, as
-// in a batch, the split at 4 waves stays faster).
-template <class T, bool VF64>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 3)))
-k_ttp_stream(TtpK<T> K, T* __restrict__ st, const T* __restrict__ Vtab,
-             const T* __restrict__ Ctab, const T* __restrict__ Ktab, TtpTabs tb,
-             tc_ionic_io io) {
-  ttp_node<T, VF64, true, sizeof(T) == 4>(K, st, Vtab, Ctab, Ktab, tb, io,
-                                          (int64_t)blockIdx.x * kBlock + threadIdx.x);
+  ttp_node<T, VF64, NT>(K, st, Vtab, Ctab, Ktab, tb, io, (int64_t)blockIdx.x * kBlock + threadIdx.x);
 }
 
 // ---------------------------------------------------------------- CRN
@@ -983,7 +961,7 @@ k_pool_ionic(tc_ms2v_params mp, TtpK<T> tk, CrnK<T> ck, T* __restrict__ st0,
   if (b < b1) {
     ms2v_node<T, VF64, NT>(mp, st0, io0, (int64_t)b * kBlock + threadIdx.x);
   } else if (b < b2) {
-    ttp_node<T, VF64, NT, false>(tk, st1, tt0, tt1, tt2, ttb, io1, (int64_t)(b - b1) * kBlock + threadIdx.x);
+    ttp_node<T, VF64, NT>(tk, st1, tt0, tt1, tt2, ttb, io1, (int64_t)(b - b1) * kBlock + threadIdx.x);
   } else {
     crn_node<T, VF64, NT>(ck, st2, ct0, ct1, ct2, ctb, io2, (int64_t)(b - b2) * kBlock + threadIdx.x);
   }
@@ -1088,15 +1066,15 @@ int tc::ttp_step(const tc_ttp_params* p, void* states, const void* V_tab, const 
   dim3 g((int)ceil_div(io->n, kBlock)), b(kBlock);
   const bool nt = io->stream_states != 0;
   if (dtype == TC_F32 && io->v_f64)
-    launch(nt ? k_ttp_stream<float, true> : k_ttp<float, true>, g, b, s, e0, e1,
+    launch(nt ? k_ttp<float, true, true> : k_ttp<float, true, false>, g, b, s, e0, e1,
            ttp_constants<float>(*p), (float*)states, (const float*)V_tab,
            (const float*)CaSS_tab, (const float*)VEk_tab, tb, *io);
   else if (dtype == TC_F32)
-    launch(nt ? k_ttp_stream<float, false> : k_ttp<float, false>, g, b, s, e0, e1,
+    launch(nt ? k_ttp<float, false, true> : k_ttp<float, false, false>, g, b, s, e0, e1,
            ttp_constants<float>(*p), (float*)states, (const float*)V_tab,
            (const float*)CaSS_tab, (const float*)VEk_tab, tb, *io);
   else
-    launch(nt ? k_ttp_stream<double, false> : k_ttp<double, false>, g, b, s, e0, e1,
+    launch(nt ? k_ttp<double, false, true> : k_ttp<double, false, false>, g, b, s, e0, e1,
            ttp_constants<double>(*p), (double*)states, (const double*)V_tab,
            (const double*)CaSS_tab, (const double*)VEk_tab, tb, *io);
   TC_LAUNCH();
