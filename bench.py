@@ -58,9 +58,9 @@ def alg_bytes(n, nnz, S, ib=4, xb=4, wb=4):
 # WRITE_SIZE x1.0.  profiles/cube101_ttp_step_hbm_traffic.md holds the
 # counter summary and the commands.
 MEASURED_TRAFFIC_101 = {
-    "ionic": 192_103_728,
-    "rhs_init": 204_164_883,
-    "pcg_spmv": 112_219_328,
+    "ionic": 173_180_357,
+    "rhs_init": 204_172_882,
+    "pcg_spmv": 112_206_273,
     "pcg_update": 37_284_347,
 }
 
