@@ -103,6 +103,16 @@ int tc_operator_values(const double* M_vals, const double* K_vals, const int32_t
  * entry j of row 64*s+l at slice_ptr[s] + 64*j + l.  Padding entries hold
  * col = row and value 0.  The CSR stays the external format; SELL is the
  * internal layout of the SpMV kernels.
+ *
+ * Shared slices: when every row of a slice has its column offsets
+ * (col - row) among those of the slice's longest row, the slice may be laid
+ * out on that row's offsets (its pattern): entry j of every row then has
+ * offset pat[slice_ptr[s]/64 + j], or is a padding entry (col = row, value
+ * 0) where the row lacks that offset.  Each row keeps its entries in
+ * column order, so every product sums the same terms in the same order.
+ * The per-row columns stay complete; the PCG and RHS passes read the
+ * pattern (one value per entry position instead of 64) on the slices
+ * flagged in slice_tag.
  * ---------------------------------------------------------------------- */
 typedef struct tc_sell {
   int64_t n;              /* rows */
@@ -112,16 +122,35 @@ typedef struct tc_sell {
   /* optional (NULL: unused): col - row as int16, from tc_sell_deltas when
    * every offset fits; the PCG and RHS passes then read these instead */
   const int16_t* dcol;
+  /* optional, used with dcol (NULL: no shared slices): the patterns
+   * (slice_ptr[n_slices]/64 int32 offsets, each within int16) and slice_ptr
+   * with bit 0 set on the shared slices (n_slices+1; slice offsets are
+   * multiples of 64) */
+  const int32_t* pat;
+  const int32_t* slice_tag;
 } tc_sell;
 
 /* Padded entry count; synchronizes `stream`.  slice_ptr (n_slices+1) is
  * filled as a side effect. */
 int tc_sell_layout(const int32_t* row_ptr, int64_t n, int32_t* slice_ptr,
                    int64_t* padded_host, void* stream);
+/* Shared-slice detection over the layout: slice s is shared when every
+ * row's offsets lie among those of the slice's longest row and fit int16;
+ * its offsets then go to pat (slice_ptr[n_slices]/64 int32) and
+ * slice_tag[s] = slice_ptr[s] | 1 (else slice_ptr[s]; n_slices+1 entries).
+ * Stream-ordered, does not synchronize. */
+int tc_sell_share(const int32_t* row_ptr, const int32_t* col_idx, int64_t n,
+                  const int32_t* slice_ptr, int32_t* pat, int32_t* slice_tag, void* stream);
+/* Columns and values in the layout; with pat/slice_tag from tc_sell_share
+ * (NULL: every slice per row) the shared slices are laid out on their
+ * pattern (tc_sell_values then needs col_idx to place each value; it may
+ * be NULL without a pattern). */
 int tc_sell_columns(const int32_t* row_ptr, const int32_t* col_idx, int64_t n,
-                    const int32_t* slice_ptr, int32_t* sell_col, void* stream);
-int tc_sell_values(const int32_t* row_ptr, int64_t n, const int32_t* slice_ptr,
-                   const void* csr_vals, void* sell_vals, int dtype, void* stream);
+                    const int32_t* slice_ptr, int32_t* sell_col, const int32_t* pat,
+                    const int32_t* slice_tag, void* stream);
+int tc_sell_values(const int32_t* row_ptr, const int32_t* col_idx, int64_t n,
+                   const int32_t* slice_ptr, const void* csr_vals, void* sell_vals, int dtype,
+                   const int32_t* pat, const int32_t* slice_tag, void* stream);
 /* dcol[e] = col[e] - row(e) as int16 for every padded entry; *max_abs_dev
  * (one device int32) receives the largest |col - row|, so dcol is usable
  * when it is <= 32767.  Stream-ordered, does not synchronize. */

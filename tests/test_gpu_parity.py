@@ -573,3 +573,48 @@ def test_runtime_int16_columns_identical(monkeypatch):
                       fibres=torch.tensor(_rand_fibres(E, 2)))
     S.set_operators(M, K, 140, 0.01, 0.5, 0.01, torch.float32)
     assert S.bandwidth > 32767 and S.sell.dcol is None
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_shared_slices_identical(monkeypatch, dtype):
+    """Slices whose rows share one stencil (the structured cube) are laid out
+    on it and read one column offset per entry position: each row keeps its
+    entries in column order with zeros where it lacks an offset, so runs,
+    products and the external CSR are bit-identical to the per-row layout;
+    an unstructured mesh gets no shared slices."""
+    from torchcor_amd.core.system import P1System
+    _say(f"[gpu] shared vs per-row slices {dtype}")
+    runs = []
+    for flag in (True, False):
+        monkeypatch.setattr(P1System, "share_slices", flag)
+        sim = _cube_sim(21, "ttp", dtype)
+        sim.start(1e-5, 1e-5, 100, snapshot_interval=1.0, depth=4)
+        S = sim.system
+        ns = (S.n + 63) // 64
+        if flag:
+            assert S.sell.slice_tag is not None and S.n_shared >= 0.95 * ns
+            assert torch.equal(S.slice_tag & ~1, S.slice_ptr)
+        else:
+            assert S.sell.slice_tag is None and S.n_shared == 0
+        sim.advance(400)
+        sim.finish()
+        rng = np.random.default_rng(3)
+        x = torch.tensor(rng.normal(size=S.n), dtype=dtype, device=DEV)
+        prods = [S.spmv(w, x) for w in ("M", "K", "A")]
+        A = sim.A
+        runs.append((sim.u.clone(), list(sim.iterations), sim.frames.clone(), prods,
+                     A.col_indices().clone(), A.values().clone()))
+    a, b = runs
+    assert torch.equal(a[0], b[0]) and a[1] == b[1] and torch.equal(a[2], b[2])
+    for p, q in zip(a[3], b[3]):
+        assert torch.equal(p, q)
+    assert torch.equal(a[4], b[4]) and torch.equal(a[5], b[5])
+    monkeypatch.setattr(P1System, "share_slices", True)
+    nodes, elems = meshes.jittered_delaunay(9, seed=3)
+    S = _sys(nodes, elems)
+    E = len(elems)
+    M, K = S.assemble(sigma_l=torch.full((E,), 0.3, dtype=torch.float64),
+                      sigma_t=torch.full((E,), 0.1, dtype=torch.float64),
+                      fibres=torch.tensor(_rand_fibres(E, 2)))
+    S.set_operators(M, K, 140, 0.01, 0.5, 0.01, torch.float32)
+    assert S.n_shared <= 0.05 * ((S.n + 63) // 64)

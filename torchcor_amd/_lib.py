@@ -28,7 +28,8 @@ dbl = C.c_double
 
 
 class Sell(C.Structure):
-    _fields_ = [("n", i64), ("n_slices", i64), ("slice_ptr", p), ("col", p), ("dcol", p)]
+    _fields_ = [("n", i64), ("n_slices", i64), ("slice_ptr", p), ("col", p), ("dcol", p),
+                ("pat", p), ("slice_tag", p)]
 
 
 class Stimulus(C.Structure):
@@ -155,8 +156,9 @@ FUNCS = {
     "tc_assemble_p1": (p, p, i64, i32, p, p, p, p, p, p, i64, p, i32, p, p, p),
     "tc_operator_values": (p, p, p, p, i64, i64, dbl, dbl, dbl, dbl, i32, p, p, p, p, p),
     "tc_sell_layout": (p, i64, p, I64_P, p),
-    "tc_sell_columns": (p, p, i64, p, p, p),
-    "tc_sell_values": (p, i64, p, p, p, i32, p),
+    "tc_sell_share": (p, p, i64, p, p, p, p),
+    "tc_sell_columns": (p, p, i64, p, p, p, p, p),
+    "tc_sell_values": (p, p, i64, p, p, p, i32, p, p, p),
     "tc_sell_deltas": (SELL_P, p, p, p),
     "tc_rcm_order": (i64, p, p, p, I64_P, I64_P),
     "tc_level_order": (i64, p, p, p, p, I64_P, I64_P),

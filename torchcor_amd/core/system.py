@@ -24,6 +24,8 @@ class P1System:
 
     #: hot passes read int16 column offsets (col - row) when every offset fits
     int16_columns = True
+    #: slices whose rows share one stencil are laid out on it (with int16 offsets)
+    share_slices = True
 
     def __init__(self, nodes, elems, device=None):
         device = torch.device(device) if device is not None else elems.device
@@ -100,11 +102,24 @@ class P1System:
         padded = C.c_int64(0)
         L.call("tc_sell_layout", L.ptr(self.row_ptr), self.n, L.ptr(self.slice_ptr), C.byref(padded), s)
         self.padded = int(padded.value)
+        # slices whose rows share one stencil (a structured grid's interior)
+        # are laid out on it: the hot passes then read one offset per entry
+        # position instead of one per row
+        self.pat = self.slice_tag = None
+        self.n_shared = 0
+        if self.int16_columns and self.share_slices:
+            self.pat = torch.zeros(max(self.padded // 64, 1), dtype=torch.int32, device=dev)
+            self.slice_tag = torch.empty(ns + 1, dtype=torch.int32, device=dev)
+            L.call("tc_sell_share", L.ptr(self.row_ptr), L.ptr(self.col), self.n,
+                   L.ptr(self.slice_ptr), L.ptr(self.pat), L.ptr(self.slice_tag), s)
+            self.n_shared = int((self.slice_tag[:ns] & 1).sum().item())
+            if self.n_shared == 0:
+                self.pat = self.slice_tag = None
         self.scol = torch.empty(self.padded, dtype=torch.int32, device=dev)
         L.call("tc_sell_columns", L.ptr(self.row_ptr), L.ptr(self.col), self.n, L.ptr(self.slice_ptr),
-               L.ptr(self.scol), s)
+               L.ptr(self.scol), L.ptr(self.pat), L.ptr(self.slice_tag), s)
         self.sM, self.sK, self.sA = (self.to_sell(v) for v in (self.Mv, self.Kv, self.Av))
-        self.sell = L.Sell(self.n, ns, L.ptr(self.slice_ptr), L.ptr(self.scol), None)
+        self.sell = L.Sell(self.n, ns, L.ptr(self.slice_ptr), L.ptr(self.scol), None, None, None)
         # int16 column offsets for the hot passes when the bandwidth allows
         self.dcol = torch.empty(self.padded, dtype=torch.int16, device=dev)
         mx = torch.empty(1, dtype=torch.int32, device=dev)
@@ -112,14 +127,18 @@ class P1System:
         self.bandwidth = int(mx.item())
         if self.int16_columns and self.bandwidth <= 32767:
             self.sell.dcol = L.ptr(self.dcol)
+            self.sell.pat = L.ptr(self.pat)
+            self.sell.slice_tag = L.ptr(self.slice_tag)
         else:
             self.dcol = None
         self._make_pcg()
 
     def to_sell(self, vals):
+        """Values (nnz, CSR order) in the system's SELL layout."""
         out = torch.empty(self.padded, dtype=vals.dtype, device=self.device)
-        L.call("tc_sell_values", L.ptr(self.row_ptr), self.n, L.ptr(self.slice_ptr), L.ptr(vals),
-               L.ptr(out), L.dtype_code(vals.dtype), L.stream_handle(self.device))
+        L.call("tc_sell_values", L.ptr(self.row_ptr), L.ptr(self.col), self.n, L.ptr(self.slice_ptr),
+               L.ptr(vals), L.ptr(out), L.dtype_code(vals.dtype), L.ptr(self.pat), L.ptr(self.slice_tag),
+               L.stream_handle(self.device))
         return out
 
     def _make_pcg(self):

@@ -365,10 +365,80 @@ __global__ void k_slice_scan(const int32_t* width, int64_t n_slices, int32_t* sl
   }
 }
 
+// Entry e of row i (offset d = col - i) goes to position e of its slice,
+// or, in a shared slice, to the position of d in the slice's pattern; the
+// other positions are padding (col = i, value 0).  pos_walk visits the w
+// positions in order and calls put(j, e) for a real entry, pad(j) else.
+template <class Put, class Pad>
+__device__ inline void pos_walk(const int32_t* __restrict__ col, int32_t b, int len, int64_t i,
+                                int w, const int32_t* pat_s, Put put, Pad pad) {
+  if (!pat_s) {
+    for (int j = 0; j < w; ++j) {
+      if (j < len) put(j, j);
+      else pad(j);
+    }
+    return;
+  }
+  int e = 0;
+  for (int j = 0; j < w; ++j) {
+    if (e < len && (int64_t)col[b + e] - i == (int64_t)pat_s[j]) put(j, e++);
+    else pad(j);
+  }
+}
+
+// One wavefront per slice: the slice is shared when every row's offsets
+// lie, in order, among those of its first longest row and all fit int16.
+// The candidate pattern goes through LDS (the rows' subset walks diverge,
+// so they read it there rather than across lanes); every thread reaches
+// the block barrier, waves past the last slice included.
+__global__ void __launch_bounds__(kBlock)
+k_sell_share(const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col, int64_t n,
+             int64_t n_slices, const int32_t* __restrict__ slice_ptr, int32_t* __restrict__ pat,
+             int32_t* __restrict__ slice_tag) {
+  __shared__ int32_t U[kBlock / kWave][kWave];
+  const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  const bool live = s < n_slices;
+  const int64_t i = s * kWave + lane;
+  const int w = live ? (slice_ptr[s + 1] - slice_ptr[s]) / kWave : 0;
+  int len = 0;
+  int32_t b = 0;
+  if (live && i < n) {
+    b = row_ptr[i];
+    len = row_ptr[i + 1] - b;
+  }
+  const unsigned long long longest = __ballot(w > 0 && len == w);
+  const bool cand = w > 0 && w <= kWave && longest != 0;
+  bool ok = true;
+  if (cand && lane < w) {
+    const int lead = __ffsll((long long)longest) - 1;
+    const int64_t li = s * kWave + lead;
+    const int64_t d = (int64_t)col[row_ptr[li] + lane] - li;
+    ok = d >= -32767 && d <= 32767;
+    U[wv][lane] = (int32_t)d;
+  }
+  __syncthreads();
+  if (cand) {
+    // this row's offsets, in order, among the pattern's
+    int p = 0;
+    for (int e = 0; e < len && ok; ++e) {
+      const int64_t d = (int64_t)col[b + e] - i;
+      while (p < w && (int64_t)U[wv][p] < d) ++p;
+      ok = p < w && (int64_t)U[wv][p] == d;
+      ++p;
+    }
+  }
+  const bool all = cand && __ballot(!ok) == 0;
+  if (live && lane == 0) slice_tag[s] = slice_ptr[s] | (all ? 1 : 0);
+  if (live && lane == 0 && s == n_slices - 1) slice_tag[n_slices] = slice_ptr[n_slices];
+  if (all && lane < w) pat[slice_ptr[s] / kWave + lane] = U[wv][lane];
+}
+
 __global__ void k_sell_columns(const int32_t* __restrict__ row_ptr,
                                const int32_t* __restrict__ col, int64_t n,
                                int64_t n_slices, const int32_t* __restrict__ slice_ptr,
-                               int32_t* __restrict__ scol) {
+                               int32_t* __restrict__ scol, const int32_t* __restrict__ pat,
+                               const int32_t* __restrict__ slice_tag) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_slices * kWave) return;
   int64_t s = i / kWave;
@@ -382,7 +452,10 @@ __global__ void k_sell_columns(const int32_t* __restrict__ row_ptr,
     len = row_ptr[i + 1] - b;
   }
   const int32_t pad = (int32_t)(i < n ? i : n - 1);
-  for (int j = 0; j < w; ++j) scol[base + (int64_t)j * kWave] = j < len ? col[b + j] : pad;
+  const int32_t* ps = (pat && slice_tag && (slice_tag[s] & 1)) ? pat + slice_ptr[s] / kWave : nullptr;
+  pos_walk(col, b, len, i, w, ps,
+           [&](int j, int e) { scol[base + (int64_t)j * kWave] = col[b + e]; },
+           [&](int j) { scol[base + (int64_t)j * kWave] = pad; });
 }
 
 __global__ void k_zero_int(int32_t* p) { *p = 0; }
@@ -415,9 +488,11 @@ __global__ void k_sell_deltas(const int32_t* __restrict__ scol, int64_t n_slices
 }
 
 template <class T>
-__global__ void k_sell_values(const int32_t* __restrict__ row_ptr, int64_t n,
-                              int64_t n_slices, const int32_t* __restrict__ slice_ptr,
-                              const T* __restrict__ v, T* __restrict__ sv) {
+__global__ void k_sell_values(const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+                              int64_t n, int64_t n_slices, const int32_t* __restrict__ slice_ptr,
+                              const T* __restrict__ v, T* __restrict__ sv,
+                              const int32_t* __restrict__ pat,
+                              const int32_t* __restrict__ slice_tag) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_slices * kWave) return;
   int64_t s = i / kWave;
@@ -430,7 +505,10 @@ __global__ void k_sell_values(const int32_t* __restrict__ row_ptr, int64_t n,
     b = row_ptr[i];
     len = row_ptr[i + 1] - b;
   }
-  for (int j = 0; j < w; ++j) sv[base + (int64_t)j * kWave] = j < len ? v[b + j] : T(0);
+  const int32_t* ps = (pat && slice_tag && (slice_tag[s] & 1)) ? pat + slice_ptr[s] / kWave : nullptr;
+  pos_walk(col, b, len, i, w, ps,
+           [&](int j, int e) { sv[base + (int64_t)j * kWave] = v[b + e]; },
+           [&](int j) { sv[base + (int64_t)j * kWave] = T(0); });
 }
 
 }  // namespace
@@ -560,13 +638,26 @@ extern "C" int tc_sell_layout(const int32_t* row_ptr, int64_t n, int32_t* slice_
   return TC_OK;
 }
 
+extern "C" int tc_sell_share(const int32_t* row_ptr, const int32_t* col, int64_t n,
+                             const int32_t* slice_ptr, int32_t* pat, int32_t* slice_tag,
+                             void* stream) {
+  TC_CHECK(row_ptr && col && slice_ptr && pat && slice_tag && n > 0, TC_ERR_ARG,
+           "tc_sell_share: bad arguments");
+  const int64_t ns = ceil_div(n, kWave);
+  hipLaunchKernelGGL(k_sell_share, dim3((int)ceil_div(ns * kWave, kBlock)), dim3(kBlock), 0,
+                     (hipStream_t)stream, row_ptr, col, n, ns, slice_ptr, pat, slice_tag);
+  TC_LAUNCH();
+  return TC_OK;
+}
+
 extern "C" int tc_sell_columns(const int32_t* row_ptr, const int32_t* col, int64_t n,
-                               const int32_t* slice_ptr, int32_t* scol, void* stream) {
+                               const int32_t* slice_ptr, int32_t* scol, const int32_t* pat,
+                               const int32_t* slice_tag, void* stream) {
   TC_CHECK(row_ptr && col && slice_ptr && scol && n > 0, TC_ERR_ARG,
            "tc_sell_columns: bad arguments");
   int64_t ns = ceil_div(n, kWave);
   hipLaunchKernelGGL(k_sell_columns, dim3((int)ceil_div(ns * kWave, kBlock)), dim3(kBlock), 0,
-                     (hipStream_t)stream, row_ptr, col, n, ns, slice_ptr, scol);
+                     (hipStream_t)stream, row_ptr, col, n, ns, slice_ptr, scol, pat, slice_tag);
   TC_LAUNCH();
   return TC_OK;
 }
@@ -584,18 +675,21 @@ extern "C" int tc_sell_deltas(const tc_sell* A, int16_t* dcol, int32_t* max_abs,
   return TC_OK;
 }
 
-extern "C" int tc_sell_values(const int32_t* row_ptr, int64_t n, const int32_t* slice_ptr,
-                              const void* v, void* sv, int dtype, void* stream) {
+extern "C" int tc_sell_values(const int32_t* row_ptr, const int32_t* col_for_values, int64_t n,
+                              const int32_t* slice_ptr, const void* v, void* sv, int dtype,
+                              const int32_t* pat, const int32_t* slice_tag, void* stream) {
   TC_CHECK(row_ptr && slice_ptr && v && sv && n > 0, TC_ERR_ARG, "tc_sell_values: bad arguments");
+  TC_CHECK(!pat || (col_for_values && slice_tag), TC_ERR_ARG,
+           "tc_sell_values: a pattern needs the CSR columns and the slice tags");
   int64_t ns = ceil_div(n, kWave);
   dim3 g((int)ceil_div(ns * kWave, kBlock)), b(kBlock);
   hipStream_t s = (hipStream_t)stream;
   if (dtype == TC_F32)
-    hipLaunchKernelGGL(k_sell_values<float>, g, b, 0, s, row_ptr, n, ns, slice_ptr,
-                       (const float*)v, (float*)sv);
+    hipLaunchKernelGGL(k_sell_values<float>, g, b, 0, s, row_ptr, col_for_values, n, ns,
+                       slice_ptr, (const float*)v, (float*)sv, pat, slice_tag);
   else if (dtype == TC_F64)
-    hipLaunchKernelGGL(k_sell_values<double>, g, b, 0, s, row_ptr, n, ns, slice_ptr,
-                       (const double*)v, (double*)sv);
+    hipLaunchKernelGGL(k_sell_values<double>, g, b, 0, s, row_ptr, col_for_values, n, ns,
+                       slice_ptr, (const double*)v, (double*)sv, pat, slice_tag);
   else
     TC_CHECK(false, TC_ERR_ARG, "bad dtype %d", dtype);
   TC_LAUNCH();

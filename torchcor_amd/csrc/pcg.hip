@@ -21,7 +21,7 @@ int pcg_solve(tc_pcg* P, const CI* sc, const T* b, T* x, const T* xp, T* xn, dou
               double r_tol, int max_iter, int lg, int* n_iter_host, hipStream_t s) {
   const int64_t n = P->n;
   const dim3 g(P->G), blk(kPcgBlock);
-  const int32_t* sp = P->A.slice_ptr;
+  const int32_t* sp = hot_slices(P->A);
   const T* Av = (const T*)P->Av;
   const T* D = (const T*)P->Dinv;
   T* r = (T*)P->r;
@@ -36,12 +36,12 @@ int pcg_solve(tc_pcg* P, const CI* sc, const T* b, T* x, const T* xp, T* xn, dou
   TC_LAUNCH();
   T* xs;
   if (lg) {
-    launch(k_pcg_init<T, true, CI>, g, blk, s, ev(0), ev(1), n, sp, sc, Av, D, b, (const T*)x, xp,
-           xn, r, P->ctl, a_tol, r_tol, P->part[0]);
+    launch(k_pcg_init<T, true, CI>, g, blk, s, ev(0), ev(1), n, sp, sc, P->A.pat, Av, D, b,
+           (const T*)x, xp, xn, r, P->ctl, a_tol, r_tol, P->part[0]);
     xs = xn;
   } else {
-    launch(k_pcg_init<T, false, CI>, g, blk, s, ev(0), ev(1), n, sp, sc, Av, D, b, (const T*)x,
-           xp, (T*)nullptr, r, P->ctl, a_tol, r_tol, P->part[0]);
+    launch(k_pcg_init<T, false, CI>, g, blk, s, ev(0), ev(1), n, sp, sc, P->A.pat, Av, D, b,
+           (const T*)x, xp, (T*)nullptr, r, P->ctl, a_tol, r_tol, P->part[0]);
     xs = x;
   }
   TC_LAUNCH();
@@ -56,7 +56,7 @@ int pcg_solve(tc_pcg* P, const CI* sc, const T* b, T* x, const T* xp, T* xn, dou
       const int k = it + j;
       T* pin = pp[k & 1];
       T* pout = pp[(k + 1) & 1];
-      launch(k_pcg_spmv<T, CI>, g, blk, s, ev(2 + 4 * k), ev(3 + 4 * k), n, sp, sc, Av, D,
+      launch(k_pcg_spmv<T, CI>, g, blk, s, ev(2 + 4 * k), ev(3 + 4 * k), n, sp, sc, P->A.pat, Av, D,
              (const T*)r, (const T*)pin, pout, Ap, k, P->ctl, (const double*)P->part[0],
              (const double*)P->part[2], P->part[1]);
       TC_LAUNCH();

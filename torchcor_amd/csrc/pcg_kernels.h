@@ -177,14 +177,74 @@ k_rhs(int64_t n, const int32_t* __restrict__ sptr, const int32_t* __restrict__ s
 // entry (a valid address) and are masked out of the sums.
 constexpr int kChunk = 4;
 
+// Column source of one slice: the lane's own entries 64 apart (per-row
+// columns), or the slice's shared pattern, one offset per entry position
+// and the same for every lane (a wave-uniform address, read through the
+// scalar cache).  A pattern position that a row lacks holds value 0 and may
+// point past the matrix at its edge, so its column is clamped to [0, hi].
+template <class CI>
+struct Cols {
+  const CI* p;
+  const int32_t* pat;  // the slice's pattern, or NULL for per-row columns
+  int32_t hi;
+};
+
+template <class CI>
+__device__ inline Cols<CI> per_row(const CI* p) {
+  return Cols<CI>{p, nullptr, 0};
+}
+
+// The slice table of the hot passes: slice_tag when the operator has
+// shared slices, else slice_ptr.
+__host__ __device__ inline const int32_t* hot_slices(const tc_sell& A) {
+  return A.slice_tag ? A.slice_tag : A.slice_ptr;
+}
+
+// Slice s of the hot passes' slice table: its entry offset b, width wd and
+// whether it is shared.  The table is slice_ptr, or slice_ptr with bit 0 set
+// on the shared slices (offsets are multiples of 64), so the flag costs no
+// load of its own.
+struct SliceHd {
+  int32_t b, wd;
+  int sh;
+};
+
+__device__ inline SliceHd slice_hd(const int32_t* __restrict__ sptr, int64_t s) {
+  const int32_t t0 = sptr[s], t1 = sptr[s + 1];
+  const int32_t b = t0 & ~1;
+  return SliceHd{b, ((t1 & ~1) - b) >> 6, t0 & 1};
+}
+
+// Column source of slice s (entries from b): its shared pattern when it has
+// one (int16 offsets only), else the lane's per-row columns.
+__device__ inline Cols<int32_t> slice_cols(const int32_t* scol, const int32_t*, int, int32_t b,
+                                           int lane, int64_t) {
+  return per_row(scol + b + lane);
+}
+
+__device__ inline Cols<int16_t> slice_cols(const int16_t* dcol, const int32_t* pat, int sh,
+                                           int32_t b, int lane, int64_t n) {
+  return Cols<int16_t>{dcol + b + lane, sh ? pat + (b >> 6) : nullptr, (int32_t)(n - 1)};
+}
+
 template <class CI, class T>
-__device__ inline void load_chunk(const CI* c, const T* v, int j0, int wd, int64_t i,
+__device__ inline void load_chunk(const Cols<CI>& c, const T* v, int j0, int wd, int64_t i,
                                   int32_t (&cj)[kChunk], T (&vj)[kChunk]) {
+  if (c.pat) {
 #pragma unroll
-  for (int u = 0; u < kChunk; ++u) {
-    const int jj = j0 + u < wd ? j0 + u : wd - 1;
-    cj[u] = col_of(c[jj * 64], i);
-    vj[u] = v[jj * 64];
+    for (int u = 0; u < kChunk; ++u) {
+      const int jj = j0 + u < wd ? j0 + u : wd - 1;
+      const int32_t x = (int32_t)i + c.pat[jj];
+      cj[u] = x < 0 ? 0 : (x > c.hi ? c.hi : x);
+      vj[u] = v[jj * 64];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+      const int jj = j0 + u < wd ? j0 + u : wd - 1;
+      cj[u] = col_of(c.p[jj * 64], i);
+      vj[u] = v[jj * 64];
+    }
   }
 }
 
@@ -261,6 +321,7 @@ __device__ inline bool stop_rule(double zn, const PcgCtl* ctl) {
 template <class T, bool LG, class CI>
 __global__ void __launch_bounds__(kPcgBlock)
 k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ scol,
+           const int32_t* __restrict__ pat,
            const T* __restrict__ Av, const T* __restrict__ Dinv, const T* __restrict__ bvec,
            const T* __restrict__ x, const T* __restrict__ xp, T* __restrict__ xnext,
            T* __restrict__ r, PcgCtl* ctl, double a_tol, double r_tol, double* part0) {
@@ -270,8 +331,9 @@ k_pcg_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
   double red[2] = {0.0, 0.0};
   for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
-    const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
-    const CI* c = scol + b + lane;
+    const SliceHd hd = slice_hd(sptr, s);
+    const int32_t b = hd.b, wd = hd.wd;
+    const Cols<CI> c = slice_cols(scol, pat, hd.sh, b, lane, n);
     const T* a = Av + b + lane;
     double ax = 0.0;
     for (int j0 = 0; j0 < wd; j0 += kChunk) {
@@ -317,7 +379,7 @@ template <class T, class CI>
 struct SpmvChunk {
   int32_t cj[kChunk];
   T av[kChunk], dj[kChunk], rj[kChunk], qj[kChunk];
-  __device__ inline void fetch(const CI* c, const T* a, int j0, int wd, int64_t i,
+  __device__ inline void fetch(const Cols<CI>& c, const T* a, int j0, int wd, int64_t i,
                                const T* __restrict__ Dinv, const T* __restrict__ r,
                                const T* __restrict__ pin, int k) {
     load_chunk(c, a, j0, wd, i, cj, av);
@@ -348,6 +410,7 @@ struct SpmvChunk {
 template <class T, class CI>
 __global__ void __launch_bounds__(kPcgBlock)
 k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ scol,
+           const int32_t* __restrict__ pat,
            const T* __restrict__ Av, const T* __restrict__ Dinv, const T* __restrict__ r,
            const T* __restrict__ pin, T* __restrict__ pout, T* __restrict__ Ap, int k,
            PcgCtl* ctl, const double* __restrict__ part0, const double* __restrict__ part2,
@@ -358,10 +421,15 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
   int64_t s = sp.s0 + wv;
   SpmvChunk<T, CI> ch;
   int32_t b = 0, wd = 0;
+  int sh = 0;
   if (s < sp.s1) {
-    b = sptr[s];
-    wd = (sptr[s + 1] - b) >> 6;
-    if (wd > 0) ch.fetch(scol + b + lane, Av + b + lane, 0, wd, s * 64 + lane, Dinv, r, pin, k);
+    const SliceHd hd = slice_hd(sptr, s);
+    b = hd.b;
+    wd = hd.wd;
+    sh = hd.sh;
+    if (wd > 0)
+      ch.fetch(slice_cols(scol, pat, sh, b, lane, n), Av + b + lane, 0, wd, s * 64 + lane, Dinv, r,
+               pin, k);
   }
   // the flags and the partial sums are read before the first wait
   __shared__ int halted;
@@ -397,10 +465,12 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
   for (; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
     if (!first) {
-      b = sptr[s];
-      wd = (sptr[s + 1] - b) >> 6;
+      const SliceHd hd = slice_hd(sptr, s);
+      b = hd.b;
+      wd = hd.wd;
+      sh = hd.sh;
     }
-    const CI* c = scol + b + lane;
+    const Cols<CI> c = slice_cols(scol, pat, sh, b, lane, n);
     const T* a = Av + b + lane;
     double acc = 0.0;
     for (int j0 = 0; j0 < wd; j0 += kChunk) {
@@ -536,8 +606,8 @@ struct RhsChunk {
   XT xj[kChunk];
   WT wj[kChunk];
   T pj[kChunk];
-  __device__ inline void fetch(const CI* c, const T* m, const T* kk, int j0, int wd, int64_t i,
-                               const XT* __restrict__ x, const WT* __restrict__ w,
+  __device__ inline void fetch(const Cols<CI>& c, const T* m, const T* kk, int j0, int wd,
+                               int64_t i, const XT* __restrict__ x, const WT* __restrict__ w,
                                const T* __restrict__ xp) {
     load_chunk(c, m, j0, wd, i, cj, mv);
 #pragma unroll
@@ -554,6 +624,7 @@ struct RhsChunk {
 template <class T, class XT, class WT, bool LG, class CI>
 __global__ void __launch_bounds__(kPcgBlock)
 k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ scol,
+           const int32_t* __restrict__ pat,
            const T* __restrict__ Mv, const T* __restrict__ Kv, const T* __restrict__ Dinv,
            const WT* __restrict__ w, const XT* __restrict__ x, const T* __restrict__ xp,
            XT* __restrict__ xnext, T* __restrict__ xpnext, T* __restrict__ r, double coef,
@@ -566,10 +637,15 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
   int64_t s = sp.s0 + wv;
   RhsChunk<T, XT, WT, LG, CI> ch;
   int32_t b = 0, wd = 0;
+  int sh = 0;
   if (s < sp.s1) {
-    b = sptr[s];
-    wd = (sptr[s + 1] - b) >> 6;
-    if (wd > 0) ch.fetch(scol + b + lane, Mv + b + lane, Kv + b + lane, 0, wd, s * 64 + lane, x, w, xp);
+    const SliceHd hd = slice_hd(sptr, s);
+    b = hd.b;
+    wd = hd.wd;
+    sh = hd.sh;
+    if (wd > 0)
+      ch.fetch(slice_cols(scol, pat, sh, b, lane, n), Mv + b + lane, Kv + b + lane, 0, wd,
+               s * 64 + lane, x, w, xp);
   }
   {
     __shared__ int st;
@@ -582,10 +658,12 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
   for (; s < sp.s1; s += kWaves) {
     const int64_t i = s * 64 + lane;
     if (!first) {
-      b = sptr[s];
-      wd = (sptr[s + 1] - b) >> 6;
+      const SliceHd hd = slice_hd(sptr, s);
+      b = hd.b;
+      wd = hd.wd;
+      sh = hd.sh;
     }
-    const CI* c = scol + b + lane;
+    const Cols<CI> c = slice_cols(scol, pat, sh, b, lane, n);
     const T* m = Mv + b + lane;
     const T* kk = Kv + b + lane;
     double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;

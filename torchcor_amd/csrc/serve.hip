@@ -225,7 +225,7 @@ __device__ inline void sv_rhs_rows(const SvBlk& bk, const SvEnt* __restrict__ en
   if (li < bk.l1) {
     e = ent[li];
     if (e.w > 0)
-      ch.fetch(cols + e.off + lane, Mv + e.off + lane, Kv + e.off + lane, 0, e.w,
+      ch.fetch(per_row(cols + e.off + lane), Mv + e.off + lane, Kv + e.off + lane, 0, e.w,
                (int64_t)e.sl * 64 + lane, x, w, xp);
   }
   __syncthreads();
@@ -234,7 +234,7 @@ __device__ inline void sv_rhs_rows(const SvBlk& bk, const SvEnt* __restrict__ en
   for (; li < bk.l1; li += kSW) {
     if (!first) e = ent[li];
     const int64_t i = (int64_t)e.sl * 64 + lane;
-    const CI* c = cols + e.off + lane;
+    const Cols<CI> c = per_row(cols + e.off + lane);
     const T* m = Mv + e.off + lane;
     const T* kk = Kv + e.off + lane;
     double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
@@ -333,8 +333,8 @@ __device__ inline void sv_spmv_block(const SvPool& pl, const SvBlk& bk, int vb,
   if (li < bk.l1) {
     e = pl.ent[li];
     if (e.w > 0)
-      ch.fetch(cols + e.off + lane, Av + e.off + lane, 0, e.w, (int64_t)e.sl * 64 + lane, Dinv, r,
-               pin, k);
+      ch.fetch(per_row(cols + e.off + lane), Av + e.off + lane, 0, e.w, (int64_t)e.sl * 64 + lane,
+               Dinv, r, pin, k);
   }
   double a = 0.0, b = 0.0;
   if (bk.q >= 0) fold_range_block(k == 0 ? P0 : P2, 2, bk.vb0, bk.vb1, a, b);
@@ -380,7 +380,7 @@ __device__ inline void sv_spmv_block(const SvPool& pl, const SvBlk& bk, int vb,
   for (; li < bk.l1; li += kSW) {
     if (!first) e = pl.ent[li];
     const int64_t i = (int64_t)e.sl * 64 + lane;
-    const CI* c = cols + e.off + lane;
+    const Cols<CI> c = per_row(cols + e.off + lane);
     const T* av = Av + e.off + lane;
     double acc = 0.0;
     for (int j0 = 0; j0 < e.w; j0 += kChunk) {

@@ -124,7 +124,8 @@ int ionic_launch(tc_sim* S, const tc_ionic_io* io, hipStream_t s, hipEvent_t e0,
 template <class T, class CI>
 void launch_spmv(tc_pcg* P, const CI* sc, T* pin, T* pout, int k, hipStream_t s, hipEvent_t e0,
                  hipEvent_t e1) {
-  launch(k_pcg_spmv<T, CI>, dim3(P->G), dim3(kPcgBlock), s, e0, e1, P->n, P->A.slice_ptr, sc,
+  launch(k_pcg_spmv<T, CI>, dim3(P->G), dim3(kPcgBlock), s, e0, e1, P->n, hot_slices(P->A), sc,
+         P->A.pat,
          (const T*)P->Av, (const T*)P->Dinv, (const T*)P->r, pin, pout, (T*)P->Ap, k, P->ctl,
          (const double*)P->part[0], (const double*)P->part[2], P->part[1]);
 }
@@ -216,8 +217,7 @@ void rhs_init_launch(tc_sim* S, const CI* sc, int xc, int pc, double coef, doubl
                      dim3 blk, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const tc_sim_desc& d = S->d;
   tc_pcg* P = d.pcg;
-  launch(k_rhs_init<T, XT, WT, LG, CI>, g, blk, s, e0, e1, d.n, (const int32_t*)P->A.slice_ptr,
-         sc, (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const WT*)d.w,
+  launch(k_rhs_init<T, XT, WT, LG, CI>, g, blk, s, e0, e1, d.n, hot_slices(P->A), sc, P->A.pat, (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const WT*)d.w,
          (const XT*)d.x[xc], LG ? (const T*)d.xp[pc] : (const T*)nullptr,
          LG ? (XT*)d.x[1 - xc] : (XT*)nullptr, LG ? (T*)d.xp[1 - pc] : (T*)nullptr, (T*)P->r,
          coef, d.Cm, tdt, P->ctl, d.a_tol, d.r_tol, P->part[0]);
