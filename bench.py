@@ -31,26 +31,43 @@ METRIC = "node-timesteps/s + PCG HBM GB/s (1M-node TTP); sims/hour & p99 lat bat
 HBM_PEAK_GBPS = 8000.0
 
 
-def alg_bytes(n, nnz, S, ib=4, xb=4, wb=4):
+def alg_bytes(n, nnz, S, col_bytes, xb=4, wb=4):
     """Algorithmic HBM bytes per launch of each kernel (each array read or
-    written once; slice pointers ignored): float32 values and states, `ib`-byte
-    column indices (2 with int16 offsets), `xb`-byte iterate x and `wb`-byte w
-    (8 in float32 runs with the float64 iterate and right-hand side)."""
+    written once; slice pointers ignored): float32 values and states,
+    `col_bytes` of column data per SELL pass (see column_bytes), `xb`-byte
+    iterate x and `wb`-byte w (8 in float32 runs with the float64 iterate and
+    right-hand side)."""
     return {
         # V (= x) and S states in; S states, w out; AT, RT read
         "ionic": n * (xb + 8 * S + wb + 8),
         # M, K values + columns; w, x, x_prev, Dinv in; x_next, x_prev_next, r out
-        "rhs_init": (8 + ib) * nnz + n * (wb + xb + 4 + 4 + xb + 4 + 4),
+        "rhs_init": 8 * nnz + col_bytes + n * (wb + xb + 4 + 4 + xb + 4 + 4),
         # A values + columns; r, Dinv, p in; p', Ap out
-        "pcg_spmv": (4 + ib) * nnz + 20 * n,
+        "pcg_spmv": 4 * nnz + col_bytes + 20 * n,
         # p', Ap, Dinv, r, x in; x, r out
         "pcg_update": n * (12 + 4 + xb + xb + 4),
     }
 
 
+def column_bytes(system):
+    """Column data one SELL pass reads: the int16 (or int32) offsets of the
+    rows in per-row slices, and one int32 pattern entry per entry position of
+    each shared slice (whose rows share one stencil)."""
+    import torch
+    ib = 2 if system.sell.dcol is not None else 4
+    deg = system.row_ptr.diff().to(torch.int64)
+    if system.sell.dcol is None or system.slice_tag is None:
+        return ib * int(deg.sum().item())
+    ns = (system.n + 63) // 64
+    sh = (system.slice_tag[:ns] & 1).to(torch.bool)
+    row_sh = sh.repeat_interleave(64)[: system.n]
+    width = system.slice_ptr.diff().to(torch.int64) // 64
+    return ib * int(deg[~row_sh].sum().item()) + 4 * int(width[sh].sum().item())
+
+
 # Memory bytes per dispatch of each kernel on the default workload (101^3 TTP
-# cube, float32 storage, float64 iterate and right-hand side, int16 column
-# offsets): what the L2 caches fetch from and write to the memory fabric
+# cube, float32 storage, float64 iterate and right-hand side, shared slices
+# for nearly all of the cube's rows): what the L2 caches fetch from and write to the memory fabric
 # (HBM, or the last-level cache in front of it), from two rocprofv3 --pmc
 # runs of this script (FETCH_SIZE, then
 # WRITE_SIZE, median KiB per dispatch) scaled by what the same counters read
@@ -58,16 +75,18 @@ def alg_bytes(n, nnz, S, ib=4, xb=4, wb=4):
 # WRITE_SIZE x1.0.  profiles/cube101_ttp_step_hbm_traffic.md holds the
 # counter summary and the commands.
 MEASURED_TRAFFIC_101 = {
-    "ionic": 173_180_357,
-    "rhs_init": 204_172_882,
-    "pcg_spmv": 112_206_273,
-    "pcg_update": 37_284_347,
+    "ionic": 173_180_373,
+    "rhs_init": 174_771_430,
+    "pcg_spmv": 82_715_612,
+    "pcg_update": 37_295_610,
 }
 
 
 def survey_electric_bytes(n, nnz):
-    """Electric-phase algorithmic bytes of the unfused formulation
-    (B_rhs, B_init, B_iter): the definition behind 'PCG HBM GB/s'."""
+    """Electric-phase algorithmic bytes of the unfused formulation with int32
+    indices (B_rhs, B_init, B_iter): the definition behind 'PCG HBM GB/s'.
+    The fused passes move fewer bytes than this (pcg_moved_GBps counts
+    those), so the rate by this definition can exceed the HBM peak."""
     return (12 * nnz + 4 * (n + 1) + 12 * n, 8 * nnz + 4 * (n + 1) + 32 * n,
             8 * nnz + 4 * (n + 1) + 48 * n)
 
@@ -391,8 +410,8 @@ def main():
     value = n * args.steps / elapsed
     kt = {} if args.no_kernel_timing else sim.kernel_times()
     S = len(sim.ionic_model.STATES)
-    ib = 2 if sim.system.sell.dcol is not None else 4
-    B = alg_bytes(n, nnz, S, ib=ib, xb=sim._X[0].element_size(), wb=sim._w.element_size())
+    B = alg_bytes(n, nnz, S, column_bytes(sim.system), xb=sim._X[0].element_size(),
+                  wb=sim._w.element_size())
     kernels = {}
     for k, (ms, cnt) in kt.items():
         if cnt == 0:
@@ -402,7 +421,7 @@ def main():
         if k in B:
             kernels[k].update(alg_bytes=B[k], GBps=round(B[k] / avg_s / 1e9, 1))
     roof = None
-    pcg_rate = None
+    pcg_rate = pcg_moved = None
     if kernels:
         dom = max((k for k in kernels if k in B), key=lambda k: kernels[k]["total_ms"])
         a = kernels[dom]["GBps"]
@@ -417,6 +436,10 @@ def main():
         ele_s = sum(kernels[k]["total_ms"] for k in ("rhs_init", "pcg_spmv", "pcg_update",
                                                      "pcg_check") if k in kernels) / 1e3
         pcg_rate = round(ele_bytes / ele_s / 1e9, 1)
+        # the same phase by the bytes the fused passes actually move
+        moved = sum(kernels[k]["launches"] * B[k] for k in ("rhs_init", "pcg_spmv", "pcg_update")
+                    if k in kernels)
+        pcg_moved = round(moved / ele_s / 1e9, 1)
     # release the single-run workload's device memory before the next parts,
     # so that each part runs on a device holding only its own data
     n_elems = int(sim.elems.shape[0])
@@ -450,6 +473,7 @@ def main():
                                                             (args.warmup + args.steps) * 0.01],
                    "pcg_iterations_per_step": sum(iters) / max(len(iters), 1) + 1},
         "pcg_hbm_GBps": pcg_rate,
+        "pcg_moved_GBps": pcg_moved,
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
