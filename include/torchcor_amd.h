@@ -531,6 +531,8 @@ typedef struct tc_serve_desc {
   int32_t* slice_sys;        /* (cap_slices) owning system */
   int32_t* col;              /* (cap_entries) pool rows */
   int16_t* dcol;             /* (cap_entries) col - row, or NULL (int32 only) */
+  int32_t* pat;              /* (cap_entries/64) shared-slice patterns by entry
+                                offset / 64, or NULL (per-row columns only) */
   void* Mv;                  /* (cap_entries) dtype: M, K/Chi, A values */
   void* Kv;
   void* Av;
@@ -558,6 +560,8 @@ typedef struct tc_serve_system {
   const int32_t* slice_ptr;  /* the system's SELL-64 operator (tc_sell_*) */
   const int32_t* col;        /* its columns (0 .. n-1) */
   const int16_t* dcol;       /* its col - row as int16 (tc_sell_deltas), or NULL */
+  const int32_t* pat;        /* its shared-slice patterns and tagged slice table */
+  const int32_t* slice_tag;  /* (tc_sell_share), or NULL */
   const void* Mv;            /* SELL values of M, K/Chi, A and Dinv, dtype */
   const void* Kv;
   const void* Av;

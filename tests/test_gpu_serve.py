@@ -98,6 +98,25 @@ def test_serve_int32_columns_identical():
         assert torch.equal(a["u"], b["u"]) and a["iterations"] == b["iterations"]
 
 
+def test_serve_shared_slices_identical(monkeypatch):
+    """Systems whose slices share one stencil bring their patterns into the
+    pool, whose passes then read them (clamped to the system's own rows):
+    bit-identical to the same systems on per-row columns."""
+    from torchcor_amd.core.system import P1System
+    _say("[gpu] serve shared vs per-row slices")
+    reqs = _requests()
+    out = []
+    for flag in (True, False):
+        monkeypatch.setattr(P1System, "share_slices", flag)
+        eng = _engine(torch.float32)
+        prs = [eng.prepare(r) for r in reqs]
+        assert all((p.system.slice_tag is not None) == flag for p in prs)
+        out.append(eng.run_until_done(prs))
+    for a, b in zip(*out):
+        assert torch.equal(a["u"], b["u"]) and a["iterations"] == b["iterations"]
+        assert torch.equal(a["AT"], b["AT"]) and torch.equal(a["RT"], b["RT"])
+
+
 def test_serve_loop_with_arrivals_matches_batch_admission():
     """The real-time loop (arrivals, assembly on a worker thread and its own
     stream, admission as room frees up) gives each run bit-identical

@@ -113,8 +113,8 @@ class BatchDesc(C.Structure):
 class ServeDesc(C.Structure):
     _fields_ = [("dtype", i32), ("x_f64", i32), ("w_f64", i32), ("cap_slices", i64),
                 ("cap_entries", i64), ("max_sys", i32), ("slice_off", p), ("slice_w", p),
-                ("slice_sys", p), ("col", p), ("dcol", p), ("Mv", p), ("Kv", p), ("Av", p),
-                ("Dinv", p), ("x", p * 2), ("xp", p * 2), ("w", p), ("r", p), ("p", p * 2), ("Ap", p),
+                ("slice_sys", p), ("col", p), ("dcol", p), ("pat", p), ("Mv", p), ("Kv", p),
+                ("Av", p), ("Dinv", p), ("x", p * 2), ("xp", p * 2), ("w", p), ("r", p), ("p", p * 2), ("Ap", p),
                 ("AT", p), ("RT", p), ("stim_mask", p), ("states", p * 3), ("params", p * 3),
                 ("tab", (p * 3) * 3), ("dt", dbl), ("dt_ionic", dbl), ("Cm", dbl),
                 ("theta", dbl), ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32),
@@ -122,8 +122,8 @@ class ServeDesc(C.Structure):
 
 
 class ServeSystem(C.Structure):
-    _fields_ = [("n", i64), ("n_entries", i64), ("slice_ptr", p), ("col", p), ("dcol", p), ("Mv", p),
-                ("Kv", p), ("Av", p), ("Dinv", p), ("model", i32), ("n_states", i32),
+    _fields_ = [("n", i64), ("n_entries", i64), ("slice_ptr", p), ("col", p), ("dcol", p),
+                ("pat", p), ("slice_tag", p), ("Mv", p), ("Kv", p), ("Av", p), ("Dinv", p), ("model", i32), ("n_states", i32),
                 ("states", p), ("V0", p), ("stim_mask", p), ("n_stim", i32),
                 ("stim", Stimulus * POOL_STIM), ("lat", i32)]
 

@@ -181,17 +181,18 @@ constexpr int kChunk = 4;
 // columns), or the slice's shared pattern, one offset per entry position
 // and the same for every lane (a wave-uniform address, read through the
 // scalar cache).  A pattern position that a row lacks holds value 0 and may
-// point past the matrix at its edge, so its column is clamped to [0, hi].
+// point past the operator's rows at its edge, so its column is clamped to
+// [lo, hi], rows whose values are finite.
 template <class CI>
 struct Cols {
   const CI* p;
   const int32_t* pat;  // the slice's pattern, or NULL for per-row columns
-  int32_t hi;
+  int32_t lo, hi;
 };
 
 template <class CI>
 __device__ inline Cols<CI> per_row(const CI* p) {
-  return Cols<CI>{p, nullptr, 0};
+  return Cols<CI>{p, nullptr, 0, 0};
 }
 
 // The slice table of the hot passes: slice_tag when the operator has
@@ -224,7 +225,7 @@ __device__ inline Cols<int32_t> slice_cols(const int32_t* scol, const int32_t*, 
 
 __device__ inline Cols<int16_t> slice_cols(const int16_t* dcol, const int32_t* pat, int sh,
                                            int32_t b, int lane, int64_t n) {
-  return Cols<int16_t>{dcol + b + lane, sh ? pat + (b >> 6) : nullptr, (int32_t)(n - 1)};
+  return Cols<int16_t>{dcol + b + lane, sh ? pat + (b >> 6) : nullptr, 0, (int32_t)(n - 1)};
 }
 
 template <class CI, class T>
@@ -235,7 +236,7 @@ __device__ inline void load_chunk(const Cols<CI>& c, const T* v, int j0, int wd,
     for (int u = 0; u < kChunk; ++u) {
       const int jj = j0 + u < wd ? j0 + u : wd - 1;
       const int32_t x = (int32_t)i + c.pat[jj];
-      cj[u] = x < 0 ? 0 : (x > c.hi ? c.hi : x);
+      cj[u] = x < c.lo ? c.lo : (x > c.hi ? c.hi : x);
       vj[u] = v[jj * 64];
     }
   } else {

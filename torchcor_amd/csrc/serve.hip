@@ -99,12 +99,13 @@ struct SvBlk {
   int64_t row_end;      // first pool row past the system
   int32_t use16;        // the system's entries have int16 column offsets
   int32_t vb0, vb1;     // the system's block range
-  int32_t pad;
+  int32_t row0;         // first pool row of the system
 };
 
-// One entry of the slice list: the slice and its SELL position and width.
+// One entry of the slice list: the slice, its SELL position and width, and
+// whether it is laid out on a shared pattern (at pat[off / 64]).
 struct SvEnt {
-  int32_t sl, off, w, pad;
+  int32_t sl, off, w, shared;
 };
 
 // Live system q: its slot and block range.
@@ -123,7 +124,21 @@ struct SvPool {
   const int32_t* slice_sys;
   const int32_t* col;       // pool rows
   const int16_t* dcol;      // col - row, for systems with use16
+  const int32_t* pat;       // shared-slice patterns by entry offset / 64, or NULL
 };
+
+// Column source of list entry e of a block: per-row columns, or (int16
+// systems) the slice's shared pattern clamped to the system's own rows.
+__device__ inline Cols<int32_t> sv_cols(const int32_t* cols, const int32_t*, const SvEnt& e,
+                                        int lane, const SvBlk&) {
+  return per_row(cols + e.off + lane);
+}
+
+__device__ inline Cols<int16_t> sv_cols(const int16_t* cols, const int32_t* pat, const SvEnt& e,
+                                        int lane, const SvBlk& bk) {
+  return Cols<int16_t>{cols + e.off + lane, e.shared ? pat + (e.off >> 6) : nullptr, bk.row0,
+                       (int32_t)(bk.row_end - 1)};
+}
 
 // Partial sums are per block (one system each) at part[v * nv + k]; system
 // q's total is the sum over its blocks in block order, which every consumer
@@ -211,7 +226,8 @@ __device__ inline void store_block_partials(const BlockView& v, double r0, doubl
 // the stall flag (*s_stop), so the flag's latency overlaps those loads.
 template <class T, class XT, class WT, bool LG, class CI>
 __device__ inline void sv_rhs_rows(const SvBlk& bk, const SvEnt* __restrict__ ent,
-                                   const CI* __restrict__ cols, const T* __restrict__ Mv,
+                                   const CI* __restrict__ cols, const int32_t* __restrict__ pat,
+                                   const T* __restrict__ Mv,
                                    const T* __restrict__ Kv, const T* __restrict__ Dinv,
                                    const WT* __restrict__ w, const XT* __restrict__ x,
                                    const T* __restrict__ xp, XT* __restrict__ xnext,
@@ -225,7 +241,7 @@ __device__ inline void sv_rhs_rows(const SvBlk& bk, const SvEnt* __restrict__ en
   if (li < bk.l1) {
     e = ent[li];
     if (e.w > 0)
-      ch.fetch(per_row(cols + e.off + lane), Mv + e.off + lane, Kv + e.off + lane, 0, e.w,
+      ch.fetch(sv_cols(cols, pat, e, lane, bk), Mv + e.off + lane, Kv + e.off + lane, 0, e.w,
                (int64_t)e.sl * 64 + lane, x, w, xp);
   }
   __syncthreads();
@@ -234,7 +250,7 @@ __device__ inline void sv_rhs_rows(const SvBlk& bk, const SvEnt* __restrict__ en
   for (; li < bk.l1; li += kSW) {
     if (!first) e = ent[li];
     const int64_t i = (int64_t)e.sl * 64 + lane;
-    const Cols<CI> c = per_row(cols + e.off + lane);
+    const Cols<CI> c = sv_cols(cols, pat, e, lane, bk);
     const T* m = Mv + e.off + lane;
     const T* kk = Kv + e.off + lane;
     double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
@@ -287,10 +303,10 @@ k_sv_rhs_init(SvPool pl, int nsys, const T* __restrict__ Mv, const T* __restrict
   if (threadIdx.x == 0) s_stop = ctl->stall;
   double red0 = 0.0, red1 = 0.0;
   if (bk.use16)
-    sv_rhs_rows<T, XT, WT, LG, int16_t>(bk, pl.ent, pl.dcol, Mv, Kv, Dinv, w, x, xp, xnext,
+    sv_rhs_rows<T, XT, WT, LG, int16_t>(bk, pl.ent, pl.dcol, pl.pat, Mv, Kv, Dinv, w, x, xp, xnext,
                                         xpnext, r, coef, Cm, tdt, &s_stop, red0, red1);
   else
-    sv_rhs_rows<T, XT, WT, LG, int32_t>(bk, pl.ent, pl.col, Mv, Kv, Dinv, w, x, xp, xnext,
+    sv_rhs_rows<T, XT, WT, LG, int32_t>(bk, pl.ent, pl.col, nullptr, Mv, Kv, Dinv, w, x, xp, xnext,
                                         xpnext, r, coef, Cm, tdt, &s_stop, red0, red1);
   if (s_stop) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->n_active = nsys;
@@ -333,7 +349,7 @@ __device__ inline void sv_spmv_block(const SvPool& pl, const SvBlk& bk, int vb,
   if (li < bk.l1) {
     e = pl.ent[li];
     if (e.w > 0)
-      ch.fetch(per_row(cols + e.off + lane), Av + e.off + lane, 0, e.w, (int64_t)e.sl * 64 + lane,
+      ch.fetch(sv_cols(cols, pl.pat, e, lane, bk), Av + e.off + lane, 0, e.w, (int64_t)e.sl * 64 + lane,
                Dinv, r, pin, k);
   }
   double a = 0.0, b = 0.0;
@@ -380,7 +396,7 @@ __device__ inline void sv_spmv_block(const SvPool& pl, const SvBlk& bk, int vb,
   for (; li < bk.l1; li += kSW) {
     if (!first) e = pl.ent[li];
     const int64_t i = (int64_t)e.sl * 64 + lane;
-    const Cols<CI> c = per_row(cols + e.off + lane);
+    const Cols<CI> c = sv_cols(cols, pl.pat, e, lane, bk);
     const T* av = Av + e.off + lane;
     double acc = 0.0;
     for (int j0 = 0; j0 < e.w; j0 += kChunk) {
@@ -486,7 +502,8 @@ __global__ void k_sv_ent(int64_t nl, const int32_t* __restrict__ list,
   const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= nl) return;
   const int32_t sl = list[li];
-  ent[li] = SvEnt{sl, slice_off[sl], slice_w[sl], 0};
+  const int32_t o = slice_off[sl];
+  ent[li] = SvEnt{sl, o & ~1, slice_w[sl], o & 1};
 }
 
 // One block: end of a launch chunk after k_end iterations.  The stopping
@@ -653,11 +670,13 @@ __global__ void k_sv_place_entries(int64_t ne, int64_t entry0, int64_t row0,
 }
 
 __global__ void k_sv_place_slices(int64_t ns, int64_t slice0, int64_t entry0, int sys,
-                                  const int32_t* __restrict__ sptr_s, int32_t* slice_off,
+                                  const int32_t* __restrict__ sptr_s,
+                                  const int32_t* __restrict__ tag_s, int32_t* slice_off,
                                   int32_t* slice_w, int32_t* slice_sys) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
-  slice_off[slice0 + s] = (int32_t)(entry0 + sptr_s[s]);
+  // bit 0: the slice is laid out on a shared pattern (offsets are multiples of 64)
+  slice_off[slice0 + s] = (int32_t)(entry0 + sptr_s[s]) | (tag_s ? (tag_s[s] & 1) : 0);
   slice_w[slice0 + s] = (sptr_s[s + 1] - sptr_s[s]) / 64;
   slice_sys[slice0 + s] = sys;
 }
@@ -822,6 +841,7 @@ SvPool sv_pool(const tc_serve* S) {
   p.slice_sys = S->d.slice_sys;
   p.col = S->d.col;
   p.dcol = S->d.dcol;
+  p.pat = S->d.pat;
   return p;
 }
 
@@ -875,7 +895,7 @@ int sv_rebuild(tc_serve* S, hipStream_t s) {
     k.use16 = h && h->use16 ? 1 : 0;
     k.vb0 = k.q >= 0 ? vb0[k.q] : 0;
     k.vb1 = k.q >= 0 ? vb1[k.q] : -1;
-    k.pad = 0;
+    k.row0 = h ? (int32_t)(h->slice0 * 64) : 0;
   }
   int rc;
   if ((rc = up(S->list, list))) return rc;
@@ -1268,10 +1288,15 @@ extern "C" int tc_serve_add(tc_serve* S, const tc_serve_system* y, int* slot_out
   const int64_t row0 = s0 * 64, nrows = ns * 64;
   const int f64 = d.dtype == TC_F64;
   const int xf = d.x_f64 || f64;
+  // slices on shared patterns: int16 systems whose patterns the pool keeps
+  const bool shared = y->dcol && d.dcol && d.pat && y->pat && y->slice_tag && e0 % 64 == 0;
   hipLaunchKernelGGL(k_sv_place_slices, dim3((int)ceil_div(ns, 256)), dim3(256), 0, s, ns, s0, e0,
-                     slot, y->slice_ptr, (int32_t*)d.slice_off, (int32_t*)d.slice_w,
-                     (int32_t*)d.slice_sys);
+                     slot, y->slice_ptr, shared ? y->slice_tag : nullptr, (int32_t*)d.slice_off,
+                     (int32_t*)d.slice_w, (int32_t*)d.slice_sys);
   TC_LAUNCH();
+  if (shared)
+    TC_HIP(hipMemcpyAsync(d.pat + e0 / 64, y->pat, (size_t)(y->n_entries / 64) * sizeof(int32_t),
+                          hipMemcpyDeviceToDevice, s));
   const dim3 ge((int)ceil_div(y->n_entries, 256)), gr((int)ceil_div(nrows, 256)), b(256);
   if (f64) {
     hipLaunchKernelGGL(k_sv_place_entries<double>, ge, b, 0, s, y->n_entries, e0, row0, y->col,

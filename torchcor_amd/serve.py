@@ -79,6 +79,7 @@ class ServeEngine:
             slice_off=z(cap_slices, torch.int32), slice_w=z(cap_slices, torch.int32),
             slice_sys=z(cap_slices, torch.int32), col=z(cap_entries, torch.int32),
             dcol=z(cap_entries, torch.int16) if int16_columns else None,
+            pat=z(max(int(cap_entries) // 64, 1), torch.int32) if int16_columns else None,
             Mv=z(cap_entries, dtype), Kv=z(cap_entries, dtype), Av=z(cap_entries, dtype),
             Dinv=z(rows, dtype), x0=z(rows, x_dtype), x1=z(rows, x_dtype), xp0=z(rows, dtype),
             xp1=z(rows, dtype), w=z(rows, torch.float64 if w_f64 else dtype), r=z(rows, dtype),
@@ -97,8 +98,8 @@ class ServeEngine:
         d.x_f64 = 1 if x_dtype == torch.float64 else 0
         d.w_f64 = 1 if w_f64 else 0
         d.cap_slices, d.cap_entries, d.max_sys = cap_slices, int(cap_entries), int(max_sys)
-        for k in ("slice_off", "slice_w", "slice_sys", "col", "dcol", "Mv", "Kv", "Av", "Dinv", "w", "r",
-                  "Ap", "AT", "RT", "stim_mask"):
+        for k in ("slice_off", "slice_w", "slice_sys", "col", "dcol", "pat", "Mv", "Kv", "Av", "Dinv",
+                  "w", "r", "Ap", "AT", "RT", "stim_mask"):
             setattr(d, k, L.ptr(P[k]))
         d.x[0], d.x[1] = L.ptr(P["x0"]), L.ptr(P["x1"])
         d.xp[0], d.xp[1] = L.ptr(P["xp0"]), L.ptr(P["xp1"])
@@ -173,6 +174,8 @@ class ServeEngine:
         y.n, y.n_entries = pr.n, pr.n_entries
         y.slice_ptr, y.col = L.ptr(S.slice_ptr), L.ptr(S.scol)
         y.dcol = L.ptr(S.dcol) if S.sell.dcol is not None else None
+        if S.sell.dcol is not None and S.slice_tag is not None:
+            y.pat, y.slice_tag = L.ptr(S.pat), L.ptr(S.slice_tag)
         y.Mv, y.Kv, y.Av, y.Dinv = L.ptr(S.sM), L.ptr(S.sK), L.ptr(S.sA), L.ptr(S.Dinv)
         y.model = pr.req.model
         y.n_states = pr.states.shape[0]
