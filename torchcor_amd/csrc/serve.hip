@@ -87,7 +87,7 @@ __device__ inline LRange block_list(int64_t nl) {
 // partial sums group the same way whatever else is in the pool (a run gives
 // bit-identical results alone or in a mix), and a fold reads at most
 // kSBlkMax partials (one per lane).
-constexpr int kSPer = 8;
+constexpr int kSPer = 16;
 constexpr int kSBlkMax = 1024;
 
 // One block's work, formed on the host when the lists are rebuilt: a
