@@ -75,9 +75,9 @@ def column_bytes(system):
 # WRITE_SIZE x1.0.  profiles/cube101_ttp_step_hbm_traffic.md holds the
 # counter summary and the commands.
 MEASURED_TRAFFIC_101 = {
-    "ionic": 173_180_373,
-    "rhs_init": 174_771_430,
-    "pcg_spmv": 82_715_612,
+    "ionic": 173_180_357,
+    "rhs_init": 174_625_266,
+    "pcg_spmv": 82_717_404,
     "pcg_update": 37_295_610,
 }
 
