@@ -292,7 +292,9 @@ __device__ inline void sv_rhs_rows(const SvBlk& bk, const SvEnt* __restrict__ en
 // of r.z and (Dinv b)^2.
 template <class T, class XT, class WT, bool LG>
 __global__ void __launch_bounds__(kSB)
-k_sv_rhs_init(SvPool pl, int nsys, const T* __restrict__ Mv, const T* __restrict__ Kv,
+k_sv_rhs_init(SvPool pl, const SvEnt* __restrict__ ent, const int32_t* __restrict__ pat,
+              const int16_t* __restrict__ dcol, const int32_t* __restrict__ col, int nsys,
+              const T* __restrict__ Mv, const T* __restrict__ Kv,
               const T* __restrict__ Dinv, const WT* __restrict__ w, const XT* __restrict__ x,
               const T* __restrict__ xp, XT* __restrict__ xnext, T* __restrict__ xpnext,
               T* __restrict__ r, double coef, double Cm, double tdt, SvCtl* ctl,
@@ -303,10 +305,10 @@ k_sv_rhs_init(SvPool pl, int nsys, const T* __restrict__ Mv, const T* __restrict
   if (threadIdx.x == 0) s_stop = ctl->stall;
   double red0 = 0.0, red1 = 0.0;
   if (bk.use16)
-    sv_rhs_rows<T, XT, WT, LG, int16_t>(bk, pl.ent, pl.dcol, pl.pat, Mv, Kv, Dinv, w, x, xp, xnext,
+    sv_rhs_rows<T, XT, WT, LG, int16_t>(bk, ent, dcol, pat, Mv, Kv, Dinv, w, x, xp, xnext,
                                         xpnext, r, coef, Cm, tdt, &s_stop, red0, red1);
   else
-    sv_rhs_rows<T, XT, WT, LG, int32_t>(bk, pl.ent, pl.col, nullptr, Mv, Kv, Dinv, w, x, xp, xnext,
+    sv_rhs_rows<T, XT, WT, LG, int32_t>(bk, ent, col, nullptr, Mv, Kv, Dinv, w, x, xp, xnext,
                                         xpnext, r, coef, Cm, tdt, &s_stop, red0, red1);
   if (s_stop) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->n_active = nsys;
@@ -321,82 +323,81 @@ k_sv_rhs_init(SvPool pl, int nsys, const T* __restrict__ Mv, const T* __restrict
 // p'.Ap'.  The flags, the partial sums and the first chunk of each
 // wavefront's first slice are all loaded before the first wait.
 template <class T, class CI>
-__device__ inline void sv_spmv_block(const SvPool& pl, const SvBlk& bk, int vb,
+__device__ inline void sv_spmv_block(const SvBlk& bk, int vb, const SvEnt* __restrict__ ent,
+                                     const int32_t* __restrict__ pat,
                                      const CI* __restrict__ cols, const T* __restrict__ Av,
                                      const T* __restrict__ Dinv, const T* __restrict__ r,
                                      const T* __restrict__ pin, T* __restrict__ pout,
                                      T* __restrict__ Ap, int k, int max_sys, SvCtl* ctl,
+                                     double* rz2, double* bnorm, int* active, int* n_iter,
                                      const double* __restrict__ P0,
                                      const double* __restrict__ P2, double* P1) {
-  __shared__ int s_stop, s_pre, s_act;
-  __shared__ double s_beta;
+  __shared__ int s_stop, s_pre;
+  __shared__ double s_sc[4];  // ||Dinv b||, previous r.z, a_tol, r_tol
   const int lane = threadIdx.x & 63, wv = wave_id();
   const int sys = bk.sys;
   // one read of the flags for the whole block (another block may clear the
-  // system's flag during this pass; the branches below must be uniform)
-  double bn = 1.0, rzp = 1.0;
+  // system's flag during this pass; the branches below must be uniform):
+  // thread 0 stages them in LDS before the fold's barrier, and every thread
+  // then takes the same decisions from the same values
   if (threadIdx.x == 0) {
     s_stop = ctl->stall != 0 || (k > 0 && ctl->n_active == 0);
-    s_pre = bk.q >= 0 ? ctl->active[sys] : 0;
+    s_pre = bk.q >= 0 ? active[sys] : 0;
+    double bn = 1.0, rzp = 1.0;
     if (k > 0 && bk.q >= 0) {
-      bn = ctl->bnorm[sys];
-      rzp = ctl->rz2[((k - 1) & 1) * max_sys + sys];
+      bn = bnorm[sys];
+      rzp = rz2[((k - 1) & 1) * max_sys + sys];
     }
+    s_sc[0] = bn;
+    s_sc[1] = rzp;
+    s_sc[2] = ctl->a_tol;
+    s_sc[3] = ctl->r_tol;
   }
   int64_t li = bk.l0 + wv;
   SpmvChunk<T, CI> ch;
   SvEnt e{0, 0, 0, 0};
   if (li < bk.l1) {
-    e = pl.ent[li];
+    e = ent[li];
     if (e.w > 0)
-      ch.fetch(sv_cols(cols, pl.pat, e, lane, bk), Av + e.off + lane, 0, e.w, (int64_t)e.sl * 64 + lane,
+      ch.fetch(sv_cols(cols, pat, e, lane, bk), Av + e.off + lane, 0, e.w, (int64_t)e.sl * 64 + lane,
                Dinv, r, pin, k);
   }
   double a = 0.0, b = 0.0;
   if (bk.q >= 0) fold_range_block(k == 0 ? P0 : P2, 2, bk.vb0, bk.vb1, a, b);
   else __syncthreads();
   if (s_stop || bk.q < 0) return;
-  const bool firstblk = bk.vb0 == vb;
-  if (threadIdx.x == 0) {
-    if (k == 0) {
-      s_act = 1;
-      s_beta = 0.0;
-      if (firstblk) {
-        ctl->rz2[sys] = a;
-        ctl->bnorm[sys] = sqrt(b);
-        ctl->active[sys] = 1;
-        ctl->n_iter[sys] = 0;
+  const bool lead = bk.vb0 == vb && threadIdx.x == 0;  // the system's first block records
+  double beta = 0.0;
+  if (k == 0) {
+    if (lead) {
+      rz2[sys] = a;
+      bnorm[sys] = sqrt(b);
+      active[sys] = 1;
+      n_iter[sys] = 0;
+    }
+  } else {
+    if (!s_pre) return;
+    const double zn = sqrt(a);
+    if (zn < s_sc[2] || zn / s_sc[0] < s_sc[3]) {
+      if (lead) {
+        active[sys] = 0;
+        n_iter[sys] = k - 1;
+        atomicSub(&ctl->n_active, 1);
       }
-    } else if (!s_pre) {
-      s_act = 0;
-    } else {
-      const double zn = sqrt(a);
-      if (zn < ctl->a_tol || zn / bn < ctl->r_tol) {
-        s_act = 0;
-        if (firstblk) {
-          ctl->active[sys] = 0;
-          ctl->n_iter[sys] = k - 1;
-          atomicSub(&ctl->n_active, 1);
-        }
-      } else {
-        s_act = 1;
-        s_beta = b / rzp;
-        if (firstblk) {
-          ctl->rz2[(k & 1) * max_sys + sys] = b;
-          ctl->n_iter[sys] = k;
-        }
-      }
+      return;
+    }
+    beta = b / s_sc[1];
+    if (lead) {
+      rz2[(k & 1) * max_sys + sys] = b;
+      n_iter[sys] = k;
     }
   }
-  __syncthreads();
-  if (!s_act) return;
-  const double beta = s_beta;
   double red = 0.0;
   bool first = true;
   for (; li < bk.l1; li += kSW) {
-    if (!first) e = pl.ent[li];
+    if (!first) e = ent[li];
     const int64_t i = (int64_t)e.sl * 64 + lane;
-    const Cols<CI> c = sv_cols(cols, pl.pat, e, lane, bk);
+    const Cols<CI> c = sv_cols(cols, pat, e, lane, bk);
     const T* av = Av + e.off + lane;
     double acc = 0.0;
     for (int j0 = 0; j0 < e.w; j0 += kChunk) {
@@ -419,43 +420,46 @@ __device__ inline void sv_spmv_block(const SvPool& pl, const SvBlk& bk, int vb,
 
 template <class T>
 __global__ void __launch_bounds__(kSB)
-k_sv_spmv(SvPool pl, const T* __restrict__ Av, const T* __restrict__ Dinv,
+k_sv_spmv(SvPool pl, const SvEnt* __restrict__ ent, const int32_t* __restrict__ pat,
+          const int16_t* __restrict__ dcol, const int32_t* __restrict__ col,
+          const T* __restrict__ Av, const T* __restrict__ Dinv,
           const T* __restrict__ r, const T* __restrict__ pin, T* __restrict__ pout,
-          T* __restrict__ Ap, int k, int max_sys, SvCtl* ctl, const double* __restrict__ P0,
+          T* __restrict__ Ap, int k, int max_sys, SvCtl* ctl, double* rz2, double* bnorm,
+          int* active, int* n_iter, const double* __restrict__ P0,
           const double* __restrict__ P2, double* P1) {
   const int vb = sv_vblock();
   const SvBlk bk = pl.blk[vb];
   if (bk.use16)
-    sv_spmv_block<T, int16_t>(pl, bk, vb, pl.dcol, Av, Dinv, r, pin, pout, Ap, k, max_sys, ctl,
-                              P0, P2, P1);
+    sv_spmv_block<T, int16_t>(bk, vb, ent, pat, dcol, Av, Dinv, r, pin, pout, Ap, k, max_sys,
+                              ctl, rz2, bnorm, active, n_iter, P0, P2, P1);
   else
-    sv_spmv_block<T, int32_t>(pl, bk, vb, pl.col, Av, Dinv, r, pin, pout, Ap, k, max_sys, ctl,
-                              P0, P2, P1);
+    sv_spmv_block<T, int32_t>(bk, vb, ent, nullptr, col, Av, Dinv, r, pin, pout, Ap, k, max_sys,
+                              ctl, rz2, bnorm, active, n_iter, P0, P2, P1);
 }
 
 // update pass of iteration k: alpha = rz / p'.Ap' per system, x += alpha p',
 // r -= alpha Ap', block partials of z.z and r.z.  The first slice's operands,
 // the flags, r.z and the partial sums load before the first wait.
 template <class T, class XT>
-__global__ void __launch_bounds__(kSB)
-k_sv_update(SvPool pl, const T* __restrict__ Dinv, const T* __restrict__ p,
-            const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, int k, int max_sys,
-            SvCtl* ctl, const double* __restrict__ P1, double* P2) {
+__device__ inline void sv_update_block(const SvBlk& bk, int vb, const SvEnt* __restrict__ ent,
+                                       const T* __restrict__ Dinv, const T* __restrict__ p,
+                                       const T* __restrict__ Ap, XT* __restrict__ x,
+                                       T* __restrict__ r, int k, int max_sys, SvCtl* ctl,
+                                       const double* rz2, const int* active,
+                                       const double* __restrict__ P1, double* P2) {
   __shared__ int s_stop, s_pre;
   const int lane = threadIdx.x & 63, wv = wave_id();
-  const int vb = sv_vblock();
-  const SvBlk bk = pl.blk[vb];
   if (threadIdx.x == 0) {
     s_stop = ctl->stall != 0 || ctl->n_active == 0;
-    s_pre = bk.q >= 0 ? ctl->active[bk.sys] : 0;
+    s_pre = bk.q >= 0 ? active[bk.sys] : 0;
   }
-  const double rzk = bk.q >= 0 ? ctl->rz2[(k & 1) * max_sys + bk.sys] : 1.0;
+  const double rzk = bk.q >= 0 ? rz2[(k & 1) * max_sys + bk.sys] : 1.0;
   int64_t li = bk.l0 + wv;
   int64_t i = 0;
   XT xv = XT(0);
   T pv = T(0), av = T(0), rv = T(0), dv = T(0);
   if (li < bk.l1) {
-    i = (int64_t)pl.ent[li].sl * 64 + lane;
+    i = (int64_t)ent[li].sl * 64 + lane;
     if (i < bk.row_end) {
       xv = x[i];
       pv = p[i];
@@ -482,7 +486,7 @@ k_sv_update(SvPool pl, const T* __restrict__ Dinv, const T* __restrict__ p,
     }
     li += kSW;
     if (li < bk.l1) {
-      i = (int64_t)pl.ent[li].sl * 64 + lane;
+      i = (int64_t)ent[li].sl * 64 + lane;
       if (i < bk.row_end) {
         xv = x[i];
         pv = p[i];
@@ -493,6 +497,18 @@ k_sv_update(SvPool pl, const T* __restrict__ Dinv, const T* __restrict__ p,
     }
   }
   store_block_partials(BlockView{vb, bk.l0, bk.l1, bk.q}, red0, red1, 2, P2);
+}
+
+template <class T, class XT>
+__global__ void __launch_bounds__(kSB)
+k_sv_update(SvPool pl, const SvEnt* __restrict__ ent, const T* __restrict__ Dinv,
+            const T* __restrict__ p,
+            const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, int k, int max_sys,
+            SvCtl* ctl, const double* rz2, const int* active, const double* __restrict__ P1,
+            double* P2) {
+  const int vb = sv_vblock();
+  sv_update_block<T, XT>(pl.blk[vb], vb, ent, Dinv, p, Ap, x, r, k, max_sys, ctl, rz2, active,
+                         P1, P2);
 }
 
 // The list entries' slice positions (after the lists change).
@@ -972,12 +988,17 @@ int sv_iters(tc_serve* S, int k0, int k1, XT* xs, hipStream_t s) {
   for (int k = k0; k < k1; ++k) {
     T* pin = pp[k & 1];
     T* pout = pp[(k + 1) & 1];
-    hipLaunchKernelGGL(k_sv_spmv<T>, g, blk, 0, s, pl, (const T*)d.Av, (const T*)d.Dinv,
+    hipLaunchKernelGGL(k_sv_spmv<T>, g, blk, 0, s, pl, (const SvEnt*)pl.ent,
+                       (const int32_t*)pl.pat, (const int16_t*)pl.dcol, (const int32_t*)pl.col,
+                       (const T*)d.Av, (const T*)d.Dinv,
                        (const T*)d.r, (const T*)pin, pout, (T*)d.Ap, k, d.max_sys, S->ctl,
+                       S->hctl.rz2, S->hctl.bnorm, S->hctl.active, S->hctl.n_iter,
                        (const double*)S->part[0], (const double*)S->part[2], S->part[1]);
     TC_LAUNCH();
-    hipLaunchKernelGGL((k_sv_update<T, XT>), g, blk, 0, s, pl, (const T*)d.Dinv, (const T*)pout,
+    hipLaunchKernelGGL((k_sv_update<T, XT>), g, blk, 0, s, pl, (const SvEnt*)pl.ent,
+                       (const T*)d.Dinv, (const T*)pout,
                        (const T*)d.Ap, xs, (T*)d.r, k, d.max_sys, S->ctl,
+                       (const double*)S->hctl.rz2, (const int*)S->hctl.active,
                        (const double*)S->part[1], S->part[2]);
     TC_LAUNCH();
   }
@@ -1025,13 +1046,17 @@ int sv_enqueue(tc_serve* S, SPlan& plan, hipStream_t s) {
   const double coef = (1.0 - d.theta) * d.dt, tdt = d.theta * d.dt;
   XT* xs;
   if (d.linear_guess) {
-    hipLaunchKernelGGL((k_sv_rhs_init<T, XT, WT, true>), g, blk, 0, s, pl, S->nsys,
+    hipLaunchKernelGGL((k_sv_rhs_init<T, XT, WT, true>), g, blk, 0, s, pl, (const SvEnt*)pl.ent,
+                       (const int32_t*)pl.pat, (const int16_t*)pl.dcol, (const int32_t*)pl.col,
+                       S->nsys,
                        (const T*)d.Mv, (const T*)d.Kv, (const T*)d.Dinv, (const WT*)d.w,
                        (const XT*)d.x[xc], (const T*)d.xp[pc], (XT*)d.x[1 - xc],
                        (T*)d.xp[1 - pc], (T*)d.r, coef, d.Cm, tdt, S->ctl, S->part[0]);
     xs = (XT*)d.x[1 - xc];
   } else {
-    hipLaunchKernelGGL((k_sv_rhs_init<T, XT, WT, false>), g, blk, 0, s, pl, S->nsys,
+    hipLaunchKernelGGL((k_sv_rhs_init<T, XT, WT, false>), g, blk, 0, s, pl, (const SvEnt*)pl.ent,
+                       (const int32_t*)pl.pat, (const int16_t*)pl.dcol, (const int32_t*)pl.col,
+                       S->nsys,
                        (const T*)d.Mv, (const T*)d.Kv, (const T*)d.Dinv, (const WT*)d.w,
                        (const XT*)d.x[xc], (const T*)nullptr, (XT*)nullptr, (T*)nullptr,
                        (T*)d.r, coef, d.Cm, tdt, S->ctl, S->part[0]);
