@@ -317,8 +317,8 @@ k_bat_pform(int64_t n, int lanes, int B, const T* __restrict__ Dinv, const T* __
 // sums of p'.Ap' (active instances only).
 template <class T, int NK, bool WU>
 __global__ void __launch_bounds__(kBB)
-k_bat_spmv(BatOp<T> op, const T* __restrict__ p, T* __restrict__ Ap, int k, BatCtl* ctl,
-           double* part) {
+k_bat_spmv(BatOp<T> op, const int32_t* __restrict__ rp, const T* __restrict__ p,
+           T* __restrict__ Ap, int k, BatCtl* ctl, double* part) {
   if (bat_halted(ctl, op.B, true)) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl->iter = k + 1;
   const int lanes = op.lanes, rpw = 64 >> op.lg;
@@ -336,7 +336,7 @@ k_bat_spmv(BatOp<T> op, const T* __restrict__ p, T* __restrict__ Ap, int k, BatC
     // values are read once, spread over the lanes, and broadcast.
     for (int64_t g = rr.g0 + wv; g < rr.g1; g += kBW) {
       const int64_t i = g;
-      const int32_t j0 = op.rp[i], j1 = op.rp[i + 1];
+      const int32_t j0 = rp[i], j1 = rp[i + 1];
       double acc = 0.0;
       for (int32_t js = j0; js < j1; js += 64) {
         const int len = j1 - js < 64 ? j1 - js : 64;
@@ -372,7 +372,7 @@ k_bat_spmv(BatOp<T> op, const T* __restrict__ p, T* __restrict__ Ap, int k, BatC
     for (int64_t g = rr.g0 + wv; g < rr.g1; g += kBW) {
       const int64_t i = g * rpw + rsub;
       if (i >= op.n) continue;
-      const int32_t j0 = op.rp[i], j1 = op.rp[i + 1];
+      const int32_t j0 = rp[i], j1 = rp[i + 1];
       double acc = 0.0;
       for (int32_t jc = j0; jc < j1; jc += kBChunk) {
         int32_t cj[kBChunk];
@@ -657,10 +657,12 @@ int bat_iters(tc_batch* S, int k0, int k1, XT* xs, hipStream_t s) {
                        (const T*)d.Dinv, (const T*)S->r, (const T*)pin, pout, k, S->ctl);
     TC_LAUNCH();
     if (d.lanes == 64)
-      hipLaunchKernelGGL((k_bat_spmv<T, NK, true>), g, blk, 0, s, op, (const T*)pout,
+      hipLaunchKernelGGL((k_bat_spmv<T, NK, true>), g, blk, 0, s, op, (const int32_t*)op.rp,
+                         (const T*)pout,
                          (T*)S->Ap, k, S->ctl, S->part[1]);
     else
-      hipLaunchKernelGGL((k_bat_spmv<T, NK, false>), g, blk, 0, s, op, (const T*)pout,
+      hipLaunchKernelGGL((k_bat_spmv<T, NK, false>), g, blk, 0, s, op, (const int32_t*)op.rp,
+                         (const T*)pout,
                          (T*)S->Ap, k, S->ctl, S->part[1]);
     TC_LAUNCH();
     hipLaunchKernelGGL(k_bat_fold, dim3(d.B), fb, 0, s, 1, k, S->G, d.lanes, d.B, S->ctl,
