@@ -169,6 +169,18 @@ __device__ inline const T& at32(const T* base, uint32_t idx) {
                                      idx * (uint32_t)sizeof(T));
 }
 
+// Pool slice list and per-system fields (tc_ionic_io's pool_* members),
+// carried separately where they come from read-only kernel arguments.
+struct PoolArgs {
+  const int32_t* slices;
+  const int32_t* slice_sys;
+  const int64_t* row0;
+  const int64_t* n;
+  const double* t;
+  const long long* steps;
+  const uint32_t* bits;
+};
+
 struct NodeCtx {
   int64_t vi;          // entry in the per-node arrays (V, w, dV, stim_mask, AT, RT)
   int64_t si, ss;      // state index and state row stride
@@ -183,23 +195,29 @@ struct NodeCtx {
 // Plain: entry i.  Batch group (lanes > 0): lane lane0 + i % lanes of node
 // i / lanes, rows of lane_stride entries.  Pool: row slice*64 + i%64 of the
 // listed slice, with its system's row range, stimulus clock and LAT time.
-__device__ inline NodeCtx node_ctx(const tc_ionic_io& io, int64_t i) {
+template <bool PA = false>
+__device__ inline NodeCtx node_ctx(const tc_ionic_io& io, int64_t i,
+                                   const PoolArgs& pa = PoolArgs{}) {
   NodeCtx c;
   if (io.pool_slices) {
     // one slice per wavefront: a scalar index, so its descriptors load as scalars
-    const int64_t sl = io.pool_slices[__builtin_amdgcn_readfirstlane((int)(i >> 6))];
+    const PoolArgs q = PA ? pa
+                          : PoolArgs{io.pool_slices, io.pool_slice_sys, io.pool_sys_row0,
+                                     io.pool_sys_n, io.pool_sys_t, io.pool_sys_steps,
+                                     io.pool_sys_bits};
+    const int64_t sl = q.slices[__builtin_amdgcn_readfirstlane((int)(i >> 6))];
     const int64_t row = sl * 64 + (i & 63);
-    const int sys = io.pool_slice_sys[sl];
+    const int sys = q.slice_sys[sl];
     c.vi = row;
     c.si = row;
     c.ss = io.state_stride;
-    c.ok = row - io.pool_sys_row0[sys] < io.pool_sys_n[sys];
-    const uint32_t on = io.pool_sys_bits[sys];
+    c.ok = row - q.row0[sys] < q.n[sys];
+    const uint32_t on = q.bits[sys];
     c.bits = (on && io.stim_mask) ? io.stim_mask[row] & on : 0u;
     c.amp = &io.pool_sys_stim[(int64_t)sys * TC_POOL_STIM].amp;
     c.amp_stride = (int)(sizeof(tc_stimulus) / sizeof(double));
-    c.lat = io.AT != nullptr && io.pool_sys_steps[sys] > 0;
-    c.lat_t = io.pool_sys_t[sys];
+    c.lat = io.AT != nullptr && q.steps[sys] > 0;
+    c.lat_t = q.t[sys];
     return c;
   }
   if (io.lanes > 0) {
@@ -306,12 +324,13 @@ __device__ inline void finish_node(const tc_ionic_io& io, const NodeCtx& c, T V,
 
 // ---------------------------------------------------------------- MS2v
 
-template <class T, bool VF64, bool NT>
+template <class T, bool VF64, bool NT, bool PA = false>
 __device__ __forceinline__ void ms2v_node(const tc_ms2v_params& p, T* __restrict__ H,
-                                          const tc_ionic_io& io, int64_t i) {
+                                          const tc_ionic_io& io, int64_t i,
+                                          const PoolArgs& pa = PoolArgs{}) {
   if (i >= io.n) return;
   const int gated = gate_value(io);
-  const NodeCtx c = node_ctx(io, i);
+  const NodeCtx c = node_ctx<PA>(io, i, pa);
   if (!c.ok) return;
   const T U = load_v<T, VF64>(io, c.vi);
   const T h = ld_state<NT>(H, c, 0);
@@ -538,15 +557,16 @@ TtpK<T> ttp_constants(const tc_ttp_params& p) {
 }
 
 // One TTP step per node (TenTusscherPanfilov.py:353-454).
-template <class T, bool VF64, bool NT>
+template <class T, bool VF64, bool NT, bool PA = false>
 __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
                                          const T* __restrict__ Vtab, const T* __restrict__ Ctab,
                                          const T* __restrict__ Ktab, const TtpTabs& tb,
-                                         const tc_ionic_io& io, int64_t i) {
+                                         const tc_ionic_io& io, int64_t i,
+                                         const PoolArgs& pa = PoolArgs{}) {
   const int64_t n = io.n;
   if (i >= n) return;
   const int gated = gate_value(io);
-  const NodeCtx c = node_ctx(io, i);
+  const NodeCtx c = node_ctx<PA>(io, i, pa);
   if (!c.ok) return;
   const T V = load_v<T, VF64>(io, c.vi);
   T s[TS_N];
@@ -838,15 +858,16 @@ CrnK<T> crn_constants(const tc_crn_params& p, double dt) {
 // One CRN step per node (CourtemancheRamirezNattel.py:358-444), in two
 // phases like the TTP step: currents and concentrations first (with the
 // V-table groups they read), then the Rush-Larsen gates.
-template <class T, bool VF64, bool NT>
+template <class T, bool VF64, bool NT, bool PA = false>
 __device__ __forceinline__ void crn_node(const CrnK<T>& K, T* __restrict__ st,
                                          const T* __restrict__ Ctab, const T* __restrict__ Vtab,
                                          const T* __restrict__ Ftab, const CrnTabs& tb,
-                                         const tc_ionic_io& io, int64_t i) {
+                                         const tc_ionic_io& io, int64_t i,
+                                         const PoolArgs& pa = PoolArgs{}) {
   const int64_t n = io.n;
   if (i >= n) return;
   const int gated = gate_value(io);
-  const NodeCtx c = node_ctx(io, i);
+  const NodeCtx c = node_ctx<PA>(io, i, pa);
   if (!c.ok) return;
   const T V = load_v<T, VF64>(io, c.vi);
   T s[CS_N];
@@ -946,6 +967,7 @@ struct PoolIonic {
   TtpTabs ttb;
   CrnTabs ctb;
   tc_ionic_io io[3];
+  tc_ionic_io sh;  // an io of a model present in the launch: the per-system fields
   int b1, b2;
 };
 
@@ -956,14 +978,29 @@ k_pool_ionic(tc_ms2v_params mp, TtpK<T> tk, CrnK<T> ck, T* __restrict__ st0,
              T* __restrict__ st1, T* __restrict__ st2, const T* __restrict__ tt0,
              const T* __restrict__ tt1, const T* __restrict__ tt2, const T* __restrict__ ct0,
              const T* __restrict__ ct1, const T* __restrict__ ct2, TtpTabs ttb, CrnTabs ctb,
-             tc_ionic_io io0, tc_ionic_io io1, tc_ionic_io io2, int b1, int b2) {
+             tc_ionic_io io0, tc_ionic_io io1, tc_ionic_io io2, int b1, int b2,
+             const int32_t* __restrict__ sl0, const int32_t* __restrict__ sl1,
+             const int32_t* __restrict__ sl2, const int32_t* __restrict__ slice_sys,
+             const int64_t* __restrict__ sys_row0, const int64_t* __restrict__ sys_n,
+             const double* __restrict__ sys_t, const long long* __restrict__ sys_steps,
+             const uint32_t* __restrict__ sys_bits) {
+  // The slice lists and per-system fields also come as read-only kernel
+  // arguments (the io copies of the same pointers are not used): loads
+  // through them, uniform per wavefront, then compile to scalar loads, so a
+  // wavefront's chain slice -> system -> row range, stimulus bits and clock
+  // does not wait on its vector memory counter.
   const int b = blockIdx.x;
   if (b < b1) {
-    ms2v_node<T, VF64, NT>(mp, st0, io0, (int64_t)b * kBlock + threadIdx.x);
+    const PoolArgs pa{sl0, slice_sys, sys_row0, sys_n, sys_t, sys_steps, sys_bits};
+    ms2v_node<T, VF64, NT, true>(mp, st0, io0, (int64_t)b * kBlock + threadIdx.x, pa);
   } else if (b < b2) {
-    ttp_node<T, VF64, NT>(tk, st1, tt0, tt1, tt2, ttb, io1, (int64_t)(b - b1) * kBlock + threadIdx.x);
+    const PoolArgs pa{sl1, slice_sys, sys_row0, sys_n, sys_t, sys_steps, sys_bits};
+    ttp_node<T, VF64, NT, true>(tk, st1, tt0, tt1, tt2, ttb, io1,
+                                (int64_t)(b - b1) * kBlock + threadIdx.x, pa);
   } else {
-    crn_node<T, VF64, NT>(ck, st2, ct0, ct1, ct2, ctb, io2, (int64_t)(b - b2) * kBlock + threadIdx.x);
+    const PoolArgs pa{sl2, slice_sys, sys_row0, sys_n, sys_t, sys_steps, sys_bits};
+    crn_node<T, VF64, NT, true>(ck, st2, ct0, ct1, ct2, ctb, io2,
+                                (int64_t)(b - b2) * kBlock + threadIdx.x, pa);
   }
 }
 
@@ -1146,7 +1183,10 @@ int pool_launch(const PoolIonic<T>& a, bool vf64, bool nt, int blocks, hipStream
                 : (nt ? k_pool_ionic<T, false, true> : k_pool_ionic<T, false, false>);
   hipLaunchKernelGGL(k, dim3(blocks), dim3(kBlock), 0, s, a.mp, a.tk, a.ck, a.st[0], a.st[1],
                      a.st[2], a.ttab[0], a.ttab[1], a.ttab[2], a.ctab[0], a.ctab[1], a.ctab[2],
-                     a.ttb, a.ctb, a.io[0], a.io[1], a.io[2], a.b1, a.b2);
+                     a.ttb, a.ctb, a.io[0], a.io[1], a.io[2], a.b1, a.b2, a.io[0].pool_slices,
+                     a.io[1].pool_slices, a.io[2].pool_slices, a.sh.pool_slice_sys,
+                     a.sh.pool_sys_row0, a.sh.pool_sys_n, a.sh.pool_sys_t, a.sh.pool_sys_steps,
+                     a.sh.pool_sys_bits);
   TC_LAUNCH();
   return TC_OK;
 }
@@ -1174,6 +1214,20 @@ int tc::pool_ionic_step(const tc_ms2v_params* mp, const tc_ttp_params* tp,
     vf = io[m].v_f64;
     ns = io[m].stream_states != 0;
   }
+  // the launch reads one set of per-system fields for all models
+  int m0 = -1;
+  for (int m = 0; m < 3; ++m) {
+    if (nb[m] == 0) continue;
+    if (m0 < 0) {
+      m0 = m;
+      continue;
+    }
+    const tc_ionic_io &a = io[m0], &b = io[m];
+    TC_CHECK(a.pool_slice_sys == b.pool_slice_sys && a.pool_sys_row0 == b.pool_sys_row0 &&
+                 a.pool_sys_n == b.pool_sys_n && a.pool_sys_t == b.pool_sys_t &&
+                 a.pool_sys_steps == b.pool_sys_steps && a.pool_sys_bits == b.pool_sys_bits,
+             TC_ERR_ARG, "pool ionic step: models with different per-system fields");
+  }
   const int64_t blocks = nb[0] + nb[1] + nb[2];
   if (blocks == 0) return TC_OK;
   TC_CHECK(blocks < (1LL << 31), TC_ERR_SIZE, "pool ionic step: too many entries");
@@ -1196,6 +1250,7 @@ int tc::pool_ionic_step(const tc_ms2v_params* mp, const tc_ttp_params* tp,
       a.ttab[m] = (const T*)ttp_tab[m];
       a.ctab[m] = (const T*)crn_tab[m];
     }
+    a.sh = io[m0];
     a.ttb = ttb;
     a.ctb = ctb;
     a.b1 = (int)nb[0];
