@@ -526,12 +526,15 @@ __global__ void k_sv_ent(int64_t nl, const int32_t* __restrict__ list,
 // test of the last iteration for every system still active; systems still
 // iterating below max_iter stall the pipeline; otherwise the step is final:
 // every live system's n_iter sum, clock (t += dt) and next-step stimulus
-// bits advance, and the status goes to pinned host memory.
+// bits advance, and the status goes to pinned host memory.  hc is the host
+// copy of *ctl: only its per-system array pointers and tolerances, fixed
+// when the pool is created, are read from it (a kernel-argument load instead
+// of a load of the pointer from device memory before every array access).
 constexpr int kSCheck = 1024;  // threads of the check pass (one wavefront per system fold)
 constexpr int kSCheckSys = 1024;  // systems of one check pass (max_sys is checked against it)
 
 __global__ void __launch_bounds__(kSCheck)
-k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
+k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, SvCtl hc,
            const double* __restrict__ P2, StatusRec* status, long long seq) {
   static_assert((TC_POOL_STIM & (TC_POOL_STIM - 1)) == 0 && TC_POOL_STIM <= 64, "stimulus lanes");
   __shared__ int stall0, any_stall, nmax, imax;
@@ -556,14 +559,14 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
   for (int q = wv; q < nsys; q += kSCheck / 64) {
     const SvSys y = pl.sysd[q];
     const int s = y.slot;
-    const int act = ctl->active[s];
-    const double bn = ctl->bnorm[s];
-    const int ni_old = ctl->n_iter[s];
-    const long long tot = ctl->total[s];
-    const double t_old = ctl->t[s];
-    const long long stp = ctl->steps[s];
-    const int nst = ctl->nstim[s];
-    const tc_stimulus sk = ctl->stim[(int64_t)s * TC_POOL_STIM + (lane & (TC_POOL_STIM - 1))];
+    const int act = hc.active[s];
+    const double bn = hc.bnorm[s];
+    const int ni_old = hc.n_iter[s];
+    const long long tot = hc.total[s];
+    const double t_old = hc.t[s];
+    const long long stp = hc.steps[s];
+    const int nst = hc.nstim[s];
+    const tc_stimulus sk = hc.stim[(int64_t)s * TC_POOL_STIM + (lane & (TC_POOL_STIM - 1))];
     constexpr int R = kSBlkMax / 64;
     double x0[R];
 #pragma unroll
@@ -591,7 +594,7 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
       int nact = act, ni = ni_old;
       if (act) {
         const double zn = sqrt(a);
-        if (zn < ctl->a_tol || zn / bn < ctl->r_tol) {
+        if (zn < hc.a_tol || zn / bn < hc.r_tol) {
           nact = 0;
           ni = k_end - 1;
         } else {
@@ -624,14 +627,14 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl,
   for (int q = threadIdx.x; q < nsys; q += blockDim.x) {
     const int s = s_slot[q];
     if (s_act[q] >> 1) {
-      ctl->active[s] = s_act[q] & 1;
-      ctl->n_iter[s] = s_ni[q];
+      hc.active[s] = s_act[q] & 1;
+      hc.n_iter[s] = s_ni[q];
     }
     if (!any_stall) {
-      ctl->total[s] = s_tot[q];
-      ctl->t[s] = s_t[q];
-      ctl->steps[s] = s_stp[q];
-      ctl->bits[s] = s_bits[q];
+      hc.total[s] = s_tot[q];
+      hc.t[s] = s_t[q];
+      hc.steps[s] = s_stp[q];
+      hc.bits[s] = s_bits[q];
     }
   }
 }
@@ -1007,7 +1010,7 @@ int sv_iters(tc_serve* S, int k0, int k1, XT* xs, hipStream_t s) {
 
 int sv_check(tc_serve* S, int k_end, int slot, long long seq, hipStream_t s) {
   hipLaunchKernelGGL(k_sv_check, dim3(1), dim3(kSCheck), 0, s, sv_pool(S), k_end, S->nsys,
-                     S->d.max_iter, S->d.dt, S->ctl, (const double*)S->part[2],
+                     S->d.max_iter, S->d.dt, S->ctl, S->hctl, (const double*)S->part[2],
                      S->dev_view + slot, seq);
   TC_LAUNCH();
   return TC_OK;
