@@ -237,6 +237,25 @@ int tc_pcg_solve(tc_pcg* plan, const void* b, void* x, const void* x_prev, void*
 int tc_pcg_set_timing(tc_pcg* plan, int on);
 int tc_pcg_get_timing(tc_pcg* plan, double* ms_host, int64_t* count_host);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code that must not call hipMalloc: tight serving loops,
+ * harnesses that forbid allocation): the solve of tc_pcg_solve, same
+ * kernels, same arguments and results, with no plan and no workspace.  The
+ * operator is passed directly (A pattern, Av values, Dinv in `dtype`).  Its
+ * scratch memory (control block, partial sums, r, p, p', Ap) is a static
+ * device array of the library, 33,603,840 bytes, which every GPU the
+ * library runs on holds until the process ends; the largest system it
+ * takes has TC_PCG_STATIC_MAX_ROWS rows (larger ones fail with
+ * TC_ERR_SIZE).  It allocates nothing (no hipMalloc/hipFree, no pinned
+ * host memory).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two host threads, or two streams) race and must be
+ * serialised by the caller.  Synchronizes `stream`. */
+#define TC_PCG_STATIC_MAX_ROWS 1048576
+int tc_pcg_solve_static(const tc_sell* A, const void* Av, const void* Dinv, int dtype,
+                        const void* b, void* x, const void* x_prev, void* x_next,
+                        double a_tol, double r_tol, int max_iter, int linear_guess,
+                        int* n_iter_host, void* stream);
+
 /* ------------------------------------------------------------------------
  * Ionic models.  State arrays are one (S, n) row-major block in `dtype`
  * (state k of node i at states[k*n + i]), in the order listed per model.

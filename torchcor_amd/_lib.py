@@ -13,6 +13,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libtorchcor_amd.so"
 
 TC_F32, TC_F64 = 0, 1
 MAX_STIMULI = 32
+# rows of the largest system tc_pcg_solve_static takes (its static scratch)
+TC_PCG_STATIC_MAX_ROWS = 1 << 20
 
 if not LIB_PATH.exists():
     raise ImportError(
@@ -173,6 +175,7 @@ FUNCS = {
     "tc_pcg_solve": (p, p, p, p, p, dbl, dbl, i32, i32, INT_P, p),
     "tc_pcg_set_timing": (p, i32),
     "tc_pcg_get_timing": (p, C.POINTER(dbl), I64_P),
+    "tc_pcg_solve_static": (SELL_P, p, p, i32, p, p, p, p, dbl, dbl, i32, i32, INT_P, p),
     "tc_ms2v_step": (C.POINTER(MS2vParams), p, IO_P, i32, p),
     "tc_ttp_tables": (C.POINTER(TTPParams), dbl, p, p, p, i32, p),
     "tc_ttp_step": (C.POINTER(TTPParams), p, p, p, p, IO_P, i32, p),

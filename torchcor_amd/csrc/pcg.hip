@@ -1,5 +1,7 @@
 // Stand-alone Jacobi-PCG solve and SpMV/RHS entry points (kernels in
 // pcg_kernels.h).
+#include <mutex>
+
 #include "pcg_kernels.h"
 
 namespace tc {
@@ -16,79 +18,117 @@ int ev_reserve(tc_pcg* P, size_t pairs) {
   return TC_OK;
 }
 
+// Device buffers and host state of one solve: a plan's (tc_pcg_create) or
+// the library's static scratch array (tc_pcg_solve_static).
+struct PcgBufs {
+  const tc_sell* A;
+  const void* Av;
+  const void* Dinv;
+  int64_t n;
+  int G;
+  PcgCtl* ctl;
+  double* part[3];
+  void* r;
+  void* p[2];
+  void* Ap;
+  PcgCtl* host_ctl;  // where the control block is copied back to
+  int* chunk;        // iterations launched before the first status read (hint)
+  tc_pcg* timed;     // plan whose event timing applies, or null
+};
+
+PcgBufs plan_bufs(tc_pcg* P) {
+  PcgBufs B;
+  B.A = &P->A;
+  B.Av = P->Av;
+  B.Dinv = P->Dinv;
+  B.n = P->n;
+  B.G = P->G;
+  B.ctl = P->ctl;
+  for (int k = 0; k < 3; ++k) B.part[k] = P->part[k];
+  B.r = P->r;
+  B.p[0] = P->p[0];
+  B.p[1] = P->p[1];
+  B.Ap = P->Ap;
+  B.host_ctl = P->host_ctl;
+  B.chunk = &P->chunk;
+  B.timed = P->timing ? P : nullptr;
+  return B;
+}
+
 template <class T, class CI>
-int pcg_solve(tc_pcg* P, const CI* sc, const T* b, T* x, const T* xp, T* xn, double a_tol,
-              double r_tol, int max_iter, int lg, int* n_iter_host, hipStream_t s) {
-  const int64_t n = P->n;
-  const dim3 g(P->G), blk(kPcgBlock);
-  const int32_t* sp = hot_slices(P->A);
-  const T* Av = (const T*)P->Av;
-  const T* D = (const T*)P->Dinv;
-  T* r = (T*)P->r;
-  T* Ap = (T*)P->Ap;
-  T* pp[2] = {(T*)P->p[0], (T*)P->p[1]};
-  if (P->timing) {
-    int rc = ev_reserve(P, 1 + 2 * (size_t)(max_iter > 0 ? max_iter : 0));
+int pcg_solve(const PcgBufs& P, const CI* sc, const T* b, T* x, const T* xp, T* xn,
+              double a_tol, double r_tol, int max_iter, int lg, int* n_iter_host, hipStream_t s) {
+  const int64_t n = P.n;
+  const dim3 g(P.G), blk(kPcgBlock);
+  const int32_t* sp = hot_slices(*P.A);
+  const T* Av = (const T*)P.Av;
+  const T* D = (const T*)P.Dinv;
+  T* r = (T*)P.r;
+  T* Ap = (T*)P.Ap;
+  T* pp[2] = {(T*)P.p[0], (T*)P.p[1]};
+  tc_pcg* tm = P.timed;
+  if (tm) {
+    int rc = ev_reserve(tm, 1 + 2 * (size_t)(max_iter > 0 ? max_iter : 0));
     if (rc) return rc;
   }
-  auto ev = [&](size_t k) { return P->timing ? P->ev[k] : (hipEvent_t) nullptr; };
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, &P->ctl->stall, 0);
+  auto ev = [&](size_t k) { return tm ? tm->ev[k] : (hipEvent_t) nullptr; };
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, &P.ctl->stall, 0);
   TC_LAUNCH();
   T* xs;
   if (lg) {
-    launch(k_pcg_init<T, true, CI>, g, blk, s, ev(0), ev(1), n, sp, sc, P->A.pat, Av, D, b,
-           (const T*)x, xp, xn, r, P->ctl, a_tol, r_tol, P->part[0]);
+    launch(k_pcg_init<T, true, CI>, g, blk, s, ev(0), ev(1), n, sp, sc, P.A->pat, Av, D, b,
+           (const T*)x, xp, xn, r, P.ctl, a_tol, r_tol, P.part[0]);
     xs = xn;
   } else {
-    launch(k_pcg_init<T, false, CI>, g, blk, s, ev(0), ev(1), n, sp, sc, P->A.pat, Av, D, b,
-           (const T*)x, xp, (T*)nullptr, r, P->ctl, a_tol, r_tol, P->part[0]);
+    launch(k_pcg_init<T, false, CI>, g, blk, s, ev(0), ev(1), n, sp, sc, P.A->pat, Av, D, b,
+           (const T*)x, xp, (T*)nullptr, r, P.ctl, a_tol, r_tol, P.part[0]);
     xs = x;
   }
   TC_LAUNCH();
   int it = 0;
-  int chunk = P->chunk < 1 ? 1 : P->chunk;
-  P->host_ctl->converged = 0;
-  P->host_ctl->iter = 0;
-  P->host_ctl->n_iter = 0;
+  int chunk = *P.chunk < 1 ? 1 : *P.chunk;
+  P.host_ctl->converged = 0;
+  P.host_ctl->iter = 0;
+  P.host_ctl->n_iter = 0;
   while (it < max_iter) {
     const int c = chunk < max_iter - it ? chunk : max_iter - it;
     for (int j = 0; j < c; ++j) {
       const int k = it + j;
       T* pin = pp[k & 1];
       T* pout = pp[(k + 1) & 1];
-      launch(k_pcg_spmv<T, CI>, g, blk, s, ev(2 + 4 * k), ev(3 + 4 * k), n, sp, sc, P->A.pat, Av, D,
-             (const T*)r, (const T*)pin, pout, Ap, k, P->ctl, (const double*)P->part[0],
-             (const double*)P->part[2], P->part[1]);
+      launch(k_pcg_spmv<T, CI>, g, blk, s, ev(2 + 4 * k), ev(3 + 4 * k), n, sp, sc, P.A->pat, Av, D,
+             (const T*)r, (const T*)pin, pout, Ap, k, P.ctl, (const double*)P.part[0],
+             (const double*)P.part[2], P.part[1]);
       TC_LAUNCH();
       launch(k_pcg_update<T, T>, g, blk, s, ev(4 + 4 * k), ev(5 + 4 * k), n, D, (const T*)pout,
-             (const T*)Ap, xs, r, k, P->ctl, (const double*)P->part[1], P->part[2]);
+             (const T*)Ap, xs, r, k, P.ctl, (const double*)P.part[1], P.part[2]);
       TC_LAUNCH();
     }
     it += c;
-    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kPcgBlock), 0, s, it - 1, P->G, P->ctl, P->part[2],
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kPcgBlock), 0, s, it - 1, P.G, P.ctl, P.part[2],
                        (StatusRec*)nullptr, 0, max_iter, 0LL);
     TC_LAUNCH();
-    TC_HIP(hipMemcpyAsync(P->host_ctl, P->ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, s));
+    TC_HIP(hipMemcpyAsync(P.host_ctl, P.ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, s));
     TC_HIP(hipStreamSynchronize(s));
-    if (P->host_ctl->converged) break;
+    if (P.host_ctl->converged) break;
     chunk = chunk * 2 < 64 ? chunk * 2 : 64;
   }
   if (max_iter == 0) TC_HIP(hipStreamSynchronize(s));
-  *n_iter_host = P->host_ctl->n_iter;
-  P->chunk = P->host_ctl->iter > 0 ? P->host_ctl->iter : 1;
-  if (P->timing) {
+  *n_iter_host = P.host_ctl->n_iter;
+  *P.chunk = P.host_ctl->iter > 0 ? P.host_ctl->iter : 1;
+  if (tm) {
     // Only iterations that ran count (later launches exit at once).
     float ms = 0.f;
-    TC_HIP(hipEventElapsedTime(&ms, P->ev[0], P->ev[1]));
-    P->acc_ms[0] += ms;
-    P->acc_n[0] += 1;
-    for (int k = 0; k < P->host_ctl->iter && k < it; ++k) {
-      TC_HIP(hipEventElapsedTime(&ms, P->ev[2 + 4 * k], P->ev[3 + 4 * k]));
-      P->acc_ms[1] += ms;
-      TC_HIP(hipEventElapsedTime(&ms, P->ev[4 + 4 * k], P->ev[5 + 4 * k]));
-      P->acc_ms[2] += ms;
-      P->acc_n[1] += 1;
-      P->acc_n[2] += 1;
+    TC_HIP(hipEventElapsedTime(&ms, tm->ev[0], tm->ev[1]));
+    tm->acc_ms[0] += ms;
+    tm->acc_n[0] += 1;
+    for (int k = 0; k < P.host_ctl->iter && k < it; ++k) {
+      TC_HIP(hipEventElapsedTime(&ms, tm->ev[2 + 4 * k], tm->ev[3 + 4 * k]));
+      tm->acc_ms[1] += ms;
+      TC_HIP(hipEventElapsedTime(&ms, tm->ev[4 + 4 * k], tm->ev[5 + 4 * k]));
+      tm->acc_ms[2] += ms;
+      tm->acc_n[1] += 1;
+      tm->acc_n[2] += 1;
     }
   }
   return TC_OK;
@@ -229,11 +269,106 @@ extern "C" int tc_pcg_solve(tc_pcg* P, const void* b, void* x, const void* x_pre
   TC_CHECK(!linear_guess || (x_prev && x_next), TC_ERR_ARG,
            "tc_pcg_solve: linear_guess needs x_prev and x_next");
   hipStream_t s = (hipStream_t)stream;
+  const PcgBufs B = plan_bufs(P);
 #define TC_PCG_SOLVE(T, SC)                                                                  \
-  pcg_solve<T>(P, SC, (const T*)b, (T*)x, (const T*)x_prev, (T*)x_next, a_tol, r_tol, max_iter, \
+  pcg_solve<T>(B, SC, (const T*)b, (T*)x, (const T*)x_prev, (T*)x_next, a_tol, r_tol, max_iter, \
                linear_guess, n_iter_host, s)
   if (P->dtype == TC_F32)
     return P->A.dcol ? TC_PCG_SOLVE(float, P->A.dcol) : TC_PCG_SOLVE(float, P->A.col);
   return P->A.dcol ? TC_PCG_SOLVE(double, P->A.dcol) : TC_PCG_SOLVE(double, P->A.col);
+#undef TC_PCG_SOLVE
+}
+
+// ------------------------------------------------------------------------
+// The same solve with no device allocation by the library or the caller:
+// the workspace a plan would hold (control block, partial sums, r, p, p',
+// Ap) lives in `scratch`, a file-scope device array of fixed size.  Every
+// GPU the library runs on holds its own copy of it for the life of the
+// process.  Host code never passes `scratch` itself (its host-side name is
+// not a device address): the first call on a device resolves the array's
+// device address with hipGetSymbolAddress, with that device current, and
+// every later call on that device uses that address.
+namespace tc {
+namespace {
+
+constexpr int64_t kStaticRows = TC_PCG_STATIC_MAX_ROWS;
+constexpr size_t kStaticCtlBytes = 256;
+constexpr size_t kStaticPartBytes = 2 * kMaxBlocks * sizeof(double);   // 2G doubles, G <= 1024
+constexpr size_t kStaticVecBytes = size_t(kStaticRows) * sizeof(double);
+constexpr size_t kStaticBytes = kStaticCtlBytes + 3 * kStaticPartBytes + 4 * kStaticVecBytes;
+static_assert(sizeof(PcgCtl) <= kStaticCtlBytes, "control block does not fit");
+static_assert(kStaticPartBytes % 256 == 0 && kStaticVecBytes % 256 == 0, "alignment");
+
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kStaticMaxDevices = 64;
+std::mutex g_static_mu;
+void* g_static_ptr[kStaticMaxDevices];     // device address of `scratch` per device
+int g_static_chunk[kStaticMaxDevices];     // launch-chunk hint per device
+PcgCtl g_static_host[kStaticMaxDevices];   // where each device's control block is read back
+
+int static_scratch(int* dev_out, void** ptr_out) {
+  int dev = 0;
+  TC_HIP(hipGetDevice(&dev));
+  TC_CHECK(dev >= 0 && dev < kStaticMaxDevices, TC_ERR_ARG,
+           "tc_pcg_solve_static: device %d beyond the %d devices it serves", dev,
+           kStaticMaxDevices);
+  std::lock_guard<std::mutex> lock(g_static_mu);
+  if (!g_static_ptr[dev]) {
+    void* ptr = nullptr;
+    TC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    TC_CHECK(ptr != nullptr, TC_ERR_HIP, "tc_pcg_solve_static: no device address for scratch");
+    g_static_ptr[dev] = ptr;
+    g_static_chunk[dev] = 1;
+  }
+  *dev_out = dev;
+  *ptr_out = g_static_ptr[dev];
+  return TC_OK;
+}
+
+}  // namespace
+}  // namespace tc
+
+extern "C" int tc_pcg_solve_static(const tc_sell* A, const void* Av, const void* Dinv, int dtype,
+                                   const void* b, void* x, const void* x_prev, void* x_next,
+                                   double a_tol, double r_tol, int max_iter, int linear_guess,
+                                   int* n_iter_host, void* stream) {
+  TC_CHECK(A && Av && Dinv && b && x && n_iter_host && max_iter >= 0 && A->n > 0, TC_ERR_ARG,
+           "tc_pcg_solve_static: bad arguments");
+  TC_CHECK(dtype == TC_F32 || dtype == TC_F64, TC_ERR_ARG, "bad dtype %d", dtype);
+  TC_CHECK(!linear_guess || (x_prev && x_next), TC_ERR_ARG,
+           "tc_pcg_solve_static: linear_guess needs x_prev and x_next");
+  TC_CHECK(A->n <= kStaticRows, TC_ERR_SIZE,
+           "tc_pcg_solve_static: %lld rows exceed the static scratch maximum of %lld rows "
+           "(use tc_pcg_create/tc_pcg_solve with a caller workspace)",
+           (long long)A->n, (long long)kStaticRows);
+  int dev = 0;
+  void* base = nullptr;
+  int rc = static_scratch(&dev, &base);
+  if (rc) return rc;
+  char* c = (char*)base;
+  PcgBufs B;
+  B.A = A;
+  B.Av = Av;
+  B.Dinv = Dinv;
+  B.n = A->n;
+  B.G = pcg_blocks(A->n);
+  B.ctl = (PcgCtl*)c;
+  size_t off = kStaticCtlBytes;
+  for (int k = 0; k < 3; ++k, off += kStaticPartBytes) B.part[k] = (double*)(c + off);
+  const size_t vb = vec_bytes(A->n, dtype);
+  B.r = c + off;
+  B.p[0] = c + off + vb;
+  B.p[1] = c + off + 2 * vb;
+  B.Ap = c + off + 3 * vb;
+  B.host_ctl = &g_static_host[dev];
+  B.chunk = &g_static_chunk[dev];
+  B.timed = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+#define TC_PCG_SOLVE(T, SC)                                                                  \
+  pcg_solve<T>(B, SC, (const T*)b, (T*)x, (const T*)x_prev, (T*)x_next, a_tol, r_tol, max_iter, \
+               linear_guess, n_iter_host, s)
+  if (dtype == TC_F32) return A->dcol ? TC_PCG_SOLVE(float, A->dcol) : TC_PCG_SOLVE(float, A->col);
+  return A->dcol ? TC_PCG_SOLVE(double, A->dcol) : TC_PCG_SOLVE(double, A->col);
 #undef TC_PCG_SOLVE
 }
