@@ -567,14 +567,19 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, 
     const long long stp = hc.steps[s];
     const int nst = hc.nstim[s];
     const tc_stimulus sk = hc.stim[(int64_t)s * TC_POOL_STIM + (lane & (TC_POOL_STIM - 1))];
-    constexpr int R = kSBlkMax / 64;
-    double x0[R];
+    // The block partials of the system, loaded as fold_range_block's
+    // threads hold them: thread t = w * 64 + lane of a kSB-thread block
+    // holds blocks vb0 + t + j * kSB, j < kSBlkMax / kSB.
+    constexpr int R = kSBlkMax / kSB;
+    double x0[kSW][R];
 #pragma unroll
-    for (int u = 0; u < R; ++u) {
-      const int v = y.vb0 + lane + u * 64;
-      const double t = P2[(int64_t)(v <= y.vb1 ? v : y.vb1) * 2];
-      x0[u] = v <= y.vb1 ? t : 0.0;
-    }
+    for (int w = 0; w < kSW; ++w)
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        const int v = y.vb0 + w * 64 + lane + j * kSB;
+        const double t = P2[(int64_t)(v <= y.vb1 ? v : y.vb1) * 2];
+        x0[w][j] = v <= y.vb1 ? t : 0.0;
+      }
     // the clock after this step (t += dt, as the reference) and the
     // stimuli on in the next one (the test of stimulus_bits, per lane)
     const double t = t_old + dt;
@@ -586,10 +591,19 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, 
       on = t_t <= sk.count && p_t >= sk.start && p_t <= sk.start + sk.duration;
     }
     const uint32_t bits = (uint32_t)__ballot(on);
+    // ||z||^2 summed in exactly the order of fold_range_block (each
+    // thread's blocks in order, the wavefront sum, then the wavefronts in
+    // order), so the stopping test at the end of a launch chunk sees the
+    // bits the next spmv pass would see: where a chunk ends (which depends
+    // on the other systems in the pool) cannot change a system's result.
     double a = 0.0;
 #pragma unroll
-    for (int u = 0; u < R; ++u) a += x0[u];
-    a = wave_sum(a);
+    for (int w = 0; w < kSW; ++w) {
+      double sw = 0.0;
+#pragma unroll
+      for (int j = 0; j < R; ++j) sw += x0[w][j];
+      a += wave_sum(sw);
+    }
     if (lane == 0) {
       int nact = act, ni = ni_old;
       if (act) {
