@@ -300,12 +300,12 @@ def test_ionic_fused_outputs():
     assert np.array_equal(AT.cpu().numpy(), ATr) and np.array_equal(RT.cpu().numpy(), RTr)
 
 
-@pytest.mark.parametrize("name", ["ttp_endo", "crn"])
+@pytest.mark.parametrize("name", ["ttp_endo", "ttp_epi", "crn"])
 def test_single_cell_vs_reference_golden(name):
     _say(f"[gpu] single cell golden {name}")
     from pathlib import Path
     gold = np.load(Path(__file__).parent / "golden" / f"ionic_{name}_f64.npz")
-    g = _gpu_model("ttp" if name == "ttp_endo" else "crn", torch.float64)
+    g = _gpu_model({"ttp_endo": "ttp", "ttp_epi": "ttp_epi", "crn": "crn"}[name], torch.float64)
     V = g.initialize(1)
     tr = [V.item()]
     t = 0.0

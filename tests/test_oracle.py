@@ -102,6 +102,33 @@ def test_assemble_pattern_and_invariants_cube():
         assert r21[-1] == 128581
 
 
+def test_sigma_tensor_eigenstructure():
+    """sigma = sigma_t I + (sigma_l - sigma_t) f f^T (conductivity.py:64-67):
+    the fibre is an eigenvector with eigenvalue sigma_l and every direction
+    normal to it one with eigenvalue sigma_t (swapping sigma_l and sigma_t,
+    or a transposed f f^T of a non-normalised outer product, fails)."""
+    rng = np.random.default_rng(21)
+    E = 200
+    f = rng.normal(size=(E, 3))
+    f /= np.linalg.norm(f, axis=1, keepdims=True)
+    sl = rng.uniform(0.3, 1.0, E)
+    st = rng.uniform(0.05, 0.25, E)
+    S = fem.sigma_tensor(sl, st, f)
+    assert S.shape == (E, 3, 3)
+    assert np.allclose(S, np.transpose(S, (0, 2, 1)), atol=1e-15)
+    Sf = np.einsum("eij,ej->ei", S, f)
+    assert np.allclose(Sf, sl[:, None] * f, rtol=1e-13, atol=1e-15)
+    # two unit vectors normal to f
+    a = np.cross(f, rng.normal(size=(E, 3)))
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    b = np.cross(f, a)
+    for v in (a, b):
+        Sv = np.einsum("eij,ej->ei", S, v)
+        assert np.allclose(Sv, st[:, None] * v, rtol=1e-13, atol=1e-15)
+    w = np.linalg.eigvalsh(S)
+    assert np.allclose(w, np.sort(np.stack([st, st, sl], axis=1), axis=1), rtol=1e-12)
+
+
 def test_conductivity_fields():
     regions = np.array([0, 1, 2, 1])
     sl, st = fem.conductivity_fields(regions, [([0], 0.3, 0.06, None, None)])
@@ -178,6 +205,7 @@ def _single_cell(model, nsteps, dt=0.01, stim=40.0, keep=10):
 
 @pytest.mark.parametrize("name,make", [
     ("ttp_endo", lambda: ionic.TenTusscherPanfilov("ENDO", 0.01)),
+    ("ttp_epi", lambda: ionic.TenTusscherPanfilov("EPI", 0.01)),
     ("crn", lambda: ionic.CourtemancheRamirezNattel(0.01)),
 ])
 def test_ionic_vs_reference_golden(name, make):
@@ -186,6 +214,19 @@ def test_ionic_vs_reference_golden(name, make):
     tr = _single_cell(make(), nsteps)
     ref = g["V"][: len(tr)]
     assert np.abs(tr - ref).max() < 1e-6
+
+
+def test_ttp_epi_whole_action_potential_vs_reference_golden():
+    """The EPI branch (TenTusscherPanfilov.py:88-89, 258: Gto 0.294 and the
+    EPI tau_S) over the reference's whole 400 ms trace, upstroke to rest;
+    the EPI and ENDO traces differ by millivolts, so a swapped branch fails."""
+    epi = np.load(GOLD / "ionic_ttp_epi_f64.npz")["V"]
+    endo = np.load(GOLD / "ionic_ttp_endo_f64.npz")["V"]
+    assert np.abs(epi - endo).max() > 1.0
+    tr = _single_cell(ionic.TenTusscherPanfilov("EPI", 0.01), 40000)
+    assert len(tr) == len(epi)
+    assert np.abs(tr - epi).max() < 1e-6
+    assert tr.max() > 40.0 and tr[-1] < -80.0
 
 
 def test_arange_grid_matches_torch():
@@ -334,3 +375,128 @@ def test_phie_needs_extracellular_conductivity():
     m.add_conductivity([0], 0.1, 0.1)
     with pytest.raises(ValueError):
         m.phie_recovery(np.zeros((1, len(nodes))))
+
+
+# ------------------------------------------- monodomain loop composition
+class _StubIonic:
+    """An ionic model whose -Iion is a given function of V (and a latch),
+    so the loop can be checked against closed forms."""
+    name = "stub"
+
+    def __init__(self, u0, rate):
+        self.u0 = np.asarray(u0, dtype=np.float64)
+        self.rate = rate
+
+    def initialize(self, n):
+        assert n == len(self.u0)
+        return self.u0.copy()
+
+    def differentiate(self, V):
+        return self.rate(V)
+
+
+def _stub_loop(N, stub, T, dt, sigma=1.0, tol=1e-13):
+    nodes, elems = meshes.kuhn_cube(N)
+    E = len(elems)
+    m = monodomain.Monodomain(stub, T=T, dt=dt)
+    m.round_f32 = False
+    m.load_arrays(nodes, elems, np.zeros(E, int), np.tile([1.0, 0, 0], (E, 1)))
+    m.add_conductivity([0], sigma, sigma)
+    return m, nodes, elems
+
+
+def test_loop_ionic_and_stimulus_scaling_closed_form():
+    """Uniform fields (K 1 = 0, so no diffusion): the step must give
+    dV/dt = (-Iion)/100/Cm + Istim/100/Cm (monodomain.py:116-127, Cm = 0.01):
+    a constant ionic rate c raises V by c mV/ms (the /100 and Cm cancel), a
+    stimulus of amplitude I by I mV/ms while its window is on (steps 1..99
+    of a 0..1 ms window).  A missing /100 or Cm factor fails."""
+    N, dt = 5, 0.01
+    n = N ** 3
+    c = 30.0
+    m, _, _ = _stub_loop(N, _StubIonic(np.full(n, -80.0), lambda V: np.full_like(V, c)),
+                         T=0.5, dt=dt)
+    r = m.solve(1e-13, 1e-13, 200)
+    assert np.allclose(r["u"], -80.0 + 50 * dt * c, rtol=0, atol=1e-9)
+    m, _, _ = _stub_loop(N, _StubIonic(np.full(n, -80.0), np.zeros_like), T=2.0, dt=dt)
+    m.add_stimulus(np.arange(n), 0.0, 1.0, 50.0)
+    r = m.solve(1e-13, 1e-13, 200, snapshot_interval=0.01)
+    assert np.allclose(r["u"], -80.0 + 99 * dt * 50.0, rtol=0, atol=1e-9)
+    rise = np.diff(r["Vm"][:, 0])
+    assert np.allclose(rise[:99], dt * 50.0, atol=1e-9) and np.allclose(rise[99:], 0.0, atol=1e-9)
+
+
+@pytest.mark.parametrize("dt", [0.01, 25.0])
+def test_loop_theta_scheme_on_a_discrete_eigenmode(dt):
+    """Ionic current off, V = -80 + eps v with K v = lam M v (the slowest
+    mode of the assembled P1 operators, found by scipy's dense symmetric
+    eigensolver): each step must multiply eps by
+    g = (Cm - (1-theta) dt lam/Chi) / (Cm + theta dt lam/Chi)
+    with Chi = 140, Cm = 0.01, theta = 1/2 (monodomain.py:44-46, 93-131).
+    At dt = 25 ms, lam dt / (Chi Cm) is about 1.7, where Crank-Nicolson
+    (g = 0.08), backward Euler (0.37) and a wrong Chi or Cm differ widely.
+    The mode's rate also matches the continuum sigma (pi/L)^2 / (Chi Cm) to
+    discretisation error, and closer on the finer mesh."""
+    import scipy.linalg as sla
+    Chi, Cm, theta, sigma = 140.0, 0.01, 0.5, 1.0
+    errs = []
+    for N in (5, 9):
+        nodes, elems = meshes.kuhn_cube(N)
+        E = len(elems)
+        S = fem.sigma_tensor(np.full(E, sigma), np.full(E, sigma), np.tile([1.0, 0, 0], (E, 1)))
+        Me, Ke = fem.tet_local(nodes[elems], S)
+        rp, col, kv = fem.assemble_csr(elems, Ke, len(nodes), round_f32=False)
+        _, _, mv = fem.assemble_csr(elems, Me, len(nodes), round_f32=False)
+        Kd = sp.csr_matrix((kv, col, rp)).toarray()
+        Md = sp.csr_matrix((mv, col, rp)).toarray()
+        w, V = sla.eigh(Kd, Md)
+        lam, v = w[1], V[:, 1]                       # w[0] = 0: the constant
+        errs.append(abs(lam / (sigma * (math.pi / 10.0) ** 2) - 1.0))
+        if N != 5:
+            continue
+        eps, steps = 5.0 / np.abs(v).max(), 4 if dt > 1 else 200
+        T = steps * dt
+        m, _, _ = _stub_loop(N, _StubIonic(-80.0 + eps * v, np.zeros_like), T=T, dt=dt,
+                             sigma=sigma)
+        r = m.solve(1e-14, 1e-14, 500, snapshot_interval=dt)
+        g = (Cm - (1 - theta) * dt * lam / Chi) / (Cm + theta * dt * lam / Chi)
+        for k in range(steps + 1):
+            want = -80.0 + eps * g ** k * v
+            assert np.abs(r["Vm"][k] - want).max() <= 1e-9 * 5.0, (k, g)
+        if dt > 1:
+            assert 0.0 < g < 0.15
+    assert errs[1] < errs[0] and errs[1] < 0.02
+
+
+def test_loop_at_rt_stamps_are_the_step_time():
+    """LAT/RT (monodomain.py:174-176): AT is the accumulated double t of the
+    first step whose solution is > 0 mV, RT the t of the first later step
+    whose solution is < -70 mV.  A uniform field driven by a latching rate
+    (+c1 until V first exceeds 10 mV, then -c2) has both times in closed
+    form; a stamp one step early or late, or >= in place of >, fails."""
+    N, dt, c1, c2 = 4, 0.01, 37.0, 23.0
+    n = N ** 3
+    latch = {"down": False}
+
+    def rate(V):
+        if V.max() > 10.0:
+            latch["down"] = True
+        return np.full_like(V, -c2 if latch["down"] else c1)
+
+    m, _, _ = _stub_loop(N, _StubIonic(np.full(n, -80.0), rate), T=10.0, dt=dt)
+    r = m.solve(1e-13, 1e-13, 200)
+    # the same recurrence on one scalar, with the reference's t += dt
+    v, t, down, at, rt = -80.0, 0.0, False, -1.0, -1.0
+    for _ in range(int(10.0 / dt)):
+        t += dt
+        if v > 10.0:
+            down = True
+        v = v + dt * (-c2 if down else c1)
+        if v > 0 and at == -1:
+            at = t
+        if at > 0 and rt == -1 and v < -70:
+            rt = t
+    assert at > 0 and rt > at
+    # no solution value sits within roundoff of a threshold
+    assert np.allclose(r["u"], v, atol=1e-9)
+    assert np.all(r["AT"] == at) and np.all(r["RT"] == rt)
