@@ -155,6 +155,30 @@ class P1System:
         if getattr(self, "_pcg", None):
             L.lib.tc_pcg_destroy(self._pcg)
             self._pcg = None
+        if getattr(self, "_pcg64", None):
+            L.lib.tc_pcg_destroy(self._pcg64)
+            self._pcg64 = None
+
+    def pcg_plan_f64(self):
+        """The PCG plan of a float64 solve on the working operator: for a
+        float32 system, float64 copies of A's SELL values and of Dinv (both
+        exact conversions), as the reference's ConjugateGradient converts A
+        to float64 and keeps the Jacobi scaling in A's dtype, so that
+        z = Dinv r multiplies by the working-dtype scaling (solver.py:24-25,
+        preconditioner.py:9-43).  A float64 system's own plan otherwise."""
+        if self.code == L.TC_F64:
+            return self._pcg
+        if getattr(self, "_pcg64", None) is None:
+            self.sA64 = self.sA.to(torch.float64)
+            self.Dinv64 = self.Dinv.to(torch.float64)
+            nb = C.c_size_t(0)
+            L.call("tc_pcg_workspace_size", self.n, L.TC_F64, C.byref(nb))
+            self.pcg64_ws = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
+            h = C.c_void_p(0)
+            L.call("tc_pcg_create", C.byref(h), C.byref(self.sell), L.ptr(self.sA64),
+                   L.ptr(self.Dinv64), L.TC_F64, L.ptr(self.pcg64_ws), nb.value)
+            self._pcg64 = h
+        return self._pcg64
 
     def __del__(self):
         try:
@@ -175,11 +199,13 @@ class P1System:
                float(coef), L.ptr(out), self.code, L.stream_handle(self.device))
         return out
 
-    def pcg_solve(self, b, x, x_prev, x_next, a_tol, r_tol, max_iter, linear_guess):
+    def pcg_solve(self, b, x, x_prev, x_next, a_tol, r_tol, max_iter, linear_guess, plan=None):
+        """One solve on the working plan (vectors in the working dtype), or
+        on ``plan`` (e.g. ``pcg_plan_f64()``, float64 vectors)."""
         it = C.c_int(0)
-        L.call("tc_pcg_solve", self._pcg, L.ptr(b), L.ptr(x), L.ptr(x_prev), L.ptr(x_next),
-               float(a_tol), float(r_tol), int(max_iter), 1 if linear_guess else 0, C.byref(it),
-               L.stream_handle(self.device))
+        L.call("tc_pcg_solve", self._pcg if plan is None else plan, L.ptr(b), L.ptr(x),
+               L.ptr(x_prev), L.ptr(x_next), float(a_tol), float(r_tol), int(max_iter),
+               1 if linear_guess else 0, C.byref(it), L.stream_handle(self.device))
         return int(it.value)
 
     def pcg_timing(self, on=None):

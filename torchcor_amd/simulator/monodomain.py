@@ -39,6 +39,8 @@ from ..ionic.base import default_device
 # to pinned host memory through a ring of RING_SLOTS device frames.
 FRAMES_DEVICE_BYTES = 16 << 30
 RING_SLOTS = 4
+# with verbose output, every VERBOSE_TIMING_PERIOD-th step is timed per kernel
+VERBOSE_TIMING_PERIOD = 16
 
 
 class Monodomain:
@@ -170,10 +172,14 @@ class Monodomain:
 
     # ------------------------------------------------------------ stepping
     def _init_solver(self, u, linear_guess=True):
-        """ConjugateGradient.initialize (solver.py:23-33): x = u, x_prev = u."""
-        self._x = u.clone()
-        self._x_prev = u.clone()
-        self._x_next = torch.empty_like(u)
+        """ConjugateGradient.initialize (solver.py:23-33): the iterate x = u in
+        float64 and x_prev = u in u's dtype.  x_prev is held in float64
+        storage with values rounded to the working dtype, as the reference's
+        float32 x_prev rounds each copy of x (solver.py:26, 35-38)."""
+        f64 = torch.float64
+        self._x = u.to(f64).clone()
+        self._x_prev = u.to(f64).clone()
+        self._x_next = torch.empty_like(self._x)
         self._w = torch.empty_like(u)
         self._b = torch.empty_like(u)
         self._linear_guess = linear_guess
@@ -199,37 +205,44 @@ class Monodomain:
         self.ionic_model.step(io)
 
     def _electric(self, u, a_tol, r_tol, max_iter):
+        """b = M w - (1-theta) dt K u in the working dtype, then the float64
+        PCG (monodomain.py:127-133, solver.py:40-76): b converted to float64,
+        the float64 iterate warm-started by the linear guess, the solution
+        returned in the working dtype."""
         sys_ = self.system
         sys_.rhs(self._w, u, (1 - self.theta) * self.dt, self._b)
-        n_iter = sys_.pcg_solve(self._b, self._x, self._x_prev, self._x_next, a_tol, r_tol,
-                                max_iter, self._linear_guess)
+        f64 = torch.float64
+        b = self._b if self._b.dtype == f64 else self._b.to(f64)
+        n_iter = sys_.pcg_solve(b, self._x, self._x_prev, self._x_next, a_tol, r_tol, max_iter,
+                                self._linear_guess, plan=sys_.pcg_plan_f64())
         if self._linear_guess:
             self._x_prev, self._x, self._x_next = self._x, self._x_next, self._x_prev
-        return self._x, n_iter
+            if self.dtype != f64:
+                self._x_prev.copy_(self._x_prev.to(self.dtype))
+        return self._x.to(self.dtype), n_iter
 
     def step(self, u, t, a_tol, r_tol, max_iter, verbose=False, _lat=None):
         """One time step (monodomain.py:109-139): returns (u, n_iter, ionic_s, electric_s).
-        ``u`` is in mesh numbering; the returned u too."""
+        ``u`` is in mesh numbering; the returned u too.  As in the reference,
+        ``u`` feeds the ionic update and K u, while the PCG keeps its own
+        float64 iterate between steps (set from the first step's u)."""
         if self.system is None:
             self.assemble()
+        if self.node_order is not None and _lat is not None:
+            raise NotImplementedError("LAT arrays in step() need the mesh's own node order")
+        ui = self._to_internal(u).to(self.dtype).contiguous()
         if self._x is None:
-            self._init_solver(self._to_internal(u).contiguous())
-        if self.node_order is not None:
-            if _lat is not None:
-                raise NotImplementedError("LAT arrays in step() need the mesh's own node order")
-            self._x.copy_(self._to_internal(u))
-        elif u.data_ptr() != self._x.data_ptr():
-            self._x.copy_(u)
+            self._init_solver(ui)
         ionic_time = electric_time = 0.0
         if verbose:
             torch.cuda.synchronize(self.device)
             t0 = time.time()
-        self._ionic(self._x, t, _lat)
+        self._ionic(ui, t, _lat)
         if verbose:
             torch.cuda.synchronize(self.device)
             ionic_time = time.time() - t0
             t0 = time.time()
-        u, n_iter = self._electric(self._x, a_tol, r_tol, max_iter)
+        u, n_iter = self._electric(ui, a_tol, r_tol, max_iter)
         if verbose:
             torch.cuda.synchronize(self.device)
             electric_time = time.time() - t0
@@ -335,6 +348,10 @@ class Monodomain:
                          snapshot_interval=snapshot_interval, verbose=verbose, n_total_iter=0,
                          t0=time.time())
         self.iterations = []
+        if verbose:
+            # per-phase totals for the closing log line, from HIP-event
+            # timings of every 16th step (timing every step slows the loop)
+            self.set_kernel_timing(True, VERBOSE_TIMING_PERIOD)
 
     def _free_sim(self):
         if getattr(self, "_sim", None):
@@ -396,6 +413,21 @@ class Monodomain:
         L.call("tc_sim_get_timing", self._sim, ms, cnt)
         names = ("ionic", "rhs_init", "pcg_spmv", "pcg_update", "pcg_check")
         return {k: (ms[i], cnt[i]) for i, k in enumerate(names)}
+
+    def phase_totals(self, steps):
+        """(ionic seconds, electric seconds, GPU-busy % of the wall time) for
+        ``steps`` steps, from the sampled kernel timings (zeros when no step
+        was timed)."""
+        kt = self.kernel_times()
+        sampled = kt["ionic"][1]
+        if sampled <= 0:
+            return 0.0, 0.0, 0.0
+        ion = kt["ionic"][0] / sampled * steps / 1e3
+        ele = sum(kt[k][0] for k in ("rhs_init", "pcg_spmv", "pcg_update", "pcg_check")) \
+            / sampled * steps / 1e3
+        wall = getattr(self, "solving_time", 0.0)
+        util = 100.0 * (ion + ele) / wall if wall > 0 else 0.0
+        return ion, ele, min(util, 100.0)
 
     # ------------------------------------------------------------ output
     def pt_to_vtk(self):
@@ -505,9 +537,16 @@ class Monodomain:
             if R["snapshot_interval"] < self.T:
                 torch.save(self.frames.cpu(), self.result_path / "Vm.pt")
         if R["verbose"]:
+            # the reference's closing line (monodomain.py:211-220): model, nodes,
+            # wall time, ionic and electric totals, CG iterations, GPU
+            # utilisation (%), memory (GB).  The phase totals and the
+            # utilisation (kernel-busy time over wall time) are the sampled
+            # per-step kernel durations scaled to all steps.
+            ion_s, ele_s, util = self.phase_totals(steps)
             mem = torch.cuda.max_memory_allocated(self.device) / 1e9
-            print(self.ionic_model.name, self.n_nodes, round(self.solving_time, 2), 0.0, 0.0,
-                  R["n_total_iter"], "n/a", f"{round(mem, 2)}", flush=True)
+            print(self.ionic_model.name, self.n_nodes, round(self.solving_time, 2),
+                  round(ion_s, 2), round(ele_s, 2), R["n_total_iter"], f"{round(util, 2)}",
+                  f"{round(mem, 2)}", flush=True)
             if R["calc"]:
                 print("ATs: ", AT.min().item(), AT.max().item(), flush=True)
                 print("RTs: ", RT.min().item(), RT.max().item(), flush=True)
