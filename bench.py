@@ -12,9 +12,15 @@ A step is one time step of the whole path (ionic kernel, RHS, PCG).  W
 warm-up steps run first (they include the stimulus onset); K steps are timed
 between synchronisations.  value = nodes * K / elapsed (node-timesteps/s).
 
-Extra fields: per-kernel times measured with HIP events on the stream the
-kernels run on, the PCG algorithmic HBM rate, the roofline of the dominant
-kernel, and the CPU oracle timed on a bounded sample of the same workload.
+Per-kernel durations come from HIP events attached to the kernel launches
+on the stream they run on.  Events on a launch add a few microseconds of
+gap after it, so the timed pass carries them on every 16th step only; when
+that gives fewer than 16 samples (short windows), a second pass over the
+same step window (same warm-up, same K steps) times every step and supplies
+the kernel figures.  A steady-state window (2000 steps after 2 ms) is
+reported beside the headline.  Also: the batched and served forms, the
+irregular shell, and the CPU baseline (a torch-CPU restatement of the step
+on the host's threads, float32 and float64).
 """
 import argparse
 import gc
@@ -82,15 +88,6 @@ MEASURED_TRAFFIC_101 = {
 }
 
 
-def survey_electric_bytes(n, nnz):
-    """Electric-phase algorithmic bytes of the unfused formulation with int32
-    indices (B_rhs, B_init, B_iter): the definition behind 'PCG HBM GB/s'.
-    The fused passes move fewer bytes than this (pcg_moved_GBps counts
-    those), so the rate by this definition can exceed the HBM peak."""
-    return (12 * nnz + 4 * (n + 1) + 12 * n, 8 * nnz + 4 * (n + 1) + 32 * n,
-            8 * nnz + 4 * (n + 1) + 48 * n)
-
-
 def build_sim(N, dev):
     import torch
     from torchcor_amd import meshes
@@ -106,32 +103,61 @@ def build_sim(N, dev):
     return sim, (nodes, elems, regions, fib)
 
 
-def cpu_baseline(N, mesh, steps):
-    """The float64 CPU oracle on the same workload: `steps` time steps after its
-    own (untimed) assembly, single-threaded numpy/scipy."""
+def host_operators(sim, N):
+    """The assembled operators of the workload in CSR order on the host (float64
+    copies of the float32 values) and its stimulus, for the CPU baseline."""
+    import types
     import numpy as np
-    from oracle import ionic as oion
-    from oracle import monodomain as omono
     from torchcor_amd import meshes
-    nodes, elems, regions, fib = mesh
-    m = omono.Monodomain(oion.TenTusscherPanfilov("ENDO", 0.01), T=100.0, dt=0.01)
-    m.load_arrays(nodes, elems, regions, fib)
-    m.add_conductivity([0], 0.5272, 0.2076)
-    m.add_conductivity([1], 0.9074, 0.3332)
-    m.add_stimulus(meshes.corner_nodes(N, 10), 0.0, 1.0, 100)
-    print("[bench] cpu oracle: assembling", flush=True)
-    m.assemble()
-    m.assemble = lambda: None
-    print(f"[bench] cpu oracle: {steps} steps", flush=True)
-    t0 = time.perf_counter()
-    with np.errstate(all="ignore"):
-        m.solve(1e-5, 1e-5, 100, nsteps=steps)
-    el = time.perf_counter() - t0
-    return {"value": nodes.shape[0] * steps / el, "unit": "node-timesteps/s", "cores": 1,
-            "kind": "port",
-            "sample": f"{steps} time steps (t = 0.01..{steps * 0.01:.2f} ms) of the same "
-                      f"{N}^3 TTP workload in float64 numpy/scipy, assembly excluded; "
-                      f"{el:.1f} s"}
+    S = sim.system
+    mask = np.zeros(S.n)
+    mask[meshes.corner_nodes(N, 10)] = 100.0
+    return types.SimpleNamespace(
+        row_ptr=S.row_ptr.cpu().numpy(), col=S.col.cpu().numpy(), n=S.n,
+        M=S.Mv.double().cpu().numpy(), K=S.Kv.double().cpu().numpy(), Cm=sim.Cm, dt=sim.dt,
+        theta=sim.theta, stimuli=[(mask, 0.0, 1.0, sim.T, 1)])
+
+
+def cpu_baseline(ops, N, seconds=12.0):
+    """The step on the host CPU: the torch-CPU restatement (oracle/torch_cpu.py:
+    torch sparse CSR products, TTP on torch tensors, float64 Jacobi-PCG, the
+    reference's operation order and storage types) on torch's thread count,
+    float32 then float64, each timed over as many steps (after 2 untimed) as
+    fit about `seconds` seconds from t = 0.  Assembly is not timed (the
+    operators are the GPU's, copied to the host)."""
+    import torch
+    from oracle import torch_cpu
+    threads = torch.get_num_threads()
+    out = {}
+    for dt_ in (torch.float32, torch.float64):
+        name = "f32" if dt_ == torch.float32 else "f64"
+        print(f"[bench] cpu baseline ({name}, torch CPU, {threads} threads)", flush=True)
+        tm = torch_cpu.Monodomain(ops, torch_cpu.TenTusscherPanfilov("ENDO", 0.01, dt_), dt_)
+        u = tm.initialize()
+        t = 0.0
+        t0 = time.perf_counter()
+        for _ in range(2):
+            t += 0.01
+            u, _ = tm.step(u, t)
+        per = (time.perf_counter() - t0) / 2
+        steps = max(3, min(400, int(seconds / max(per, 1e-6))))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t += 0.01
+            u, _ = tm.step(u, t)
+        el = time.perf_counter() - t0
+        out[name] = {"value": ops.n * steps / el, "steps": steps, "seconds": round(el, 1),
+                     "t_window_ms": [0.02, round(t, 2)]}
+    f32, f64 = out["f32"], out["f64"]
+    return {"value": f32["value"], "unit": "node-timesteps/s", "cores": threads, "kind": "port",
+            "sample": f"{f32['steps']} steps (t = 0.03..{f32['t_window_ms'][1]:.2f} ms) of the "
+                      f"same {N}^3 TTP workload, float32, torch CPU restatement of the step "
+                      f"(oracle/torch_cpu.py) on {threads} threads, {f32['seconds']} s; "
+                      f"assembly excluded",
+            "cpu_model": torch_cpu.cpu_model_name(),
+            "physical_cores": torch_cpu.physical_cores(),
+            "float64": {"value": f64["value"], "steps": f64["steps"],
+                        "seconds": f64["seconds"]}}
 
 
 def batched_config(dev, steps, warmup):
@@ -204,7 +230,7 @@ def batched_config(dev, steps, warmup):
             "pcg_iterations_per_step_max": sum(bat.iterations[warmup:]) / steps + 1}
 
 
-def irregular_config(dev, steps, warmup, n_target=640_000):
+def irregular_config(dev, steps, warmup, n_target=640_000, orders=(None, "rcm", "levels")):
     """Irregular volume: the thick ellipsoidal shell (semi-axes 40/30/60 mm,
     wall 10 mm; jittered-lattice Delaunay, about 640k nodes, randomly
     permuted node numbers), four conductivity regions, transmural fibres,
@@ -226,7 +252,7 @@ def irregular_config(dev, steps, warmup, n_target=640_000):
                        f"4 regions, 5 patches",
            "n_nodes": int(nodes.shape[0]), "n_tets": int(elems.shape[0]), "mesh_s": round(t_mesh, 1),
            "steps_timed": steps, "t_window_ms": [warmup * 0.01, (warmup + steps) * 0.01]}
-    for reorder in (None, "rcm", "levels"):
+    for reorder in orders:
         sim = Monodomain(TenTusscherPanfilov("ENDO", 0.01, device=dev, dtype=torch.float32),
                          T=(warmup + steps) * 0.01 + 1.0, dt=0.01, device=dev, dtype=torch.float32,
                          reorder=reorder)
@@ -258,8 +284,9 @@ def irregular_config(dev, steps, warmup, n_target=640_000):
                     "pcg_iterations_per_step": sum(sim.iterations[warmup:]) / steps + 1}
         sim.finish()
         del sim
-    out["speedup_rcm"] = round(out["mesh_order"]["ms_per_step"] / out["rcm"]["ms_per_step"], 2)
-    out["speedup_levels"] = round(out["mesh_order"]["ms_per_step"] / out["levels"]["ms_per_step"], 2)
+    for o in ("rcm", "levels"):
+        if o in out and "mesh_order" in out:
+            out["speedup_" + o] = round(out["mesh_order"]["ms_per_step"] / out[o]["ms_per_step"], 2)
     return out
 
 
@@ -351,21 +378,82 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": a.numel() * 4}), flush=True)
 
 
+def kernel_stats(kt, B):
+    """{kernel: launches, average microseconds, and for the bandwidth-bound
+    passes their algorithmic bytes and GB/s} from kernel_times()."""
+    kernels = {}
+    for k, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        avg_s = ms / cnt / 1e3
+        kernels[k] = {"launches": cnt, "avg_us": round(avg_s * 1e6, 2), "total_ms": round(ms, 3)}
+        if k in B:
+            kernels[k].update(alg_bytes=B[k], GBps=round(B[k] / avg_s / 1e9, 1))
+    return kernels
+
+
+def roofline(kernels, B, N):
+    """The dominant kernel (largest total time) against the HBM peak."""
+    dom = max((k for k in kernels if k in B), key=lambda k: kernels[k]["total_ms"])
+    a = kernels[dom]["GBps"]
+    traffic = MEASURED_TRAFFIC_101.get(dom) if N == 101 else None
+    return {"kernel": dom, "bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s", "frac": round(a / HBM_PEAK_GBPS, 4), "traffic": traffic,
+            "alg_bytes": B[dom]}
+
+
+def pcg_rate(kernels, B):
+    """The PCG phase (RHS + start pass, spmv and update passes, check) by the
+    algorithmic bytes its fused passes move, over its measured time."""
+    ks = ("rhs_init", "pcg_spmv", "pcg_update")
+    if not all(k in kernels for k in ks):
+        return None
+    moved = sum(kernels[k]["launches"] * B[k] for k in ks)
+    t = sum(kernels[k]["total_ms"] for k in ks + ("pcg_check",) if k in kernels) / 1e3
+    return round(moved / t / 1e9, 1)
+
+
+def window(N, dev, warmup, steps, period):
+    """A fresh run of the workload: `warmup` steps, then `steps` steps timed
+    as a whole, with per-kernel HIP events on every `period`-th step (0:
+    none).  Returns (sim, elapsed s, iterations of the timed steps, t_setup)."""
+    import torch
+    sim, _ = build_sim(N, dev)
+    t0 = time.perf_counter()
+    sim.start(1e-5, 1e-5, 100, calculate_AT_RT=True, linear_guess=True,
+              snapshot_interval=1e9, verbose=False)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    sim.advance(warmup)
+    torch.cuda.synchronize()
+    if period:
+        sim.set_kernel_timing(True, period=period)
+    it0 = len(sim.iterations)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.advance(steps)
+    torch.cuda.synchronize()
+    return sim, time.perf_counter() - t0, sim.iterations[it0:], t_setup
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=101, help="nodes per cube side")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU work per storage type in the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-steady", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-shell", action="store_true")
     ap.add_argument("--served-requests", type=int, default=32)
-    ap.add_argument("--shell", action="store_true",
-                    help="add the irregular ellipsoidal-shell workload (mesh order vs RCM)")
-    ap.add_argument("--shell-steps", type=int, default=1000)
+    ap.add_argument("--shell-rcm", action="store_true",
+                    help="also run the shell in plain reverse Cuthill-McKee order")
+    ap.add_argument("--shell-steps", type=int, default=200)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-warmup", type=int, default=200)
     ap.add_argument("--timing-period", type=int, default=16,
@@ -386,66 +474,54 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     N = args.n
-    print(f"[bench] building {N}^3 workload", flush=True)
-    sim, mesh = build_sim(N, dev)
-    t0 = time.perf_counter()
-    sim.start(1e-5, 1e-5, 100, calculate_AT_RT=True, linear_guess=True,
-              snapshot_interval=1e9, verbose=False)
-    torch.cuda.synchronize()
-    t_setup = time.perf_counter() - t0
+    # 1. the timed pass: events only when they leave >= 16 samples
+    period = args.timing_period
+    if args.no_kernel_timing or args.steps // max(period, 1) < 16:
+        period = 0
+    print(f"[bench] building {N}^3 workload; warmup {args.warmup}, timing {args.steps} steps",
+          flush=True)
+    sim, elapsed, iters, t_setup = window(N, dev, args.warmup, args.steps, period)
     n, nnz = sim.n_nodes, sim.system.nnz
-    print(f"[bench] assembled n={n} nnz={nnz} in {t_setup:.2f} s; warmup {args.warmup}", flush=True)
-    sim.advance(args.warmup)
-    torch.cuda.synchronize()
-    if not args.no_kernel_timing:
-        sim.set_kernel_timing(True, period=args.timing_period)
-    it0 = len(sim.iterations)
-    print(f"[bench] timing {args.steps} steps", flush=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    sim.advance(args.steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    iters = sim.iterations[it0:]
     value = n * args.steps / elapsed
-    kt = {} if args.no_kernel_timing else sim.kernel_times()
     S = len(sim.ionic_model.STATES)
     B = alg_bytes(n, nnz, S, column_bytes(sim.system), xb=sim._X[0].element_size(),
                   wb=sim._w.element_size())
-    kernels = {}
-    for k, (ms, cnt) in kt.items():
-        if cnt == 0:
-            continue
-        avg_s = ms / cnt / 1e3
-        kernels[k] = {"launches": cnt, "avg_us": round(avg_s * 1e6, 2), "total_ms": round(ms, 2)}
-        if k in B:
-            kernels[k].update(alg_bytes=B[k], GBps=round(B[k] / avg_s / 1e9, 1))
-    roof = None
-    pcg_rate = pcg_moved = None
-    if kernels:
-        dom = max((k for k in kernels if k in B), key=lambda k: kernels[k]["total_ms"])
-        a = kernels[dom]["GBps"]
-        traffic = MEASURED_TRAFFIC_101.get(dom) if N == 101 else None
-        roof = {"kernel": dom, "bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(a / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                "alg_bytes": B[dom]}
-        b_rhs, b_init, b_iter = survey_electric_bytes(n, nnz)
-        n_it = kernels.get("pcg_spmv", {}).get("launches", 0)
-        n_st = kernels.get("rhs_init", {}).get("launches", 0)
-        ele_bytes = n_st * (b_rhs + b_init) + n_it * b_iter
-        ele_s = sum(kernels[k]["total_ms"] for k in ("rhs_init", "pcg_spmv", "pcg_update",
-                                                     "pcg_check") if k in kernels) / 1e3
-        pcg_rate = round(ele_bytes / ele_s / 1e9, 1)
-        # the same phase by the bytes the fused passes actually move
-        moved = sum(kernels[k]["launches"] * B[k] for k in ("rhs_init", "pcg_spmv", "pcg_update")
-                    if k in kernels)
-        pcg_moved = round(moved / ele_s / 1e9, 1)
-    # release the single-run workload's device memory before the next parts,
-    # so that each part runs on a device holding only its own data
+    kt = sim.kernel_times() if period else {}
     n_elems = int(sim.elems.shape[0])
+    ops = host_operators(sim, N) if not args.no_cpu_baseline else None
     sim._free_sim()
     del sim
     torch.cuda.empty_cache()
+    timing_note = f"timed pass, every {period}th step"
+    if not period and not args.no_kernel_timing:
+        # 2. a second pass over the same step window with every step timed
+        print(f"[bench] kernel timing: the same {args.steps}-step window, every step", flush=True)
+        sim, _, it2, _ = window(N, dev, args.warmup, args.steps, 1)
+        kt = sim.kernel_times()
+        timing_note = ("second pass over the same step window (same warm-up and steps), every "
+                       f"step timed; its n_iter matched the timed pass: {it2 == iters}")
+        sim._free_sim()
+        del sim
+        torch.cuda.empty_cache()
+    kernels = kernel_stats(kt, B)
+    roof = roofline(kernels, B, N) if kernels else None
+    pcg = pcg_rate(kernels, B) if kernels else None
+    # 3. a steady-state window (the wave well under way)
+    steady = None
+    if not args.no_steady:
+        ws, ss = 200, 2000
+        print(f"[bench] steady state: {ss} steps after {ws}", flush=True)
+        sim, el, it3, _ = window(N, dev, ws, ss, 0 if args.no_kernel_timing else 16)
+        kst = kernel_stats(sim.kernel_times(), B) if not args.no_kernel_timing else {}
+        steady = {"steps": ss, "t_window_ms": [ws * 0.01, (ws + ss) * 0.01],
+                  "ms_per_step": round(el / ss * 1e3, 4), "node_timesteps_per_s": n * ss / el,
+                  "pcg_iterations_per_step": sum(it3) / ss + 1,
+                  "kernel_avg_us": {k: v["avg_us"] for k, v in kst.items()},
+                  "roofline_frac": roofline(kst, B, N)["frac"] if kst else None,
+                  "pcg_hbm_GBps": pcg_rate(kst, B) if kst else None}
+        sim._free_sim()
+        del sim
+        torch.cuda.empty_cache()
     batched = None
     if not args.no_batched:
         batched = batched_config(dev, args.batch_steps, args.batch_warmup)
@@ -457,11 +533,14 @@ def main():
         gc.collect()
         torch.cuda.empty_cache()
     irregular = None
-    if args.shell:
-        irregular = irregular_config(dev, args.shell_steps, 700)
+    if not args.no_shell:
+        orders = (None, "rcm", "levels") if args.shell_rcm else (None, "levels")
+        irregular = irregular_config(dev, args.shell_steps, 700, orders=orders)
+        gc.collect()
+        torch.cuda.empty_cache()
     cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(N, mesh, args.cpu_steps)
+    if ops is not None:
+        cpu = cpu_baseline(ops, N, args.cpu_seconds)
     line = {
         "metric": METRIC, "value": value, "unit": "node-timesteps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -472,10 +551,11 @@ def main():
                    "n_nodes": n, "nnz": nnz, "t_window_ms": [args.warmup * 0.01,
                                                             (args.warmup + args.steps) * 0.01],
                    "pcg_iterations_per_step": sum(iters) / max(len(iters), 1) + 1},
-        "pcg_hbm_GBps": pcg_rate,
-        "pcg_moved_GBps": pcg_moved,
+        "pcg_hbm_GBps": pcg,
+        "kernel_timing": timing_note,
         "kernels": kernels,
         "roofline": roof,
+        "steady_state": steady,
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,

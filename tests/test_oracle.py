@@ -500,3 +500,39 @@ def test_loop_at_rt_stamps_are_the_step_time():
     # no solution value sits within roundoff of a threshold
     assert np.allclose(r["u"], v, atol=1e-9)
     assert np.all(r["AT"] == at) and np.all(r["RT"] == rt)
+
+
+# -------------------------------------------------- torch-CPU baseline
+def test_torch_cpu_baseline_matches_numpy_oracle():
+    """The CPU baseline's torch restatement of the step (oracle/torch_cpu.py)
+    follows the numpy oracle: float64, a 7^3 TTP cube with a corner stimulus,
+    100 steps: same n_iter per step, V within 1e-9 mV; float32 within 0.5 mV."""
+    torch = pytest.importorskip("torch")
+    from oracle import torch_cpu
+    N, steps = 7, 100
+    nodes, elems, regions, fib = meshes.cube_workload(N)
+    stim = meshes.corner_nodes(N, 3)
+
+    def oracle_model():
+        om = monodomain.Monodomain(ionic.TenTusscherPanfilov("ENDO", 0.01), T=1.0, dt=0.01)
+        om.load_arrays(nodes, elems, regions, fib)
+        om.add_conductivity([0], 0.5272, 0.2076)
+        om.add_conductivity([1], 0.9074, 0.3332)
+        om.add_stimulus(stim, 0.0, 1.0, 100)
+        return om
+
+    ref = oracle_model().solve(1e-5, 1e-5, 100)
+    for dtype, tol in ((torch.float64, 1e-9), (torch.float32, 0.5)):
+        om = oracle_model()
+        om.assemble()
+        tm = torch_cpu.Monodomain(om, torch_cpu.TenTusscherPanfilov("ENDO", 0.01, dtype), dtype)
+        u = tm.initialize()
+        t, its = 0.0, []
+        for _ in range(steps):
+            t += 0.01
+            u, it = tm.step(u, t)
+            its.append(it)
+        assert u.dtype == dtype
+        assert np.abs(u.double().numpy() - ref["u"]).max() < tol
+        if dtype == torch.float64:
+            assert its == ref["iters"].tolist()

@@ -52,6 +52,7 @@ struct Plan {
   int K;
   long long seq;  // sequence number the check pass writes for this step
   bool timed;
+  int evi;        // event set of a timed step (index into tc_sim::ev_sets)
   int64_t frame;  // frame written after this step or -1
   HostState after;
 };
@@ -73,7 +74,14 @@ struct tc_sim {
   tc::HostState vs;  // validated
   std::deque<int> recent;
   int timing;
-  std::vector<hipEvent_t> tev[tc::kRing];
+  // Event sets of timed steps: a step takes a free set at enqueue; when its
+  // status is consumed the set waits in ev_done, with the iterations it
+  // timed, until the end of the advance call, where the elapsed times are
+  // read (the events are complete by then, so no query waits and the host
+  // loop that keeps the pipeline fed does no timing work).
+  std::vector<std::vector<hipEvent_t>> ev_sets;
+  std::vector<int> ev_free;
+  std::vector<std::pair<int, int>> ev_done;
   double acc_ms[tc::kTimers];
   int64_t acc_n[tc::kTimers];
   // frame buffer of the current advance call (host memory in ring mode)
@@ -94,12 +102,19 @@ namespace {
 
 uint32_t active_bits(const tc_sim_desc& d, double t) { return stimulus_bits(d.stim, d.n_stim, t); }
 
-int ev_reserve(tc_sim* S, int slot, size_t count) {
-  while (S->tev[slot].size() < count) {
+int ev_take(tc_sim* S, size_t count, int* evi) {
+  if (S->ev_free.empty()) {
+    S->ev_sets.emplace_back();
+    S->ev_free.push_back((int)S->ev_sets.size() - 1);
+  }
+  const int slot = S->ev_free.back();
+  S->ev_free.pop_back();
+  *evi = slot;
+  while (S->ev_sets[slot].size() < count) {
     hipEvent_t e;
     // timing only: no system-scope fence (no cache write-back per record)
     TC_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
-    S->tev[slot].push_back(e);
+    S->ev_sets[slot].push_back(e);
   }
   return TC_OK;
 }
@@ -162,9 +177,26 @@ int check_launch(tc_sim* S, int k, int slot, long long seq, hipStream_t s,
 }
 
 // Wait until the check pass has published record `seq` in slot `slot`.
+void accumulate_timing(tc_sim* S, const std::vector<hipEvent_t>& ev, int kk);
+
+// Reads the elapsed times of one consumed timed step whose events are
+// complete (false when there is none), so that the host does its timing
+// work while it waits for the device anyway.
+bool drain_one_timing(tc_sim* S) {
+  if (S->ev_done.empty()) return false;
+  const std::pair<int, int> dn = S->ev_done.back();
+  const std::vector<hipEvent_t>& ev = S->ev_sets[dn.first];
+  if (hipEventQuery(ev[5]) != hipSuccess) return false;
+  accumulate_timing(S, ev, dn.second);
+  S->ev_free.push_back(dn.first);
+  S->ev_done.pop_back();
+  return true;
+}
+
 int wait_status(tc_sim* S, int slot, long long seq) {
   const auto t0 = std::chrono::steady_clock::now();
   for (long spins = 0; !status_take(S->host_status + slot, seq, S->status + slot); ++spins) {
+    if (drain_one_timing(S)) continue;
     if ((spins & 1023) == 1023) {
       if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(120)) {
         set_error("time-step runtime: no status for record %lld after 120 s", seq);
@@ -235,10 +267,11 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   const int xc = ps.xc, pc = ps.pc;
   std::vector<hipEvent_t>* ev = nullptr;
   plan.timed = S->timing > 0 && step % S->timing == 0;
+  plan.evi = -1;
   if (plan.timed) {
-    int rc = ev_reserve(S, slot, 6 + 4 * (size_t)K);
+    int rc = ev_take(S, 6 + 4 * (size_t)K, &plan.evi);
     if (rc) return rc;
-    ev = &S->tev[slot];
+    ev = &S->ev_sets[plan.evi];
   }
   // 1. ionic
   tc_ionic_io io{};
@@ -354,8 +387,9 @@ int resolve_stall(tc_sim* S, const Plan& plan, hipStream_t s) {
   return TC_OK;
 }
 
-void accumulate_timing(tc_sim* S, const Plan& plan, int iters_run) {
-  std::vector<hipEvent_t>& ev = S->tev[plan.slot];
+// Elapsed times of one timed step: ionic, rhs_init and check, and the
+// first kk of its speculative iterations (those that ran).
+void accumulate_timing(tc_sim* S, const std::vector<hipEvent_t>& ev, int kk) {
   (void)hipEventSynchronize(ev[5]);
   float ms = 0.f;
   const int pairs[3][2] = {{0, 1}, {2, 3}, {4, 5}};
@@ -366,7 +400,6 @@ void accumulate_timing(tc_sim* S, const Plan& plan, int iters_run) {
       S->acc_n[which[k]] += 1;
     }
   }
-  const int kk = iters_run < plan.K ? iters_run : plan.K;
   for (int k = 0; k < kk; ++k) {
     const size_t e = 6 + 4 * (size_t)k;
     if (hipEventElapsedTime(&ms, ev[e], ev[e + 1]) == hipSuccess) {
@@ -398,11 +431,15 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     int rc = wait_status(S, plan.slot, plan.seq);
     if (rc) return rc;
     StepStatus* st = S->status + plan.slot;
-    if (plan.timed) accumulate_timing(S, plan, st->iter);
+    if (plan.timed)
+      S->ev_done.emplace_back(plan.evi, st->iter < plan.K ? st->iter : plan.K);
     if (st->stalled) {
       int rc = resolve_stall<T, XT>(S, plan, s);
       if (rc) return rc;
       enq -= (int64_t)inflight.size() - 1;
+      // the later steps are re-issued: their event sets are free again
+      for (size_t j = 1; j < inflight.size(); ++j)
+        if (inflight[j].timed) S->ev_free.push_back(inflight[j].evi);
       inflight.clear();
       ps = plan.after;
     } else {
@@ -415,6 +452,11 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     ++done;
   }
   TC_HIP(hipStreamSynchronize(s));
+  for (const auto& dn : S->ev_done) {
+    accumulate_timing(S, S->ev_sets[dn.first], dn.second);
+    S->ev_free.push_back(dn.first);
+  }
+  S->ev_done.clear();
   if (S->ring) {
     TC_HIP(hipStreamSynchronize(S->cs));
     for (int j = 0; j < kMaxRingSlots; ++j) S->slot_busy[j] = false;
@@ -501,8 +543,8 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
 
 extern "C" int tc_sim_destroy(tc_sim* S) {
   if (!S) return TC_OK;
-  for (int k = 0; k < kRing; ++k)
-    for (hipEvent_t e : S->tev[k]) (void)hipEventDestroy(e);
+  for (auto& set : S->ev_sets)
+    for (hipEvent_t e : set) (void)hipEventDestroy(e);
   if (S->cs) {
     (void)hipStreamSynchronize(S->cs);
     for (int j = 0; j < kMaxRingSlots; ++j) {
