@@ -366,6 +366,37 @@ def served_config(dev, n_req, budget_nodes=4_000_000):
             "speedup_vs_back_to_back": round(summary["sims_per_hour"] / b2b["sims_per_hour"], 2)}
 
 
+def copy_ceiling(nbytes, reps=10):
+    """GB/s of a plain device copy that reads nbytes/2 and writes nbytes/2
+    (the library's tc_stream_copy: one 16-byte load and store per thread),
+    with the 256 MB last-level cache evicted before each timed copy by a
+    1 GiB copy: what HBM delivers to a kernel that moves the same bytes
+    once.  Median of `reps` copies timed with events on the copy's stream."""
+    import torch
+    from torchcor_amd import _lib as L
+    half = int(nbytes // 2) // 16 * 16
+    a = torch.ones(half // 4, device="cuda:0")
+    b = torch.empty_like(a)
+    fa = torch.ones(1 << 28, device="cuda:0")
+    fb = torch.empty_like(fa)
+    s = L.stream_handle()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ts = []
+    for r in range(reps + 2):
+        L.call("tc_stream_copy", fa.data_ptr(), fb.data_ptr(), fa.numel() * 4, s)
+        e0.record()
+        L.call("tc_stream_copy", a.data_ptr(), b.data_ptr(), half, s)
+        e1.record()
+        e1.synchronize()
+        if r >= 2:
+            ts.append(e0.elapsed_time(e1) / 1e3)
+    ts.sort()
+    t = ts[len(ts) // 2]
+    del a, b, fa, fb
+    torch.cuda.empty_cache()
+    return round(2 * half / t / 1e9, 1)
+
+
 def copy_probe(mib):
     """Five copies of `mib` MiB between two device buffers: each reads and
     writes exactly mib MiB, which scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -505,6 +536,11 @@ def main():
         torch.cuda.empty_cache()
     kernels = kernel_stats(kt, B)
     roof = roofline(kernels, B, N) if kernels else None
+    if roof is not None:
+        # what a plain copy of the dominant kernel's bytes reaches from HBM
+        cc = copy_ceiling(roof["alg_bytes"])
+        roof["copy_ceiling_GBps"] = cc
+        roof["frac_of_copy"] = round(roof["achieved"] / cc, 4)
     pcg = pcg_rate(kernels, B) if kernels else None
     # 3. a steady-state window (the wave well under way)
     steady = None

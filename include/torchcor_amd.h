@@ -234,6 +234,13 @@ int tc_pcg_solve(tc_pcg* plan, const void* b, void* x, const void* x_prev, void*
  * get_timing returns milliseconds and launch counts for
  * [0] the start pass, [1] the SpMV pass, [2] the update pass, counting only
  * iterations that ran.  No counterpart in the reference. */
+/* Blocks of the plan's PCG passes (a tuning choice: it groups the partial
+ * sums, so results may differ in the last bits): 1 .. the default
+ * (min(ceil(n/512), 1024), rounded down to a multiple of 8), or 0 for the
+ * default.  A time-step runtime on the plan may be switched between
+ * tc_sim_advance calls (its pipeline is drained then), not during one.
+ * No counterpart in the reference. */
+int tc_pcg_set_blocks(tc_pcg* plan, int blocks);
 int tc_pcg_set_timing(tc_pcg* plan, int on);
 int tc_pcg_get_timing(tc_pcg* plan, double* ms_host, int64_t* count_host);
 
@@ -513,6 +520,8 @@ typedef struct tc_batch_desc {
   int depth;                 /* steps enqueued ahead of the status read */
   void* ws;                  /* workspace of tc_batch_workspace_size bytes */
   size_t ws_bytes;
+  int max_blocks;            /* cap on the blocks of the PCG passes, 8..1024, or 0 for
+                                1024 (a tuning choice; it groups the partial sums) */
 } tc_batch_desc;
 typedef struct tc_batch tc_batch;
 int tc_batch_workspace_size(int64_t n, int lanes, int dtype, size_t* bytes_host);
@@ -593,6 +602,9 @@ typedef struct tc_serve_system {
   int n_stim;
   tc_stimulus stim[TC_POOL_STIM];
   int lat;                   /* track AT/RT */
+  int slices_per_block;      /* 64-row slices per block of this system's passes: 8..64, or 0
+                                for the default 16 (a tuning choice; a system's results depend
+                                on it, never on the other systems in the pool) */
 } tc_serve_system;
 typedef struct tc_serve tc_serve;
 int tc_serve_workspace_size(int64_t cap_slices, int max_sys, size_t* bytes_host);
@@ -612,6 +624,11 @@ int tc_serve_advance(tc_serve* serve, int64_t nsteps, int32_t* iters_host, void*
 /* Engine steps since creation, and how many of them needed more PCG
  * iterations than the pipeline had launched (each drains the pipeline). */
 int tc_serve_stats(const tc_serve* serve, int64_t* steps_host, int64_t* stalls_host);
+
+/* Device copy of nbytes (a multiple of 16, 16-byte aligned buffers), one
+ * 16-byte load and store per thread: the bandwidth reference of the
+ * benchmark.  No counterpart in the reference. */
+int tc_stream_copy(const void* src, void* dst, int64_t nbytes, void* stream);
 
 /* LAT/RT update alone (monodomain.py:174-176), for the last step of a run. */
 int tc_lat_update(const void* u, void* AT, void* RT, int64_t n, double t, int dtype,

@@ -8,7 +8,9 @@ Tolerances:
   * float32 (float64 PCG reductions, float64 iterate and right-hand side)
     vs the float64 oracle: max |dVm| <= 0.5 mV over every frame, |dAT| <=
     dt, |dRT| <= dt, the same nodes activated and repolarised (measured:
-    MS2v 0.0013 mV, TTP 0.021 mV; RT off by one step at some nodes).
+    MS2v 0.0013 mV, TTP 0.021 mV; RT off by one step at some nodes).  The
+    times are stored in float32, so dt is widened by the float32 spacing of
+    the latest time (3e-5 ms at 400 ms).
   * 101^3 TTP float32 vs the float64 oracle over the first 20 steps, every
     step's Vm compared: max |dVm| <= 0.5 mV; total n_iter within 2.
 """
@@ -92,8 +94,9 @@ def _compare(gm, ro, dtype):
             assert np.array_equal(got > 0, ref > 0)
             on = ref > 0
             d = np.abs(got[on] - ref[on]).max() if on.any() else 0.0
-            _say(f"[gpu] float32: max |dVm| {err:.4f} mV, max |dT| {d:.4f} ms")
-            assert d <= DT + 1e-6
+            _say(f"[gpu] float32: max |dVm| {err:.4f} mV, max |dT| {d:.6f} ms")
+            # one step, plus the float32 rounding of the stored time stamp
+            assert d <= DT + float(np.spacing(np.float32(ref.max()))) + 1e-9
     return err
 
 

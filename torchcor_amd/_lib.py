@@ -109,7 +109,7 @@ class BatchDesc(C.Structure):
                 ("stim", Stimulus * MAX_STIMULI), ("n_groups", i32),
                 ("groups", BatchGroup * BATCH_MAX_GROUPS), ("a_tol", dbl), ("r_tol", dbl),
                 ("max_iter", i32), ("linear_guess", i32), ("depth", i32), ("ws", p),
-                ("ws_bytes", C.c_size_t)]
+                ("ws_bytes", C.c_size_t), ("max_blocks", i32)]
 
 
 class ServeDesc(C.Structure):
@@ -127,7 +127,7 @@ class ServeSystem(C.Structure):
     _fields_ = [("n", i64), ("n_entries", i64), ("slice_ptr", p), ("col", p), ("dcol", p),
                 ("pat", p), ("slice_tag", p), ("Mv", p), ("Kv", p), ("Av", p), ("Dinv", p), ("model", i32), ("n_states", i32),
                 ("states", p), ("V0", p), ("stim_mask", p), ("n_stim", i32),
-                ("stim", Stimulus * POOL_STIM), ("lat", i32)]
+                ("stim", Stimulus * POOL_STIM), ("lat", i32), ("slices_per_block", i32)]
 
 
 MODEL_MS2V, MODEL_TTP, MODEL_CRN = 0, 1, 2
@@ -173,6 +173,7 @@ FUNCS = {
     "tc_pcg_create": (C.POINTER(p), SELL_P, p, p, i32, p, C.c_size_t),
     "tc_pcg_destroy": (p,),
     "tc_pcg_solve": (p, p, p, p, p, dbl, dbl, i32, i32, INT_P, p),
+    "tc_pcg_set_blocks": (p, i32),
     "tc_pcg_set_timing": (p, i32),
     "tc_pcg_get_timing": (p, C.POINTER(dbl), I64_P),
     "tc_pcg_solve_static": (SELL_P, p, p, i32, p, p, p, p, dbl, dbl, i32, i32, INT_P, p),
@@ -182,6 +183,7 @@ FUNCS = {
     "tc_crn_tables": (C.POINTER(CRNParams), dbl, p, p, p, i32, p),
     "tc_crn_step": (C.POINTER(CRNParams), p, p, p, p, IO_P, i32, p),
     "tc_lat_update": (p, p, p, i64, dbl, i32, p),
+    "tc_stream_copy": (p, p, i64, p),
     "tc_sim_create": (C.POINTER(p), C.POINTER(SimDesc)),
     "tc_sim_destroy": (p,),
     "tc_sim_advance": (p, i64, C.POINTER(C.c_int32), p, i64, i64, p),

@@ -188,7 +188,9 @@ class SharedMeshBatch:
 
     # ------------------------------------------------------------- run
     def start(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True, depth=4,
-              x_f64=True, precise_rhs=True):
+              x_f64=True, precise_rhs=True, max_blocks=None):
+        """Set up the run.  ``max_blocks`` caps the blocks of the PCG passes
+        (default: the tuning table's value for n x lanes)."""
         self.assemble()
         dev, dt_ = self.device, self.dtype
         n, B = self.n_nodes, len(self.instances)
@@ -279,6 +281,10 @@ class SharedMeshBatch:
         d.a_tol, d.r_tol, d.max_iter = float(a_tol), float(r_tol), int(max_iter)
         d.linear_guess = 1 if linear_guess else 0
         d.depth = int(depth)
+        from . import autotune
+        d.max_blocks = int(max_blocks) if max_blocks else \
+            autotune.lookup("batch_max_blocks", n * lanes)
+        self.max_blocks = d.max_blocks
         nb = C.c_size_t(0)
         L.call("tc_batch_workspace_size", n, lanes, d.dtype, C.byref(nb))
         self._ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)

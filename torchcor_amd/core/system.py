@@ -150,6 +150,14 @@ class P1System:
         L.call("tc_pcg_create", C.byref(h), C.byref(self.sell), L.ptr(self.sA), L.ptr(self.Dinv),
                self.code, L.ptr(self.pcg_ws), nb.value)
         self._pcg = h
+        self._tune_plan(h)
+
+    def _tune_plan(self, plan):
+        """Blocks of a PCG plan's passes from the tuning table (by rows)."""
+        from .. import autotune
+        spb = autotune.lookup("pcg_slices_per_block", self.n)
+        self.pcg_blocks = autotune.pcg_blocks(self.n, spb)
+        L.call("tc_pcg_set_blocks", plan, self.pcg_blocks)
 
     def _free_pcg(self):
         if getattr(self, "_pcg", None):
@@ -178,6 +186,7 @@ class P1System:
             L.call("tc_pcg_create", C.byref(h), C.byref(self.sell), L.ptr(self.sA64),
                    L.ptr(self.Dinv64), L.TC_F64, L.ptr(self.pcg64_ws), nb.value)
             self._pcg64 = h
+            self._tune_plan(h)
         return self._pcg64
 
     def __del__(self):

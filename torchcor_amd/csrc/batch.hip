@@ -530,11 +530,13 @@ __global__ void k_bat_init_ctl(BatCtl* ctl, double a_tol, double r_tol) {
   }
 }
 
-inline int bat_blocks(int64_t n, int lanes) {
+// Blocks of the passes: one wavefront per row group, at most `cap` blocks
+// (the workspace is sized for kBMaxBlocks).
+inline int bat_blocks(int64_t n, int lanes, int cap = kBMaxBlocks) {
   const int64_t rpw = 64 / lanes;
   const int64_t ng = ceil_div(n, rpw);
   int64_t g = ceil_div(ng, kBW);
-  if (g > kBMaxBlocks) g = kBMaxBlocks;
+  if (g > cap) g = cap;
   if (g >= kXcd) g -= g % kXcd;
   return (int)(g < 1 ? 1 : g);
 }
@@ -936,6 +938,9 @@ extern "C" int tc_batch_create(tc_batch** out, const tc_batch_desc* d, void* str
     const unsigned long long all = d->B == 64 ? ~0ull : (1ull << d->B) - 1;
     TC_CHECK(seen == all, TC_ERR_ARG, "the model groups do not cover all %d instances", d->B);
   }
+  TC_CHECK(d->max_blocks == 0 || (d->max_blocks >= kXcd && d->max_blocks <= kBMaxBlocks),
+           TC_ERR_ARG, "max_blocks %d outside %d..%d (0: default)", d->max_blocks, kXcd,
+           kBMaxBlocks);
   size_t need = 0;
   int rc = tc_batch_workspace_size(d->n, d->lanes, d->dtype, &need);
   if (rc) return rc;
@@ -956,7 +961,7 @@ extern "C" int tc_batch_create(tc_batch** out, const tc_batch_desc* d, void* str
     S->d.groups[g].params = nullptr;
   }
   S->lg = log2_exact(d->lanes);
-  S->G = bat_blocks(d->n, d->lanes);
+  S->G = bat_blocks(d->n, d->lanes, d->max_blocks > 0 ? d->max_blocks : kBMaxBlocks);
   {
     const int64_t gp = ceil_div(d->n * d->lanes, 256 * 4);
     S->Gp = (int)(gp < 1 ? 1 : (gp > 8192 ? 8192 : gp));

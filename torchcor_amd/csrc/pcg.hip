@@ -235,6 +235,15 @@ extern "C" int tc_pcg_create(tc_pcg** plan, const tc_sell* A, const void* Av, co
   return TC_OK;
 }
 
+extern "C" int tc_pcg_set_blocks(tc_pcg* P, int blocks) {
+  TC_CHECK(P, TC_ERR_ARG, "tc_pcg_set_blocks: null plan");
+  const int mx = pcg_blocks(P->n);  // the workspace holds this many partial slots
+  TC_CHECK(blocks >= 0 && blocks <= mx, TC_ERR_ARG,
+           "tc_pcg_set_blocks: %d blocks outside 1..%d (0: default)", blocks, mx);
+  P->G = blocks > 0 ? blocks : mx;
+  return TC_OK;
+}
+
 extern "C" int tc_pcg_set_timing(tc_pcg* P, int on) {
   TC_CHECK(P, TC_ERR_ARG, "tc_pcg_set_timing: null plan");
   P->timing = on ? 1 : 0;

@@ -87,7 +87,9 @@ __device__ inline LRange block_list(int64_t nl) {
 // partial sums group the same way whatever else is in the pool (a run gives
 // bit-identical results alone or in a mix), and a fold reads at most
 // kSBlkMax partials (one per lane).
-constexpr int kSPer = 16;
+constexpr int kSPer = 16;       // default slices per block
+constexpr int kSPerMin = 8;     // smallest per-system choice (tc_serve_system.slices_per_block)
+constexpr int kSPerMax = 64;
 constexpr int kSBlkMax = 1024;
 
 // One block's work, formed on the host when the lists are rebuilt: a
@@ -789,6 +791,7 @@ struct SysHost {
   int64_t slice0 = 0, ns = 0, entry0 = 0, ne = 0, n = 0;
   bool lat = false;
   bool use16 = false;
+  int spb = kSPer;  // slices per block of this system's passes
 };
 
 struct SPlan {
@@ -848,8 +851,8 @@ SvLayout sv_layout(int64_t cap_slices, int max_sys) {
   // rz2 (2 per system) bnorm alpha beta t (double), total steps (ll), row0 n (i64),
   // active n_iter nstim (int), bits (u32), stim
   L.arrays = take(ms * (6 * 8 + 2 * 8 + 2 * 8 + 4 * 4 + 4) + ms * TC_POOL_STIM * sizeof(tc_stimulus));
-  // blocks (and partial slots): sum over systems of ceil(slices / kSPer), padded to 8
-  const size_t slots = (size_t)cap_slices / kSPer + ms + 16;
+  // blocks (and partial slots): sum over systems of ceil(slices / spb), padded to 8
+  const size_t slots = (size_t)cap_slices / kSPerMin + ms + 16;
   L.part[0] = take(slots * 2 * sizeof(double));
   L.part[1] = take(slots * sizeof(double));
   L.part[2] = take(slots * 2 * sizeof(double));
@@ -878,7 +881,7 @@ SvPool sv_pool(const tc_serve* S) {
   return p;
 }
 
-// Rebuild the slice lists, the blocks (kSPer slices of one system each) and
+// Rebuild the slice lists, the blocks (spb slices of one system each) and
 // the per-system block ranges (pipeline drained).
 int sv_rebuild(tc_serve* S, hipStream_t s) {
   std::vector<int32_t> list, ml[3], sl, vb0, vb1, bl0, bq;
@@ -888,7 +891,7 @@ int sv_rebuild(tc_serve* S, hipStream_t s) {
     const int qi = (int)sl.size();
     sl.push_back(q);
     vb0.push_back((int32_t)bq.size());
-    int64_t nb = ceil_div(h.ns, kSPer);
+    int64_t nb = ceil_div(h.ns, h.spb);
     if (nb > kSBlkMax) nb = kSBlkMax;
     const int64_t per = ceil_div(h.ns, nb);
     for (int64_t k = 0; k < h.ns; ++k) {
@@ -1299,6 +1302,10 @@ extern "C" int tc_serve_add(tc_serve* S, const tc_serve_system* y, int* slot_out
   TC_CHECK(y->n_stim >= 0 && y->n_stim <= TC_POOL_STIM && (y->n_stim == 0 || y->stim_mask),
            TC_ERR_ARG, "tc_serve_add: at most %d stimuli, with a mask", TC_POOL_STIM);
   TC_CHECK(!y->lat || S->d.AT, TC_ERR_ARG, "tc_serve_add: LAT needs AT/RT pool arrays");
+  TC_CHECK(y->slices_per_block == 0 ||
+               (y->slices_per_block >= kSPerMin && y->slices_per_block <= kSPerMax),
+           TC_ERR_ARG, "tc_serve_add: slices_per_block %d outside %d..%d (0: default %d)",
+           y->slices_per_block, kSPerMin, kSPerMax, kSPer);
   const tc_serve_desc& d = S->d;
   hipStream_t s = (hipStream_t)stream;
   TC_HIP(hipStreamSynchronize(s));
@@ -1327,6 +1334,7 @@ extern "C" int tc_serve_add(tc_serve* S, const tc_serve_system* y, int* slot_out
   h.ne = y->n_entries;
   h.n = y->n;
   h.lat = y->lat != 0;
+  h.spb = y->slices_per_block > 0 ? y->slices_per_block : kSPer;
   const int64_t row0 = s0 * 64, nrows = ns * 64;
   const int f64 = d.dtype == TC_F64;
   const int xf = d.x_f64 || f64;

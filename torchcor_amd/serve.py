@@ -65,7 +65,7 @@ class ServeEngine:
 
     def __init__(self, cap_nodes, cap_entries, max_sys=256, dt=0.01, dtype=torch.float32,
                  device=None, a_tol=1e-5, r_tol=1e-5, max_iter=100, x_f64=True,
-                 precise_rhs=True, lat=True, depth=4, int16_columns=True):
+                 precise_rhs=True, lat=True, depth=4, int16_columns=True, slices_per_block=None):
         self.device = torch.device(device) if device is not None else default_device()
         dev = self.device
         self.dtype = dtype
@@ -121,6 +121,9 @@ class ServeEngine:
         self._h = h
         self._desc = d
         self.lat = lat
+        # slices per block of each system's passes: fixed, or by its size
+        # from the tuning table
+        self.slices_per_block = slices_per_block
         self.live = {}
         self.step_iters = []      # largest n_iter among the live runs, per engine step
 
@@ -168,7 +171,10 @@ class ServeEngine:
         L.call("tc_serve_fits", self._h, pr.n_slices, pr.n_entries, C.byref(f))
         return bool(f.value)
 
-    def add(self, pr):
+    def add(self, pr, slices_per_block=None):
+        """Admit a prepared run.  ``slices_per_block`` (8..64) overrides the
+        engine's choice for this system (engine value, else the tuning
+        table by the system's size)."""
         S = pr.system
         y = L.ServeSystem()
         y.n, y.n_entries = pr.n, pr.n_entries
@@ -186,6 +192,9 @@ class ServeEngine:
         for k, s in enumerate(pr.stim):
             y.stim[k] = s
         y.lat = 1 if self.lat else 0
+        from . import autotune
+        spb = slices_per_block or self.slices_per_block
+        y.slices_per_block = int(spb) if spb else autotune.lookup("serve_slices_per_block", pr.n)
         slot = C.c_int(-1)
         L.call("tc_serve_add", self._h, C.byref(y), C.byref(slot), L.stream_handle(self.device))
         self.live[slot.value] = pr
