@@ -358,6 +358,8 @@ def served_config(dev, n_req, budget_nodes=4_000_000):
             "requests": n_req, "wall_s": round(summary["wall_s"], 2),
             "assembly_s": round(summary["prepare_s"], 2),
             "engine_steps": summary["steps"], "stalls": summary["stalls"],
+            "advance_calls": summary["advance_calls"],
+            "advance_s": round(summary["advance_s"], 2),
             "sims_per_hour": round(summary["sims_per_hour"], 1),
             "latency_per_sim_s_p50": round(summary["latency_per_sim_s_p50"], 3),
             "latency_per_sim_s_p99": round(summary["latency_per_sim_s_p99"], 3),

@@ -7,7 +7,8 @@ Three shapes decide how rows map onto blocks in the bandwidth-bound passes:
   wavefronts, a slice being 64 rows.  Default s = 8 (one slice per wavefront
   per block round).
 * ``serve_slices_per_block``: a system in the served pool gets
-  min(ceil(slices / s), 1024) blocks of its own.  Default s = 16.
+  min(ceil(slices / s), 1024) blocks of its own.  Default s = 16.  Tuned as
+  one value for all systems, on a replay of a seeded mixed-size stream.
 * ``batch_max_blocks``: the cap on the blocks of the shared-mesh batch's
   passes.  Default 1024.
 
@@ -136,50 +137,57 @@ def _tune_pcg(dev, sizes, rounds, steps, log):
     return out
 
 
-def _tune_serve(dev, classes, rounds, steps, log, background=61):
-    """A size class's systems beside a large background system (a 61^3 cube,
-    default blocks) that keeps the GPU busy, as the served stream's large
-    runs do: the class's slices-per-block is judged by the engine step time
-    it leaves, not by how it would fill an otherwise idle GPU."""
-    import torch
+def stream_requests(n_req=24, seed=7, t_range=(20.0, 60.0)):
+    """A seeded stream like the served benchmark's: cubes of 11..81 nodes a
+    side (log-uniform), MS2v/TTP/CRN, T uniform in t_range, scaled
+    conductivities, one boundary patch stimulus each; all present at once."""
+    import numpy as np
     from . import meshes
-    from .serve import Request, ServeEngine
-    cond = [([0], 0.5272, 0.2076), ([1], 0.9074, 0.3332)]
-
-    def request(N, model):
+    from .serve import Request
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(n_req):
+        N = int(round(float(np.exp(rng.uniform(np.log(11), np.log(81))))))
+        model = ("ms2v", "ttp", "crn")[int(rng.integers(3))]
+        T = float(rng.uniform(*t_range))
+        sc = float(rng.uniform(0.5, 1.5))
         nodes, elems, regions, fib = meshes.cube_workload(N)
-        return Request(nodes, elems, regions, fib, cond, model,
-                       [(meshes.corner_nodes(N, max(2, N // 10)), 0.0, 1.0, 100.0)], 1000.0)
+        surf = meshes.cube_surface_nodes(N)
+        patch = meshes.surface_patch(nodes, surf, min(200, len(surf) // 4), rng)
+        cond = [([0], 0.5272 * sc, 0.2076 * sc), ([1], 0.9074 * sc, 0.3332 * sc)]
+        reqs.append(Request(nodes, elems, regions, fib, cond, model,
+                            [(patch, 0.0, 1.0, 100.0)], T, tag=k))
+    return reqs
 
-    bg = request(background, "ttp")
-    out = []
-    for N, count in classes:
-        reqs = [request(N, ("ms2v", "ttp", "crn")[k % 3]) for k in range(count)]
-        rows = count * (N ** 3 + 64) + background ** 3 + 64
-        times = {s: [] for s in CANDIDATES["serve_slices_per_block"]}
-        for _ in range(rounds):
-            for s in times:
-                eng = ServeEngine(cap_nodes=rows, cap_entries=16 * rows, max_sys=count + 4,
-                                  dtype=torch.float32, device=dev)
-                eng.add(eng.prepare(bg), slices_per_block=DEFAULTS["serve_slices_per_block"])
-                for r in reqs:
-                    eng.add(eng.prepare(r), slices_per_block=s)
-                eng.advance(100)
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                eng.advance(steps)
-                torch.cuda.synchronize()
-                times[s].append((time.perf_counter() - t0) / steps * 1e3)
-                eng.close()
-                del eng
-                torch.cuda.empty_cache()
-        best = min(times, key=lambda s: _median(times[s]))
-        row = {"rows": N ** 3, "systems": count, "background": f"{background}^3",
-               "value": best,
-               "ms_per_step": {str(s): round(_median(t), 5) for s, t in times.items()}}
-        log(f"[autotune] serve {count} x {N}^3 beside {background}^3: {row}")
-        out.append(row)
-    return out
+
+def _tune_serve(dev, rounds, log, n_req=24):
+    """One slices-per-block for every system, judged on a replay of a seeded
+    stream of mixed sizes (all runs admitted up front, to the end): in the
+    pool the large systems keep the GPU busy, and a shape that helps a
+    small system fill an idle GPU can slow the pool as a whole."""
+    import torch
+    from .serve import ServeEngine
+    reqs = stream_requests(n_req)
+    rows = sum(-(-r.nodes.shape[0] // 64) * 64 for r in reqs)
+    times = {s: [] for s in CANDIDATES["serve_slices_per_block"]}
+    for _ in range(rounds):
+        for s in times:
+            eng = ServeEngine(cap_nodes=rows, cap_entries=16 * rows, max_sys=n_req + 4,
+                              dtype=torch.float32, device=dev, slices_per_block=s)
+            prs = [eng.prepare(r) for r in reqs]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            eng.run_until_done(prs)
+            torch.cuda.synchronize()
+            times[s].append(time.perf_counter() - t0)
+            eng.close()
+            del eng, prs
+            torch.cuda.empty_cache()
+    best = min(times, key=lambda s: _median(times[s]))
+    row = {"rows": 2 ** 62, "stream_requests": n_req, "value": best,
+           "wall_s": {str(s): round(_median(t), 4) for s, t in times.items()}}
+    log(f"[autotune] serve stream of {n_req} runs: {row}")
+    return [row]
 
 
 def _tune_batch(dev, sizes, rounds, steps, log):
@@ -244,8 +252,7 @@ def tune(device=None, quick=False, log=print):
     t0 = time.perf_counter()
     pcg = _tune_pcg(dev, (21, 41, 61, 81, 101) if not quick else (21, 61),
                     rounds, 100 if quick else 200, log)
-    serve = _tune_serve(dev, ((11, 16), (21, 8), (41, 4), (61, 2), (81, 1)) if not quick
-                        else ((11, 8), (41, 2)), rounds, 100 if quick else 300, log)
+    serve = _tune_serve(dev, rounds, log, n_req=12 if quick else 24)
     batch = _tune_batch(dev, (21, 41) if not quick else (21,), rounds, 50 if quick else 150, log)
     return {"device": torch.cuda.get_device_name(dev),
             "seconds": round(time.perf_counter() - t0, 1),

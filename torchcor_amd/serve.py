@@ -284,6 +284,7 @@ class ServeEngine:
 
         worker = concurrent.futures.ThreadPoolExecutor(max_workers=1)
         s_per_step = None                   # measured, smoothed
+        n_calls, t_adv = 0, 0.0             # advance calls and the time spent in them
         st0 = self.stats()
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
@@ -316,7 +317,10 @@ class ServeEngine:
                     k = min(k, max(admit_chunk, int((pending[0].arrival - clock()) / s_per_step) + 1))
                 ta = time.perf_counter()
                 self.advance(k)
-                dt_step = (time.perf_counter() - ta) / k
+                el = time.perf_counter() - ta
+                n_calls += 1
+                t_adv += el
+                dt_step = el / k
                 s_per_step = dt_step if s_per_step is None else 0.7 * s_per_step + 0.3 * dt_step
                 for s in [s for s, pr in self.live.items() if pr.steps_left <= 0]:
                     res = self.remove(s)
@@ -339,6 +343,7 @@ class ServeEngine:
         summary = dict(requests=len(records), wall_s=wall, prepare_s=spent["prepare"],
                        steps=st1["steps"] - st0["steps"], stalls=st1["stalls"] - st0["stalls"],
                        sims_per_hour=len(records) * 3600.0 / wall,
+                       advance_calls=n_calls, advance_s=t_adv,
                        latency_per_sim_s_p50=float(np.percentile(lat, 50)),
                        latency_per_sim_s_p99=float(np.percentile(lat, 99)))
         return records, summary
