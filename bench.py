@@ -227,6 +227,7 @@ def batched_config(dev, steps, warmup):
             "plain_ms_per_step": {k: round(v * 1e3, 4) for k, v in plain.items()},
             "plain_sims_per_hour": round(sph_plain, 1),
             "speedup_vs_back_to_back": round(sph / sph_plain, 2),
+            "max_blocks": bat.max_blocks,
             "pcg_iterations_per_step_max": sum(bat.iterations[warmup:]) / steps + 1}
 
 
@@ -521,6 +522,10 @@ def main():
                   wb=sim._w.element_size())
     kt = sim.kernel_times() if period else {}
     n_elems = int(sim.elems.shape[0])
+    from torchcor_amd import autotune
+    tuning = {"table": str(autotune.TABLE_PATH.relative_to(ROOT)) if autotune._enabled else None,
+              "pcg_blocks": sim.system.pcg_blocks,
+              "pcg_slices_per_block": autotune.lookup("pcg_slices_per_block", n)}
     ops = host_operators(sim, N) if not args.no_cpu_baseline else None
     sim._free_sim()
     del sim
@@ -563,11 +568,14 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_config(dev, args.batch_steps, args.batch_warmup)
+        tuning["batch_max_blocks"] = batched["max_blocks"]
         gc.collect()
         torch.cuda.empty_cache()
     served = None
     if not args.no_served:
         served = served_config(dev, args.served_requests)
+        tuning["serve_slices_per_block"] = sorted({autotune.lookup("serve_slices_per_block", m ** 3)
+                                                   for m in range(11, 82)})
         gc.collect()
         torch.cuda.empty_cache()
     irregular = None
@@ -594,6 +602,7 @@ def main():
         "kernels": kernels,
         "roofline": roof,
         "steady_state": steady,
+        "tuning": tuning,
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
