@@ -250,7 +250,7 @@ int tc_pcg_get_timing(tc_pcg* plan, double* ms_host, int64_t* count_host);
  * kernels, same arguments and results, with no plan and no workspace.  The
  * operator is passed directly (A pattern, Av values, Dinv in `dtype`).  Its
  * scratch memory (control block, partial sums, r, p, p', Ap) is a static
- * device array of the library, 33,603,840 bytes, which every GPU the
+ * device array of the library, 33,605,632 bytes, which every GPU the
  * library runs on holds until the process ends; the largest system it
  * takes has TC_PCG_STATIC_MAX_ROWS rows (larger ones fail with
  * TC_ERR_SIZE).  It allocates nothing (no hipMalloc/hipFree, no pinned
@@ -427,6 +427,8 @@ typedef struct tc_sim_desc {
   int max_iter, linear_guess;
   int depth;                 /* steps enqueued ahead of the status read, >= 1 */
   int64_t state_stride;      /* row stride of the state block (0 = n) */
+  int separate_check;        /* 0: each step's stopping test runs in its last update pass;
+                                1: in a one-block pass of its own (same results) */
 } tc_sim_desc;
 typedef struct tc_sim tc_sim;
 int tc_sim_create(tc_sim** sim, const tc_sim_desc* desc);

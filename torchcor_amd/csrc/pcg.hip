@@ -101,7 +101,8 @@ int pcg_solve(const PcgBufs& P, const CI* sc, const T* b, T* x, const T* xp, T* 
              (const double*)P.part[2], P.part[1]);
       TC_LAUNCH();
       launch(k_pcg_update<T, T>, g, blk, s, ev(4 + 4 * k), ev(5 + 4 * k), n, D, (const T*)pout,
-             (const T*)Ap, xs, r, k, P.ctl, (const double*)P.part[1], P.part[2]);
+             (const T*)Ap, xs, r, k, P.ctl, (const double*)P.part[1], P.part[2],
+             (StatusRec*)nullptr, 0, 0, 0LL);
       TC_LAUNCH();
     }
     it += c;
@@ -301,7 +302,7 @@ namespace tc {
 namespace {
 
 constexpr int64_t kStaticRows = TC_PCG_STATIC_MAX_ROWS;
-constexpr size_t kStaticCtlBytes = 256;
+constexpr size_t kStaticCtlBytes = 2048;
 constexpr size_t kStaticPartBytes = 2 * kMaxBlocks * sizeof(double);   // 2G doubles, G <= 1024
 constexpr size_t kStaticVecBytes = size_t(kStaticRows) * sizeof(double);
 constexpr size_t kStaticBytes = kStaticCtlBytes + 3 * kStaticPartBytes + 4 * kStaticVecBytes;

@@ -30,6 +30,8 @@
 
 namespace tc {
 
+constexpr int kDoneGroups = 8;
+
 struct PcgCtl {
   double rz[2];   // r.z of iteration k in rz[k & 1]
   double bnorm;   // ||Dinv b||
@@ -39,6 +41,14 @@ struct PcgCtl {
   int iter;       // iterations executed
   int stall;      // set when a pipelined step ran out of launched iterations;
                   // every later gated kernel then returns at once
+  // Completion counts of a checking update pass: block b counts itself in
+  // group[b % kDoneGroups] (one group per XCD, each on a cache line of its
+  // own, so no more than G/8 atomic adds queue on one address); the last
+  // block of a group resets that count and counts the group in `done`.
+  unsigned done;
+  struct alignas(128) Count {
+    unsigned n;
+  } group[kDoneGroups];
 };
 
 // Step status in pinned host memory: two 8-byte words, each a 32-bit
@@ -276,11 +286,18 @@ __device__ inline void store_partials(double (&v)[NV], double* partials) {
   }
 }
 
+// A partial sum read at device scope (past the reading XCD's L2), for sums
+// that other blocks of the running kernel stored at device scope.
+__device__ inline double load_dev(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Sum of the previous pass's G partials, in a fixed order, returned to
 // every thread of the block.
 // Blocks of kPcgBlock threads and G <= kMaxBlocks: each thread's loads are
-// all issued before its (in-order) additions.
-template <int NV>
+// all issued before its (in-order) additions.  DEV reads them at device
+// scope (partials stored by other blocks of the same kernel).
+template <int NV, bool DEV = false>
 __device__ inline void fold_partials(const double* __restrict__ partials, int G,
                                      double (&out)[NV]) {
   __shared__ double bc[NV];
@@ -293,7 +310,8 @@ __device__ inline void fold_partials(const double* __restrict__ partials, int G,
       // unconditional loads (a clamped index, then a select) keep the
       // loads of one thread in flight together
       const int b = (int)threadIdx.x + q * kPcgBlock;
-      const double t = partials[k * G + (b < G ? b : G - 1)];
+      const double* src = &partials[k * G + (b < G ? b : G - 1)];
+      const double t = DEV ? load_dev(src) : *src;
       v[k][q] = b < G ? t : 0.0;
     }
 #pragma unroll
@@ -492,69 +510,17 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
   store_partials<1>(red, part1);
 }
 
-// XT is the storage type of the iterate x (float64 in float32 runs of the
-// time-step runtime, as the reference keeps its CG iterate in float64).
-template <class T, class XT = T>
-__global__ void __launch_bounds__(kPcgBlock)
-k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
-             const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, int k, PcgCtl* ctl,
-             const double* __restrict__ part1, double* part2) {
-  const int64_t ns = (n + 63) >> 6;
-  const Split sp = block_slices(ns);
-  const int lane = threadIdx.x & 63, wv = wave_id();
-  // the first slice's operands load while the block reads the flags and
-  // folds the partial sums
-  int64_t s = sp.s0 + wv;
-  int64_t i = s * 64 + lane;
-  XT xv = XT(0);
-  T pv = T(0), av = T(0), rv = T(0), dv = T(0);
-  if (s < sp.s1 && i < n) {
-    xv = x[i];
-    pv = p[i];
-    av = Ap[i];
-    rv = r[i];
-    dv = Dinv[i];
-  }
-  // the flags, r.z and the partial sums are all read before the first wait
-  __shared__ int halted;
-  if (threadIdx.x == 0) halted = ctl->converged | ctl->stall;
-  const double rzk = ctl->rz[k & 1];
-  double v[1];
-  fold_partials<1>(part1, gridDim.x, v);
-  if (halted) return;
-  const double alpha = rzk / v[0];
-  double red[2] = {0.0, 0.0};
-  while (s < sp.s1) {
-    if (i < n) {
-      const XT xi = (XT)((double)xv + alpha * (double)pv);
-      const T ri = (T)((double)rv - alpha * (double)av);
-      x[i] = xi;
-      r[i] = ri;
-      const double z = (double)dv * (double)ri;
-      red[0] += z * z;
-      red[1] += (double)ri * z;
-    }
-    s += kWaves;
-    i = s * 64 + lane;
-    if (s < sp.s1 && i < n) {
-      xv = x[i];
-      pv = p[i];
-      av = Ap[i];
-      rv = r[i];
-      dv = Dinv[i];
-    }
-  }
-  store_partials<2>(red, part2);
-}
-
-// One block: stopping test of the last update of a launch chunk (iteration
-// k; k < 0 when no iteration ran), so the host can read the state.  With
-// `status` the result is also stored there; with `stall_open` an unconverged
-// result below max_iter sets ctl->stall, which gates every later launch of
-// a pipelined run until the host has launched more iterations.
-__global__ void __launch_bounds__(kPcgBlock)
-k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StatusRec* status,
-            int stall_open, int max_iter, long long seq) {
+// Stopping test of the last update of a launch chunk (iteration k; k < 0
+// when no iteration ran), so the host can read the state.  With `status`
+// the result is also stored there; with `stall_open` an unconverged result
+// below max_iter sets ctl->stall, which gates every later launch of a
+// pipelined run until the host has launched more iterations.  Run by one
+// block: the one-block check pass, or the last block of a checking update
+// pass (DEV: the partial sums come from the running kernel's other blocks).
+template <bool DEV>
+__device__ inline void check_block(int k, int G, PcgCtl* ctl, const double* __restrict__ part2,
+                                   StatusRec* status, int stall_open, int max_iter,
+                                   long long seq) {
   __shared__ int flags[2];
   if (threadIdx.x == 0) {
     flags[0] = ctl->stall;
@@ -563,7 +529,7 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StatusR
   // the fold reads the partials while the flags arrive (its result is used
   // only when the solve is still running)
   double v[2] = {0.0, 0.0};
-  if (k >= 0) fold_partials<2>(part2, G, v);
+  if (k >= 0) fold_partials<2, DEV>(part2, G, v);
   __syncthreads();
   if (flags[0]) return;
   int n_iter, iter, conv, stalled = 0;
@@ -587,6 +553,113 @@ k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StatusR
   if (threadIdx.x == 0 && status)
     put_status(status, (uint32_t)n_iter,
                (uint32_t)iter | (conv ? kStConverged : 0u) | (stalled ? kStStalled : 0u), seq);
+}
+
+// XT is the storage type of the iterate x (float64 in float32 runs of the
+// time-step runtime, as the reference keeps its CG iterate in float64).
+// CHECK: this is the last update of a launch chunk and also runs the
+// stopping test (check_block) in the block that finishes last, so no
+// separate check pass is launched.  Every block stores its partial sums at
+// device scope, waits until the stores are complete and then counts itself
+// in its group's count, the last of a group counts the group in ctl->done;
+// the block that counts the last group has every partial sum in memory,
+// folds them exactly as the check pass does (same order, same bits) and
+// resets the counter.  No block waits for another.
+template <class T, class XT = T, bool CHECK = false>
+__global__ void __launch_bounds__(kPcgBlock)
+k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
+             const T* __restrict__ Ap, XT* __restrict__ x, T* __restrict__ r, int k, PcgCtl* ctl,
+             const double* __restrict__ part1, double* part2, StatusRec* status = nullptr,
+             int stall_open = 0, int max_iter = 0, long long seq = 0) {
+  const int64_t ns = (n + 63) >> 6;
+  const Split sp = block_slices(ns);
+  const int lane = threadIdx.x & 63, wv = wave_id();
+  // the first slice's operands load while the block reads the flags and
+  // folds the partial sums
+  int64_t s = sp.s0 + wv;
+  int64_t i = s * 64 + lane;
+  XT xv = XT(0);
+  T pv = T(0), av = T(0), rv = T(0), dv = T(0);
+  if (s < sp.s1 && i < n) {
+    xv = x[i];
+    pv = p[i];
+    av = Ap[i];
+    rv = r[i];
+    dv = Dinv[i];
+  }
+  // the flags, r.z and the partial sums are all read before the first wait
+  __shared__ int halted;
+  if (threadIdx.x == 0) halted = ctl->converged | ctl->stall;
+  const double rzk = ctl->rz[k & 1];
+  double v[1];
+  fold_partials<1>(part1, gridDim.x, v);
+  if (halted && !CHECK) return;
+  double red[2] = {0.0, 0.0};
+  if (!halted) {
+    const double alpha = rzk / v[0];
+    while (s < sp.s1) {
+      if (i < n) {
+        const XT xi = (XT)((double)xv + alpha * (double)pv);
+        const T ri = (T)((double)rv - alpha * (double)av);
+        x[i] = xi;
+        r[i] = ri;
+        const double z = (double)dv * (double)ri;
+        red[0] += z * z;
+        red[1] += (double)ri * z;
+      }
+      s += kWaves;
+      i = s * 64 + lane;
+      if (s < sp.s1 && i < n) {
+        xv = x[i];
+        pv = p[i];
+        av = Ap[i];
+        rv = r[i];
+        dv = Dinv[i];
+      }
+    }
+  }
+  if constexpr (!CHECK) {
+    store_partials<2>(red, part2);
+  } else {
+    __shared__ int last;
+    block_sum<2>(red);
+    if (threadIdx.x == 0) {
+      const int G = gridDim.x;
+      if (!halted) {
+        __hip_atomic_store(&part2[blockIdx.x], red[0], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&part2[G + blockIdx.x], red[1], __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // both stores complete before the block counts itself
+      __builtin_amdgcn_s_waitcnt(0);
+      const int g = blockIdx.x % kDoneGroups;
+      const unsigned members = (unsigned)((G - 1 - g) / kDoneGroups + 1);
+      const unsigned groups = (unsigned)min(G, kDoneGroups);
+      int fin = 0;
+      if (__hip_atomic_fetch_add(&ctl->group[g].n, 1u, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
+        // every block of the group has stored its sums
+        __hip_atomic_store(&ctl->group[g].n, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        fin = __hip_atomic_fetch_add(&ctl->done, 1u, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) == groups - 1;
+      }
+      last = fin;
+    }
+    __syncthreads();
+    if (!last) return;
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&ctl->done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    check_block<true>(k, gridDim.x, ctl, part2, status, stall_open, max_iter, seq);
+  }
+}
+
+// One block: the stopping test of the last update of a launch chunk
+// (check_block) as a pass of its own.
+__global__ void __launch_bounds__(kPcgBlock)
+k_pcg_check(int k, int G, PcgCtl* ctl, const double* __restrict__ part2, StatusRec* status,
+            int stall_open, int max_iter, long long seq) {
+  check_block<false>(k, G, ctl, part2, status, stall_open, max_iter, seq);
 }
 
 // Fused right-hand side and PCG start of one time step:

@@ -52,6 +52,7 @@ struct Plan {
   int K;
   long long seq;  // sequence number the check pass writes for this step
   bool timed;
+  bool fused;     // the stopping test ran in the last update pass
   int evi;        // event set of a timed step (index into tc_sim::ev_sets)
   int64_t frame;  // frame written after this step or -1
   HostState after;
@@ -81,7 +82,13 @@ struct tc_sim {
   // loop that keeps the pipeline fed does no timing work).
   std::vector<std::vector<hipEvent_t>> ev_sets;
   std::vector<int> ev_free;
-  std::vector<std::pair<int, int>> ev_done;
+  struct EvDone {
+    int evi, kk, fused, K;
+  };
+  std::vector<EvDone> ev_done;
+  // the stopping test of a step runs in its last update pass (1, default)
+  // or in a one-block check pass (0)
+  int fuse_check;
   double acc_ms[tc::kTimers];
   int64_t acc_n[tc::kTimers];
   // frame buffer of the current advance call (host memory in ring mode)
@@ -145,9 +152,13 @@ void launch_spmv(tc_pcg* P, const CI* sc, T* pin, T* pout, int k, hipStream_t s,
          (const double*)P->part[0], (const double*)P->part[2], P->part[1]);
 }
 
+// Iterations k0 .. k1-1 (spmv and update passes).  With check_slot >= 0
+// the last update also runs the stopping test and publishes record seq in
+// that status slot (no separate check pass); its events are then
+// (*ev)[4], (*ev)[5], the check pass's.
 template <class T, class XT>
 int launch_iters(tc_sim* S, int k0, int k1, XT* xs, hipStream_t s, std::vector<hipEvent_t>* ev,
-                 size_t ev_base) {
+                 size_t ev_base, int check_slot = -1, long long seq = 0) {
   tc_pcg* P = S->d.pcg;
   const dim3 g(P->G), blk(kPcgBlock);
   T* pp[2] = {(T*)P->p[0], (T*)P->p[1]};
@@ -159,9 +170,17 @@ int launch_iters(tc_sim* S, int k0, int k1, XT* xs, hipStream_t s, std::vector<h
     if (P->A.dcol) launch_spmv<T>(P, P->A.dcol, pin, pout, k, s, ev0, ev1);
     else launch_spmv<T>(P, P->A.col, pin, pout, k, s, ev0, ev1);
     TC_LAUNCH();
-    launch(k_pcg_update<T, XT>, g, blk, s, ev ? (*ev)[e + 2] : nullptr,
-           ev ? (*ev)[e + 3] : nullptr, P->n, (const T*)P->Dinv, (const T*)pout,
-           (const T*)P->Ap, xs, (T*)P->r, k, P->ctl, (const double*)P->part[1], P->part[2]);
+    if (check_slot >= 0 && k == k1 - 1) {
+      launch(k_pcg_update<T, XT, true>, g, blk, s, ev ? (*ev)[4] : nullptr,
+             ev ? (*ev)[5] : nullptr, P->n, (const T*)P->Dinv, (const T*)pout, (const T*)P->Ap,
+             xs, (T*)P->r, k, P->ctl, (const double*)P->part[1], P->part[2],
+             S->dev_view + check_slot, 1, S->d.max_iter, seq);
+    } else {
+      launch(k_pcg_update<T, XT>, g, blk, s, ev ? (*ev)[e + 2] : nullptr,
+             ev ? (*ev)[e + 3] : nullptr, P->n, (const T*)P->Dinv, (const T*)pout,
+             (const T*)P->Ap, xs, (T*)P->r, k, P->ctl, (const double*)P->part[1], P->part[2],
+             (StatusRec*)nullptr, 0, 0, 0LL);
+    }
     TC_LAUNCH();
   }
   return TC_OK;
@@ -177,18 +196,18 @@ int check_launch(tc_sim* S, int k, int slot, long long seq, hipStream_t s,
 }
 
 // Wait until the check pass has published record `seq` in slot `slot`.
-void accumulate_timing(tc_sim* S, const std::vector<hipEvent_t>& ev, int kk);
+void accumulate_timing(tc_sim* S, const std::vector<hipEvent_t>& ev, int kk, int fused, int K);
 
 // Reads the elapsed times of one consumed timed step whose events are
 // complete (false when there is none), so that the host does its timing
 // work while it waits for the device anyway.
 bool drain_one_timing(tc_sim* S) {
   if (S->ev_done.empty()) return false;
-  const std::pair<int, int> dn = S->ev_done.back();
-  const std::vector<hipEvent_t>& ev = S->ev_sets[dn.first];
+  const tc_sim::EvDone dn = S->ev_done.back();
+  const std::vector<hipEvent_t>& ev = S->ev_sets[dn.evi];
   if (hipEventQuery(ev[5]) != hipSuccess) return false;
-  accumulate_timing(S, ev, dn.second);
-  S->ev_free.push_back(dn.first);
+  accumulate_timing(S, ev, dn.kk, dn.fused, dn.K);
+  S->ev_free.push_back(dn.evi);
   S->ev_done.pop_back();
   return true;
 }
@@ -322,14 +341,18 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
     xs = (XT*)d.x[xc];
   }
   TC_LAUNCH();
-  // 3. speculative iterations
-  rc = launch_iters<T, XT>(S, 0, K, xs, s, ev, 6);
-  if (rc) return rc;
-  // 4. check
+  // 3. speculative iterations; 4. the stopping test, in the last update
+  // pass (or a check pass of its own when no iteration is launched or
+  // fusing is off)
   plan.seq = ++S->seq;
-  rc = check_launch(S, K - 1, slot, plan.seq, s, ev ? (*ev)[4] : nullptr,
-                    ev ? (*ev)[5] : nullptr);
+  plan.fused = S->fuse_check && K > 0;
+  rc = launch_iters<T, XT>(S, 0, K, xs, s, ev, 6, plan.fused ? slot : -1, plan.seq);
   if (rc) return rc;
+  if (!plan.fused) {
+    rc = check_launch(S, K - 1, slot, plan.seq, s, ev ? (*ev)[4] : nullptr,
+                      ev ? (*ev)[5] : nullptr);
+    if (rc) return rc;
+  }
   // state after the step
   HostState nx = ps;
   nx.steps = step;
@@ -388,12 +411,16 @@ int resolve_stall(tc_sim* S, const Plan& plan, hipStream_t s) {
 }
 
 // Elapsed times of one timed step: ionic, rhs_init and check, and the
-// first kk of its speculative iterations (those that ran).
-void accumulate_timing(tc_sim* S, const std::vector<hipEvent_t>& ev, int kk) {
+// first kk of its K speculative iterations (those that ran).  With the
+// check fused into the last update pass, events 4-5 bracket that pass: it
+// counts as an update when all K iterations ran, else as the check (the
+// solve had stopped and the pass only ran the test).
+void accumulate_timing(tc_sim* S, const std::vector<hipEvent_t>& ev, int kk, int fused, int K) {
   (void)hipEventSynchronize(ev[5]);
   float ms = 0.f;
+  const bool last_fused = fused && kk >= K;
   const int pairs[3][2] = {{0, 1}, {2, 3}, {4, 5}};
-  const int which[3] = {0, 1, 4};
+  const int which[3] = {0, 1, last_fused ? 3 : 4};
   for (int k = 0; k < 3; ++k) {
     if (hipEventElapsedTime(&ms, ev[pairs[k][0]], ev[pairs[k][1]]) == hipSuccess) {
       S->acc_ms[which[k]] += ms;
@@ -406,6 +433,7 @@ void accumulate_timing(tc_sim* S, const std::vector<hipEvent_t>& ev, int kk) {
       S->acc_ms[2] += ms;
       S->acc_n[2] += 1;
     }
+    if (last_fused && k == K - 1) continue;  // timed as the 4-5 pair
     if (hipEventElapsedTime(&ms, ev[e + 2], ev[e + 3]) == hipSuccess) {
       S->acc_ms[3] += ms;
       S->acc_n[3] += 1;
@@ -432,7 +460,8 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     if (rc) return rc;
     StepStatus* st = S->status + plan.slot;
     if (plan.timed)
-      S->ev_done.emplace_back(plan.evi, st->iter < plan.K ? st->iter : plan.K);
+      S->ev_done.push_back(
+          {plan.evi, st->iter < plan.K ? st->iter : plan.K, plan.fused ? 1 : 0, plan.K});
     if (st->stalled) {
       int rc = resolve_stall<T, XT>(S, plan, s);
       if (rc) return rc;
@@ -453,8 +482,8 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
   }
   TC_HIP(hipStreamSynchronize(s));
   for (const auto& dn : S->ev_done) {
-    accumulate_timing(S, S->ev_sets[dn.first], dn.second);
-    S->ev_free.push_back(dn.first);
+    accumulate_timing(S, S->ev_sets[dn.evi], dn.kk, dn.fused, dn.K);
+    S->ev_free.push_back(dn.evi);
   }
   S->ev_done.clear();
   if (S->ring) {
@@ -533,6 +562,7 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
     return TC_ERR_HIP;
   }
   S->timing = 0;
+  S->fuse_check = d->separate_check ? 0 : 1;
   for (int k = 0; k < kTimers; ++k) {
     S->acc_ms[k] = 0.0;
     S->acc_n[k] = 0;

@@ -269,7 +269,7 @@ class Monodomain:
     # on the device and no host round trip is needed per step.
     def start(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True,
               snapshot_interval=1, verbose=False, depth=4, x_f64=True, precise_rhs=True,
-              frames="auto"):
+              frames="auto", separate_check=False):
         """Set up a run.  ``x_f64`` keeps the PCG iterate in float64 in float32
         runs, as the reference's ConjugateGradient does (solver.py:24-26);
         x_prev stays in the working dtype.  ``precise_rhs`` also keeps the
@@ -279,7 +279,9 @@ class Monodomain:
         memory, as the reference's list of snapshots does
         (monodomain.py:162,186-187); "host" streams each one through a
         device ring into pinned host memory while the run goes on; "auto"
-        is "device" up to FRAMES_DEVICE_BYTES of frames, else "host"."""
+        is "device" up to FRAMES_DEVICE_BYTES of frames, else "host".
+        ``separate_check`` runs each step's PCG stopping test in a pass of
+        its own instead of in the step's last update pass (same results)."""
         import ctypes as C
         self.assemble()
         ion = self.ionic_model
@@ -340,6 +342,7 @@ class Monodomain:
         d.a_tol, d.r_tol, d.max_iter = float(a_tol), float(r_tol), int(max_iter)
         d.linear_guess = 1 if linear_guess else 0
         d.depth = int(depth)
+        d.separate_check = 1 if separate_check else 0
         self._free_sim()
         h = C.c_void_p(0)
         L.call("tc_sim_create", C.byref(h), C.byref(d))
