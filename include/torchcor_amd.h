@@ -429,6 +429,9 @@ typedef struct tc_sim_desc {
   int64_t state_stride;      /* row stride of the state block (0 = n) */
   int separate_check;        /* 0: each step's stopping test runs in its last update pass;
                                 1: in a one-block pass of its own (same results) */
+  int k_history;             /* K (PCG iterations launched per step before its status is
+                                read) = the most any of the last k_history steps needed;
+                                0 = 4, at most 256 (same results for any value) */
 } tc_sim_desc;
 typedef struct tc_sim tc_sim;
 int tc_sim_create(tc_sim** sim, const tc_sim_desc* desc);

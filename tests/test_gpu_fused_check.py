@@ -68,3 +68,29 @@ def test_fused_check_kernel_timing_counts():
     assert kt["ionic"][1] == 200 and kt["rhs_init"][1] == 200
     assert kt["pcg_spmv"][1] >= sum(sim.iterations) + 200 - 1 - 10
     assert kt["pcg_update"][1] + kt["pcg_check"][1] >= 200
+
+
+@pytest.mark.parametrize("model,dtype,N", [("ttp", torch.float32, 21), ("crn", torch.float64, 13)])
+def test_k_history_identical(model, dtype, N):
+    """The number of iterations launched ahead of each status read (set by
+    k_history) changes how often a step stalls or runs halted passes, never
+    the results."""
+    runs = []
+    for H in (1, 2, 16):
+        _say(f"[gpu] k_history {H} {model} {dtype} {N}^3")
+        sim = _sim(N, model, dtype, 15.0)
+        sim.start(1e-5, 1e-5, 100, snapshot_interval=0.5, depth=4, k_history=H)
+        sim.advance(sim.nt)
+        sim.finish()
+        runs.append((sim.u.clone(), list(sim.iterations), sim.activation_time.clone(),
+                     sim.frames.clone()))
+    for other in runs[1:]:
+        assert torch.equal(runs[0][0], other[0])
+        assert runs[0][1] == other[1]
+        assert torch.equal(runs[0][2], other[2]) and torch.equal(runs[0][3], other[3])
+
+
+def test_k_history_out_of_range():
+    sim = _sim(9, "ms2v", torch.float32, 1.0)
+    with pytest.raises(RuntimeError, match="k_history"):
+        sim.start(1e-5, 1e-5, 100, k_history=257)

@@ -88,7 +88,7 @@ class SimDesc(C.Structure):
                 ("w", p), ("AT", p), ("RT", p),
                 ("stim_mask", p), ("n_stim", i32), ("stim", Stimulus * MAX_STIMULI),
                 ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32), ("linear_guess", i32),
-                ("depth", i32), ("state_stride", i64), ("separate_check", i32)]
+                ("depth", i32), ("state_stride", i64), ("separate_check", i32), ("k_history", i32)]
 
 
 BATCH_MAX_K = 5

@@ -424,8 +424,14 @@ struct SpmvChunk {
 
 // spmv pass of iteration k.  pin is the previous p (unused when k == 0),
 // pout receives p'.  Iteration k-1's stopping test happens here first.  The
-// first chunk of each wave's first slice is fetched before the block reads
-// the flags and folds the partial sums, so those latencies overlap.
+// first chunk of each wave's first slice is fetched before the block folds
+// the partial sums, so those latencies overlap.  Each wave reads the flags
+// beside its slice header and skips that fetch (two dependent loads: the
+// columns, then the gathers) when the pass is halted, so a pass launched
+// for a step that already stopped costs little more than the launch.  A
+// wave and the block can only disagree on the flags if `converged` is set
+// during this pass, which block 0 does only when the stopping test holds,
+// and then every block returns at that test before using a fetched chunk.
 template <class T, class CI>
 __global__ void __launch_bounds__(kPcgBlock)
 k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ scol,
@@ -441,14 +447,18 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
   SpmvChunk<T, CI> ch;
   int32_t b = 0, wd = 0;
   int sh = 0;
+  const int wave_halted = ctl->converged | ctl->stall;
+  bool fetched = false;
   if (s < sp.s1) {
     const SliceHd hd = slice_hd(sptr, s);
     b = hd.b;
     wd = hd.wd;
     sh = hd.sh;
-    if (wd > 0)
+    if (wd > 0 && !wave_halted) {
       ch.fetch(slice_cols(scol, pat, sh, b, lane, n), Av + b + lane, 0, wd, s * 64 + lane, Dinv, r,
                pin, k);
+      fetched = true;
+    }
   }
   // the flags and the partial sums are read before the first wait
   __shared__ int halted;
@@ -493,7 +503,7 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
     const T* a = Av + b + lane;
     double acc = 0.0;
     for (int j0 = 0; j0 < wd; j0 += kChunk) {
-      if (!(first && j0 == 0)) ch.fetch(c, a, j0, wd, i, Dinv, r, pin, k);
+      if (!(first && j0 == 0 && fetched)) ch.fetch(c, a, j0, wd, i, Dinv, r, pin, k);
       acc = ch.add(acc, j0, wd, k, beta);
     }
     first = false;
@@ -593,9 +603,17 @@ k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
   const double rzk = ctl->rz[k & 1];
   double v[1];
   fold_partials<1>(part1, gridDim.x, v);
-  if (halted && !CHECK) return;
+  if (halted) {
+    // The flags are the same in every block: nothing writes them during
+    // this pass before the last count.  A pass halted by convergence
+    // publishes the status the check would (from the control block), one
+    // halted by a stall publishes nothing; no block counts itself.
+    if (CHECK && blockIdx.x == 0 && threadIdx.x == 0 && status && !ctl->stall)
+      put_status(status, (uint32_t)ctl->n_iter, (uint32_t)ctl->iter | kStConverged, seq);
+    return;
+  }
   double red[2] = {0.0, 0.0};
-  if (!halted) {
+  {
     const double alpha = rzk / v[0];
     while (s < sp.s1) {
       if (i < n) {
@@ -625,12 +643,9 @@ k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
     block_sum<2>(red);
     if (threadIdx.x == 0) {
       const int G = gridDim.x;
-      if (!halted) {
-        __hip_atomic_store(&part2[blockIdx.x], red[0], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&part2[G + blockIdx.x], red[1], __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      }
+      __hip_atomic_store(&part2[blockIdx.x], red[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&part2[G + blockIdx.x], red[1], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
       // both stores complete before the block counts itself
       __builtin_amdgcn_s_waitcnt(0);
       const int g = blockIdx.x % kDoneGroups;

@@ -33,7 +33,7 @@ namespace {
 
 constexpr int kRing = 64;
 constexpr int kTimers = 5;  // ionic, rhs+start, spmv, update, check
-constexpr int kHistory = 16;
+constexpr int kHistoryDefault = 4, kHistoryMax = 256;
 constexpr int kMaxRingSlots = 16;
 
 struct HostState {
@@ -89,6 +89,7 @@ struct tc_sim {
   // the stopping test of a step runs in its last update pass (1, default)
   // or in a one-block check pass (0)
   int fuse_check;
+  int history;  // steps whose iteration counts set K
   double acc_ms[tc::kTimers];
   int64_t acc_n[tc::kTimers];
   // frame buffer of the current advance call (host memory in ring mode)
@@ -476,7 +477,7 @@ int advance(tc_sim* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     }
     iters[done] = st->n_iter;
     S->recent.push_back(st->iter > 0 ? st->iter : 1);
-    if ((int)S->recent.size() > kHistory) S->recent.pop_front();
+    if ((int)S->recent.size() > S->history) S->recent.pop_front();
     S->vs = plan.after;
     ++done;
   }
@@ -527,6 +528,8 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
   TC_CHECK(d->n_stim == 0 || d->stim_mask, TC_ERR_ARG, "stimuli need a mask");
   TC_CHECK(d->state_stride == 0 || d->state_stride >= d->n, TC_ERR_ARG,
            "tc_sim_create: state stride below n");
+  TC_CHECK(d->k_history >= 0 && d->k_history <= kHistoryMax, TC_ERR_ARG,
+           "k_history %d outside 0..%d", d->k_history, kHistoryMax);
   TC_CHECK(d->max_iter >= 0 && d->max_iter <= kMaxIterLimit, TC_ERR_ARG,
            "max_iter %d outside 0..%d", d->max_iter, kMaxIterLimit);
   TC_CHECK(!d->w_f64 || d->x_f64 || d->dtype == TC_F64, TC_ERR_ARG, "w_f64 needs x_f64");
@@ -563,6 +566,7 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
   }
   S->timing = 0;
   S->fuse_check = d->separate_check ? 0 : 1;
+  S->history = d->k_history > 0 ? d->k_history : kHistoryDefault;
   for (int k = 0; k < kTimers; ++k) {
     S->acc_ms[k] = 0.0;
     S->acc_n[k] = 0;

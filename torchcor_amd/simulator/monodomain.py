@@ -269,7 +269,7 @@ class Monodomain:
     # on the device and no host round trip is needed per step.
     def start(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True,
               snapshot_interval=1, verbose=False, depth=4, x_f64=True, precise_rhs=True,
-              frames="auto", separate_check=False):
+              frames="auto", separate_check=False, k_history=0):
         """Set up a run.  ``x_f64`` keeps the PCG iterate in float64 in float32
         runs, as the reference's ConjugateGradient does (solver.py:24-26);
         x_prev stays in the working dtype.  ``precise_rhs`` also keeps the
@@ -281,7 +281,11 @@ class Monodomain:
         device ring into pinned host memory while the run goes on; "auto"
         is "device" up to FRAMES_DEVICE_BYTES of frames, else "host".
         ``separate_check`` runs each step's PCG stopping test in a pass of
-        its own instead of in the step's last update pass (same results)."""
+        its own instead of in the step's last update pass (same results).
+        ``k_history``: each step launches as many PCG iterations as the most
+        any of the last ``k_history`` steps needed (0: the runtime's
+        default) before its status is read; any value gives the same
+        results."""
         import ctypes as C
         self.assemble()
         ion = self.ionic_model
@@ -343,6 +347,7 @@ class Monodomain:
         d.linear_guess = 1 if linear_guess else 0
         d.depth = int(depth)
         d.separate_check = 1 if separate_check else 0
+        d.k_history = int(k_history)
         self._free_sim()
         h = C.c_void_p(0)
         L.call("tc_sim_create", C.byref(h), C.byref(d))
