@@ -400,6 +400,72 @@ def copy_ceiling(nbytes, reps=10):
     return round(2 * half / t / 1e9, 1)
 
 
+STEP_KERNELS = ("k_ttp", "k_crn", "k_ms2v", "k_rhs_init", "k_pcg_spmv", "k_pcg_update",
+                "k_pcg_check", "k_set_int", "k_copy_gated")
+
+
+def _kernel_label(name):
+    """Short label of a kernel name from a rocprofv3 trace; the update pass
+    that also runs the stopping test (last template argument true) is
+    labelled k_pcg_update+check."""
+    base = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+    head = base.split("<")[0]
+    for k in STEP_KERNELS:
+        if head.endswith(k):
+            if k == "k_pcg_update" and base[base.find("<") + 1:base.rfind(">")].split(",")[-1].strip() == "true":
+                return "k_pcg_update+check"
+            return k
+    return head[-40:]
+
+
+def trace_summary(path, steps):
+    """Per-kernel statistics of the time-step windows in a rocprofv3
+    kernel_trace.csv of this script.  A pass is a run of time-step kernels
+    with no other kernel between them; its window starts at the `steps`-th
+    last ionic launch that ran (launches of a stalled pipeline's gated steps
+    return early and are not counted) and ends with the pass's last kernel."""
+    import csv
+    import statistics as st
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]))
+    rows.sort()
+    names = [_kernel_label(n) for _, _, n in rows]
+    step = set(STEP_KERNELS) | {"k_pcg_update+check"}
+    passes, cur = [], []
+    for i, n in enumerate(names):
+        if n in step:
+            cur.append(i)
+        elif cur:
+            passes.append(cur)
+            cur = []
+    if cur:
+        passes.append(cur)
+    for p, idx in enumerate(passes):
+        ion = [i for i in idx if names[i] in ("k_ttp", "k_crn", "k_ms2v")]
+        if len(ion) < steps:
+            continue
+        med = st.median(rows[i][1] - rows[i][0] for i in ion)
+        ran = [i for i in ion if rows[i][1] - rows[i][0] > 0.6 * med]
+        if len(ran) < steps:
+            continue
+        first = ran[-steps]
+        win = [i for i in idx if i >= first]
+        t0, t1 = rows[first][0], max(rows[i][1] for i in win)
+        per, busy = {}, 0.0
+        for i in win:
+            d = (rows[i][1] - rows[i][0]) / 1e3
+            per.setdefault(names[i], []).append(d)
+            busy += d
+        print(f"pass {p}: last {steps} steps, {(t1 - t0) / 1e3 / steps:.2f} us/step under the "
+              f"profiler, GPU busy {min(busy * 1e3 / (t1 - t0), 1.0):.3f}")
+        print(f"{'kernel':20s} {'calls':>6s} {'mean_us':>8s} {'median_us':>9s} {'min_us':>7s}")
+        for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+            print(f"{k:20s} {len(v):6d} {st.mean(v):8.2f} {st.median(v):9.2f} {min(v):7.2f}")
+        print()
+
+
 def copy_probe(mib):
     """Five copies of `mib` MiB between two device buffers: each reads and
     writes exactly mib MiB, which scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -495,6 +561,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 5 plain device copies of this many MiB (the "
                          "calibration for HBM byte counters) and exit")
+    ap.add_argument("--trace-summary", default="",
+                    help="only summarise the last --steps steps of each pass in this "
+                         "rocprofv3 kernel_trace.csv (no GPU needed) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -502,6 +571,9 @@ def main():
         sys.exit(2)
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
+        return
+    if args.trace_summary:
+        trace_summary(args.trace_summary, args.steps)
         return
 
     import torch

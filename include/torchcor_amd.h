@@ -586,6 +586,8 @@ typedef struct tc_serve_desc {
   int max_iter, linear_guess, depth;
   void* ws;                  /* tc_serve_workspace_size bytes */
   size_t ws_bytes;
+  int k_history;             /* as tc_sim_desc: K = the most any of the last k_history
+                                engine steps needed (0 = 4, at most 256) */
 } tc_serve_desc;
 typedef struct tc_serve_system {
   int64_t n;                 /* nodes */

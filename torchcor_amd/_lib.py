@@ -120,7 +120,8 @@ class ServeDesc(C.Structure):
                 ("AT", p), ("RT", p), ("stim_mask", p), ("states", p * 3), ("params", p * 3),
                 ("tab", (p * 3) * 3), ("dt", dbl), ("dt_ionic", dbl), ("Cm", dbl),
                 ("theta", dbl), ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32),
-                ("linear_guess", i32), ("depth", i32), ("ws", p), ("ws_bytes", C.c_size_t)]
+                ("linear_guess", i32), ("depth", i32), ("ws", p), ("ws_bytes", C.c_size_t),
+                ("k_history", i32)]
 
 
 class ServeSystem(C.Structure):

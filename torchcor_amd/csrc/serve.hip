@@ -40,7 +40,7 @@ namespace {
 constexpr int kSB = 512;            // threads per block of the passes
 constexpr int kSW = kSB / 64;
 constexpr int kSRing = 64;
-constexpr int kSHistory = 16;
+constexpr int kSHistoryDefault = 4, kSHistoryMax = 256;
 
 // Per-system device state (arrays of max_sys), plus the pipeline gate.
 struct SvCtl {
@@ -827,6 +827,7 @@ struct tc_serve {
   tc::StepStatus status[tc::kSRing];  // records as read by sv_wait
   long long seq;
   std::deque<int> recent;
+  int history;  // engine steps whose iteration counts set K
   int64_t steps;
   int64_t stalls = 0;    // steps that needed more iterations than were launched
 };
@@ -1167,7 +1168,7 @@ int sv_advance(tc_serve* S, int64_t nsteps, int32_t* iters, hipStream_t s) {
     }
     iters[done] = st->n_iter;
     S->recent.push_back(st->iter > 0 ? st->iter : 1);
-    if ((int)S->recent.size() > kSHistory) S->recent.pop_front();
+    if ((int)S->recent.size() > S->history) S->recent.pop_front();
     ++done;
   }
   TC_HIP(hipStreamSynchronize(s));
@@ -1196,6 +1197,8 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
            TC_ERR_SIZE, "pool too large for int32 indices");
   TC_CHECK(d->max_iter >= 1 && d->max_iter <= kMaxIterLimit, TC_ERR_ARG,
            "max_iter %d outside 1..%d", d->max_iter, kMaxIterLimit);
+  TC_CHECK(d->k_history >= 0 && d->k_history <= kSHistoryMax, TC_ERR_ARG,
+           "k_history %d outside 0..%d", d->k_history, kSHistoryMax);
   TC_CHECK(d->slice_off && d->slice_w && d->slice_sys && d->col && d->Mv && d->Kv && d->Av &&
                d->Dinv && d->x[0] && d->x[1] && d->xp[0] && d->xp[1] && d->w && d->r &&
                d->p[0] && d->p[1] && d->Ap && d->stim_mask,
@@ -1207,6 +1210,7 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
            d->ws_bytes, need);
   tc_serve* S = new tc_serve();
   S->d = *d;
+  S->history = d->k_history > 0 ? d->k_history : kSHistoryDefault;
   for (int m = 0; m < 3; ++m) {
     if (!d->params[m]) continue;
     if (m == TC_MODEL_MS2V) S->mp[m].ms2v = *(const tc_ms2v_params*)d->params[m];

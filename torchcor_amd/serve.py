@@ -65,7 +65,8 @@ class ServeEngine:
 
     def __init__(self, cap_nodes, cap_entries, max_sys=256, dt=0.01, dtype=torch.float32,
                  device=None, a_tol=1e-5, r_tol=1e-5, max_iter=100, x_f64=True,
-                 precise_rhs=True, lat=True, depth=4, int16_columns=True, slices_per_block=None):
+                 precise_rhs=True, lat=True, depth=4, int16_columns=True, slices_per_block=None,
+                 k_history=0):
         self.device = torch.device(device) if device is not None else default_device()
         dev = self.device
         self.dtype = dtype
@@ -112,6 +113,7 @@ class ServeEngine:
         d.dt, d.dt_ionic, d.Cm, d.theta = dt, dt, self.Cm, self.theta
         d.a_tol, d.r_tol, d.max_iter = float(a_tol), float(r_tol), int(max_iter)
         d.linear_guess, d.depth = 1, int(depth)
+        d.k_history = int(k_history)
         nb = C.c_size_t(0)
         L.call("tc_serve_workspace_size", cap_slices, int(max_sys), C.byref(nb))
         self._ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)
