@@ -179,3 +179,40 @@ def test_sigma_split_reassembles_rotated_tensor():
         d = math.radians(deg)
         got = S[0] + math.cos(d) * S[1] + math.sin(d) * S[2] + math.cos(2 * d) * S[3] + math.sin(2 * d) * S[4]
         assert np.abs(got - ref).max() < 1e-14
+
+
+def test_bench_trace_summary(tmp_path):
+    """bench.py --trace-summary: a pass is a run of time-step kernels; its
+    window starts at the K-th last ionic launch that ran (short launches
+    of gated steps do not count) and takes every kernel after it."""
+    import subprocess
+    import sys
+    hdr = '"Kind","Kernel_Name","Start_Timestamp","End_Timestamp"\n'
+    t = [1000]
+    lines = []
+
+    def k(name, us):
+        lines.append(f'"KERNEL_DISPATCH","{name}",{t[0]},{t[0] + us * 1000}\n')
+        t[0] += us * 1000 + 500
+
+    ttp = "void tc::(anonymous namespace)::k_ttp<float, true, true>(long, float*)"
+    step = [("void tc::(anonymous namespace)::k_rhs_init<float, double>(long)", 30),
+            ("void tc::(anonymous namespace)::k_pcg_spmv<float, short>(long)", 20),
+            ("void tc::(anonymous namespace)::k_pcg_update<float, double, true>(long)", 12)]
+    k("void tc::k_local_tet(int)", 100)                    # assembly: not a step kernel
+    for s in range(6):                                      # 2 warm-up + 4 timed
+        k(ttp, 40)
+        for nm, us in step:
+            k(nm, us)
+    k(ttp, 10)                                              # a gated launch
+    k("void tc::(anonymous namespace)::k_pcg_update<float, double>(long)", 9)
+    k("__amd_rocclr_copyBuffer", 3)                         # ends the pass
+    f = tmp_path / "kt_kernel_trace.csv"
+    f.write_text(hdr + "".join(lines))
+    out = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--trace-summary", str(f),
+                          "--steps", "4"], capture_output=True, text=True, check=True).stdout
+    rows = {ln.split()[0]: ln.split() for ln in out.splitlines() if ln.startswith("k_")}
+    assert rows["k_ttp"][1] == "5" and rows["k_rhs_init"][1] == "4"
+    assert rows["k_pcg_update+check"][1] == "4" and rows["k_pcg_update"][1] == "1"
+    assert float(rows["k_ttp"][4]) == 10.0 and float(rows["k_rhs_init"][2]) == 30.0
+    assert out.startswith("pass 0: last 4 steps")
