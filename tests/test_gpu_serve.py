@@ -138,3 +138,33 @@ def test_serve_loop_with_arrivals_matches_batch_admission():
         assert got[k]["iterations"] == ref[k]["iterations"]
     for rec in records:
         assert rec["admitted"] >= rec["arrival"] and rec["done"] > rec["admitted"]
+
+
+def test_serve_many_systems_alone_equals_in_mix():
+    """40 live systems: the check pass runs over 3 blocks (16 systems a
+    block) and the block that finishes last commits every system's clock
+    and counts; runs alone give the same bits."""
+    from torchcor_amd.serve import Request, ServeEngine
+    _say("[gpu] serve 40 systems alone vs mix f32")
+    rng = np.random.default_rng(11)
+    reqs = []
+    for k in range(40):
+        N = int(rng.integers(6, 10))
+        model = ("ms2v", "ttp", "crn")[k % 3]
+        nodes, elems, regions, fib = meshes.cube_workload(N)
+        st = [(meshes.corner_nodes(N, 2 + k % 2), 0.0, 1.0, 100.0)]
+        reqs.append(Request(nodes, elems, regions, fib, COND, model, st, float(rng.uniform(1.0, 4.0)),
+                            tag=k))
+    eng = ServeEngine(cap_nodes=40000, cap_entries=640000, max_sys=64, dtype=torch.float32,
+                      device=DEV)
+    mix = eng.run_until_done([eng.prepare(r) for r in reqs], max_chunk=29)
+    assert eng.stats()["steps"] >= 300
+    for k in (0, 17, 39):
+        alone_eng = _engine(torch.float32)
+        alone = alone_eng.run_until_done([alone_eng.prepare(reqs[k])])[0]
+        assert alone["steps"] == mix[k]["steps"] == int(reqs[k].T / 0.01)
+        assert torch.equal(alone["u"], mix[k]["u"]), k
+        assert torch.equal(alone["AT"], mix[k]["AT"]) and torch.equal(alone["RT"], mix[k]["RT"])
+        assert alone["iterations"] == mix[k]["iterations"]
+    for k, r in enumerate(reqs):
+        assert mix[k]["steps"] == int(r.T / 0.01), k

@@ -524,26 +524,55 @@ __global__ void k_sv_ent(int64_t nl, const int32_t* __restrict__ list,
   ent[li] = SvEnt{sl, o & ~1, slice_w[sl], o & 1};
 }
 
-// One block: end of a launch chunk after k_end iterations.  The stopping
-// test of the last iteration for every system still active; systems still
-// iterating below max_iter stall the pipeline; otherwise the step is final:
-// every live system's n_iter sum, clock (t += dt) and next-step stimulus
-// bits advance, and the status goes to pinned host memory.  hc is the host
-// copy of *ctl: only its per-system array pointers and tolerances, fixed
-// when the pool is created, are read from it (a kernel-argument load instead
-// of a load of the pointer from device memory before every array access).
-constexpr int kSCheck = 1024;  // threads of the check pass (one wavefront per system fold)
+// End of a launch chunk after k_end iterations.  The stopping test of the
+// last iteration for every system still active; systems still iterating
+// below max_iter stall the pipeline; otherwise the step is final: every live
+// system's n_iter sum, clock (t += dt) and next-step stimulus bits advance,
+// and the status goes to pinned host memory.  hc is the host copy of *ctl:
+// only its per-system array pointers and tolerances, fixed when the pool is
+// created, are read from it (a kernel-argument load instead of a load of the
+// pointer from device memory before every array access).
+//
+// One wavefront per system over ceil(nsys / 16) blocks.  Each wavefront
+// stores its system's results (SvChk) and each block its aggregates (stall,
+// largest n_iter and iteration count) at device scope; every block waits
+// until its stores are complete and counts itself; the block that counts
+// last publishes the status and commits the per-system results, so the
+// pass costs one round of per-system latency whatever the number of
+// systems.  No block waits for another.
+constexpr int kSCheck = 1024;     // threads of a check block
+constexpr int kSCheckWaves = kSCheck / 64;
 constexpr int kSCheckSys = 1024;  // systems of one check pass (max_sys is checked against it)
+constexpr int kSCheckBlocks = kSCheckSys / kSCheckWaves;
+static_assert(kSCheckBlocks <= 64, "one wavefront folds the block aggregates");
+
+struct SvChk {
+  long long tot, stp;
+  double t;
+  int slot, act, ni;
+  uint32_t bits;
+};
+
+struct SvChkCtl {
+  int any[kSCheckBlocks], nmax[kSCheckBlocks], imax[kSCheckBlocks];
+  unsigned done;
+};
+
+template <class U>
+__device__ inline void st_dev(U* p, U v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class U>
+__device__ inline U ld_dev(const U* p) {
+  return __hip_atomic_load(const_cast<U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __global__ void __launch_bounds__(kSCheck)
 k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, SvCtl hc,
-           const double* __restrict__ P2, StatusRec* status, long long seq) {
+           const double* __restrict__ P2, StatusRec* status, long long seq, SvChk* chk,
+           SvChkCtl* cc) {
   static_assert((TC_POOL_STIM & (TC_POOL_STIM - 1)) == 0 && TC_POOL_STIM <= 64, "stimulus lanes");
-  __shared__ int stall0, any_stall, nmax, imax;
-  __shared__ int s_slot[kSCheckSys], s_act[kSCheckSys], s_ni[kSCheckSys];
-  __shared__ uint32_t s_bits[kSCheckSys];
-  __shared__ long long s_tot[kSCheckSys], s_stp[kSCheckSys];
-  __shared__ double s_t[kSCheckSys];
+  __shared__ int stall0, any_stall, nmax, imax, last;
   if (threadIdx.x == 0) {
     stall0 = ctl->stall;
     any_stall = 0;
@@ -551,6 +580,10 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, 
     imax = 0;
   }
   __syncthreads();
+  // A gated pass (the pipeline stalled before it) returns in every block:
+  // the flag is the same in all of them, since only the last block of this
+  // pass writes it, after every block has read it and counted itself.
+  if (stall0) return;
   const int lane = threadIdx.x & 63, wv = wave_id();
   // phase 1, one wavefront per system: the stopping test of the last
   // iteration and the system's clock and next-step stimulus bits (lane k
@@ -558,7 +591,8 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, 
   // of it (flags, norms, clock, stimulus table and block partials; each
   // lane's at once, at clamped in-range addresses) is issued before the
   // first wait.
-  for (int q = wv; q < nsys; q += kSCheck / 64) {
+  const int q = blockIdx.x * kSCheckWaves + wv;
+  if (q < nsys) {
     const SvSys y = pl.sysd[q];
     const int s = y.slot;
     const int act = hc.active[s];
@@ -619,38 +653,67 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, 
           else nact = 0;  // max_iter reached
         }
       }
-      s_slot[q] = s;
-      s_act[q] = nact | (act << 1);
-      s_ni[q] = ni;
-      s_bits[q] = bits;
-      s_tot[q] = tot + ni;
-      s_t[q] = t;
-      s_stp[q] = stp + 1;
+      SvChk* c = chk + q;
+      st_dev(&c->slot, s);
+      st_dev(&c->act, nact | (act << 1));
+      st_dev(&c->ni, ni);
+      st_dev(&c->bits, bits);
+      st_dev(&c->tot, tot + (long long)ni);
+      st_dev(&c->t, t);
+      st_dev(&c->stp, stp + 1);
       atomicMax(&nmax, ni);
       atomicMax(&imax, nact ? k_end : (ni + 1 < k_end ? ni + 1 : k_end));
     }
   }
+  // every wavefront's stores complete before its block counts itself
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (stall0) return;
-  // phase 2: the step status (the next kernels and host copies are ordered
-  // after this kernel on the stream) and, one thread per system, the flags,
-  // iteration counts and, when no system stalls, the clock and bits
   if (threadIdx.x == 0) {
-    if (any_stall) ctl->stall = 1;
-    if (status)
-      put_status(status, (uint32_t)nmax, (uint32_t)imax | (any_stall ? kStStalled : 0u), seq);
+    st_dev(&cc->any[blockIdx.x], any_stall);
+    st_dev(&cc->nmax[blockIdx.x], nmax);
+    st_dev(&cc->imax[blockIdx.x], imax);
+    __builtin_amdgcn_s_waitcnt(0);
+    last = __hip_atomic_fetch_add(&cc->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+           gridDim.x - 1;
   }
-  for (int q = threadIdx.x; q < nsys; q += blockDim.x) {
-    const int s = s_slot[q];
-    if (s_act[q] >> 1) {
-      hc.active[s] = s_act[q] & 1;
-      hc.n_iter[s] = s_ni[q];
+  __syncthreads();
+  if (!last) return;
+  // phase 2 in the block that counted last: the step status (the next
+  // kernels and host copies are ordered after this kernel on the stream)
+  // and, one thread per system, the flags, iteration counts and, when no
+  // system stalls, the clock and bits
+  if (wv == 0) {
+    const bool in = lane < (int)gridDim.x;
+    int any = in ? ld_dev(&cc->any[lane]) : 0;
+    int nm = in ? ld_dev(&cc->nmax[lane]) : 0;
+    int im = in ? ld_dev(&cc->imax[lane]) : 0;
+    for (int o = 32; o > 0; o >>= 1) {
+      any |= __shfl_xor(any, o);
+      nm = max(nm, __shfl_xor(nm, o));
+      im = max(im, __shfl_xor(im, o));
+    }
+    if (lane == 0) {
+      st_dev(&cc->done, 0u);
+      any_stall = any;
+      if (any) ctl->stall = 1;
+      if (status)
+        put_status(status, (uint32_t)nm, (uint32_t)im | (any ? kStStalled : 0u), seq);
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < nsys; r += blockDim.x) {
+    const SvChk* c = chk + r;
+    const int s = ld_dev(&c->slot);
+    const int a = ld_dev(&c->act);
+    if (a >> 1) {
+      hc.active[s] = a & 1;
+      hc.n_iter[s] = ld_dev(&c->ni);
     }
     if (!any_stall) {
-      hc.total[s] = s_tot[q];
-      hc.t[s] = s_t[q];
-      hc.steps[s] = s_stp[q];
-      hc.bits[s] = s_bits[q];
+      hc.total[s] = ld_dev(&c->tot);
+      hc.t[s] = ld_dev(&c->t);
+      hc.steps[s] = ld_dev(&c->stp);
+      hc.bits[s] = ld_dev(&c->bits);
     }
   }
 }
@@ -817,6 +880,8 @@ struct tc_serve {
   tc::SvBlk* blk_desc;  // device: per block
   tc::SvEnt* ent;       // device: per list entry
   tc::SvSys* sysd;      // device: per live system
+  tc::SvChk* chk;       // device: the check pass's per-system results
+  tc::SvChkCtl* chkctl; // device: its per-block aggregates and completion count
   int64_t nl, nml[3];
   int nsys, G;
   std::vector<tc::SysHost> sys;
@@ -836,7 +901,7 @@ namespace tc {
 namespace {
 
 struct SvLayout {
-  size_t ctl, arrays, part[3], list, mlist[3], blk_desc, ent, sysd, total;
+  size_t ctl, arrays, chk, chkctl, part[3], list, mlist[3], blk_desc, ent, sysd, total;
 };
 
 SvLayout sv_layout(int64_t cap_slices, int max_sys) {
@@ -852,6 +917,10 @@ SvLayout sv_layout(int64_t cap_slices, int max_sys) {
   // rz2 (2 per system) bnorm alpha beta t (double), total steps (ll), row0 n (i64),
   // active n_iter nstim (int), bits (u32), stim
   L.arrays = take(ms * (6 * 8 + 2 * 8 + 2 * 8 + 4 * 4 + 4) + ms * TC_POOL_STIM * sizeof(tc_stimulus));
+  // the check pass's per-system results and per-block aggregates (zeroed
+  // with the arrays: its completion count starts at 0)
+  L.chk = take(ms * sizeof(SvChk));
+  L.chkctl = take(sizeof(SvChkCtl));
   // blocks (and partial slots): sum over systems of ceil(slices / spb), padded to 8
   const size_t slots = (size_t)cap_slices / kSPerMin + ms + 16;
   L.part[0] = take(slots * 2 * sizeof(double));
@@ -1027,9 +1096,10 @@ int sv_iters(tc_serve* S, int k0, int k1, XT* xs, hipStream_t s) {
 }
 
 int sv_check(tc_serve* S, int k_end, int slot, long long seq, hipStream_t s) {
-  hipLaunchKernelGGL(k_sv_check, dim3(1), dim3(kSCheck), 0, s, sv_pool(S), k_end, S->nsys,
+  const int blocks = S->nsys > 0 ? (S->nsys + kSCheckWaves - 1) / kSCheckWaves : 1;
+  hipLaunchKernelGGL(k_sv_check, dim3(blocks), dim3(kSCheck), 0, s, sv_pool(S), k_end, S->nsys,
                      S->d.max_iter, S->d.dt, S->ctl, S->hctl, (const double*)S->part[2],
-                     S->dev_view + slot, seq);
+                     S->dev_view + slot, seq, S->chk, S->chkctl);
   TC_LAUNCH();
   return TC_OK;
 }
@@ -1253,6 +1323,8 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
   S->blk_desc = (SvBlk*)(b + L.blk_desc);
   S->ent = (SvEnt*)(b + L.ent);
   S->sysd = (SvSys*)(b + L.sysd);
+  S->chk = (SvChk*)(b + L.chk);
+  S->chkctl = (SvChkCtl*)(b + L.chkctl);
   S->nl = 0;
   S->nsys = 0;
   S->G = kXcd;
