@@ -573,6 +573,8 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, 
            SvChkCtl* cc) {
   static_assert((TC_POOL_STIM & (TC_POOL_STIM - 1)) == 0 && TC_POOL_STIM <= 64, "stimulus lanes");
   __shared__ int stall0, any_stall, nmax, imax, last;
+  __shared__ SvChk s_chk[kSCheckWaves];   // the results when one block covers every system
+  const bool one = gridDim.x == 1;
   if (threadIdx.x == 0) {
     stall0 = ctl->stall;
     any_stall = 0;
@@ -653,21 +655,48 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, 
           else nact = 0;  // max_iter reached
         }
       }
-      SvChk* c = chk + q;
-      st_dev(&c->slot, s);
-      st_dev(&c->act, nact | (act << 1));
-      st_dev(&c->ni, ni);
-      st_dev(&c->bits, bits);
-      st_dev(&c->tot, tot + (long long)ni);
-      st_dev(&c->t, t);
-      st_dev(&c->stp, stp + 1);
+      if (one) {
+        s_chk[wv] = SvChk{tot + (long long)ni, stp + 1, t, s, nact | (act << 1), ni, bits};
+      } else {
+        SvChk* c = chk + q;
+        st_dev(&c->slot, s);
+        st_dev(&c->act, nact | (act << 1));
+        st_dev(&c->ni, ni);
+        st_dev(&c->bits, bits);
+        st_dev(&c->tot, tot + (long long)ni);
+        st_dev(&c->t, t);
+        st_dev(&c->stp, stp + 1);
+      }
       atomicMax(&nmax, ni);
       atomicMax(&imax, nact ? k_end : (ni + 1 < k_end ? ni + 1 : k_end));
     }
   }
   // every wavefront's stores complete before its block counts itself
-  __builtin_amdgcn_s_waitcnt(0);
+  if (!one) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
+  if (one) {
+    // one block covers every system: it is the last, with the results in
+    // its own memory
+    if (threadIdx.x == 0) {
+      if (any_stall) ctl->stall = 1;
+      if (status)
+        put_status(status, (uint32_t)nmax, (uint32_t)imax | (any_stall ? kStStalled : 0u), seq);
+    }
+    if (threadIdx.x < nsys) {
+      const SvChk& c = s_chk[threadIdx.x];
+      if (c.act >> 1) {
+        hc.active[c.slot] = c.act & 1;
+        hc.n_iter[c.slot] = c.ni;
+      }
+      if (!any_stall) {
+        hc.total[c.slot] = c.tot;
+        hc.t[c.slot] = c.t;
+        hc.steps[c.slot] = c.stp;
+        hc.bits[c.slot] = c.bits;
+      }
+    }
+    return;
+  }
   if (threadIdx.x == 0) {
     st_dev(&cc->any[blockIdx.x], any_stall);
     st_dev(&cc->nmax[blockIdx.x], nmax);
