@@ -415,6 +415,8 @@ def _kernel_label(name):
             if k == "k_pcg_update" and base[base.find("<") + 1:base.rfind(">")].split(",")[-1].strip() == "true":
                 return "k_pcg_update+check"
             return k
+    if head.startswith("tc::"):
+        head = head[4:]
     return head[-40:]
 
 
@@ -464,6 +466,21 @@ def trace_summary(path, steps):
         for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
             print(f"{k:20s} {len(v):6d} {st.mean(v):8.2f} {st.median(v):9.2f} {min(v):7.2f}")
         print()
+    # the served engine's passes (k_pool_ionic, k_sv_*): first to last of them
+    sv = [i for i, n in enumerate(names) if n == "k_pool_ionic" or n.startswith("k_sv_")]
+    if sv:
+        t0, t1 = rows[sv[0]][0], max(rows[i][1] for i in sv)
+        per, busy = {}, 0.0
+        for i in range(sv[0], sv[-1] + 1):
+            d = (rows[i][1] - rows[i][0]) / 1e3
+            busy += d
+            per.setdefault(names[i], []).append(d)
+        steps = len(per.get("k_pool_ionic", []))
+        print(f"served engine: {steps} engine steps, window {(t1 - t0) / 1e9:.2f} s, "
+              f"GPU busy {min(busy * 1e3 / (t1 - t0), 1.0):.3f}")
+        print(f"{'kernel':28s} {'calls':>7s} {'mean_us':>8s} {'total_s':>8s}")
+        for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:12]:
+            print(f"{k:28s} {len(v):7d} {st.mean(v):8.2f} {sum(v) / 1e6:8.3f}")
 
 
 def copy_probe(mib):

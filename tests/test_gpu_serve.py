@@ -140,15 +140,17 @@ def test_serve_loop_with_arrivals_matches_batch_admission():
         assert rec["admitted"] >= rec["arrival"] and rec["done"] > rec["admitted"]
 
 
-def test_serve_many_systems_alone_equals_in_mix():
-    """40 live systems: the check pass runs over 3 blocks (16 systems a
-    block) and the block that finishes last commits every system's clock
-    and counts; runs alone give the same bits."""
+@pytest.mark.parametrize("nreq", [24, 40])
+def test_serve_many_systems_alone_equals_in_mix(nreq):
+    """24 live systems: one check block takes them in two rounds; 40: the
+    check pass runs over 3 blocks (16 systems a block) and the block that
+    finishes last commits every system's clock and counts.  Runs alone give
+    the same bits."""
     from torchcor_amd.serve import Request, ServeEngine
-    _say("[gpu] serve 40 systems alone vs mix f32")
+    _say(f"[gpu] serve {nreq} systems alone vs mix f32")
     rng = np.random.default_rng(11)
     reqs = []
-    for k in range(40):
+    for k in range(nreq):
         N = int(rng.integers(6, 10))
         model = ("ms2v", "ttp", "crn")[k % 3]
         nodes, elems, regions, fib = meshes.cube_workload(N)
@@ -159,7 +161,7 @@ def test_serve_many_systems_alone_equals_in_mix():
                       device=DEV)
     mix = eng.run_until_done([eng.prepare(r) for r in reqs], max_chunk=29)
     assert eng.stats()["steps"] >= 300
-    for k in (0, 17, 39):
+    for k in (0, 17, nreq - 1):
         alone_eng = _engine(torch.float32)
         alone = alone_eng.run_until_done([alone_eng.prepare(reqs[k])])[0]
         assert alone["steps"] == mix[k]["steps"] == int(reqs[k].T / 0.01)

@@ -533,7 +533,8 @@ __global__ void k_sv_ent(int64_t nl, const int32_t* __restrict__ list,
 // created, are read from it (a kernel-argument load instead of a load of the
 // pointer from device memory before every array access).
 //
-// One wavefront per system over ceil(nsys / 16) blocks.  Each wavefront
+// One wavefront per system over ceil(nsys / 16) blocks (one block looping
+// over at most kSCheckOneMax systems).  Each wavefront
 // stores its system's results (SvChk) and each block its aggregates (stall,
 // largest n_iter and iteration count) at device scope; every block waits
 // until its stores are complete and counts itself; the block that counts
@@ -544,6 +545,10 @@ constexpr int kSCheck = 1024;     // threads of a check block
 constexpr int kSCheckWaves = kSCheck / 64;
 constexpr int kSCheckSys = 1024;  // systems of one check pass (max_sys is checked against it)
 constexpr int kSCheckBlocks = kSCheckSys / kSCheckWaves;
+// Up to this many systems one block runs the pass, its wavefronts taking
+// the systems in turn: two rounds of per-system latency cost less than the
+// device-scope completion count of several blocks.
+constexpr int kSCheckOneMax = 2 * kSCheckWaves;
 static_assert(kSCheckBlocks <= 64, "one wavefront folds the block aggregates");
 
 struct SvChk {
@@ -567,14 +572,15 @@ __device__ inline U ld_dev(const U* p) {
   return __hip_atomic_load(const_cast<U*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <bool ONE>
 __global__ void __launch_bounds__(kSCheck)
 k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, SvCtl hc,
            const double* __restrict__ P2, StatusRec* status, long long seq, SvChk* chk,
            SvChkCtl* cc) {
   static_assert((TC_POOL_STIM & (TC_POOL_STIM - 1)) == 0 && TC_POOL_STIM <= 64, "stimulus lanes");
   __shared__ int stall0, any_stall, nmax, imax, last;
-  __shared__ SvChk s_chk[kSCheckWaves];   // the results when one block covers every system
-  const bool one = gridDim.x == 1;
+  __shared__ SvChk s_chk[kSCheckOneMax];  // the results when one block covers every system
+  constexpr bool one = ONE;  // a single block covering every system
   if (threadIdx.x == 0) {
     stall0 = ctl->stall;
     any_stall = 0;
@@ -593,8 +599,8 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, 
   // of it (flags, norms, clock, stimulus table and block partials; each
   // lane's at once, at clamped in-range addresses) is issued before the
   // first wait.
-  const int q = blockIdx.x * kSCheckWaves + wv;
-  if (q < nsys) {
+  const int q_end = one ? nsys : min(nsys, (int)(blockIdx.x + 1) * kSCheckWaves);
+  for (int q = blockIdx.x * kSCheckWaves + wv; q < q_end; q += kSCheckWaves) {
     const SvSys y = pl.sysd[q];
     const int s = y.slot;
     const int act = hc.active[s];
@@ -655,8 +661,8 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, 
           else nact = 0;  // max_iter reached
         }
       }
-      if (one) {
-        s_chk[wv] = SvChk{tot + (long long)ni, stp + 1, t, s, nact | (act << 1), ni, bits};
+      if constexpr (one) {
+        s_chk[q] = SvChk{tot + (long long)ni, stp + 1, t, s, nact | (act << 1), ni, bits};
       } else {
         SvChk* c = chk + q;
         st_dev(&c->slot, s);
@@ -672,9 +678,9 @@ k_sv_check(SvPool pl, int k_end, int nsys, int max_iter, double dt, SvCtl* ctl, 
     }
   }
   // every wavefront's stores complete before its block counts itself
-  if (!one) __builtin_amdgcn_s_waitcnt(0);
+  if constexpr (!one) __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (one) {
+  if constexpr (one) {
     // one block covers every system: it is the last, with the results in
     // its own memory
     if (threadIdx.x == 0) {
@@ -1125,10 +1131,15 @@ int sv_iters(tc_serve* S, int k0, int k1, XT* xs, hipStream_t s) {
 }
 
 int sv_check(tc_serve* S, int k_end, int slot, long long seq, hipStream_t s) {
-  const int blocks = S->nsys > 0 ? (S->nsys + kSCheckWaves - 1) / kSCheckWaves : 1;
-  hipLaunchKernelGGL(k_sv_check, dim3(blocks), dim3(kSCheck), 0, s, sv_pool(S), k_end, S->nsys,
-                     S->d.max_iter, S->d.dt, S->ctl, S->hctl, (const double*)S->part[2],
-                     S->dev_view + slot, seq, S->chk, S->chkctl);
+  const int blocks = S->nsys > kSCheckOneMax ? (S->nsys + kSCheckWaves - 1) / kSCheckWaves : 1;
+  if (blocks == 1)
+    hipLaunchKernelGGL(k_sv_check<true>, dim3(1), dim3(kSCheck), 0, s, sv_pool(S), k_end,
+                       S->nsys, S->d.max_iter, S->d.dt, S->ctl, S->hctl,
+                       (const double*)S->part[2], S->dev_view + slot, seq, S->chk, S->chkctl);
+  else
+    hipLaunchKernelGGL(k_sv_check<false>, dim3(blocks), dim3(kSCheck), 0, s, sv_pool(S), k_end,
+                       S->nsys, S->d.max_iter, S->d.dt, S->ctl, S->hctl,
+                       (const double*)S->part[2], S->dev_view + slot, seq, S->chk, S->chkctl);
   TC_LAUNCH();
   return TC_OK;
 }
