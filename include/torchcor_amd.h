@@ -250,7 +250,7 @@ int tc_pcg_get_timing(tc_pcg* plan, double* ms_host, int64_t* count_host);
  * kernels, same arguments and results, with no plan and no workspace.  The
  * operator is passed directly (A pattern, Av values, Dinv in `dtype`).  Its
  * scratch memory (control block, partial sums, r, p, p', Ap) is a static
- * device array of the library, 33,605,632 bytes, which every GPU the
+ * device array of the library, 33,611,776 bytes, which every GPU the
  * library runs on holds until the process ends; the largest system it
  * takes has TC_PCG_STATIC_MAX_ROWS rows (larger ones fail with
  * TC_ERR_SIZE).  It allocates nothing (no hipMalloc/hipFree, no pinned

@@ -109,7 +109,7 @@ int pcg_solve(const PcgBufs& P, const CI* sc, const T* b, T* x, const T* xp, T* 
     hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kPcgBlock), 0, s, it - 1, P.G, P.ctl, P.part[2],
                        (StatusRec*)nullptr, 0, max_iter, 0LL);
     TC_LAUNCH();
-    TC_HIP(hipMemcpyAsync(P.host_ctl, P.ctl, sizeof(PcgCtl), hipMemcpyDeviceToHost, s));
+    TC_HIP(hipMemcpyAsync(P.host_ctl, P.ctl, kCtlHostBytes, hipMemcpyDeviceToHost, s));
     TC_HIP(hipStreamSynchronize(s));
     if (P.host_ctl->converged) break;
     chunk = chunk * 2 < 64 ? chunk * 2 : 64;
@@ -302,7 +302,7 @@ namespace tc {
 namespace {
 
 constexpr int64_t kStaticRows = TC_PCG_STATIC_MAX_ROWS;
-constexpr size_t kStaticCtlBytes = 2048;
+constexpr size_t kStaticCtlBytes = 8192;
 constexpr size_t kStaticPartBytes = 2 * kMaxBlocks * sizeof(double);   // 2G doubles, G <= 1024
 constexpr size_t kStaticVecBytes = size_t(kStaticRows) * sizeof(double);
 constexpr size_t kStaticBytes = kStaticCtlBytes + 3 * kStaticPartBytes + 4 * kStaticVecBytes;

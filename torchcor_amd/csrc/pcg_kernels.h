@@ -24,13 +24,14 @@
 // XCD-aware work split: block b runs on XCD b % 8; blocks of one XCD get one
 // contiguous range of slices so that neighbour gathers stay in that XCD's L2.
 #pragma once
+#include <cstddef>
 #include <vector>
 
 #include "tc_internal.h"
 
 namespace tc {
 
-constexpr int kDoneGroups = 8;
+constexpr int kDoneGroups = 32;
 
 struct PcgCtl {
   double rz[2];   // r.z of iteration k in rz[k & 1]
@@ -42,9 +43,11 @@ struct PcgCtl {
   int stall;      // set when a pipelined step ran out of launched iterations;
                   // every later gated kernel then returns at once
   // Completion counts of a checking update pass: block b counts itself in
-  // group[b % kDoneGroups] (one group per XCD, each on a cache line of its
-  // own, so no more than G/8 atomic adds queue on one address); the last
-  // block of a group resets that count and counts the group in `done`.
+  // group[b % kDoneGroups] (each on a cache line of its own, so no more than
+  // G/32 atomic adds queue on one address: adds on one address serialize at
+  // about 10 ns each); the last block of a group resets that count and
+  // counts the group in `done`.  The counts follow every field the host
+  // reads back (kCtlHostBytes).
   unsigned done;
   struct alignas(128) Count {
     unsigned n;
@@ -58,6 +61,9 @@ struct PcgCtl {
 // sequence number carries that record's payload: no fence is needed between
 // payload and flag (a system-scope fence would also write back the L2 of
 // the writer's XCD, microseconds after a pass that wrote megabytes).
+// The leading fields of PcgCtl, which the host reads back.
+constexpr size_t kCtlHostBytes = offsetof(PcgCtl, done);
+
 struct StatusRec {
   unsigned long long w[2];
 };
