@@ -466,6 +466,20 @@ def trace_summary(path, steps):
         for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
             print(f"{k:20s} {len(v):6d} {st.mean(v):8.2f} {st.median(v):9.2f} {min(v):7.2f}")
         print()
+    # the batched runs and their back-to-back single runs: every launch from
+    # the first batch kernel (k_bat_*) up to the first served-engine kernel
+    bat = [i for i, n in enumerate(names) if n.startswith("k_bat_")]
+    if bat:
+        end = next((i for i, n in enumerate(names)
+                    if i > bat[0] and (n == "k_pool_ionic" or n.startswith("k_sv_"))), len(rows))
+        per = {}
+        for i in range(bat[0], end):
+            per.setdefault(names[i], []).append((rows[i][1] - rows[i][0]) / 1e3)
+        print(f"batched runs and their single-run baseline: {end - bat[0]} launches")
+        print(f"{'kernel':28s} {'calls':>7s} {'mean_us':>8s} {'median_us':>9s} {'total_ms':>9s}")
+        for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1]))[:16]:
+            print(f"{k:28s} {len(v):7d} {st.mean(v):8.2f} {st.median(v):9.2f} {sum(v) / 1e3:9.2f}")
+        print()
     # the served engine's passes (k_pool_ionic, k_sv_*): first to last of them
     sv = [i for i, n in enumerate(names) if n == "k_pool_ionic" or n.startswith("k_sv_")]
     if sv:
