@@ -516,6 +516,36 @@ def test_runtime_matches_python_step_loop(model):
     assert torch.equal(AT, ref_at)
 
 
+@pytest.mark.parametrize("max_iter", [1, 2])
+def test_runtime_small_max_iter_matches_step_loop(max_iter):
+    """max_iter below what the steps need: steps end unconverged at
+    n_iter = max_iter (solver.py:57-76) and must not stall the pipeline;
+    the native loop equals the per-step Python path, float64."""
+    _say(f"[gpu] runtime vs step() with max_iter {max_iter}")
+    N, steps = 11, 300
+    sim = _cube_sim(N, "ttp", torch.float64)
+    sim.start(1e-7, 1e-7, max_iter, snapshot_interval=1.0, depth=4)
+    sim.advance(steps)
+    sim.finish()
+    ref_u, ref_it = sim.u.clone(), list(sim.iterations)
+    sim = _cube_sim(N, "ttp", torch.float64)
+    sim.assemble()
+    u = sim.ionic_model.initialize(sim.n_nodes)
+    sim._init_solver(u, True)
+    AT = torch.full_like(u, -1.0)
+    RT = torch.full_like(u, -1.0)
+    t, prev, its = 0, None, []
+    for n in range(steps):
+        t += 0.01
+        lat = (AT, RT, prev) if prev is not None else None
+        u, it, _, _ = sim.step(sim._x, t, 1e-7, 1e-7, max_iter, False, lat)
+        prev = t
+        its.append(it)
+    assert its == ref_it
+    assert max(its) == max_iter and its.count(max_iter) > steps // 2
+    assert (u - ref_u).abs().max().item() < 1e-9
+
+
 @pytest.mark.parametrize("name", ["ttp_endo", "crn"])
 def test_single_cell_float32_vs_reference_float32(name):
     """float32 kernels against the reference's own float32 single-cell trace
