@@ -18,7 +18,8 @@ gap after it, so the timed pass carries them on every 16th step only; when
 that gives fewer than 16 samples (short windows), a second pass over the
 same step window (same warm-up, same K steps) times every step and supplies
 the kernel figures.  A steady-state window (2000 steps after 2 ms) is
-reported beside the headline.  Also: the batched and served forms, the
+reported beside the headline, and the headline window run again with each
+step's fixed launches replayed as a captured HIP graph (graph_replay).  Also: the batched and served forms, the
 irregular shell, and the CPU baseline (a torch-CPU restatement of the step
 on the host's threads, float32 and float64).
 """
@@ -160,7 +161,7 @@ def cpu_baseline(ops, N, seconds=12.0):
                         "seconds": f64["seconds"]}}
 
 
-def batched_config(dev, steps, warmup):
+def batched_config(dev, steps, warmup, graph=False):
     """64 simulations sharing the 41^3 cube (68,921 nodes): 32 ModifiedMS2v
     and 32 TTP ENDO, conductivity scales U[0.5, 1.5], fibre rotations
     U[-30, 30] degrees, one seeded 200-node boundary patch stimulus each
@@ -193,7 +194,7 @@ def batched_config(dev, steps, warmup):
         bat.add_instance(ms if k < B // 2 else ttp, rng.uniform(0.5, 1.5), rng.uniform(-30, 30),
                          [(patch, 0.0, 1.0, 100.0)])
     print(f"[bench] batch: {B} x {N}^3, warmup {warmup}", flush=True)
-    bat.start(1e-5, 1e-5, 100)
+    bat.start(1e-5, 1e-5, 100, graph=graph)
     bat.advance(warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -208,7 +209,7 @@ def batched_config(dev, steps, warmup):
         for c in cond:
             sim.add_conductivity(*c)
         sim.add_stimulus(patches[0], 0.0, 1.0, 100.0)
-        sim.start(1e-5, 1e-5, 100, snapshot_interval=1e9)
+        sim.start(1e-5, 1e-5, 100, snapshot_interval=1e9, graph=graph)
         sim.advance(warmup)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -291,7 +292,7 @@ def irregular_config(dev, steps, warmup, n_target=640_000, orders=(None, "rcm", 
     return out
 
 
-def served_config(dev, n_req, budget_nodes=4_000_000):
+def served_config(dev, n_req, budget_nodes=4_000_000, graph=False):
     """Served stream: Poisson arrivals (seeded, mean 20 req/s) of runs on
     cubes of 11..81 nodes a side (log-uniform), ionic model uniform over
     {MS2v, TTP ENDO, CRN}, T ~ U[50, 300] ms, conductivity scale U[0.5, 1.5],
@@ -321,7 +322,7 @@ def served_config(dev, n_req, budget_nodes=4_000_000):
         reqs.append(Request(nodes, elems, regions, fib, cond, model, [(patch, 0.0, 1.0, 100.0)], T,
                             arrival=float(arrivals[k]), tag=k))
     eng = ServeEngine(cap_nodes=budget_nodes, cap_entries=16 * budget_nodes, max_sys=128, dtype=f32,
-                      device=dev)
+                      device=dev, graph=graph)
     # warm-up on a tiny run (kernel and table set-up outside the clock)
     eng.run_until_done([eng.prepare(Request(*meshes.cube_workload(5), reqs[0].conductivities,
                                             "ttp", [], 0.5))])
@@ -339,7 +340,7 @@ def served_config(dev, n_req, budget_nodes=4_000_000):
         for c in r.conductivities:
             sim.add_conductivity(*c)
         sim.add_stimulus(r.stimuli[0][0], *r.stimuli[0][1:])
-        sim.start(1e-5, 1e-5, 100, snapshot_interval=1e9)
+        sim.start(1e-5, 1e-5, 100, snapshot_interval=1e9, graph=graph)
         sim.advance(sim.nt)
         sim.finish()
         torch.cuda.synchronize()
@@ -544,7 +545,7 @@ def pcg_rate(kernels, B):
     return round(moved / t / 1e9, 1)
 
 
-def window(N, dev, warmup, steps, period):
+def window(N, dev, warmup, steps, period, graph=False):
     """A fresh run of the workload: `warmup` steps, then `steps` steps timed
     as a whole, with per-kernel HIP events on every `period`-th step (0:
     none).  Returns (sim, elapsed s, iterations of the timed steps, t_setup)."""
@@ -552,7 +553,7 @@ def window(N, dev, warmup, steps, period):
     sim, _ = build_sim(N, dev)
     t0 = time.perf_counter()
     sim.start(1e-5, 1e-5, 100, calculate_AT_RT=True, linear_guess=True,
-              snapshot_interval=1e9, verbose=False)
+              snapshot_interval=1e9, verbose=False, graph=graph)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
     sim.advance(warmup)
@@ -592,6 +593,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 5 plain device copies of this many MiB (the "
                          "calibration for HBM byte counters) and exit")
+    ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
+                    help="1: every leg replays each step's fixed launches as a captured "
+                         "HIP graph (timed steps launch eagerly)")
     ap.add_argument("--trace-summary", default="",
                     help="only summarise the last --steps steps of each pass in this "
                          "rocprofv3 kernel_trace.csv (no GPU needed) and exit")
@@ -617,7 +621,8 @@ def main():
         period = 0
     print(f"[bench] building {N}^3 workload; warmup {args.warmup}, timing {args.steps} steps",
           flush=True)
-    sim, elapsed, iters, t_setup = window(N, dev, args.warmup, args.steps, period)
+    graph = bool(args.graph)
+    sim, elapsed, iters, t_setup = window(N, dev, args.warmup, args.steps, period, graph)
     n, nnz = sim.n_nodes, sim.system.nnz
     value = n * args.steps / elapsed
     S = len(sim.ionic_model.STATES)
@@ -652,12 +657,23 @@ def main():
         roof["copy_ceiling_GBps"] = cc
         roof["frac_of_copy"] = round(roof["achieved"] / cc, 4)
     pcg = pcg_rate(kernels, B) if kernels else None
+    # the same window with the step's fixed launches replayed as a HIP graph
+    # (and eagerly when the timed pass used graphs), no kernel timing
+    other = not graph
+    print(f"[bench] the same window, graph replay {other}", flush=True)
+    sim, el_o, it_o, _ = window(N, dev, args.warmup, args.steps, 0, other)
+    graph_replay = {"ms_per_step": round((el_o if other else elapsed) / args.steps * 1e3, 4),
+                    "eager_ms_per_step": round((elapsed if other else el_o) / args.steps * 1e3, 4),
+                    "same_iterations": it_o == iters}
+    sim._free_sim()
+    del sim
+    torch.cuda.empty_cache()
     # 3. a steady-state window (the wave well under way)
     steady = None
     if not args.no_steady:
         ws, ss = 200, 2000
         print(f"[bench] steady state: {ss} steps after {ws}", flush=True)
-        sim, el, it3, _ = window(N, dev, ws, ss, 0 if args.no_kernel_timing else 16)
+        sim, el, it3, _ = window(N, dev, ws, ss, 0 if args.no_kernel_timing else 16, graph)
         kst = kernel_stats(sim.kernel_times(), B) if not args.no_kernel_timing else {}
         steady = {"steps": ss, "t_window_ms": [ws * 0.01, (ws + ss) * 0.01],
                   "ms_per_step": round(el / ss * 1e3, 4), "node_timesteps_per_s": n * ss / el,
@@ -670,13 +686,13 @@ def main():
         torch.cuda.empty_cache()
     batched = None
     if not args.no_batched:
-        batched = batched_config(dev, args.batch_steps, args.batch_warmup)
+        batched = batched_config(dev, args.batch_steps, args.batch_warmup, graph)
         tuning["batch_max_blocks"] = batched["max_blocks"]
         gc.collect()
         torch.cuda.empty_cache()
     served = None
     if not args.no_served:
-        served = served_config(dev, args.served_requests)
+        served = served_config(dev, args.served_requests, graph=graph)
         tuning["serve_slices_per_block"] = sorted({autotune.lookup("serve_slices_per_block", m ** 3)
                                                    for m in range(11, 82)})
         gc.collect()
@@ -710,6 +726,8 @@ def main():
         "batched": batched,
         "served": served,
         "irregular": irregular,
+        "graph": graph,
+        "graph_replay": graph_replay,
         "setup_s": round(t_setup, 2),
     }
     print(json.dumps(line), flush=True)

@@ -42,7 +42,8 @@ enum tc_dtype { TC_F32 = 0, TC_F64 = 1 };
 
 /* Text of the last error on this thread ("" if none). */
 const char* tc_last_error(void);
-/* ABI version of this library (bumped on any signature change). */
+/* ABI version of this library (bumped on any signature or descriptor layout
+ * change). */
 int tc_abi_version(void);
 
 /* ------------------------------------------------------------------------
@@ -432,6 +433,10 @@ typedef struct tc_sim_desc {
   int k_history;             /* K (PCG iterations launched per step before its status is
                                 read) = the most any of the last k_history steps needed;
                                 0 = 4, at most 256 (same results for any value) */
+  int graph;                 /* 1: each step's RHS + PCG launches (fixed by the buffer
+                                parity and the iteration count) are replayed as a captured
+                                HIP graph; same results.  Steps with timing events launch
+                                eagerly. */
 } tc_sim_desc;
 typedef struct tc_sim tc_sim;
 int tc_sim_create(tc_sim** sim, const tc_sim_desc* desc);
@@ -468,6 +473,10 @@ int tc_sim_status(tc_sim* sim, double* t_host, int64_t* steps_host, int* x_index
  * counting only PCG iterations that ran. */
 int tc_sim_set_timing(tc_sim* sim, int period);
 int tc_sim_get_timing(tc_sim* sim, double* ms_host, int64_t* count_host);
+/* Graph mode: graphs recorded and replayed so far.  No counterpart in the
+ * reference (its loop launches every operation eagerly,
+ * torchcor/simulator/monodomain.py:164-176). */
+int tc_sim_graph_stats(const tc_sim* sim, int64_t* captures_host, int64_t* replays_host);
 
 /* ------------------------------------------------------------------------
  * Batched simulations on one shared mesh (no counterpart in the reference:
@@ -527,6 +536,9 @@ typedef struct tc_batch_desc {
   size_t ws_bytes;
   int max_blocks;            /* cap on the blocks of the PCG passes, 8..1024, or 0 for
                                 1024 (a tuning choice; it groups the partial sums) */
+  int graph;                 /* 1: each step's RHS + PCG launches (fixed by the buffer
+                                parity and the iteration count) are replayed as a captured
+                                HIP graph; same results */
 } tc_batch_desc;
 typedef struct tc_batch tc_batch;
 int tc_batch_workspace_size(int64_t n, int lanes, int dtype, size_t* bytes_host);
@@ -536,6 +548,10 @@ int tc_batch_destroy(tc_batch* batch);
 /* nsteps time steps; iters[k] (host) = the largest n_iter over the
  * instances in step k.  Synchronizes at the end. */
 int tc_batch_advance(tc_batch* batch, int64_t nsteps, int32_t* iters_host, void* stream);
+/* Graph mode: graphs recorded and replayed so far.  No counterpart in the
+ * reference (its loop launches every operation eagerly,
+ * torchcor/simulator/monodomain.py:164-176). */
+int tc_batch_graph_stats(const tc_batch* batch, int64_t* captures_host, int64_t* replays_host);
 /* LAT/RT test of the last solution; per-instance sums of n_iter (B). */
 int tc_batch_finish(tc_batch* batch, int64_t* total_iters_host, void* stream);
 /* t, steps done, index 0..1 of desc.x holding the solution. */
@@ -588,6 +604,10 @@ typedef struct tc_serve_desc {
   size_t ws_bytes;
   int k_history;             /* as tc_sim_desc: K = the most any of the last k_history
                                 engine steps needed (0 = 4, at most 256) */
+  int graph;                 /* 1: each engine step's ionic, RHS + PCG launches (fixed by
+                                the live systems, the buffer parity and the iteration
+                                count) are replayed as a captured HIP graph, recorded anew
+                                after every admission or retirement; same results */
 } tc_serve_desc;
 typedef struct tc_serve_system {
   int64_t n;                 /* nodes */
@@ -631,6 +651,9 @@ int tc_serve_advance(tc_serve* serve, int64_t nsteps, int32_t* iters_host, void*
 /* Engine steps since creation, and how many of them needed more PCG
  * iterations than the pipeline had launched (each drains the pipeline). */
 int tc_serve_stats(const tc_serve* serve, int64_t* steps_host, int64_t* stalls_host);
+/* Graph mode: graphs recorded and replayed so far.  No counterpart in the
+ * reference. */
+int tc_serve_graph_stats(const tc_serve* serve, int64_t* captures_host, int64_t* replays_host);
 
 /* Device copy of nbytes (a multiple of 16, 16-byte aligned buffers), one
  * 16-byte load and store per thread: the bandwidth reference of the

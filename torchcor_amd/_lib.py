@@ -88,7 +88,8 @@ class SimDesc(C.Structure):
                 ("w", p), ("AT", p), ("RT", p),
                 ("stim_mask", p), ("n_stim", i32), ("stim", Stimulus * MAX_STIMULI),
                 ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32), ("linear_guess", i32),
-                ("depth", i32), ("state_stride", i64), ("separate_check", i32), ("k_history", i32)]
+                ("depth", i32), ("state_stride", i64), ("separate_check", i32), ("k_history", i32),
+                ("graph", i32)]
 
 
 BATCH_MAX_K = 5
@@ -109,7 +110,7 @@ class BatchDesc(C.Structure):
                 ("stim", Stimulus * MAX_STIMULI), ("n_groups", i32),
                 ("groups", BatchGroup * BATCH_MAX_GROUPS), ("a_tol", dbl), ("r_tol", dbl),
                 ("max_iter", i32), ("linear_guess", i32), ("depth", i32), ("ws", p),
-                ("ws_bytes", C.c_size_t), ("max_blocks", i32)]
+                ("ws_bytes", C.c_size_t), ("max_blocks", i32), ("graph", i32)]
 
 
 class ServeDesc(C.Structure):
@@ -121,7 +122,7 @@ class ServeDesc(C.Structure):
                 ("tab", (p * 3) * 3), ("dt", dbl), ("dt_ionic", dbl), ("Cm", dbl),
                 ("theta", dbl), ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32),
                 ("linear_guess", i32), ("depth", i32), ("ws", p), ("ws_bytes", C.c_size_t),
-                ("k_history", i32)]
+                ("k_history", i32), ("graph", i32)]
 
 
 class ServeSystem(C.Structure):
@@ -193,12 +194,14 @@ FUNCS = {
     "tc_sim_status": (p, C.POINTER(dbl), I64_P, INT_P, I64_P),
     "tc_sim_set_timing": (p, i32),
     "tc_sim_get_timing": (p, C.POINTER(dbl), I64_P),
+    "tc_sim_graph_stats": (p, I64_P, I64_P),
     "tc_batch_workspace_size": (i64, i32, i32, SIZE_P),
     "tc_batch_create": (C.POINTER(p), C.POINTER(BatchDesc), p),
     "tc_batch_destroy": (p,),
     "tc_batch_advance": (p, i64, C.POINTER(C.c_int32), p),
     "tc_batch_finish": (p, I64_P, p),
     "tc_batch_status": (p, C.POINTER(dbl), I64_P, INT_P),
+    "tc_batch_graph_stats": (p, I64_P, I64_P),
     "tc_serve_workspace_size": (i64, i32, SIZE_P),
     "tc_serve_create": (C.POINTER(p), C.POINTER(ServeDesc), p),
     "tc_serve_destroy": (p,),
@@ -207,6 +210,7 @@ FUNCS = {
     "tc_serve_remove": (p, i32, p, p, p, I64_P, I64_P, p),
     "tc_serve_advance": (p, i64, C.POINTER(C.c_int32), p),
     "tc_serve_stats": (p, I64_P, I64_P),
+    "tc_serve_graph_stats": (p, I64_P, I64_P),
 }
 for _n, _a in FUNCS.items():
     _proto(_n, *_a)

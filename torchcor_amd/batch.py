@@ -188,9 +188,11 @@ class SharedMeshBatch:
 
     # ------------------------------------------------------------- run
     def start(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True, depth=4,
-              x_f64=True, precise_rhs=True, max_blocks=None):
+              x_f64=True, precise_rhs=True, max_blocks=None, graph=False):
         """Set up the run.  ``max_blocks`` caps the blocks of the PCG passes
-        (default: the tuning table's value for n x lanes)."""
+        (default: the tuning table's value for n x lanes).  ``graph``
+        replays each step's RHS and PCG launches as a captured HIP graph
+        (same results)."""
         self.assemble()
         dev, dt_ = self.device, self.dtype
         n, B = self.n_nodes, len(self.instances)
@@ -285,6 +287,7 @@ class SharedMeshBatch:
         d.max_blocks = int(max_blocks) if max_blocks else \
             autotune.lookup("batch_max_blocks", n * lanes)
         self.max_blocks = d.max_blocks
+        d.graph = 1 if graph else 0
         nb = C.c_size_t(0)
         L.call("tc_batch_workspace_size", n, lanes, d.dtype, C.byref(nb))
         self._ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)
@@ -296,6 +299,12 @@ class SharedMeshBatch:
         self._desc = d
         self.iterations = []
         self.lanes = lanes
+
+    def graph_stats(self):
+        """(graphs recorded, graph replays) of the current run."""
+        cap, rep = C.c_int64(0), C.c_int64(0)
+        L.call("tc_batch_graph_stats", self._h, C.byref(cap), C.byref(rep))
+        return cap.value, rep.value
 
     def advance(self, k):
         k = int(k)

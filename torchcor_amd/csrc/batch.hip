@@ -584,6 +584,7 @@ struct tc_batch {
   long long seq;
   tc::BHost vs;
   std::deque<int> recent;
+  tc::GraphCache graphs;  // graph mode: the step's RHS + PCG launches
 };
 
 namespace tc {
@@ -720,6 +721,27 @@ void bat_rhs_launch(tc_batch* S, const BatOp<T>& op, int xc, int pc, hipStream_t
                        xn, xpn, (T*)S->r, (const T*)d.Dinv, S->ctl, S->part[0]);
 }
 
+// The RHS + PCG start pass, its fold and iterations 0 .. K-1 of a step whose
+// solution is in x[xc] (x_prev in xp[pc]).
+template <class T, class XT, class WT, int NK>
+int bat_pcg_seq(tc_batch* S, int xc, int pc, int K, hipStream_t s) {
+  const tc_batch_desc& d = S->d;
+  const BatOp<T> op = make_op<T>(S);
+  XT* xs;
+  if (d.linear_guess) {
+    bat_rhs_launch<T, XT, WT, true, NK>(S, op, xc, pc, s);
+    xs = (XT*)d.x[1 - xc];
+  } else {
+    bat_rhs_launch<T, XT, WT, false, NK>(S, op, xc, pc, s);
+    xs = (XT*)d.x[xc];
+  }
+  TC_LAUNCH();
+  hipLaunchKernelGGL(k_bat_fold, dim3(d.B), dim3(kBB), 0, s, 0, 0, S->G, d.lanes, d.B, S->ctl,
+                     (const double*)S->part[0]);
+  TC_LAUNCH();
+  return bat_iters<T, XT, NK>(S, 0, K, xs, s);
+}
+
 template <class T, class XT, class WT, int NK>
 int bat_enqueue(tc_batch* S, BHost& ps, BPlan& plan, hipStream_t s) {
   const tc_batch_desc& d = S->d;
@@ -759,24 +781,16 @@ int bat_enqueue(tc_batch* S, BHost& ps, BPlan& plan, hipStream_t s) {
     int rc = group_ionic(S, g, &io, s);
     if (rc) return rc;
   }
-  // 2. RHS + PCG start
-  const BatOp<T> op = make_op<T>(S);
-  const dim3 g(S->G), blk(kBB);
-  XT* xs;
-  if (d.linear_guess) {
-    bat_rhs_launch<T, XT, WT, true, NK>(S, op, xc, pc, s);
-    xs = (XT*)d.x[1 - xc];
-  } else {
-    bat_rhs_launch<T, XT, WT, false, NK>(S, op, xc, pc, s);
-    xs = (XT*)d.x[xc];
-  }
-  TC_LAUNCH();
-  hipLaunchKernelGGL(k_bat_fold, dim3(d.B), dim3(kBB), 0, s, 0, 0, S->G, d.lanes, d.B, S->ctl,
-                     (const double*)S->part[0]);
-  TC_LAUNCH();
-  // 3. speculative iterations, 4. check
-  int rc = bat_iters<T, XT, NK>(S, 0, K, xs, s);
+  // 2. RHS + PCG start, 3. speculative iterations: one launch sequence
+  // fixed by the buffer parity and K, replayed as a graph in graph mode
+  int rc;
+  if (d.graph)
+    rc = graph_replay(S->graphs, (long long)((xc << 1) | pc) | ((long long)K << 2), s,
+                      [&](hipStream_t cs) { return bat_pcg_seq<T, XT, WT, NK>(S, xc, pc, K, cs); });
+  else
+    rc = bat_pcg_seq<T, XT, WT, NK>(S, xc, pc, K, s);
   if (rc) return rc;
+  // 4. check
   plan.seq = ++S->seq;
   rc = bat_check(S, K, slot, plan.seq, s);
   if (rc) return rc;
@@ -1006,8 +1020,16 @@ extern "C" int tc_batch_create(tc_batch** out, const tc_batch_desc* d, void* str
 
 extern "C" int tc_batch_destroy(tc_batch* S) {
   if (!S) return TC_OK;
+  S->graphs.release();
   if (S->host_status) (void)hipHostFree(S->host_status);
   delete S;
+  return TC_OK;
+}
+
+extern "C" int tc_batch_graph_stats(const tc_batch* S, int64_t* captures, int64_t* replays) {
+  TC_CHECK(S && captures && replays, TC_ERR_ARG, "tc_batch_graph_stats: bad arguments");
+  *captures = S->graphs.captures;
+  *replays = S->graphs.replays;
   return TC_OK;
 }
 

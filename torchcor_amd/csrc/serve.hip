@@ -930,6 +930,7 @@ struct tc_serve {
   int history;  // engine steps whose iteration counts set K
   int64_t steps;
   int64_t stalls = 0;    // steps that needed more iterations than were launched
+  tc::GraphCache graphs; // graph mode: each engine step's fixed launches
 };
 
 namespace tc {
@@ -989,6 +990,7 @@ SvPool sv_pool(const tc_serve* S) {
 // Rebuild the slice lists, the blocks (spb slices of one system each) and
 // the per-system block ranges (pipeline drained).
 int sv_rebuild(tc_serve* S, hipStream_t s) {
+  S->graphs.clear();  // their launches carry the old lists, blocks and counts
   std::vector<int32_t> list, ml[3], sl, vb0, vb1, bl0, bq;
   for (int q = 0; q < (int)S->sys.size(); ++q) {
     const SysHost& h = S->sys[q];
@@ -1159,14 +1161,11 @@ int sv_wait(tc_serve* S, int slot, long long seq) {
   return TC_OK;
 }
 
+// The ionic pass, the RHS + PCG start pass and iterations 0 .. K-1 of an
+// engine step whose solution is in x[S->xc] (x_prev in xp[S->pc]).
 template <class T, class XT, class WT>
-int sv_enqueue(tc_serve* S, SPlan& plan, hipStream_t s) {
+int sv_step_seq(tc_serve* S, int K, hipStream_t s) {
   const tc_serve_desc& d = S->d;
-  const int64_t step = S->steps + 1;
-  const int slot = (int)(step % kSRing);
-  int K = 1;
-  for (int v : S->recent) K = v > K ? v : K;
-  if (K > d.max_iter) K = d.max_iter;
   {
     int rc = sv_ionic(S, s);
     if (rc) return rc;
@@ -1194,7 +1193,26 @@ int sv_enqueue(tc_serve* S, SPlan& plan, hipStream_t s) {
     xs = (XT*)d.x[xc];
   }
   TC_LAUNCH();
-  int rc = sv_iters<T, XT>(S, 0, K, xs, s);
+  return sv_iters<T, XT>(S, 0, K, xs, s);
+}
+
+template <class T, class XT, class WT>
+int sv_enqueue(tc_serve* S, SPlan& plan, hipStream_t s) {
+  const tc_serve_desc& d = S->d;
+  const int64_t step = S->steps + 1;
+  const int slot = (int)(step % kSRing);
+  int K = 1;
+  for (int v : S->recent) K = v > K ? v : K;
+  if (K > d.max_iter) K = d.max_iter;
+  const int xc = S->xc, pc = S->pc;
+  // the launches fixed by the live systems, the buffer parity and K: a
+  // graph replay in graph mode
+  int rc;
+  if (d.graph)
+    rc = graph_replay(S->graphs, (long long)((xc << 1) | pc) | ((long long)K << 2), s,
+                      [&](hipStream_t cs) { return sv_step_seq<T, XT, WT>(S, K, cs); });
+  else
+    rc = sv_step_seq<T, XT, WT>(S, K, s);
   if (rc) return rc;
   plan.seq = ++S->seq;
   plan.slot = slot;
@@ -1396,6 +1414,7 @@ extern "C" int tc_serve_create(tc_serve** out, const tc_serve_desc* d, void* str
 
 extern "C" int tc_serve_destroy(tc_serve* S) {
   if (!S) return TC_OK;
+  S->graphs.release();
   if (S->host_status) (void)hipHostFree(S->host_status);
   delete S;
   return TC_OK;
@@ -1578,6 +1597,13 @@ extern "C" int tc_serve_stats(const tc_serve* S, int64_t* steps, int64_t* stalls
   TC_CHECK(S && steps && stalls, TC_ERR_ARG, "tc_serve_stats: bad arguments");
   *steps = S->steps;
   *stalls = S->stalls;
+  return TC_OK;
+}
+
+extern "C" int tc_serve_graph_stats(const tc_serve* S, int64_t* captures, int64_t* replays) {
+  TC_CHECK(S && captures && replays, TC_ERR_ARG, "tc_serve_graph_stats: bad arguments");
+  *captures = S->graphs.captures;
+  *replays = S->graphs.replays;
   return TC_OK;
 }
 

@@ -103,6 +103,7 @@ struct tc_sim {
   hipStream_t cs;
   hipEvent_t written[tc::kMaxRingSlots], copied[tc::kMaxRingSlots];
   bool slot_busy[tc::kMaxRingSlots];
+  tc::GraphCache graphs;  // graph mode: each step's fixed RHS + PCG launches
 };
 
 namespace tc {
@@ -151,6 +152,30 @@ void launch_spmv(tc_pcg* P, const CI* sc, T* pin, T* pout, int k, hipStream_t s,
          P->A.pat,
          (const T*)P->Av, (const T*)P->Dinv, (const T*)P->r, pin, pout, (T*)P->Ap, k, P->ctl,
          (const double*)P->part[0], (const double*)P->part[2], P->part[1]);
+}
+
+// The spmv pass of iteration k.
+template <class T>
+int launch_spmv_at(tc_pcg* P, int k, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  T* pp[2] = {(T*)P->p[0], (T*)P->p[1]};
+  if (P->A.dcol) launch_spmv<T>(P, P->A.dcol, pp[k & 1], pp[(k + 1) & 1], k, s, e0, e1);
+  else launch_spmv<T>(P, P->A.col, pp[k & 1], pp[(k + 1) & 1], k, s, e0, e1);
+  TC_LAUNCH();
+  return TC_OK;
+}
+
+// The update pass of iteration k with the stopping test, publishing record
+// seq in status slot `slot`.
+template <class T, class XT>
+int launch_check_update(tc_sim* S, int k, XT* xs, int slot, long long seq, hipStream_t s,
+                        hipEvent_t e0, hipEvent_t e1) {
+  tc_pcg* P = S->d.pcg;
+  T* pout = (T*)P->p[(k + 1) & 1];
+  launch(k_pcg_update<T, XT, true>, dim3(P->G), dim3(kPcgBlock), s, e0, e1, P->n,
+         (const T*)P->Dinv, (const T*)pout, (const T*)P->Ap, xs, (T*)P->r, k, P->ctl,
+         (const double*)P->part[1], P->part[2], S->dev_view + slot, 1, S->d.max_iter, seq);
+  TC_LAUNCH();
+  return TC_OK;
 }
 
 // Iterations k0 .. k1-1 (spmv and update passes).  With check_slot >= 0
@@ -275,6 +300,30 @@ void rhs_init_launch(tc_sim* S, const CI* sc, int xc, int pc, double coef, doubl
          coef, d.Cm, tdt, P->ctl, d.a_tol, d.r_tol, P->part[0]);
 }
 
+// The fused RHS + PCG start pass of a step whose solution is in x[xc]
+// (x_prev in xp[pc]).
+template <class T, class XT, class WT>
+int rhs_init_seq(tc_sim* S, int xc, int pc, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  const tc_sim_desc& d = S->d;
+  tc_pcg* P = d.pcg;
+  const dim3 g(P->G), blk(kPcgBlock);
+  const double coef = (1.0 - d.theta) * d.dt;
+  const double tdt = d.theta * d.dt;
+  if (d.linear_guess) {
+    if (P->A.dcol)
+      rhs_init_launch<T, XT, WT, true>(S, P->A.dcol, xc, pc, coef, tdt, g, blk, s, e0, e1);
+    else
+      rhs_init_launch<T, XT, WT, true>(S, P->A.col, xc, pc, coef, tdt, g, blk, s, e0, e1);
+  } else {
+    if (P->A.dcol)
+      rhs_init_launch<T, XT, WT, false>(S, P->A.dcol, xc, pc, coef, tdt, g, blk, s, e0, e1);
+    else
+      rhs_init_launch<T, XT, WT, false>(S, P->A.col, xc, pc, coef, tdt, g, blk, s, e0, e1);
+  }
+  TC_LAUNCH();
+  return TC_OK;
+}
+
 // Enqueue the step after planned state ps; ps becomes the state after it.
 template <class T, class XT, class WT>
 int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
@@ -318,37 +367,36 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   io.state_stride = d.state_stride;
   int rc = ionic_launch(S, &io, s, ev ? (*ev)[0] : nullptr, ev ? (*ev)[1] : nullptr);
   if (rc) return rc;
-  // 2. RHS + PCG start
-  const dim3 g(P->G), blk(kPcgBlock);
-  const double coef = (1.0 - d.theta) * d.dt;
-  const double tdt = d.theta * d.dt;
-  XT* xs;
-  hipEvent_t r0 = ev ? (*ev)[2] : nullptr, r1 = ev ? (*ev)[3] : nullptr;
-  if (d.linear_guess) {
-    if (P->A.dcol)
-      rhs_init_launch<T, XT, WT, true>(S, P->A.dcol, xc, pc, coef, tdt, g, blk, s, r0,
-                                           r1);
-    else
-      rhs_init_launch<T, XT, WT, true>(S, P->A.col, xc, pc, coef, tdt, g, blk, s, r0,
-                                           r1);
-    xs = (XT*)d.x[1 - xc];
-  } else {
-    if (P->A.dcol)
-      rhs_init_launch<T, XT, WT, false>(S, P->A.dcol, xc, pc, coef, tdt, g, blk, s, r0,
-                                           r1);
-    else
-      rhs_init_launch<T, XT, WT, false>(S, P->A.col, xc, pc, coef, tdt, g, blk, s, r0,
-                                           r1);
-    xs = (XT*)d.x[xc];
-  }
-  TC_LAUNCH();
-  // 3. speculative iterations; 4. the stopping test, in the last update
-  // pass (or a check pass of its own when no iteration is launched or
-  // fusing is off)
+  // 2. RHS + PCG start; 3. speculative iterations; 4. the stopping test,
+  // in the last update pass (or a check pass of its own when no iteration
+  // is launched or fusing is off)
+  XT* xs = (XT*)d.x[d.linear_guess ? 1 - xc : xc];
   plan.seq = ++S->seq;
   plan.fused = S->fuse_check && K > 0;
-  rc = launch_iters<T, XT>(S, 0, K, xs, s, ev, 6, plan.fused ? slot : -1, plan.seq);
-  if (rc) return rc;
+  if (d.graph && !ev) {
+    // the launches fixed by the buffer parity and K (all but the last
+    // update when it runs the test, which takes this step's record) replayed
+    // as one graph
+    const int kg = plan.fused ? K - 1 : K;
+    const long long key =
+        (long long)((xc << 1) | pc) | ((long long)K << 2) | ((long long)P->G << 20);
+    rc = graph_replay(S->graphs, key, s, [&](hipStream_t cs) {
+      int r = rhs_init_seq<T, XT, WT>(S, xc, pc, cs, nullptr, nullptr);
+      if (r == TC_OK) r = launch_iters<T, XT>(S, 0, kg, xs, cs, nullptr, 0);
+      if (r == TC_OK && plan.fused) r = launch_spmv_at<T>(P, K - 1, cs, nullptr, nullptr);
+      return r;
+    });
+    if (rc) return rc;
+    if (plan.fused) {
+      rc = launch_check_update<T, XT>(S, K - 1, xs, slot, plan.seq, s, nullptr, nullptr);
+      if (rc) return rc;
+    }
+  } else {
+    rc = rhs_init_seq<T, XT, WT>(S, xc, pc, s, ev ? (*ev)[2] : nullptr, ev ? (*ev)[3] : nullptr);
+    if (rc) return rc;
+    rc = launch_iters<T, XT>(S, 0, K, xs, s, ev, 6, plan.fused ? slot : -1, plan.seq);
+    if (rc) return rc;
+  }
   if (!plan.fused) {
     rc = check_launch(S, K - 1, slot, plan.seq, s, ev ? (*ev)[4] : nullptr,
                       ev ? (*ev)[5] : nullptr);
@@ -577,6 +625,7 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
 
 extern "C" int tc_sim_destroy(tc_sim* S) {
   if (!S) return TC_OK;
+  S->graphs.release();
   for (auto& set : S->ev_sets)
     for (hipEvent_t e : set) (void)hipEventDestroy(e);
   if (S->cs) {
@@ -659,6 +708,13 @@ extern "C" int tc_sim_set_timing(tc_sim* S, int on) {
     S->acc_ms[k] = 0.0;
     S->acc_n[k] = 0;
   }
+  return TC_OK;
+}
+
+extern "C" int tc_sim_graph_stats(const tc_sim* S, int64_t* captures, int64_t* replays) {
+  TC_CHECK(S && captures && replays, TC_ERR_ARG, "tc_sim_graph_stats: bad arguments");
+  *captures = S->graphs.captures;
+  *replays = S->graphs.replays;
   return TC_OK;
 }
 

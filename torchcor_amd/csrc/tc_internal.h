@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdarg>
 #include <string>
+#include <vector>
 
 #include "../../include/torchcor_amd.h"
 
@@ -105,7 +106,68 @@ __host__ __device__ inline uint32_t stimulus_bits(const tc_stimulus* st, int n_s
 template <class F, class... Args>
 inline void launch(F kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
                    Args... args) {
-  hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, args...);
+  if (e0 || e1) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, args...);
+  else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
+// Instantiated HIP graphs of an engine's fixed launch sequences, keyed by
+// what a sequence depends on (buffer parity, iteration count).  A sequence
+// is recorded once by stream capture on a private stream and then replayed
+// with one hipGraphLaunch on the caller's stream.  Recorded sequences hold
+// kernel launches only (no memset or copy nodes).  An engine clears its
+// cache whenever an argument of a recorded launch changes.
+struct GraphCache {
+  static constexpr size_t kMax = 64;
+  struct Ent {
+    long long key;
+    hipGraphExec_t exec;
+  };
+  std::vector<Ent> ents;
+  hipStream_t cap = nullptr;
+  long long captures = 0, replays = 0;
+  void clear() {
+    for (Ent& e : ents) (void)hipGraphExecDestroy(e.exec);
+    ents.clear();
+  }
+  void release() {
+    clear();
+    if (cap) (void)hipStreamDestroy(cap);
+    cap = nullptr;
+  }
+};
+
+// Replays the graph of `key`, recording it first with rec(stream) (which
+// returns a TC_* code) when the cache has none.
+template <class F>
+int graph_replay(GraphCache& c, long long key, hipStream_t s, F&& rec) {
+  hipGraphExec_t exec = nullptr;
+  for (const GraphCache::Ent& e : c.ents)
+    if (e.key == key) exec = e.exec;
+  if (!exec) {
+    if (c.ents.size() >= GraphCache::kMax) c.clear();
+    if (!c.cap) TC_HIP(hipStreamCreateWithFlags(&c.cap, hipStreamNonBlocking));
+    TC_HIP(hipStreamBeginCapture(c.cap, hipStreamCaptureModeRelaxed));
+    const int rc = rec(c.cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c.cap, &g);
+    if (rc != TC_OK || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      if (rc != TC_OK) return rc;
+      set_error("graph capture -> %s", hipGetErrorString(e));
+      return TC_ERR_HIP;
+    }
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      set_error("graph instantiate -> %s", hipGetErrorString(ei));
+      return TC_ERR_HIP;
+    }
+    c.ents.push_back({key, exec});
+    ++c.captures;
+  }
+  TC_HIP(hipGraphLaunch(exec, s));
+  ++c.replays;
+  return TC_OK;
 }
 
 // Fused ionic steps (the tc_*_step entry points), the kernel optionally

@@ -66,7 +66,7 @@ class ServeEngine:
     def __init__(self, cap_nodes, cap_entries, max_sys=256, dt=0.01, dtype=torch.float32,
                  device=None, a_tol=1e-5, r_tol=1e-5, max_iter=100, x_f64=True,
                  precise_rhs=True, lat=True, depth=4, int16_columns=True, slices_per_block=None,
-                 k_history=0):
+                 k_history=0, graph=False):
         self.device = torch.device(device) if device is not None else default_device()
         dev = self.device
         self.dtype = dtype
@@ -114,6 +114,7 @@ class ServeEngine:
         d.a_tol, d.r_tol, d.max_iter = float(a_tol), float(r_tol), int(max_iter)
         d.linear_guess, d.depth = 1, int(depth)
         d.k_history = int(k_history)
+        d.graph = 1 if graph else 0
         nb = C.c_size_t(0)
         L.call("tc_serve_workspace_size", cap_slices, int(max_sys), C.byref(nb))
         self._ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)
@@ -233,6 +234,12 @@ class ServeEngine:
         steps, stalls = C.c_int64(0), C.c_int64(0)
         L.call("tc_serve_stats", self._h, C.byref(steps), C.byref(stalls))
         return dict(steps=int(steps.value), stalls=int(stalls.value))
+
+    def graph_stats(self):
+        """(graphs recorded, graph replays) so far (graph mode)."""
+        cap, rep = C.c_int64(0), C.c_int64(0)
+        L.call("tc_serve_graph_stats", self._h, C.byref(cap), C.byref(rep))
+        return cap.value, rep.value
 
     # -------------------------------------------------------------- loop
     def run_until_done(self, prepared, max_chunk=200):

@@ -269,7 +269,7 @@ class Monodomain:
     # on the device and no host round trip is needed per step.
     def start(self, a_tol, r_tol, max_iter, calculate_AT_RT=True, linear_guess=True,
               snapshot_interval=1, verbose=False, depth=4, x_f64=True, precise_rhs=True,
-              frames="auto", separate_check=False, k_history=0):
+              frames="auto", separate_check=False, k_history=0, graph=False):
         """Set up a run.  ``x_f64`` keeps the PCG iterate in float64 in float32
         runs, as the reference's ConjugateGradient does (solver.py:24-26);
         x_prev stays in the working dtype.  ``precise_rhs`` also keeps the
@@ -285,7 +285,9 @@ class Monodomain:
         ``k_history``: each step launches as many PCG iterations as the most
         any of the last ``k_history`` steps needed (0: the runtime's
         default) before its status is read; any value gives the same
-        results."""
+        results.  ``graph`` replays each step's RHS and PCG launches as a
+        captured HIP graph (one per buffer parity and iteration count; same
+        results)."""
         import ctypes as C
         self.assemble()
         ion = self.ionic_model
@@ -348,6 +350,7 @@ class Monodomain:
         d.depth = int(depth)
         d.separate_check = 1 if separate_check else 0
         d.k_history = int(k_history)
+        d.graph = 1 if graph else 0
         self._free_sim()
         h = C.c_void_p(0)
         L.call("tc_sim_create", C.byref(h), C.byref(d))
@@ -412,6 +415,13 @@ class Monodomain:
         """Time each kernel of the runtime with HIP events around its launch,
         on every ``period``-th step."""
         L.call("tc_sim_set_timing", self._sim, int(period) if on else 0)
+
+    def graph_stats(self):
+        """(graphs recorded, graph replays) of the current run."""
+        import ctypes as C
+        cap, rep = C.c_int64(0), C.c_int64(0)
+        L.call("tc_sim_graph_stats", self._sim, C.byref(cap), C.byref(rep))
+        return cap.value, rep.value
 
     def kernel_times(self):
         """{kernel: (total_ms, launches)} since ``set_kernel_timing(True)``."""
