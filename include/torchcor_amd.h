@@ -242,6 +242,40 @@ int tc_pcg_solve(tc_pcg* plan, const void* b, void* x, const void* x_prev, void*
  * tc_sim_advance calls (its pipeline is drained then), not during one.
  * No counterpart in the reference. */
 int tc_pcg_set_blocks(tc_pcg* plan, int blocks);
+
+/* Staged gathers for operators whose slices do not share a stencil (an
+ * unstructured mesh): the spmv pass of block v first forms p' once for
+ * every distinct column its rows reference (cols[col_ptr[v] ..
+ * col_ptr[v+1]), ascending) into LDS, then reads each SELL entry's p'
+ * through its 16-bit position lidx in that set, instead of gathering
+ * Dinv, r and p per entry.  Same products in the same order: bit-identical
+ * results.  Block v owns slices [v*per, (v+1)*per), per = ceil(n_slices /
+ * blocks), as the passes split them.  Replaces the gathers inside the
+ * reference's CSR product A @ p (torchcor/core/solver.py:52,58); no
+ * counterpart otherwise. */
+typedef struct tc_stage {
+  int blocks;               /* the plan's block count the stage was built for */
+  int max_cols;             /* largest set of one block (<= TC_STAGE_MAX_COLS) */
+  const int32_t* col_ptr;   /* (blocks+1) device */
+  const int32_t* own;       /* (blocks) position of the block's first row in its set */
+  const int32_t* cols;      /* (col_ptr[blocks]) device */
+  const uint16_t* lidx;     /* (padded entries) device */
+} tc_stage;
+#define TC_STAGE_MAX_COLS 4096
+/* Host: the stage of a SELL pattern (slice_ptr, col: host copies, per-row
+ * int32 columns) split over `blocks` blocks.  cols_host needs room for
+ * slice_ptr[n_slices] entries (an upper bound); col_ptr_host (blocks+1),
+ * own_host (blocks), lidx_host (slice_ptr[n_slices]).  *max_cols_host
+ * receives the largest set; TC_ERR_SIZE when it exceeds TC_STAGE_MAX_COLS
+ * (the pattern is then left unstaged). */
+int tc_sell_stage(int64_t n, int64_t n_slices, const int32_t* slice_ptr_host,
+                  const int32_t* col_host, int blocks, int32_t* col_ptr_host, int32_t* own_host,
+                  int32_t* cols_host, uint16_t* lidx_host, int32_t* max_cols_host);
+/* Attach a stage (device arrays owned by the caller; NULL detaches).  The
+ * spmv passes of a time-step runtime on the plan (tc_sim_*) use it while
+ * the plan's block count equals stage->blocks and its slices are not
+ * shared; may be changed between tc_sim_advance calls. */
+int tc_pcg_set_stage(tc_pcg* plan, const tc_stage* stage);
 int tc_pcg_set_timing(tc_pcg* plan, int on);
 int tc_pcg_get_timing(tc_pcg* plan, double* ms_host, int64_t* count_host);
 

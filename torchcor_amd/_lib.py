@@ -12,6 +12,7 @@ import torch  # noqa: F401  (must precede loading the HIP library)
 LIB_PATH = Path(__file__).resolve().parent / "libtorchcor_amd.so"
 
 TC_F32, TC_F64 = 0, 1
+TC_ERR_ARG, TC_ERR_HIP, TC_ERR_SIZE, TC_ERR_STATE = -1, -2, -3, -4
 MAX_STIMULI = 32
 # rows of the largest system tc_pcg_solve_static takes (its static scratch)
 TC_PCG_STATIC_MAX_ROWS = 1 << 20
@@ -32,6 +33,11 @@ dbl = C.c_double
 class Sell(C.Structure):
     _fields_ = [("n", i64), ("n_slices", i64), ("slice_ptr", p), ("col", p), ("dcol", p),
                 ("pat", p), ("slice_tag", p)]
+
+
+class Stage(C.Structure):
+    _fields_ = [("blocks", C.c_int), ("max_cols", C.c_int), ("col_ptr", C.c_void_p),
+                ("own", C.c_void_p), ("cols", C.c_void_p), ("lidx", C.c_void_p)]
 
 
 class Stimulus(C.Structure):
@@ -176,6 +182,8 @@ FUNCS = {
     "tc_pcg_destroy": (p,),
     "tc_pcg_solve": (p, p, p, p, p, dbl, dbl, i32, i32, INT_P, p),
     "tc_pcg_set_blocks": (p, i32),
+    "tc_pcg_set_stage": (p, C.POINTER(Stage)),
+    "tc_sell_stage": (i64, i64, p, p, i32, p, p, p, p, C.POINTER(C.c_int32)),
     "tc_pcg_set_timing": (p, i32),
     "tc_pcg_get_timing": (p, C.POINTER(dbl), I64_P),
     "tc_pcg_solve_static": (SELL_P, p, p, i32, p, p, p, p, dbl, dbl, i32, i32, INT_P, p),

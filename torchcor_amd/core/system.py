@@ -26,6 +26,9 @@ class P1System:
     int16_columns = True
     #: slices whose rows share one stencil are laid out on it (with int16 offsets)
     share_slices = True
+    #: operators without shared slices get staged gathers in the time-step
+    #: runtime's spmv pass (tc_sell_stage) when every block's column set fits
+    stage_gathers = True
 
     def __init__(self, nodes, elems, device=None):
         device = torch.device(device) if device is not None else elems.device
@@ -151,6 +154,52 @@ class P1System:
                self.code, L.ptr(self.pcg_ws), nb.value)
         self._pcg = h
         self._tune_plan(h)
+        self.staged = self._stage_plan(h) if self.stage_gathers else False
+
+    def _stage_plan(self, plan):
+        """Staged gathers for the plan's spmv pass (operators without shared
+        slices): each block's distinct columns, ascending, and every SELL
+        entry's 16-bit position in its block's set, built on the host over
+        the plan's block split (tc_sell_stage).  When a block of the tuned
+        split references more than TC_STAGE_MAX_COLS columns, the split of
+        8 slices per block (more, smaller blocks) is tried and, if it fits,
+        becomes the plan's.  False when the operator has shared slices or
+        no split fits (the plan then gathers per entry)."""
+        import numpy as np
+        from .. import autotune
+        if self.slice_tag is not None:
+            return False
+        ns = (self.n + 63) // 64
+        sp = self.slice_ptr.cpu().numpy()
+        cl = self.scol.cpu().numpy()
+        col_ptr = own = None
+        for G in dict.fromkeys((int(self.pcg_blocks), autotune.pcg_blocks(self.n, 8))):
+            col_ptr = np.empty(G + 1, np.int32)
+            own = np.empty(G, np.int32)
+            cols = np.empty(self.padded, np.int32)
+            lidx = np.empty(self.padded, np.uint16)
+            mx = C.c_int32(0)
+            rc = L.lib.tc_sell_stage(self.n, ns, sp.ctypes.data, cl.ctypes.data, G,
+                                     col_ptr.ctypes.data, own.ctypes.data, cols.ctypes.data,
+                                     lidx.ctypes.data, C.byref(mx))
+            if rc != L.TC_ERR_SIZE:
+                break
+        if rc == L.TC_ERR_SIZE:
+            return False
+        L.check(rc, "tc_sell_stage")
+        if G != self.pcg_blocks:
+            L.call("tc_pcg_set_blocks", plan, G)
+            self.pcg_blocks = G
+        dev = self.device
+        self.st_col_ptr = torch.from_numpy(col_ptr).to(dev)
+        self.st_own = torch.from_numpy(own).to(dev)
+        self.st_cols = torch.from_numpy(cols[:int(col_ptr[G])].copy()).to(dev)
+        self.st_lidx = torch.from_numpy(lidx.view(np.int16)).to(dev)
+        st = L.Stage(G, int(mx.value), L.ptr(self.st_col_ptr), L.ptr(self.st_own),
+                     L.ptr(self.st_cols), L.ptr(self.st_lidx))
+        L.call("tc_pcg_set_stage", plan, C.byref(st))
+        self.stage_max_cols = int(mx.value)
+        return True
 
     def _tune_plan(self, plan):
         """Blocks of a PCG plan's passes from the tuning table (by rows)."""

@@ -287,7 +287,7 @@ k_bat_rhs_init(BatOp<T> op, const WT* __restrict__ w, const XT* __restrict__ x,
     double bi = am - op.ck * ak;
     if (sizeof(WT) != 8) bi = (double)(T)bi;
     const double d = (double)Dinv[e];
-    const T ri = (T)(bi - (op.Cm * mx + op.tdt * kx));
+    const T ri = (T)(bi - ax0(op.Cm, mx, op.tdt, kx));
     r[e] = ri;
     red[0] += (double)ri * (d * (double)ri);
     const double db = d * bi;
@@ -309,7 +309,7 @@ k_bat_pform(int64_t n, int lanes, int B, const T* __restrict__ Dinv, const T* __
     const int b = (int)(e & (lanes - 1));
     if (b >= B || !ctl->active[b]) continue;
     if (k == 0) pout[e] = (T)((double)Dinv[e] * (double)r[e]);
-    else pout[e] = (T)(ctl->beta[b] * (double)pin[e] + (double)Dinv[e] * (double)r[e]);
+    else pout[e] = (T)pdir(ctl->beta[b], (double)pin[e], (double)Dinv[e], (double)r[e]);
   }
 }
 

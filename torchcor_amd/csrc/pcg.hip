@@ -1,6 +1,8 @@
 // Stand-alone Jacobi-PCG solve and SpMV/RHS entry points (kernels in
 // pcg_kernels.h).
+#include <climits>
 #include <mutex>
+#include <vector>
 
 #include "pcg_kernels.h"
 
@@ -242,6 +244,74 @@ extern "C" int tc_pcg_set_blocks(tc_pcg* P, int blocks) {
   TC_CHECK(blocks >= 0 && blocks <= mx, TC_ERR_ARG,
            "tc_pcg_set_blocks: %d blocks outside 1..%d (0: default)", blocks, mx);
   P->G = blocks > 0 ? blocks : mx;
+  ++P->version;
+  return TC_OK;
+}
+
+extern "C" int tc_pcg_set_stage(tc_pcg* P, const tc_stage* st) {
+  TC_CHECK(P, TC_ERR_ARG, "tc_pcg_set_stage: null plan");
+  if (!st) {
+    P->stage = tc_stage{};
+    ++P->version;
+    return TC_OK;
+  }
+  TC_CHECK(st->col_ptr && st->own && st->cols && st->lidx && st->blocks >= 1 &&
+               st->max_cols >= 1 && st->max_cols <= TC_STAGE_MAX_COLS,
+           TC_ERR_ARG, "tc_pcg_set_stage: bad stage (max_cols 1..%d)", TC_STAGE_MAX_COLS);
+  P->stage = *st;
+  ++P->version;
+  return TC_OK;
+}
+
+// Host: the distinct columns of each block's rows (ascending), each row's
+// first-row position and every entry's position in its block's set.  A
+// block's columns lie in a window [lo, hi] of the row range; a marker array
+// over that window finds the set in one pass over the entries and one over
+// the window.
+extern "C" int tc_sell_stage(int64_t n, int64_t ns, const int32_t* sptr, const int32_t* col,
+                             int G, int32_t* col_ptr, int32_t* own, int32_t* cols,
+                             uint16_t* lidx, int32_t* max_cols) {
+  TC_CHECK(n > 0 && ns == (n + 63) / 64 && sptr && col && G >= 1 && col_ptr && own && cols &&
+               lidx && max_cols,
+           TC_ERR_ARG, "tc_sell_stage: bad arguments");
+  const int64_t per = (ns + G - 1) / G;
+  std::vector<int32_t> pos;
+  int64_t total = 0;
+  int32_t mx = 0;
+  for (int v = 0; v < G; ++v) {
+    col_ptr[v] = (int32_t)total;
+    const int64_t s0 = (int64_t)v * per, s1 = s0 + per < ns ? s0 + per : ns;
+    own[v] = 0;
+    if (s0 >= s1) continue;
+    const int64_t e0 = sptr[s0], e1 = sptr[s1];
+    int32_t lo = INT32_MAX, hi = -1;
+    for (int64_t e = e0; e < e1; ++e) {
+      TC_CHECK(col[e] >= 0 && col[e] < n, TC_ERR_ARG, "tc_sell_stage: column %d out of range",
+               col[e]);
+      lo = col[e] < lo ? col[e] : lo;
+      hi = col[e] > hi ? col[e] : hi;
+    }
+    pos.assign((size_t)(hi - lo + 1), -1);
+    for (int64_t e = e0; e < e1; ++e) pos[col[e] - lo] = 0;
+    int32_t cnt = 0;
+    for (int32_t c = lo; c <= hi; ++c)
+      if (pos[c - lo] == 0) {
+        pos[c - lo] = cnt++;
+        cols[total + cnt - 1] = c;
+      }
+    const int32_t r0 = (int32_t)(s0 * 64);
+    TC_CHECK(r0 >= lo && r0 <= hi && pos[r0 - lo] >= 0, TC_ERR_ARG,
+             "tc_sell_stage: block %d does not reference its first row", v);
+    own[v] = pos[r0 - lo];
+    if (cnt > mx) mx = cnt;
+    if (cnt <= 65536)
+      for (int64_t e = e0; e < e1; ++e) lidx[e] = (uint16_t)pos[col[e] - lo];
+    total += cnt;
+  }
+  col_ptr[G] = (int32_t)total;
+  *max_cols = mx;
+  TC_CHECK(mx <= TC_STAGE_MAX_COLS, TC_ERR_SIZE,
+           "tc_sell_stage: a block references %d columns (at most %d)", mx, TC_STAGE_MAX_COLS);
   return TC_OK;
 }
 

@@ -124,6 +124,8 @@ struct tc_pcg {
   std::vector<hipEvent_t> ev;
   double acc_ms[3];      // init, spmv, update
   int64_t acc_n[3];
+  tc_stage stage;        // staged gathers (stage.cols NULL: none)
+  int version;           // bumped when the blocks or the stage change
 };
 
 namespace tc {
@@ -282,6 +284,12 @@ __device__ inline Split block_slices(int64_t ns) {
   return sp;
 }
 
+// This block's position in the block order (the index of its slice range).
+__device__ inline int block_vindex() {
+  const int G = gridDim.x, b = blockIdx.x;
+  return G % kXcd == 0 ? (b % kXcd) * (G / kXcd) + b / kXcd : b;
+}
+
 // Block partial sums -> partials[k*G + blockIdx.x].
 template <int NV>
 __device__ inline void store_partials(double (&v)[NV], double* partials) {
@@ -335,6 +343,19 @@ __device__ inline void fold_partials(const double* __restrict__ partials, int G,
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < NV; ++k) out[k] = bc[k];
+}
+
+// Sums of two products, written with one explicit fused multiply-add so
+// that every pass, and every template instance of a pass, rounds them the
+// same way (left to contraction, the compiler fuses one product or the
+// other depending on register allocation):
+// p' = beta p + Dinv r
+__device__ inline double pdir(double beta, double p, double d, double r) {
+  return __builtin_fma(beta, p, d * r);
+}
+// Cm (M x0) + theta dt (K x0)
+__device__ inline double ax0(double Cm, double mx, double tdt, double kx) {
+  return __builtin_fma(Cm, mx, tdt * kx);
 }
 
 __device__ inline bool stop_rule(double zn, const PcgCtl* ctl) {
@@ -420,7 +441,7 @@ struct SpmvChunk {
     for (int u = 0; u < kChunk; ++u) {
       T pj;
       if (k == 0) pj = (T)((double)dj[u] * (double)rj[u]);
-      else pj = (T)(beta * (double)qj[u] + (double)dj[u] * (double)rj[u]);
+      else pj = (T)pdir(beta, (double)qj[u], (double)dj[u], (double)rj[u]);
       const double t = acc + (double)av[u] * (double)pj;
       acc = j0 + u < wd ? t : acc;
     }
@@ -516,7 +537,122 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
     if (i < n) {
       T pi;
       if (k == 0) pi = (T)((double)Dinv[i] * (double)r[i]);
-      else pi = (T)(beta * (double)pin[i] + (double)Dinv[i] * (double)r[i]);
+      else pi = (T)pdir(beta, (double)pin[i], (double)Dinv[i], (double)r[i]);
+      const T api = (T)acc;
+      pout[i] = pi;
+      Ap[i] = api;
+      red[0] += (double)pi * (double)api;
+    }
+  }
+  store_partials<1>(red, part1);
+}
+
+// spmv pass of iteration k with staged gathers (a tc_stage; operators
+// without shared slices).  The block forms p' = beta p + Dinv r once for
+// every distinct column its rows reference (its set, ascending, at most
+// kStageR * kPcgBlock) into LDS, then walks its rows as k_pcg_spmv does,
+// reading each entry's p' from LDS at the entry's 16-bit position: the same
+// products in the same order, so the same bits, with one load of Dinv, r
+// and p per distinct column instead of one per entry.  The set's columns
+// and the flags load first, then the partial-sum fold and the set's Dinv,
+// r, p gathers are in flight together.
+constexpr int kStageR = (TC_STAGE_MAX_COLS + kPcgBlock - 1) / kPcgBlock;
+
+template <class T>
+__global__ void __launch_bounds__(kPcgBlock)
+k_pcg_spmv_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* __restrict__ lidx,
+                  const int32_t* __restrict__ st_ptr, const int32_t* __restrict__ st_own,
+                  const int32_t* __restrict__ st_col, const T* __restrict__ Av,
+                  const T* __restrict__ Dinv, const T* __restrict__ r,
+                  const T* __restrict__ pin, T* __restrict__ pout, T* __restrict__ Ap, int k,
+                  PcgCtl* ctl, const double* __restrict__ part0, const double* __restrict__ part2,
+                  double* part1) {
+  extern __shared__ __align__(16) unsigned char stage_lds[];
+  T* ps = reinterpret_cast<T*>(stage_lds);
+  const int64_t ns = (n + 63) >> 6;
+  const Split sp = block_slices(ns);
+  const int vb = block_vindex();
+  const int lane = threadIdx.x & 63, wv = wave_id();
+  const int32_t u0 = st_ptr[vb], nu = st_ptr[vb + 1] - u0;
+  const int halted_early = ctl->converged | ctl->stall;
+  T dv[kStageR], rv[kStageR], qv[kStageR];
+  if (!halted_early && nu > 0) {
+    int32_t cv[kStageR];
+#pragma unroll
+    for (int q = 0; q < kStageR; ++q) {
+      const int t = (int)threadIdx.x + q * kPcgBlock;
+      cv[q] = st_col[u0 + (t < nu ? t : nu - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < kStageR; ++q) {
+      dv[q] = Dinv[cv[q]];
+      rv[q] = r[cv[q]];
+      qv[q] = k == 0 ? T(0) : pin[cv[q]];
+    }
+  }
+  __shared__ int halted;
+  if (threadIdx.x == 0) halted = ctl->converged | ctl->stall;
+  const int G = gridDim.x;
+  const double rz_prev = ctl->rz[(k - 1) & 1];
+  double v[2];
+  fold_partials<2>(k == 0 ? part0 : part2, G, v);
+  if (halted) return;
+  double beta = 0.0, rz;
+  if (k == 0) {
+    rz = v[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->bnorm = sqrt(v[1]);
+  } else {
+    if (stop_rule(sqrt(v[0]), ctl)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->converged = 1;
+        ctl->n_iter = k - 1;
+        ctl->iter = k;
+      }
+      return;
+    }
+    rz = v[1];
+    beta = rz / rz_prev;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->rz[k & 1] = rz;
+    ctl->n_iter = k;
+    ctl->iter = k;
+  }
+#pragma unroll
+  for (int q = 0; q < kStageR; ++q) {
+    const int t = (int)threadIdx.x + q * kPcgBlock;
+    if (t < nu) {
+      if (k == 0) ps[t] = (T)((double)dv[q] * (double)rv[q]);
+      else ps[t] = (T)pdir(beta, (double)qv[q], (double)dv[q], (double)rv[q]);
+    }
+  }
+  __syncthreads();
+  const int64_t r0 = sp.s0 * 64;
+  const int32_t own = st_own[vb];
+  double red[1] = {0.0};
+  for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
+    const int64_t i = s * 64 + lane;
+    const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
+    const uint16_t* li = lidx + b + lane;
+    const T* a = Av + b + lane;
+    double acc = 0.0;
+    for (int j0 = 0; j0 < wd; j0 += kChunk) {
+      uint16_t lj[kChunk];
+      T aj[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const int jj = j0 + u < wd ? j0 + u : wd - 1;
+        lj[u] = li[jj * 64];
+        aj[u] = a[jj * 64];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const double t = acc + (double)aj[u] * (double)ps[lj[u]];
+        acc = j0 + u < wd ? t : acc;
+      }
+    }
+    if (i < n) {
+      const T pi = ps[own + (int32_t)(i - r0)];
       const T api = (T)acc;
       pout[i] = pi;
       Ap[i] = api;
@@ -787,7 +923,134 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
       double bi = am - coef * ak;
       if (sizeof(WT) != 8) bi = (double)(T)bi;
       const double d = (double)Dinv[i];
-      const T ri = (T)(bi - (Cm * mx + tdt * kx));
+      const T ri = (T)(bi - ax0(Cm, mx, tdt, kx));
+      r[i] = ri;
+      red[0] += (double)ri * (d * (double)ri);
+      const double db = d * bi;
+      red[1] += db * db;
+    }
+  }
+  store_partials<2>(red, part0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->converged = 0;
+    ctl->n_iter = 0;
+    ctl->iter = 0;
+    ctl->a_tol = a_tol;
+    ctl->r_tol = r_tol;
+  }
+}
+
+// Bytes of LDS the staged RHS + PCG start pass needs for a set of nu
+// columns: x (XT), w (WT) and, with the linear guess, x_prev (T) per column.
+template <class T, class XT, class WT, bool LG>
+__host__ __device__ inline size_t rhs_stage_bytes(int nu) {
+  const size_t a = ((size_t)nu * sizeof(XT) + 15) & ~(size_t)15;
+  const size_t b = ((size_t)nu * sizeof(WT) + 15) & ~(size_t)15;
+  return a + b + (LG ? (size_t)nu * sizeof(T) : 0);
+}
+
+// Fused RHS + PCG start with staged gathers (a tc_stage; operators without
+// shared slices): the block first loads x, w and x_prev once for every
+// distinct column its rows reference into LDS, then walks its rows as
+// k_rhs_init does, reading each entry's neighbour values from LDS at the
+// entry's 16-bit position; the same terms in the same order, so the same
+// bits.  The set's values load while the block reads the stall flag.
+template <class T, class XT, class WT, bool LG>
+__global__ void __launch_bounds__(kPcgBlock)
+k_rhs_init_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* __restrict__ lidx,
+                  const int32_t* __restrict__ st_ptr, const int32_t* __restrict__ st_own,
+                  const int32_t* __restrict__ st_col, const T* __restrict__ Mv,
+                  const T* __restrict__ Kv, const T* __restrict__ Dinv, const WT* __restrict__ w,
+                  const XT* __restrict__ x, const T* __restrict__ xp, XT* __restrict__ xnext,
+                  T* __restrict__ xpnext, T* __restrict__ r, double coef, double Cm,
+                  double tdt, PcgCtl* ctl, double a_tol, double r_tol, double* part0) {
+  extern __shared__ __align__(16) unsigned char stage_lds[];
+  const int64_t ns = (n + 63) >> 6;
+  const Split sp = block_slices(ns);
+  const int vb = block_vindex();
+  const int lane = threadIdx.x & 63, wv = wave_id();
+  const int32_t u0 = st_ptr[vb], nu = st_ptr[vb + 1] - u0;
+  XT* xs = reinterpret_cast<XT*>(stage_lds);
+  WT* ws = reinterpret_cast<WT*>(stage_lds + (((size_t)nu * sizeof(XT) + 15) & ~(size_t)15));
+  T* ps = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(ws) +
+                               (((size_t)nu * sizeof(WT) + 15) & ~(size_t)15));
+  __shared__ int st;
+  if (threadIdx.x == 0) st = ctl->stall;
+  if (nu > 0) {
+    int32_t cv[kStageR];
+#pragma unroll
+    for (int q = 0; q < kStageR; ++q) {
+      const int t = (int)threadIdx.x + q * kPcgBlock;
+      cv[q] = st_col[u0 + (t < nu ? t : nu - 1)];
+    }
+    XT xv[kStageR];
+    WT wv_[kStageR];
+    T pv[kStageR];
+#pragma unroll
+    for (int q = 0; q < kStageR; ++q) {
+      xv[q] = x[cv[q]];
+      wv_[q] = w[cv[q]];
+      if (LG) pv[q] = xp[cv[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < kStageR; ++q) {
+      const int t = (int)threadIdx.x + q * kPcgBlock;
+      if (t < nu) {
+        xs[t] = xv[q];
+        ws[t] = wv_[q];
+        if (LG) ps[t] = pv[q];
+      }
+    }
+  }
+  __syncthreads();
+  if (st) return;
+  const int64_t r0 = sp.s0 * 64;
+  const int32_t own = st_own[vb];
+  double red[2] = {0.0, 0.0};
+  for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
+    const int64_t i = s * 64 + lane;
+    const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
+    const uint16_t* li = lidx + b + lane;
+    const T* m = Mv + b + lane;
+    const T* kk = Kv + b + lane;
+    double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
+    for (int j0 = 0; j0 < wd; j0 += kChunk) {
+      uint16_t lj[kChunk];
+      T mv[kChunk], kv[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        const int jj = j0 + u < wd ? j0 + u : wd - 1;
+        lj[u] = li[jj * 64];
+        mv[u] = m[jj * 64];
+        kv[u] = kk[jj * 64];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; ++u) {
+        if (j0 + u >= wd) break;
+        const int q = lj[u];
+        const XT xq = xs[q];
+        double x0;
+        if (LG) x0 = (double)(XT)(2.0 * (double)xq - (double)ps[q]);
+        else x0 = (double)xq;
+        const double mj = (double)mv[u], kj = (double)kv[u];
+        am += mj * (double)ws[q];
+        if (sizeof(WT) == 8) ak += kj * (double)xq;
+        else ak += kj * (double)(T)xq;
+        mx += mj * x0;
+        kx += kj * x0;
+      }
+    }
+    if (i < n) {
+      const int q = own + (int32_t)(i - r0);
+      const XT xi = xs[q];
+      if (LG) {
+        xnext[i] = (XT)(2.0 * (double)xi - (double)ps[q]);
+        xpnext[i] = (T)xi;
+      }
+      double bi = am - coef * ak;
+      if (sizeof(WT) != 8) bi = (double)(T)bi;
+      const double d = (double)Dinv[i];
+      const T ri = (T)(bi - ax0(Cm, mx, tdt, kx));
       r[i] = ri;
       red[0] += (double)ri * (d * (double)ri);
       const double db = d * bi;

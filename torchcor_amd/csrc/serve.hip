@@ -281,7 +281,7 @@ __device__ inline void sv_rhs_rows(const SvBlk& bk, const SvEnt* __restrict__ en
       double bi = am - coef * ak;
       if (sizeof(WT) != 8) bi = (double)(T)bi;
       const double d = (double)Dinv[i];
-      const T ri = (T)(bi - (Cm * mx + tdt * kx));
+      const T ri = (T)(bi - ax0(Cm, mx, tdt, kx));
       r[i] = ri;
       red0 += (double)ri * (d * (double)ri);
       const double db = d * bi;
@@ -410,7 +410,7 @@ __device__ inline void sv_spmv_block(const SvBlk& bk, int vb, const SvEnt* __res
     if (i < bk.row_end) {
       T pi;
       if (k == 0) pi = (T)((double)Dinv[i] * (double)r[i]);
-      else pi = (T)(beta * (double)pin[i] + (double)Dinv[i] * (double)r[i]);
+      else pi = (T)pdir(beta, (double)pin[i], (double)Dinv[i], (double)r[i]);
       const T api = (T)acc;
       pout[i] = pi;
       Ap[i] = api;

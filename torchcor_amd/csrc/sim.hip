@@ -154,12 +154,31 @@ void launch_spmv(tc_pcg* P, const CI* sc, T* pin, T* pout, int k, hipStream_t s,
          (const double*)P->part[0], (const double*)P->part[2], P->part[1]);
 }
 
-// The spmv pass of iteration k.
+// The plan's stage applies: built for its block count, on an operator
+// without shared slices.
+inline bool staged(const tc_pcg* P) {
+  return P->stage.cols && P->stage.blocks == P->G && !P->A.slice_tag;
+}
+
+// The spmv pass of iteration k: staged gathers when the plan has a stage,
+// else int16 offsets or int32 columns.
 template <class T>
 int launch_spmv_at(tc_pcg* P, int k, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   T* pp[2] = {(T*)P->p[0], (T*)P->p[1]};
-  if (P->A.dcol) launch_spmv<T>(P, P->A.dcol, pp[k & 1], pp[(k + 1) & 1], k, s, e0, e1);
-  else launch_spmv<T>(P, P->A.col, pp[k & 1], pp[(k + 1) & 1], k, s, e0, e1);
+  T* pin = pp[k & 1];
+  T* pout = pp[(k + 1) & 1];
+  if (staged(P)) {
+    const tc_stage& st = P->stage;
+    launch_lds(k_pcg_spmv_staged<T>, dim3(P->G), dim3(kPcgBlock),
+               (size_t)st.max_cols * sizeof(T), s, e0, e1, P->n, P->A.slice_ptr, st.lidx,
+               st.col_ptr, st.own, st.cols, (const T*)P->Av, (const T*)P->Dinv, (const T*)P->r,
+               (const T*)pin, pout, (T*)P->Ap, k, P->ctl, (const double*)P->part[0],
+               (const double*)P->part[2], P->part[1]);
+  } else if (P->A.dcol) {
+    launch_spmv<T>(P, P->A.dcol, pin, pout, k, s, e0, e1);
+  } else {
+    launch_spmv<T>(P, P->A.col, pin, pout, k, s, e0, e1);
+  }
   TC_LAUNCH();
   return TC_OK;
 }
@@ -189,13 +208,11 @@ int launch_iters(tc_sim* S, int k0, int k1, XT* xs, hipStream_t s, std::vector<h
   const dim3 g(P->G), blk(kPcgBlock);
   T* pp[2] = {(T*)P->p[0], (T*)P->p[1]};
   for (int k = k0; k < k1; ++k) {
-    T* pin = pp[k & 1];
     T* pout = pp[(k + 1) & 1];
     const size_t e = ev_base + 4 * (size_t)(k - k0);
     hipEvent_t ev0 = ev ? (*ev)[e] : nullptr, ev1 = ev ? (*ev)[e + 1] : nullptr;
-    if (P->A.dcol) launch_spmv<T>(P, P->A.dcol, pin, pout, k, s, ev0, ev1);
-    else launch_spmv<T>(P, P->A.col, pin, pout, k, s, ev0, ev1);
-    TC_LAUNCH();
+    int rc = launch_spmv_at<T>(P, k, s, ev0, ev1);
+    if (rc) return rc;
     if (check_slot >= 0 && k == k1 - 1) {
       launch(k_pcg_update<T, XT, true>, g, blk, s, ev ? (*ev)[4] : nullptr,
              ev ? (*ev)[5] : nullptr, P->n, (const T*)P->Dinv, (const T*)pout, (const T*)P->Ap,
@@ -289,11 +306,34 @@ int desired_K(tc_sim* S) {
   return K;
 }
 
+// Staged gathers in the RHS + PCG start pass: the plan's stage applies and
+// the set's x, w and x_prev fit the LDS of one block.
+constexpr size_t kStageLdsMax = 65536;
+
+template <class T, class XT, class WT, bool LG>
+void rhs_init_staged_launch(tc_sim* S, int xc, int pc, double coef, double tdt, hipStream_t s,
+                            hipEvent_t e0, hipEvent_t e1) {
+  const tc_sim_desc& d = S->d;
+  tc_pcg* P = d.pcg;
+  const tc_stage& st = P->stage;
+  launch_lds(k_rhs_init_staged<T, XT, WT, LG>, dim3(P->G), dim3(kPcgBlock),
+             rhs_stage_bytes<T, XT, WT, LG>(st.max_cols), s, e0, e1, d.n, P->A.slice_ptr,
+             st.lidx, st.col_ptr, st.own, st.cols, (const T*)d.Mv, (const T*)d.Kv,
+             (const T*)P->Dinv, (const WT*)d.w, (const XT*)d.x[xc],
+             LG ? (const T*)d.xp[pc] : (const T*)nullptr, LG ? (XT*)d.x[1 - xc] : (XT*)nullptr,
+             LG ? (T*)d.xp[1 - pc] : (T*)nullptr, (T*)P->r, coef, d.Cm, tdt, P->ctl, d.a_tol,
+             d.r_tol, P->part[0]);
+}
+
 template <class T, class XT, class WT, bool LG, class CI>
 void rhs_init_launch(tc_sim* S, const CI* sc, int xc, int pc, double coef, double tdt, dim3 g,
                      dim3 blk, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const tc_sim_desc& d = S->d;
   tc_pcg* P = d.pcg;
+  if (staged(P) && rhs_stage_bytes<T, XT, WT, LG>(P->stage.max_cols) <= kStageLdsMax) {
+    rhs_init_staged_launch<T, XT, WT, LG>(S, xc, pc, coef, tdt, s, e0, e1);
+    return;
+  }
   launch(k_rhs_init<T, XT, WT, LG, CI>, g, blk, s, e0, e1, d.n, hot_slices(P->A), sc, P->A.pat, (const T*)d.Mv, (const T*)d.Kv, (const T*)P->Dinv, (const WT*)d.w,
          (const XT*)d.x[xc], LG ? (const T*)d.xp[pc] : (const T*)nullptr,
          LG ? (XT*)d.x[1 - xc] : (XT*)nullptr, LG ? (T*)d.xp[1 - pc] : (T*)nullptr, (T*)P->r,
@@ -378,8 +418,8 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
     // update when it runs the test, which takes this step's record) replayed
     // as one graph
     const int kg = plan.fused ? K - 1 : K;
-    const long long key =
-        (long long)((xc << 1) | pc) | ((long long)K << 2) | ((long long)P->G << 20);
+    const long long key = (long long)((xc << 1) | pc) | ((long long)K << 2) |
+                          ((long long)P->G << 20) | ((long long)P->version << 32);
     rc = graph_replay(S->graphs, key, s, [&](hipStream_t cs) {
       int r = rhs_init_seq<T, XT, WT>(S, xc, pc, cs, nullptr, nullptr);
       if (r == TC_OK) r = launch_iters<T, XT>(S, 0, kg, xs, cs, nullptr, 0);

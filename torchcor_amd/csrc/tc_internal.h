@@ -110,6 +110,14 @@ inline void launch(F kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e0
   else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
 }
 
+// Kernel launch with dynamic LDS bytes (as launch).
+template <class F, class... Args>
+inline void launch_lds(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, hipEvent_t e0,
+                       hipEvent_t e1, Args... args) {
+  if (e0 || e1) hipExtLaunchKernelGGL(kernel, grid, block, lds, s, e0, e1, 0, args...);
+  else hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+}
+
 // Instantiated HIP graphs of an engine's fixed launch sequences, keyed by
 // what a sequence depends on (buffer parity, iteration count).  A sequence
 // is recorded once by stream capture on a private stream and then replayed
