@@ -239,7 +239,8 @@ def irregular_config(dev, steps, warmup, n_target=640_000, orders=(None, "rcm", 
     five stimulus patches (0 and 5 ms onsets), TTP ENDO float32.  Runs the
     same steps in the mesh's own (scattered) order, in reverse Cuthill-McKee
     order and in the simulator's default locality order (Cuthill-McKee levels
-    along a Z-order curve)."""
+    along a Z-order curve); the locality orders run with staged gathers
+    (per-block column sets in LDS) when their sets fit."""
     import torch
     from torchcor_amd import meshes
     from torchcor_amd.ionic import TenTusscherPanfilov
@@ -282,6 +283,10 @@ def irregular_config(dev, steps, warmup, n_target=640_000, orders=(None, "rcm", 
                     "bandwidth": sim.system.bandwidth,
                     "int16_columns": sim.system.dcol is not None,
                     "sell_padding": round(sim.system.padded / sim.system.nnz - 1, 4),
+                    "pcg_blocks": int(sim.system.pcg_blocks),
+                    "staged_gathers": bool(sim.system.staged),
+                    "stage_max_cols": int(getattr(sim.system, "stage_max_cols", 0))
+                    if sim.system.staged else None,
                     "kernel_avg_us": kt,
                     "pcg_iterations_per_step": sum(sim.iterations[warmup:]) / steps + 1}
         sim.finish()
