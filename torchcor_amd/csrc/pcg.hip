@@ -1,5 +1,6 @@
 // Stand-alone Jacobi-PCG solve and SpMV/RHS entry points (kernels in
 // pcg_kernels.h).
+#include <algorithm>
 #include <climits>
 #include <mutex>
 #include <vector>
@@ -291,6 +292,19 @@ extern "C" int tc_sell_stage(int64_t n, int64_t ns, const int32_t* sptr, const i
       lo = col[e] < lo ? col[e] : lo;
       hi = col[e] > hi ? col[e] : hi;
     }
+    if (e1 - e0 > TC_STAGE_MAX_COLS && hi - lo + 1 > 16 * (int64_t)TC_STAGE_MAX_COLS) {
+      // a window this wide (a scattered node order) is all but certain to
+      // exceed the limit: count exactly only when it could fit
+      std::vector<int32_t> c(col + e0, col + e1);
+      std::sort(c.begin(), c.end());
+      const int64_t distinct = std::unique(c.begin(), c.end()) - c.begin();
+      if (distinct > TC_STAGE_MAX_COLS) {
+        *max_cols = (int32_t)distinct;
+        set_error("tc_sell_stage: a block references %lld columns (at most %d)",
+                  (long long)distinct, TC_STAGE_MAX_COLS);
+        return TC_ERR_SIZE;
+      }
+    }
     pos.assign((size_t)(hi - lo + 1), -1);
     for (int64_t e = e0; e < e1; ++e) pos[col[e] - lo] = 0;
     int32_t cnt = 0;
@@ -304,6 +318,7 @@ extern "C" int tc_sell_stage(int64_t n, int64_t ns, const int32_t* sptr, const i
              "tc_sell_stage: block %d does not reference its first row", v);
     own[v] = pos[r0 - lo];
     if (cnt > mx) mx = cnt;
+    if (cnt > TC_STAGE_MAX_COLS) break;  // refused below
     if (cnt <= 65536)
       for (int64_t e = e0; e < e1; ++e) lidx[e] = (uint16_t)pos[col[e] - lo];
     total += cnt;
