@@ -925,9 +925,9 @@ k_rhs_init(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
       const double d = (double)Dinv[i];
       const T ri = (T)(bi - ax0(Cm, mx, tdt, kx));
       r[i] = ri;
-      red[0] += (double)ri * (d * (double)ri);
+      red[0] = __builtin_fma((double)ri, d * (double)ri, red[0]);
       const double db = d * bi;
-      red[1] += db * db;
+      red[1] = __builtin_fma(db, db, red[1]);
     }
   }
   store_partials<2>(red, part0);
@@ -955,8 +955,15 @@ __host__ __device__ inline size_t rhs_stage_bytes(int nu) {
 // k_rhs_init does, reading each entry's neighbour values from LDS at the
 // entry's 16-bit position; the same terms in the same order, so the same
 // bits.  The set's values load while the block reads the stall flag.
-template <class T, class XT, class WT, bool LG>
-__global__ void __launch_bounds__(kPcgBlock)
+//
+// NT threads per block: kPcgBlock (8 waves, wave w taking slices s0 + w,
+// s0 + w + 8, ...) or 2 kPcgBlock (16 waves, one slice each, for blocks of
+// at most 16 slices: the set's LDS allows few blocks per CU, and one slice
+// per wave halves each block's chain of dependent loads).  The 16-wave form
+// adds wave w + 8's per-lane sums to wave w's and folds the first 8 waves,
+// which is the 8-wave form's order of additions: the same partial sums.
+template <class T, class XT, class WT, bool LG, int NT>
+__global__ void __launch_bounds__(NT)
 k_rhs_init_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* __restrict__ lidx,
                   const int32_t* __restrict__ st_ptr, const int32_t* __restrict__ st_own,
                   const int32_t* __restrict__ st_col, const T* __restrict__ Mv,
@@ -976,25 +983,28 @@ k_rhs_init_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* _
                                (((size_t)nu * sizeof(WT) + 15) & ~(size_t)15));
   __shared__ int st;
   if (threadIdx.x == 0) st = ctl->stall;
+  static_assert(NT == kPcgBlock || NT == 2 * kPcgBlock, "8 or 16 waves");
+  constexpr int R = (TC_STAGE_MAX_COLS + NT - 1) / NT;
+  constexpr int NW = NT / 64;
   if (nu > 0) {
-    int32_t cv[kStageR];
+    int32_t cv[R];
 #pragma unroll
-    for (int q = 0; q < kStageR; ++q) {
-      const int t = (int)threadIdx.x + q * kPcgBlock;
+    for (int q = 0; q < R; ++q) {
+      const int t = (int)threadIdx.x + q * NT;
       cv[q] = st_col[u0 + (t < nu ? t : nu - 1)];
     }
-    XT xv[kStageR];
-    WT wv_[kStageR];
-    T pv[kStageR];
+    XT xv[R];
+    WT wv_[R];
+    T pv[R];
 #pragma unroll
-    for (int q = 0; q < kStageR; ++q) {
+    for (int q = 0; q < R; ++q) {
       xv[q] = x[cv[q]];
       wv_[q] = w[cv[q]];
       if (LG) pv[q] = xp[cv[q]];
     }
 #pragma unroll
-    for (int q = 0; q < kStageR; ++q) {
-      const int t = (int)threadIdx.x + q * kPcgBlock;
+    for (int q = 0; q < R; ++q) {
+      const int t = (int)threadIdx.x + q * NT;
       if (t < nu) {
         xs[t] = xv[q];
         ws[t] = wv_[q];
@@ -1007,7 +1017,9 @@ k_rhs_init_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* _
   const int64_t r0 = sp.s0 * 64;
   const int32_t own = st_own[vb];
   double red[2] = {0.0, 0.0};
-  for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
+  // the factors of this lane's last row's terms (0 when it had none)
+  double f_ri = 0.0, f_dri = 0.0, f_db = 0.0;
+  for (int64_t s = sp.s0 + wv; s < sp.s1; s += NW) {
     const int64_t i = s * 64 + lane;
     const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
     const uint16_t* li = lidx + b + lane;
@@ -1052,12 +1064,52 @@ k_rhs_init_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* _
       const double d = (double)Dinv[i];
       const T ri = (T)(bi - ax0(Cm, mx, tdt, kx));
       r[i] = ri;
-      red[0] += (double)ri * (d * (double)ri);
-      const double db = d * bi;
-      red[1] += db * db;
+      f_ri = (double)ri;
+      f_dri = d * (double)ri;
+      f_db = d * bi;
+      red[0] = __builtin_fma(f_ri, f_dri, red[0]);
+      red[1] = __builtin_fma(f_db, f_db, red[1]);
     }
   }
-  store_partials<2>(red, part0);
+  if constexpr (NT == kPcgBlock) {
+    store_partials<2>(red, part0);
+  } else {
+    // waves 8..15 (one row per lane at most) hand that row's factors to
+    // waves 0..7 (through the LDS of the set, free once every wave has
+    // read it), which fuse them into their sums as the 8-wave block's
+    // second slice does, and fold as the 8-wave block does
+    double* comb = reinterpret_cast<double*>(stage_lds);
+    __syncthreads();
+    if (wv >= kWaves) {
+      double* c = comb + ((wv - kWaves) * 64 + lane) * 3;
+      c[0] = f_ri;
+      c[1] = f_dri;
+      c[2] = f_db;
+    }
+    __syncthreads();
+    __shared__ double wsum[2][kWaves];
+    if (wv < kWaves) {
+      const double* c = comb + (wv * 64 + lane) * 3;
+      red[0] = __builtin_fma(c[0], c[1], red[0]);
+      red[1] = __builtin_fma(c[2], c[2], red[1]);
+      red[0] = wave_sum(red[0]);
+      red[1] = wave_sum(red[1]);
+      if (lane == 0) {
+        wsum[0][wv] = red[0];
+        wsum[1][wv] = red[1];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s0 = 0.0, s1 = 0.0;
+      for (int q = 0; q < kWaves; ++q) {
+        s0 += wsum[0][q];
+        s1 += wsum[1][q];
+      }
+      part0[blockIdx.x] = s0;
+      part0[gridDim.x + blockIdx.x] = s1;
+    }
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ctl->converged = 0;
     ctl->n_iter = 0;

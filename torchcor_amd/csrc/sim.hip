@@ -316,10 +316,19 @@ void rhs_init_staged_launch(tc_sim* S, int xc, int pc, double coef, double tdt, 
   const tc_sim_desc& d = S->d;
   tc_pcg* P = d.pcg;
   const tc_stage& st = P->stage;
-  launch_lds(k_rhs_init_staged<T, XT, WT, LG>, dim3(P->G), dim3(kPcgBlock),
-             rhs_stage_bytes<T, XT, WT, LG>(st.max_cols), s, e0, e1, d.n, P->A.slice_ptr,
-             st.lidx, st.col_ptr, st.own, st.cols, (const T*)d.Mv, (const T*)d.Kv,
-             (const T*)P->Dinv, (const WT*)d.w, (const XT*)d.x[xc],
+  // blocks of at most 16 slices run as 16 waves, one slice each (handing
+  // their rows' factors over takes 3 doubles per lane of 8 waves, 12 KB,
+  // of the set's LDS)
+  const int64_t ns = (d.n + 63) / 64;
+  const bool wide = (ns + P->G - 1) / P->G <= 2 * kWaves;
+  constexpr size_t kHandOver = 3 * kWaves * 64 * sizeof(double);
+  size_t lds = rhs_stage_bytes<T, XT, WT, LG>(st.max_cols);
+  if (wide && lds < kHandOver) lds = kHandOver;
+  auto kern = wide ? k_rhs_init_staged<T, XT, WT, LG, 2 * kPcgBlock>
+                   : k_rhs_init_staged<T, XT, WT, LG, kPcgBlock>;
+  launch_lds(kern, dim3(P->G), dim3(wide ? 2 * kPcgBlock : kPcgBlock), lds, s, e0, e1, d.n,
+             P->A.slice_ptr, st.lidx, st.col_ptr, st.own, st.cols, (const T*)d.Mv,
+             (const T*)d.Kv, (const T*)P->Dinv, (const WT*)d.w, (const XT*)d.x[xc],
              LG ? (const T*)d.xp[pc] : (const T*)nullptr, LG ? (XT*)d.x[1 - xc] : (XT*)nullptr,
              LG ? (T*)d.xp[1 - pc] : (T*)nullptr, (T*)P->r, coef, d.Cm, tdt, P->ctl, d.a_tol,
              d.r_tol, P->part[0]);
