@@ -785,7 +785,7 @@ int bat_enqueue(tc_batch* S, BHost& ps, BPlan& plan, hipStream_t s) {
   // fixed by the buffer parity and K, replayed as a graph in graph mode
   int rc;
   if (d.graph)
-    rc = graph_replay(S->graphs, (long long)((xc << 1) | pc) | ((long long)K << 2), s,
+    rc = graph_replay(S->graphs, (long long)((xc << 1) | pc) | ((long long)K << 2), 0, s,
                       [&](hipStream_t cs) { return bat_pcg_seq<T, XT, WT, NK>(S, xc, pc, K, cs); });
   else
     rc = bat_pcg_seq<T, XT, WT, NK>(S, xc, pc, K, s);

@@ -1209,7 +1209,7 @@ int sv_enqueue(tc_serve* S, SPlan& plan, hipStream_t s) {
   // graph replay in graph mode
   int rc;
   if (d.graph)
-    rc = graph_replay(S->graphs, (long long)((xc << 1) | pc) | ((long long)K << 2), s,
+    rc = graph_replay(S->graphs, (long long)((xc << 1) | pc) | ((long long)K << 2), 0, s,
                       [&](hipStream_t cs) { return sv_step_seq<T, XT, WT>(S, K, cs); });
   else
     rc = sv_step_seq<T, XT, WT>(S, K, s);

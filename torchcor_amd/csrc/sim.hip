@@ -427,9 +427,11 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
     // update when it runs the test, which takes this step's record) replayed
     // as one graph
     const int kg = plan.fused ? K - 1 : K;
-    const long long key = (long long)((xc << 1) | pc) | ((long long)K << 2) |
-                          ((long long)P->G << 20) | ((long long)P->version << 32);
-    rc = graph_replay(S->graphs, key, s, [&](hipStream_t cs) {
+    // K < 2^30 (the status word's iteration field); the plan's block count
+    // and version (bumped by tc_pcg_set_blocks / tc_pcg_set_stage) in key2
+    const long long key = (long long)((xc << 1) | pc) | ((long long)K << 2);
+    const long long key2 = (long long)P->G | ((long long)P->version << 16);
+    rc = graph_replay(S->graphs, key, key2, s, [&](hipStream_t cs) {
       int r = rhs_init_seq<T, XT, WT>(S, xc, pc, cs, nullptr, nullptr);
       if (r == TC_OK) r = launch_iters<T, XT>(S, 0, kg, xs, cs, nullptr, 0);
       if (r == TC_OK && plan.fused) r = launch_spmv_at<T>(P, K - 1, cs, nullptr, nullptr);

@@ -127,7 +127,7 @@ inline void launch_lds(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t 
 struct GraphCache {
   static constexpr size_t kMax = 64;
   struct Ent {
-    long long key;
+    long long key, key2;
     hipGraphExec_t exec;
   };
   std::vector<Ent> ents;
@@ -144,13 +144,13 @@ struct GraphCache {
   }
 };
 
-// Replays the graph of `key`, recording it first with rec(stream) (which
-// returns a TC_* code) when the cache has none.
+// Replays the graph of (key, key2), recording it first with rec(stream)
+// (which returns a TC_* code) when the cache has none.
 template <class F>
-int graph_replay(GraphCache& c, long long key, hipStream_t s, F&& rec) {
+int graph_replay(GraphCache& c, long long key, long long key2, hipStream_t s, F&& rec) {
   hipGraphExec_t exec = nullptr;
   for (const GraphCache::Ent& e : c.ents)
-    if (e.key == key) exec = e.exec;
+    if (e.key == key && e.key2 == key2) exec = e.exec;
   if (!exec) {
     if (c.ents.size() >= GraphCache::kMax) c.clear();
     if (!c.cap) TC_HIP(hipStreamCreateWithFlags(&c.cap, hipStreamNonBlocking));
@@ -170,7 +170,7 @@ int graph_replay(GraphCache& c, long long key, hipStream_t s, F&& rec) {
       set_error("graph instantiate -> %s", hipGetErrorString(ei));
       return TC_ERR_HIP;
     }
-    c.ents.push_back({key, exec});
+    c.ents.push_back({key, key2, exec});
     ++c.captures;
   }
   TC_HIP(hipGraphLaunch(exec, s));
