@@ -407,7 +407,8 @@ def copy_ceiling(nbytes, reps=10):
 
 
 STEP_KERNELS = ("k_ttp", "k_crn", "k_ms2v", "k_rhs_init", "k_pcg_spmv", "k_pcg_update",
-                "k_pcg_check", "k_set_int", "k_copy_gated")
+                "k_pcg_check", "k_set_int", "k_copy_gated", "k_rhs_init_staged",
+                "k_pcg_spmv_staged")
 
 
 def _kernel_label(name):
@@ -416,7 +417,7 @@ def _kernel_label(name):
     labelled k_pcg_update+check."""
     base = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
     head = base.split("<")[0]
-    for k in STEP_KERNELS:
+    for k in sorted(STEP_KERNELS, key=len, reverse=True):
         if head.endswith(k):
             if k == "k_pcg_update" and base[base.find("<") + 1:base.rfind(">")].split(",")[-1].strip() == "true":
                 return "k_pcg_update+check"
