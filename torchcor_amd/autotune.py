@@ -161,7 +161,7 @@ def stream_requests(n_req=24, seed=7, t_range=(20.0, 60.0)):
 
 
 def _tune_serve(dev, rounds, log, n_req=24):
-    """One slices-per-block for every system, judged on a replay of a seeded
+    """One slices-per-block for every system, chosen on a replay of a seeded
     stream of mixed sizes (all runs admitted up front, to the end): in the
     pool the large systems keep the GPU busy, and a shape that helps a
     small system fill an idle GPU can slow the pool as a whole."""
