@@ -232,15 +232,17 @@ def batched_config(dev, steps, warmup, graph=False):
             "pcg_iterations_per_step_max": sum(bat.iterations[warmup:]) / steps + 1}
 
 
-def irregular_config(dev, steps, warmup, n_target=640_000, orders=(None, "rcm", "levels")):
+def irregular_config(dev, steps, warmup, n_target=640_000, orders=(None, "rcm", "levels", "auto")):
     """Irregular volume: the thick ellipsoidal shell (semi-axes 40/30/60 mm,
     wall 10 mm; jittered-lattice Delaunay, about 640k nodes, randomly
     permuted node numbers), four conductivity regions, transmural fibres,
     five stimulus patches (0 and 5 ms onsets), TTP ENDO float32.  Runs the
-    same steps in the mesh's own (scattered) order, in reverse Cuthill-McKee
-    order and in the simulator's default locality order (Cuthill-McKee levels
-    along a Z-order curve); the locality orders run with staged gathers
-    (per-block column sets in LDS) when their sets fit."""
+    same steps in the mesh's own (scattered) order and in locality orders:
+    reverse Cuthill-McKee, Cuthill-McKee levels along a Z-order curve
+    ("levels"), and the simulator's default ("auto": the level order with
+    rows grouped by length inside each PCG block's window); the locality
+    orders run with staged gathers (per-block column sets in LDS) when their
+    sets fit."""
     import torch
     from torchcor_amd import meshes
     from torchcor_amd.ionic import TenTusscherPanfilov
@@ -285,13 +287,14 @@ def irregular_config(dev, steps, warmup, n_target=640_000, orders=(None, "rcm", 
                     "sell_padding": round(sim.system.padded / sim.system.nnz - 1, 4),
                     "pcg_blocks": int(sim.system.pcg_blocks),
                     "staged_gathers": bool(sim.system.staged),
+                    "row_window": int(sim.row_window),
                     "stage_max_cols": int(getattr(sim.system, "stage_max_cols", 0))
                     if sim.system.staged else None,
                     "kernel_avg_us": kt,
                     "pcg_iterations_per_step": sum(sim.iterations[warmup:]) / steps + 1}
         sim.finish()
         del sim
-    for o in ("rcm", "levels"):
+    for o in ("rcm", "levels", "auto"):
         if o in out and "mesh_order" in out:
             out["speedup_" + o] = round(out["mesh_order"]["ms_per_step"] / out[o]["ms_per_step"], 2)
     return out
@@ -590,7 +593,8 @@ def main():
     ap.add_argument("--no-shell", action="store_true")
     ap.add_argument("--served-requests", type=int, default=32)
     ap.add_argument("--shell-rcm", action="store_true",
-                    help="also run the shell in plain reverse Cuthill-McKee order")
+                    help="also run the shell in plain reverse Cuthill-McKee order and in the "
+                         "level order alone")
     ap.add_argument("--shell-steps", type=int, default=200)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-warmup", type=int, default=200)
@@ -705,7 +709,7 @@ def main():
         torch.cuda.empty_cache()
     irregular = None
     if not args.no_shell:
-        orders = (None, "rcm", "levels") if args.shell_rcm else (None, "levels")
+        orders = (None, "rcm", "levels", "auto") if args.shell_rcm else (None, "auto")
         irregular = irregular_config(dev, args.shell_steps, 700, orders=orders)
         gc.collect()
         torch.cuda.empty_cache()

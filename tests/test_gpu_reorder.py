@@ -1,7 +1,8 @@
 """Internal node orders on the irregular shell (run with -m gpu).
 
 The shell mesh comes with randomly permuted node numbers.  With
-reorder="levels" (what "auto" picks here) or "rcm" the simulator runs in
+reorder="levels", "auto" (here the level order with rows grouped by length
+inside the staged split's block windows) or "rcm" the simulator runs in
 that order and maps every input and output through the permutation; with
 reorder=None it runs in the mesh's own order.  Tolerances:
   * M, K, A returned in mesh numbering: bit-identical between the two orders
@@ -91,10 +92,23 @@ def test_auto_reorder_choice():
     s = _sim(mesh, "ms2v", torch.float64, 1.0, "auto")
     s.assemble()
     assert s.node_order is not None and s.bandwidth_reordered * 2 < s.bandwidth_natural
-    from torchcor_amd.core.reorder import element_pattern, level_order
-    rp, col = element_pattern(mesh[1], mesh[0].shape[0])
-    lo, _, _ = level_order(rp, col, mesh[0].shape[0], np.asarray(mesh[0], dtype=np.float64))
-    assert np.array_equal(s.node_order.cpu().numpy(), lo)   # "auto" applies the level order
+    from torchcor_amd import autotune
+    from torchcor_amd.core.reorder import element_pattern, level_order, slice_sort
+    from torchcor_amd.core.system import P1System
+    n = mesh[0].shape[0]
+    rp, col = element_pattern(mesh[1], n)
+    lo, _, _ = level_order(rp, col, n, np.asarray(mesh[0], dtype=np.float64))
+    # "auto" applies the level order, then groups rows by length inside the
+    # windows of the staged split, which the stage then uses
+    w = P1System.sort_window(n)
+    assert s.row_window == w
+    assert np.array_equal(s.node_order.cpu().numpy(), slice_sort(lo, rp, w))
+    assert s.system.staged and s.system.pcg_blocks == autotune.pcg_blocks(n, 8)
+    assert s.system.bandwidth == s.bandwidth_reordered
+    lv = _sim(mesh, "ms2v", torch.float64, 1.0, "levels")
+    lv.assemble()
+    assert lv.row_window == 0 and np.array_equal(lv.node_order.cpu().numpy(), lo)
+    assert s.system.padded < 0.9 * lv.system.padded      # less SELL padding
     nodes, elems, regions, fib = meshes.cube_workload(21)
     c = Monodomain(_gpu_model("ms2v", torch.float64), T=1.0, dt=0.01, device=DEV,
                    dtype=torch.float64)
@@ -106,7 +120,7 @@ def test_auto_reorder_choice():
 
 
 @pytest.mark.parametrize("model,tol,order", [("ttp", 1e-6, "levels"), ("ms2v", 0.5, "levels"),
-                                             ("ttp", 1e-6, "rcm")])
+                                             ("ttp", 1e-6, "rcm"), ("ttp", 1e-6, "auto")])
 def test_shell_reordered_matches_mesh_order_float64(model, tol, order):
     _say(f"[gpu] shell f64 {model} {order} vs mesh order")
     mesh = _shell()

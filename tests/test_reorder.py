@@ -13,6 +13,11 @@ coordinates) is checked against its definition computed independently:
 breadth-first levels from its first node (scipy), positions sorted by level,
 and inside a level by a Morton key formed here by plain bit loops; its
 bandwidth is then at most two level widths.
+
+The slice sort (rows grouped by length inside fixed windows of an order) is
+checked against its definition by plain loops, with the properties that make
+it safe for the staged split: no node leaves its window, and the bandwidth
+moves by at most twice the window.
 """
 import numpy as np
 import pytest
@@ -22,7 +27,8 @@ from scipy.sparse.csgraph import reverse_cuthill_mckee
 torch = pytest.importorskip("torch")
 
 from torchcor_amd import meshes  # noqa: E402
-from torchcor_amd.core.reorder import RCM, element_pattern, level_order, rcm_order  # noqa: E402
+from torchcor_amd.core.reorder import (RCM, element_pattern, level_order, rcm_order,  # noqa: E402
+                                       slice_sort)
 
 
 def _csr(n, edges):
@@ -227,3 +233,45 @@ def test_level_order_rejects_bad_coordinates():
     bad[1, 2] = np.nan
     with pytest.raises(Exception):
         level_order(rp, col, 3, bad)
+
+
+def _sell_slots(rp, order):
+    """Entry slots of the SELL layout of ``order``: 64 x the longest row of
+    every slice of 64 positions."""
+    deg = np.diff(rp)[order]
+    n = order.shape[0]
+    d = np.zeros(-(-n // 64) * 64, dtype=np.int64)
+    d[:n] = deg
+    return int(d.reshape(-1, 64).max(axis=1).sum() * 64)
+
+
+def test_slice_sort_definition_and_padding():
+    nodes, elems = meshes.jittered_delaunay(16, seed=3)
+    n = nodes.shape[0]
+    rp, col = element_pattern(elems, n)
+    lo, _, bw1 = level_order(rp, col, n, nodes)
+    deg = np.diff(rp)
+    for window in (1, 7, 64, 640, n + 5):
+        got = slice_sort(lo, rp, window)
+        expect = []
+        for w0 in range(0, n, window):
+            seg = lo[w0:w0 + window].tolist()
+            ks = sorted(range(len(seg)), key=lambda k, seg=seg: (-deg[seg[k]], k))
+            expect += [seg[k] for k in ks]
+            assert sorted(got[w0:w0 + window].tolist()) == sorted(seg)   # stays in its window
+        assert np.array_equal(got, np.asarray(expect))
+        b, _ = _band_profile(rp, col, got)
+        assert b <= bw1 + 2 * (window - 1)
+        if window == 1:
+            assert np.array_equal(got, lo)
+    # grouping rows of similar length: fewer padded slots than the level order
+    assert _sell_slots(rp, slice_sort(lo, rp, 640)) < 0.9 * _sell_slots(rp, lo)
+    assert _sell_slots(rp, lo) > rp[-1]
+
+
+def test_slice_sort_rejects_bad_arguments():
+    rp, col = _csr(3, [(0, 1), (1, 2)])
+    with pytest.raises(ValueError):
+        slice_sort(np.arange(3), rp, 0)
+    with pytest.raises(ValueError):
+        slice_sort(np.arange(4), rp, 2)

@@ -4,9 +4,10 @@
 ``map``, ``inverse``); the order itself comes from the library's native
 reverse Cuthill-McKee (``tc_rcm_order``, host code), not from scipy.
 ``rcm_order`` orders an assembled pattern by reverse Cuthill-McKee;
-``level_order`` (what the simulator applies) keeps the Cuthill-McKee level
-structure and orders each level along a Z-order curve of the node
-coordinates (``tc_level_order``).
+``level_order`` keeps the Cuthill-McKee level structure and orders each
+level along a Z-order curve of the node coordinates (``tc_level_order``);
+``slice_sort`` then groups rows of similar length inside windows of that
+order (the simulator's default applies both).
 """
 import ctypes as C
 from itertools import combinations
@@ -51,6 +52,32 @@ def level_order(row_ptr, col, n, xyz):
     L.call("tc_level_order", n, rp.ctypes.data, cl.ctypes.data, p.ctypes.data, order.ctypes.data,
            C.byref(b0), C.byref(b1))
     return order.astype(np.int64), int(b0.value), int(b1.value)
+
+
+def slice_sort(order, row_ptr, window):
+    """``order`` (position -> old node index) with the nodes of each window
+    of ``window`` consecutive positions sorted by their pattern row length,
+    longest first; equal lengths keep their order, and no node leaves its
+    window.
+
+    A SELL slice of 64 rows is as wide as its longest row, so ragged rows
+    (an unstructured mesh: 5..25 entries a row) pad every slice to its
+    longest; grouping rows of similar length cuts that padding (on the
+    640k-node shell from 37 % of the nonzeros to 6 % with 640-row windows).
+    With windows equal to the PCG blocks' row ranges each block keeps the
+    same nodes, so its staged column set is unchanged.  ``row_ptr`` is the
+    pattern in the old numbering."""
+    order = np.asarray(order, dtype=np.int64)
+    window = int(window)
+    if window < 1:
+        raise ValueError("window must be at least 1")
+    deg = np.diff(np.asarray(torch.as_tensor(row_ptr).cpu().numpy(), dtype=np.int64))
+    if deg.shape[0] != order.shape[0]:
+        raise ValueError("row_ptr must have n + 1 entries")
+    d = deg[order]
+    top = int(d.max()) if d.size else 0
+    key = (np.arange(order.shape[0], dtype=np.int64) // window) * (top + 1) + (top - d)
+    return order[np.argsort(key, kind="stable")]
 
 
 def element_pattern(elems, n):

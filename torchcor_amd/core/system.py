@@ -30,11 +30,23 @@ class P1System:
     #: runtime's spmv pass (tc_sell_stage) when every block's column set fits
     stage_gathers = True
 
+    @staticmethod
+    def sort_window(n):
+        """Rows of one block of the staged split of n rows (8 slices a block,
+        at most 1024 blocks): the window in which ``slice_sort`` groups rows
+        by length, so that each block keeps its own nodes."""
+        from .. import autotune
+        ns = -(-int(n) // 64)
+        return -(-ns // autotune.pcg_blocks(n, 8)) * 64
+
     def __init__(self, nodes, elems, device=None):
         device = torch.device(device) if device is not None else elems.device
         if device.type != "cuda":
             raise L.TorchcorError("P1System needs a GPU device (no CPU path)")
         self.device = device
+        #: rows sorted by length inside windows of this many rows (0: not
+        #: sorted); the staged split aligned with them is tried first
+        self.row_window = 0
         self.elems = torch.as_tensor(elems).to(device=device, dtype=torch.int32).contiguous()
         self.nodes = torch.as_tensor(nodes).to(device=device, dtype=torch.float64).contiguous()
         self.n = int(self.nodes.shape[0])
@@ -163,8 +175,9 @@ class P1System:
         the plan's block split (tc_sell_stage).  When a block of the tuned
         split references more than TC_STAGE_MAX_COLS columns, the split of
         8 slices per block (more, smaller blocks) is tried and, if it fits,
-        becomes the plan's.  False when the operator has shared slices or
-        no split fits (the plan then gathers per entry)."""
+        becomes the plan's; rows sorted by length inside that split's block
+        windows (``row_window``) try it first.  False when the operator has
+        shared slices or no split fits (the plan then gathers per entry)."""
         import numpy as np
         from .. import autotune
         if self.slice_tag is not None:
@@ -173,7 +186,10 @@ class P1System:
         sp = self.slice_ptr.cpu().numpy()
         cl = self.scol.cpu().numpy()
         col_ptr = own = None
-        for G in dict.fromkeys((int(self.pcg_blocks), autotune.pcg_blocks(self.n, 8))):
+        cands = [int(self.pcg_blocks), autotune.pcg_blocks(self.n, 8)]
+        if self.row_window and self.row_window == self.sort_window(self.n):
+            cands.insert(0, autotune.pcg_blocks(self.n, 8))
+        for G in dict.fromkeys(cands):
             col_ptr = np.empty(G + 1, np.int32)
             own = np.empty(G, np.int32)
             cols = np.empty(self.padded, np.int32)

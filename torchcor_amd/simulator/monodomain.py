@@ -16,10 +16,12 @@ Node order.  When the mesh's own numbering scatters neighbours (bandwidth of
 the pattern above n/8 and at least twice what the new order gives), the
 operators and every per-node array live in a locality order on the device:
 the Cuthill-McKee level structure with each level along a Z-order curve of
-the node coordinates (``level_order``).  The mesh arrays, stimulus node ids
-and every output (u, frames, AT, RT, M, K, A) stay in the caller's
-numbering.  ``reorder`` selects this: "auto" (default), "levels" (always),
-"rcm" (always, plain reverse Cuthill-McKee) or None (never).
+the node coordinates (``level_order``), and in "auto" mode the rows of each
+PCG block's window then sorted by length (``slice_sort``: less SELL padding,
+the same column set a block).  The mesh arrays, stimulus node ids and every
+output (u, frames, AT, RT, M, K, A) stay in the caller's numbering.
+``reorder`` selects this: "auto" (default), "levels" (always, the level
+order alone), "rcm" (always, plain reverse Cuthill-McKee) or None (never).
 """
 import time
 from pathlib import Path
@@ -30,7 +32,7 @@ import torch
 from .. import _lib as L
 from ..core.conductivity import Conductivity
 from ..core.mesh import MeshReader
-from ..core.reorder import level_order, rcm_order
+from ..core.reorder import level_order, rcm_order, slice_sort
 from ..core.stimulation import Stimuli
 from ..core.system import P1System
 from ..ionic.base import default_device
@@ -68,6 +70,7 @@ class Monodomain:
         self.reorder = reorder
         self.node_order = None   # internal position i holds mesh node node_order[i]
         self.node_map = None     # mesh node j sits at internal position node_map[j]
+        self.row_window = 0      # rows sorted by length in windows of this many (0: not)
 
     # ------------------------------------------------------------ set-up
     def load_mesh(self, path="Data/ventricle/Case_1", unit_conversion=1000):
@@ -99,6 +102,7 @@ class Monodomain:
     def _choose_order(self, sys_):
         """Node order for the pattern of ``sys_`` if the mode asks for one."""
         self.node_order = self.node_map = None
+        self.row_window = 0
         if self.reorder is None:
             return None
         if self.reorder == "auto":
@@ -115,10 +119,19 @@ class Monodomain:
         if self.reorder == "auto" and 2 * bw1 > bw0:
             return None
         n = sys_.n
+        if self.reorder == "auto" and P1System.stage_gathers:
+            # rows grouped by length inside the windows of the staged split
+            self.row_window = P1System.sort_window(n)
+            order = slice_sort(order, sys_.row_ptr, self.row_window)
         dev = self.device
         self.node_order = torch.as_tensor(order, dtype=torch.long, device=dev)
         self.node_map = torch.empty_like(self.node_order)
         self.node_map[self.node_order] = torch.arange(n, dtype=torch.long, device=dev)
+        if self.row_window:
+            # the sort moves rows inside their windows: the order's own bandwidth
+            m = self.node_map.to(sys_.device)
+            self.bandwidth_reordered = int(
+                (m[sys_.col.to(torch.int64)] - m[rows]).abs().max().item())
         return self.node_order
 
     def _to_mesh_order(self, v):
@@ -135,6 +148,7 @@ class Monodomain:
         if self._choose_order(sys_) is not None:
             elems = self.node_map[self.elems.to(torch.long)].to(torch.int32)
             sys_ = P1System(self.nodes[self.node_order], elems, self.device)
+            sys_.row_window = self.row_window
         # The reference forms elements from coordinates and fibres held in the
         # working dtype; the same rounded inputs are used here.
         M, K = sys_.assemble(sigma_l=self.conductivity.sigma_l, sigma_t=self.conductivity.sigma_t,
@@ -497,6 +511,7 @@ class Monodomain:
         if order is not None:
             elems = self.node_map[self.elems.to(torch.long)].to(torch.int32)
             P = P1System(self.nodes[order], elems, dev)
+            P.row_window = self.row_window
             Vm = Vm[:, order]
         else:
             P = P1System(self.nodes, self.elems, dev)
