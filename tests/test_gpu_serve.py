@@ -9,6 +9,10 @@ Tolerances:
   * against the single-run path (Monodomain, same assembly and operators),
     float64: max |dVm| < 1e-9 mV, LAT equal, n_iter sums equal (only the
     order of the PCG partial sums differs).
+  * a mesh with scattered node numbers (a small ellipsoidal shell) runs in
+    the level order inside the pool: float64 against Monodomain in the same
+    order as above; against the pool in the mesh's own order max |dVm| <
+    1e-6 mV with the same iteration sum (summation order only).
 """
 import numpy as np
 import pytest
@@ -170,3 +174,63 @@ def test_serve_many_systems_alone_equals_in_mix(nreq):
         assert alone["iterations"] == mix[k]["iterations"]
     for k, r in enumerate(reqs):
         assert mix[k]["steps"] == int(r.T / 0.01), k
+
+
+SMALL_SHELL = dict(outer=(10.0, 7.5, 15.0), wall=2.5)
+
+
+def _shell_request(T, model="ttp", tag=0):
+    from torchcor_amd.serve import Request
+    nodes, elems, regions, fib = meshes.ellipsoid_shell(3000, 11, **SMALL_SHELL)
+    by_start = {}
+    for ids, start in meshes.shell_stimuli(nodes, 12, radius_mm=1.5, **SMALL_SHELL):
+        by_start.setdefault(start, []).append(np.asarray(ids))
+    st = [(np.concatenate(v), start, 1.0, 100.0) for start, v in sorted(by_start.items())]
+    return Request(nodes, elems, regions, fib, meshes.SHELL_CONDUCTIVITIES, model, st, T, tag=tag)
+
+
+def test_serve_scattered_mesh_in_locality_order():
+    from torchcor_amd.serve import MODEL_CLASSES
+    from torchcor_amd.simulator import Monodomain
+    _say("[gpu] serve scattered shell: level order vs Monodomain and mesh order (f64)")
+    r = _shell_request(8.0)
+    runs = {}
+    for mode in ("auto", None):
+        eng = _engine(torch.float64, cap_nodes=12000, reorder=mode)
+        pr = eng.prepare(r)
+        assert (pr.node_map is not None) == (mode == "auto")
+        assert (pr.system.bandwidth < 0.1 * pr.n) == (mode == "auto")
+        runs[mode] = eng.run_until_done([pr])[0]
+        eng.close()
+    a, b = runs["auto"], runs[None]
+    assert a["iterations"] == b["iterations"]
+    assert (a["u"] - b["u"]).abs().max().item() < 1e-6
+    both = (a["AT"] > 0) & (b["AT"] > 0)
+    assert both.sum().item() > 50
+    assert (a["AT"][both] - b["AT"][both]).abs().max().item() <= 0.01 + 1e-9
+    model = MODEL_CLASSES[r.model](0.01, DEV, torch.float64)
+    sim = Monodomain(model, T=r.T, dt=0.01, device=DEV, dtype=torch.float64, reorder="levels")
+    sim.load_arrays(r.nodes, r.elems, r.regions, r.fibres)
+    for c in r.conductivities:
+        sim.add_conductivity(*c)
+    for st in r.stimuli:
+        sim.add_stimulus(st[0], *st[1:])
+    sim.start(1e-5, 1e-5, 100, snapshot_interval=1e9)
+    sim.advance(sim.nt)
+    sim.finish()
+    assert sim.node_order is not None
+    assert (sim.u - a["u"]).abs().max().item() < 1e-9
+    assert torch.equal(sim.activation_time, a["AT"])
+    assert sum(sim.iterations) == a["iterations"]
+
+
+def test_serve_scattered_mesh_alone_equals_in_mix():
+    _say("[gpu] serve scattered shell alone vs among cube runs (f32)")
+    reqs = _requests()[:3] + [_shell_request(6.0, "ms2v", tag=9)]
+    eng = _engine(torch.float32, cap_nodes=12000)
+    mix = eng.run_until_done([eng.prepare(q) for q in reqs], max_chunk=29)
+    alone_eng = _engine(torch.float32, cap_nodes=12000)
+    alone = alone_eng.run_until_done([alone_eng.prepare(reqs[-1])])[0]
+    assert torch.equal(alone["u"], mix[-1]["u"])
+    assert torch.equal(alone["AT"], mix[-1]["AT"]) and torch.equal(alone["RT"], mix[-1]["RT"])
+    assert alone["iterations"] == mix[-1]["iterations"]

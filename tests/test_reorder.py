@@ -27,8 +27,8 @@ from scipy.sparse.csgraph import reverse_cuthill_mckee
 torch = pytest.importorskip("torch")
 
 from torchcor_amd import meshes  # noqa: E402
-from torchcor_amd.core.reorder import (RCM, element_pattern, level_order, rcm_order,  # noqa: E402
-                                       slice_sort)
+from torchcor_amd.core.reorder import (RCM, element_pattern, level_order, pick_order,  # noqa: E402
+                                       rcm_order, slice_sort)
 
 
 def _csr(n, edges):
@@ -275,3 +275,29 @@ def test_slice_sort_rejects_bad_arguments():
         slice_sort(np.arange(3), rp, 0)
     with pytest.raises(ValueError):
         slice_sort(np.arange(4), rp, 2)
+
+
+def test_pick_order_modes():
+    nodes, elems = meshes.jittered_delaunay(12, seed=5)           # randomly numbered
+    n = nodes.shape[0]
+    rp, col = element_pattern(elems, n)
+    rpt, colt = torch.as_tensor(rp), torch.as_tensor(col)
+    lo, b0, b1 = level_order(rp, col, n, nodes)
+    ro, _, r1 = rcm_order(rp, col, n)
+    assert pick_order(rpt, colt, n, nodes, None) is None
+    o, a0, a1 = pick_order(rpt, colt, n, nodes, "auto")
+    assert np.array_equal(o, lo) and (a0, a1) == (b0, b1) and 2 * a1 <= a0 and 8 * a0 > n
+    o, _, a1 = pick_order(rpt, colt, n, nodes, "rcm")
+    assert np.array_equal(o, ro) and a1 == r1
+    o, _, a1 = pick_order(rpt, colt, n, nodes, "auto", window=64)
+    assert np.array_equal(o, slice_sort(lo, rp, 64))
+    assert a1 == _band_profile(rp, col, o)[0]
+    # a mesh numbered along its lattice is already banded: "auto" keeps it,
+    # "levels" still reorders
+    nodes2, elems2 = meshes.jittered_delaunay(12, seed=5, permute=False)
+    rp2, col2 = element_pattern(elems2, n)
+    args = (torch.as_tensor(rp2), torch.as_tensor(col2), n, nodes2)
+    assert pick_order(*args, "auto") is None
+    assert pick_order(*args, "levels") is not None
+    with pytest.raises(ValueError):
+        pick_order(rpt, colt, n, nodes, "bfs")

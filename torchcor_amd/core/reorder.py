@@ -80,6 +80,43 @@ def slice_sort(order, row_ptr, window):
     return order[np.argsort(key, kind="stable")]
 
 
+def pick_order(row_ptr, col, n, xyz, mode, window=0):
+    """The internal node order that ``mode`` asks for, as (order,
+    bandwidth_before, bandwidth_after), or None to keep the mesh's numbering.
+
+    ``row_ptr``/``col``: the symmetric pattern (tensors; its device does the
+    bandwidth arithmetic); ``xyz`` (n, 3) the node coordinates.  "levels":
+    ``level_order``; "rcm": ``rcm_order``; "auto": the level order when the
+    mesh's own numbering scatters neighbours (bandwidth above n/8) and the
+    level order at least halves that bandwidth, else None; None: None.
+    With ``window`` > 0 the order is then ``slice_sort``-ed in windows of
+    that many rows, and bandwidth_after is the sorted order's."""
+    if mode not in ("auto", "levels", "rcm", None):
+        raise ValueError("mode must be 'auto', 'levels', 'rcm' or None")
+    if mode is None:
+        return None
+    n = int(n)
+    rp = torch.as_tensor(row_ptr)
+    cl = torch.as_tensor(col).to(torch.int64)
+    rows = torch.repeat_interleave(torch.arange(n, device=rp.device), rp.diff().to(torch.int64))
+    if mode == "auto":
+        bw0 = int((cl - rows).abs().max().item()) if cl.numel() else 0
+        if 8 * bw0 <= n:
+            return None
+    if mode == "rcm":
+        order, bw0, bw1 = rcm_order(row_ptr, col, n)
+    else:
+        order, bw0, bw1 = level_order(row_ptr, col, n, xyz)
+    if mode == "auto" and 2 * bw1 > bw0:
+        return None
+    if window:
+        order = slice_sort(order, row_ptr, window)
+        pos = torch.empty(n, dtype=torch.int64, device=rp.device)
+        pos[torch.as_tensor(order, device=rp.device)] = torch.arange(n, device=rp.device)
+        bw1 = int((pos[cl] - pos[rows]).abs().max().item()) if cl.numel() else 0
+    return order, bw0, bw1
+
+
 def element_pattern(elems, n):
     """Symmetric CSR pattern (row_ptr, col) of the node pairs of the elements."""
     e = np.asarray(elems, dtype=np.int64)

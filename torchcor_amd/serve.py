@@ -13,7 +13,11 @@ simulated second of every request and the throughput.
 Each run follows the single-run path exactly: the same GPU assembly (with
 the reference's float32 rounding of element contributions), the same SELL
 operator values and Jacobi scaling, the same time loop semantics; its
-result does not depend on which other runs share the pool.
+result does not depend on which other runs share the pool.  A mesh whose
+node numbering scatters neighbours runs in a locality order inside the pool
+(as ``Monodomain(reorder=...)``, without the row grouping of the staged
+single-run passes, which the pool's per-entry gathers do not use); its
+stimulus node ids and its results stay in the mesh's numbering.
 """
 import collections
 import concurrent.futures
@@ -25,6 +29,7 @@ import torch
 
 from . import _lib as L
 from .core.conductivity import Conductivity
+from .core.reorder import pick_order
 from .core.system import P1System
 from .ionic.base import default_device
 from .ionic.models import CourtemancheRamirezNattel, ModifiedMS2v, TenTusscherPanfilov
@@ -66,7 +71,10 @@ class ServeEngine:
     def __init__(self, cap_nodes, cap_entries, max_sys=256, dt=0.01, dtype=torch.float32,
                  device=None, a_tol=1e-5, r_tol=1e-5, max_iter=100, x_f64=True,
                  precise_rhs=True, lat=True, depth=4, int16_columns=True, slices_per_block=None,
-                 k_history=0, graph=False):
+                 k_history=0, graph=False, reorder="auto"):
+        if reorder not in ("auto", "levels", "rcm", None):
+            raise ValueError("reorder must be 'auto', 'levels', 'rcm' or None")
+        self.reorder = reorder
         self.device = torch.device(device) if device is not None else default_device()
         dev = self.device
         self.dtype = dtype
@@ -142,6 +150,15 @@ class ServeEngine:
         for c in req.conductivities:
             cond.add(*c)
         S = P1System(nodes, elems, dev)
+        node_map = None
+        res = pick_order(S.row_ptr, S.col, S.n, nodes, self.reorder)
+        if res is not None:
+            # a locality order: internal position i holds mesh node order[i]
+            order = torch.as_tensor(res[0], dtype=torch.long, device=dev)
+            node_map = torch.empty_like(order)
+            node_map[order] = torch.arange(order.shape[0], dtype=torch.long, device=dev)
+            elems = node_map[elems.to(torch.long)].to(torch.int32)
+            S = P1System(nodes[order], elems, dev)
         M, K = S.assemble(sigma_l=cond.sigma_l, sigma_t=cond.sigma_t, fibres=fibres)
         S.release_assembly_maps()
         S.set_operators(M, K, self.Chi, self.Cm, self.theta, self.dt, dt_)
@@ -150,6 +167,7 @@ class ServeEngine:
         obj._alloc_states(n)
         pr = Prepared()
         pr.req, pr.system, pr.n = req, S, n
+        pr.node_map = node_map          # mesh node j at internal row node_map[j] (None: same)
         pr.states = obj.states
         pr.V0 = torch.full((n,), float(obj.V_init), dtype=dt_, device=dev)
         pr.mask = torch.zeros(n, dtype=torch.int32, device=dev)
@@ -161,7 +179,10 @@ class ServeEngine:
             ids, start, duration, amp = s[:4]
             period = s[4] if len(s) > 4 and s[4] is not None else req.T
             count = s[5] if len(s) > 5 else 1
-            pr.mask[torch.as_tensor(np.asarray(ids), device=dev)] |= (1 << k)
+            ids = torch.as_tensor(np.asarray(ids), dtype=torch.long, device=dev)
+            if node_map is not None:
+                ids = node_map[ids]
+            pr.mask[ids] |= (1 << k)
             pr.stim.append(L.Stimulus(float(start), float(duration), float(period), float(count),
                                       float(amp)))
         pr.nt = int(req.T / self.dt)
@@ -213,6 +234,10 @@ class ServeEngine:
         tot, steps = C.c_int64(0), C.c_int64(0)
         L.call("tc_serve_remove", self._h, slot, L.ptr(V), L.ptr(AT), L.ptr(RT), C.byref(tot),
                C.byref(steps), L.stream_handle(dev))
+        if pr.node_map is not None:      # back to the mesh's numbering
+            V = V[pr.node_map]
+            AT = AT[pr.node_map] if AT is not None else None
+            RT = RT[pr.node_map] if RT is not None else None
         return dict(u=V, AT=AT, RT=RT, iterations=int(tot.value), steps=int(steps.value),
                     req=pr.req)
 

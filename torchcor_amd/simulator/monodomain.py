@@ -32,7 +32,7 @@ import torch
 from .. import _lib as L
 from ..core.conductivity import Conductivity
 from ..core.mesh import MeshReader
-from ..core.reorder import level_order, rcm_order, slice_sort
+from ..core.reorder import pick_order
 from ..core.stimulation import Stimuli
 from ..core.system import P1System
 from ..ionic.base import default_device
@@ -100,38 +100,23 @@ class Monodomain:
         self.conductivity.add(region_ids, il, it, el, et)
 
     def _choose_order(self, sys_):
-        """Node order for the pattern of ``sys_`` if the mode asks for one."""
+        """Node order for the pattern of ``sys_`` if the mode asks for one
+        (``pick_order``; "auto" with staged gathers also groups rows by
+        length inside the staged split's block windows)."""
         self.node_order = self.node_map = None
         self.row_window = 0
-        if self.reorder is None:
-            return None
-        if self.reorder == "auto":
-            rows = torch.repeat_interleave(torch.arange(sys_.n, device=sys_.device),
-                                           sys_.row_ptr.diff().to(torch.int64))
-            bw0 = int((sys_.col.to(torch.int64) - rows).abs().max().item()) if sys_.nnz else 0
-            if 8 * bw0 <= sys_.n:
-                return None
-        if self.reorder == "rcm":
-            order, bw0, bw1 = rcm_order(sys_.row_ptr, sys_.col, sys_.n)
-        else:
-            order, bw0, bw1 = level_order(sys_.row_ptr, sys_.col, sys_.n, self.nodes)
-        self.bandwidth_natural, self.bandwidth_reordered = bw0, bw1
-        if self.reorder == "auto" and 2 * bw1 > bw0:
-            return None
         n = sys_.n
-        if self.reorder == "auto" and P1System.stage_gathers:
-            # rows grouped by length inside the windows of the staged split
-            self.row_window = P1System.sort_window(n)
-            order = slice_sort(order, sys_.row_ptr, self.row_window)
+        window = P1System.sort_window(n) if (self.reorder == "auto"
+                                             and P1System.stage_gathers) else 0
+        res = pick_order(sys_.row_ptr, sys_.col, n, self.nodes, self.reorder, window)
+        if res is None:
+            return None
+        order, self.bandwidth_natural, self.bandwidth_reordered = res
+        self.row_window = window
         dev = self.device
         self.node_order = torch.as_tensor(order, dtype=torch.long, device=dev)
         self.node_map = torch.empty_like(self.node_order)
         self.node_map[self.node_order] = torch.arange(n, dtype=torch.long, device=dev)
-        if self.row_window:
-            # the sort moves rows inside their windows: the order's own bandwidth
-            m = self.node_map.to(sys_.device)
-            self.bandwidth_reordered = int(
-                (m[sys_.col.to(torch.int64)] - m[rows]).abs().max().item())
         return self.node_order
 
     def _to_mesh_order(self, v):
