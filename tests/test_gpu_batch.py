@@ -9,6 +9,10 @@ Tolerances:
     |dVm| < 1e-8 mV, LAT/RT equal, n_iter sums equal.
   * float32 (float64 PCG reductions, float64 iterate and right-hand side)
     against the float64 oracle: max |dVm| <= 0.5 mV, |dLAT| <= dt.
+  * a mesh with scattered node numbers (jittered Delaunay, randomly
+    renumbered) runs in the level order: float64 against the oracle in the
+    mesh's numbering as above, and against the batch in the mesh's order
+    max |dVm| < 1e-8 mV with the same iteration sums.
 """
 import numpy as np
 import pytest
@@ -136,3 +140,44 @@ def test_batch_float32_vs_float64_oracle():
         both = (at > 0) & (ro["AT"] > 0)
         assert np.array_equal(at > 0, ro["AT"] > 0)
         assert np.abs(at[both] - ro["AT"][both]).max() <= 0.01 + 1e-6
+
+
+def _scattered_mesh():
+    nodes, elems = meshes.jittered_delaunay(10, seed=21)         # randomly renumbered
+    regions = (nodes[elems].mean(axis=1)[:, 0] > 5.0).astype(np.int32)
+    fib = np.tile(np.array([[1.0, 0.0, 0.0]]), (elems.shape[0], 1))
+    corner = np.nonzero((nodes < 2.5).all(axis=1))[0]
+    far = np.nonzero(nodes[:, 0] > 8.5)[0]
+    specs = [("ms2v", 1.0, 0.0, [(corner, 0.0, 1.0, 100.0)]),
+             ("ttp", 1.2, 20.0, [(corner, 0.0, 1.0, 100.0)]),
+             ("ms2v", 0.8, -30.0, [(far, 1.0, 1.0, 80.0)])]
+    return (nodes, elems, regions, fib), specs
+
+
+def test_batch_scattered_mesh_in_locality_order():
+    from torchcor_amd.batch import SharedMeshBatch
+    _say("[gpu] batch scattered mesh: level order vs oracle and mesh order (f64)")
+    mesh, specs = _scattered_mesh()
+    T = 6.0
+    runs = {}
+    for mode in ("auto", None):
+        b = SharedMeshBatch(T=T, dt=0.01, device=DEV, dtype=torch.float64, reorder=mode)
+        b.load_arrays(*mesh)
+        for c in COND:
+            b.add_conductivity(*c)
+        for name, sc, rot, st in specs:
+            b.add_instance(_gpu_model(name, torch.float64), sc, rot, st)
+        tot = b.solve(1e-5, 1e-5, 100)
+        assert (b.node_order is not None) == (mode == "auto")
+        runs[mode] = (b, tot)
+    (a, ta), (m, tm) = runs["auto"], runs[None]
+    assert ta == tm
+    assert (a.u - m.u).abs().max().item() < 1e-8
+    assert torch.equal(a.activation_time, m.activation_time)
+    for k, spec in enumerate(specs):
+        ro = _oracle(mesh, spec, T)
+        err = np.abs(a.u[k].cpu().numpy() - ro["u"]).max()
+        assert err < 1e-6, (k, err)
+        assert np.array_equal(a.activation_time[k].cpu().numpy(), ro["AT"]), k
+        assert ta[k] == int(ro["iters"].sum()), k
+    assert (a.activation_time > 0).sum().item() > 20

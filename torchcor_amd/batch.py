@@ -29,6 +29,7 @@ from . import _lib as L
 from .core.conductivity import Conductivity
 from .core.mesh import MeshReader
 from .core.stimulation import MAX_STIMULI
+from .core.reorder import pick_order
 from .core.system import P1System
 from .ionic.base import default_device
 
@@ -93,13 +94,20 @@ class SharedMeshBatch:
     intensity[, period[, count]]) like ``Monodomain.add_stimulus``.
     ``solve`` returns the per-instance sums of the PCG iteration counts;
     results are (B, n) tensors in instance order: ``u``,
-    ``activation_time``, ``repolarization_time``.
+    ``activation_time``, ``repolarization_time``.  A mesh whose node
+    numbering scatters neighbours runs in a locality order (``reorder`` as
+    for ``Monodomain``: "auto", "levels", "rcm" or None); stimulus node ids
+    and results stay in the mesh's numbering.
     """
     Chi = 140
     Cm = 0.01
     theta = 0.5
 
-    def __init__(self, T, dt, device=None, dtype=None):
+    def __init__(self, T, dt, device=None, dtype=None, reorder="auto"):
+        if reorder not in ("auto", "levels", "rcm", None):
+            raise ValueError("reorder must be 'auto', 'levels', 'rcm' or None")
+        self.reorder = reorder
+        self.node_order = self.node_map = None
         self.device = torch.device(device) if device is not None else default_device()
         self.dtype = dtype if dtype is not None else torch.float32
         self.T = T
@@ -151,6 +159,15 @@ class SharedMeshBatch:
             raise ValueError("no instances")
         dev = self.device
         S = P1System(self.nodes, self.elems, dev)
+        self.node_order = self.node_map = None
+        res = pick_order(S.row_ptr, S.col, S.n, self.nodes, self.reorder)
+        if res is not None:
+            # internal position i holds mesh node node_order[i]
+            self.node_order = torch.as_tensor(res[0], dtype=torch.long, device=dev)
+            self.node_map = torch.empty_like(self.node_order)
+            self.node_map[self.node_order] = torch.arange(S.n, dtype=torch.long, device=dev)
+            elems = self.node_map[self.elems.to(torch.long)].to(torch.int32)
+            S = P1System(self.nodes[self.node_order], elems, dev)
         sl, st = self.conductivity.sigma_l, self.conductivity.sigma_t
         comps = sigma_components(sl, st, self.fibres)
         rots = [i.rot for i in self.instances]
@@ -243,6 +260,8 @@ class SharedMeshBatch:
                 k = slots.index(key)
                 bit = 1 << k if k < 31 else -(1 << 31)
                 nd = torch.as_tensor(nodes, device=dev)
+                if self.node_map is not None:
+                    nd = self.node_map[nd]
                 mask[nd, int(self.lane_of[idx])] |= bit
         self._mask = mask
         self._coefs = self._coef(lanes, order)
@@ -321,10 +340,12 @@ class SharedMeshBatch:
         t, steps, xi = C.c_double(), C.c_int64(), C.c_int()
         L.call("tc_batch_status", self._h, C.byref(t), C.byref(steps), C.byref(xi))
         lanes = torch.as_tensor(self.lane_of, device=self.device)
-        self.u = self._X[xi.value][:, lanes].T.to(self.dtype).contiguous()
+        # (n, lanes) in the internal order -> (B, n) in the mesh's numbering
+        rows = self.node_map if self.node_map is not None else slice(None)
+        self.u = self._X[xi.value][rows][:, lanes].T.to(self.dtype).contiguous()
         if self.activation_time is not None:
-            self.activation_time = self.activation_time[:, lanes].T.contiguous()
-            self.repolarization_time = self.repolarization_time[:, lanes].T.contiguous()
+            self.activation_time = self.activation_time[rows][:, lanes].T.contiguous()
+            self.repolarization_time = self.repolarization_time[rows][:, lanes].T.contiguous()
         self.instance_iterations = [int(tot[self.lane_of[b]]) for b in range(B)]
         self.t = t.value
         return self.instance_iterations
