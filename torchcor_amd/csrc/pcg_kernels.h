@@ -726,18 +726,24 @@ k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
   const int64_t ns = (n + 63) >> 6;
   const Split sp = block_slices(ns);
   const int lane = threadIdx.x & 63, wv = wave_id();
-  // the first slice's operands load while the block reads the flags and
-  // folds the partial sums
+  // the operands of the wave's first two slices (s, s + kWaves) load while
+  // the block reads the flags and folds the partial sums
+  constexpr int kPre = 2;
   int64_t s = sp.s0 + wv;
-  int64_t i = s * 64 + lane;
-  XT xv = XT(0);
-  T pv = T(0), av = T(0), rv = T(0), dv = T(0);
-  if (s < sp.s1 && i < n) {
-    xv = x[i];
-    pv = p[i];
-    av = Ap[i];
-    rv = r[i];
-    dv = Dinv[i];
+  XT xv[kPre];
+  T pv[kPre], av[kPre], rv[kPre], dv[kPre];
+#pragma unroll
+  for (int u = 0; u < kPre; ++u) {
+    const int64_t su = s + u * kWaves, iu = su * 64 + lane;
+    xv[u] = XT(0);
+    pv[u] = av[u] = rv[u] = dv[u] = T(0);
+    if (su < sp.s1 && iu < n) {
+      xv[u] = x[iu];
+      pv[u] = p[iu];
+      av[u] = Ap[iu];
+      rv[u] = r[iu];
+      dv[u] = Dinv[iu];
+    }
   }
   // the flags, r.z and the partial sums are all read before the first wait
   __shared__ int halted;
@@ -757,25 +763,25 @@ k_pcg_update(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ p,
   double red[2] = {0.0, 0.0};
   {
     const double alpha = rzk / v[0];
-    while (s < sp.s1) {
-      if (i < n) {
-        const XT xi = (XT)((double)xv + alpha * (double)pv);
-        const T ri = (T)((double)rv - alpha * (double)av);
-        x[i] = xi;
-        r[i] = ri;
-        const double z = (double)dv * (double)ri;
-        red[0] += z * z;
-        red[1] += (double)ri * z;
-      }
-      s += kWaves;
-      i = s * 64 + lane;
-      if (s < sp.s1 && i < n) {
-        xv = x[i];
-        pv = p[i];
-        av = Ap[i];
-        rv = r[i];
-        dv = Dinv[i];
-      }
+    auto upd = [&](int64_t i, XT xo, T po, T ao, T ro, T dvo) {
+      const XT xi = (XT)((double)xo + alpha * (double)po);
+      const T ri = (T)((double)ro - alpha * (double)ao);
+      x[i] = xi;
+      r[i] = ri;
+      const double z = (double)dvo * (double)ri;
+      red[0] += z * z;
+      red[1] += (double)ri * z;
+    };
+    // the prefetched slices s and s + kWaves, then any further ones: each
+    // lane's sums run over its slices in the same order as one at a time
+#pragma unroll
+    for (int u = 0; u < kPre; ++u) {
+      const int64_t su = s + u * kWaves, iu = su * 64 + lane;
+      if (su < sp.s1 && iu < n) upd(iu, xv[u], pv[u], av[u], rv[u], dv[u]);
+    }
+    for (s += kPre * kWaves; s < sp.s1; s += kWaves) {
+      const int64_t i = s * 64 + lane;
+      if (i < n) upd(i, x[i], p[i], Ap[i], r[i], Dinv[i]);
     }
   }
   if constexpr (!CHECK) {
