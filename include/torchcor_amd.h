@@ -379,6 +379,12 @@ typedef struct tc_ionic_io {
    * time-step runtime, the served pool); a batch on a small shared mesh is
    * faster without. */
   int32_t stream_states;
+  /* Non-zero: the states lie in blocks of 64 entries, entry i's state k at
+   * (i / 64) * S * 64 + k * 64 + i % 64 (S = the model's state count;
+   * state_stride is then unused; not with lanes).  A wavefront's S state
+   * accesses then fall in one contiguous span instead of S rows far apart,
+   * which keeps the memory's row buffers open (the time-step runtime). */
+  int32_t state_block;
 } tc_ionic_io;
 
 /* ModifiedMS2v (ionic/ModifiedMS2v.py:4-44).  States: H. */
@@ -471,6 +477,8 @@ typedef struct tc_sim_desc {
                                 parity and the iteration count) are replayed as a captured
                                 HIP graph; same results.  Steps with timing events launch
                                 eagerly. */
+  int state_block;           /* 1: states in blocks of 64 nodes (tc_ionic_io.state_block),
+                                a (ceil(n/64), S, 64) array; state_stride unused */
 } tc_sim_desc;
 typedef struct tc_sim tc_sim;
 int tc_sim_create(tc_sim** sim, const tc_sim_desc* desc);

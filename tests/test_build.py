@@ -17,7 +17,7 @@ def test_library_exports_header_symbols():
     for nm in sorted(names):
         assert hasattr(L.lib, nm), nm
     assert set(L.EXPORTS) == names
-    assert L.lib.tc_abi_version() == 2
+    assert L.lib.tc_abi_version() == 3
 
 
 def test_error_path_reports_text():

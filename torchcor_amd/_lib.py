@@ -55,7 +55,7 @@ class IonicIO(C.Structure):
                 ("lane_stride", i32), ("lane0", i32), ("pool_slices", p), ("pool_slice_sys", p),
                 ("pool_sys_row0", p), ("pool_sys_n", p), ("pool_sys_t", p), ("pool_sys_steps", p),
                 ("pool_sys_bits", p), ("pool_sys_stim", p), ("state_stride", i64),
-                ("stream_states", i32)]
+                ("stream_states", i32), ("state_block", i32)]
 
 
 MS2V_FIELDS = ["tau_in", "tau_out", "tau_open", "tau_close", "u_gate", "u_crit", "vmin", "vmax"]
@@ -95,7 +95,7 @@ class SimDesc(C.Structure):
                 ("stim_mask", p), ("n_stim", i32), ("stim", Stimulus * MAX_STIMULI),
                 ("a_tol", dbl), ("r_tol", dbl), ("max_iter", i32), ("linear_guess", i32),
                 ("depth", i32), ("state_stride", i64), ("separate_check", i32), ("k_history", i32),
-                ("graph", i32)]
+                ("graph", i32), ("state_block", i32)]
 
 
 BATCH_MAX_K = 5

@@ -19,7 +19,7 @@ void set_error(const char* fmt, ...) {
 
 extern "C" const char* tc_last_error(void) { return tc::g_last_error.c_str(); }
 
-extern "C" int tc_abi_version(void) { return 2; }
+extern "C" int tc_abi_version(void) { return 3; }
 
 // ------------------------------------------------------------------------
 // Plain device copy with one 16-byte load and store per thread: the

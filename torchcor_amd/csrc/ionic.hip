@@ -184,6 +184,7 @@ struct PoolArgs {
 struct NodeCtx {
   int64_t vi;          // entry in the per-node arrays (V, w, dV, stim_mask, AT, RT)
   int64_t si, ss;      // state index and state row stride
+  int64_t sp0;         // state block offset (blocks of 64 entries; 0 in rows)
   uint32_t bits;       // stimuli on at this step for this entry
   const double* amp;   // intensity of stimulus k at amp[k * amp_stride]
   int amp_stride;
@@ -211,6 +212,7 @@ __device__ inline NodeCtx node_ctx(const tc_ionic_io& io, int64_t i,
     c.vi = row;
     c.si = row;
     c.ss = io.state_stride;
+    c.sp0 = 0;
     c.ok = row - q.row0[sys] < q.n[sys];
     const uint32_t on = q.bits[sys];
     c.bits = (on && io.stim_mask) ? io.stim_mask[row] & on : 0u;
@@ -228,6 +230,7 @@ __device__ inline NodeCtx node_ctx(const tc_ionic_io& io, int64_t i,
   }
   c.si = i;
   c.ss = io.state_stride > 0 ? io.state_stride : io.n;
+  c.sp0 = 0;
   c.ok = 1;
   c.bits = (io.stim_active && io.stim_mask) ? at32(io.stim_mask, (uint32_t)c.vi) & io.stim_active
                                              : 0u;
@@ -238,10 +241,24 @@ __device__ inline NodeCtx node_ctx(const tc_ionic_io& io, int64_t i,
   return c;
 }
 
-// State k of this node: row k of the (S, stride) block.
+// States in blocks of 64 entries (io.state_block): entry i's state k at
+// (i / 64) * S * 64 + k * 64 + i % 64, so the S loads and stores of a
+// wavefront all fall in one contiguous S * 256-byte (float32) span instead
+// of S rows n entries apart.  S is the model's state count.
+template <int S>
+__device__ inline void state_blocks(NodeCtx& c, const tc_ionic_io& io) {
+  if (io.state_block) {
+    c.sp0 = (c.si >> 6) * (int64_t)(S * 64);
+    c.si &= 63;
+    c.ss = 64;
+  }
+}
+
+// State k of this node: row k of the (S, stride) block, or its place in
+// the node's block of 64.
 template <class T>
 __device__ inline T& state(T* st, const NodeCtx& c, int k) {
-  return at32(st + (int64_t)k * c.ss, (uint32_t)c.si);
+  return at32(st + c.sp0 + (int64_t)k * c.ss, (uint32_t)c.si);
 }
 
 // State k of this node, with non-temporal loads and stores when NT (the
@@ -251,14 +268,14 @@ __device__ inline T& state(T* st, const NodeCtx& c, int k) {
 // cache step after step.
 template <bool NT, class T>
 __device__ inline T ld_state(const T* st, const NodeCtx& c, int k) {
-  const T* q = &at32(st + (int64_t)k * c.ss, (uint32_t)c.si);
+  const T* q = &at32(st + c.sp0 + (int64_t)k * c.ss, (uint32_t)c.si);
   if constexpr (NT) return __builtin_nontemporal_load(q);
   return *q;
 }
 
 template <bool NT, class T>
 __device__ inline void st_state(T* st, const NodeCtx& c, int k, T v) {
-  T* q = &at32(st + (int64_t)k * c.ss, (uint32_t)c.si);
+  T* q = &at32(st + c.sp0 + (int64_t)k * c.ss, (uint32_t)c.si);
   if constexpr (NT) __builtin_nontemporal_store(v, q);
   else *q = v;
 }
@@ -330,7 +347,8 @@ __device__ __forceinline__ void ms2v_node(const tc_ms2v_params& p, T* __restrict
                                           const PoolArgs& pa = PoolArgs{}) {
   if (i >= io.n) return;
   const int gated = gate_value(io);
-  const NodeCtx c = node_ctx<PA>(io, i, pa);
+  NodeCtx c = node_ctx<PA>(io, i, pa);
+  state_blocks<1>(c, io);
   if (!c.ok) return;
   const T U = load_v<T, VF64>(io, c.vi);
   const T h = ld_state<NT>(H, c, 0);
@@ -566,7 +584,8 @@ __device__ __forceinline__ void ttp_node(const TtpK<T>& K, T* __restrict__ st,
   const int64_t n = io.n;
   if (i >= n) return;
   const int gated = gate_value(io);
-  const NodeCtx c = node_ctx<PA>(io, i, pa);
+  NodeCtx c = node_ctx<PA>(io, i, pa);
+  state_blocks<TS_N>(c, io);
   if (!c.ok) return;
   const T V = load_v<T, VF64>(io, c.vi);
   T s[TS_N];
@@ -867,7 +886,8 @@ __device__ __forceinline__ void crn_node(const CrnK<T>& K, T* __restrict__ st,
   const int64_t n = io.n;
   if (i >= n) return;
   const int gated = gate_value(io);
-  const NodeCtx c = node_ctx<PA>(io, i, pa);
+  NodeCtx c = node_ctx<PA>(io, i, pa);
+  state_blocks<CS_N>(c, io);
   if (!c.ok) return;
   const T V = load_v<T, VF64>(io, c.vi);
   T s[CS_N];
@@ -1026,9 +1046,12 @@ int check_io(const tc_ionic_io* io, int dtype) {
   int64_t top = io->n;
   if (io->pool_slices) top = io->state_stride;
   else if (io->lanes > 0) top = (io->n - 1) / io->lanes * io->lane_stride + io->lane0 + io->lanes;
-  TC_CHECK(io->pool_slices || io->state_stride == 0 || io->state_stride >= io->n, TC_ERR_ARG,
-           "ionic step: state stride %lld below n %lld", (long long)io->state_stride,
-           (long long)io->n);
+  TC_CHECK(io->pool_slices || io->state_block || io->state_stride == 0 ||
+               io->state_stride >= io->n,
+           TC_ERR_ARG, "ionic step: state stride %lld below n %lld",
+           (long long)io->state_stride, (long long)io->n);
+  TC_CHECK(!io->state_block || io->lanes == 0, TC_ERR_ARG,
+           "ionic step: state blocks are not supported with lanes");
   TC_CHECK(top < ((int64_t)1 << 29), TC_ERR_SIZE,
            "ionic step: %lld per-node entries exceed the 2^29 limit", (long long)top);
   return TC_OK;

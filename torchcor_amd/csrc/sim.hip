@@ -414,6 +414,7 @@ int enqueue(tc_sim* S, HostState& ps, Plan& plan, hipStream_t s) {
   io.gate = &P->ctl->stall;
   io.stream_states = 1;
   io.state_stride = d.state_stride;
+  io.state_block = d.state_block;
   int rc = ionic_launch(S, &io, s, ev ? (*ev)[0] : nullptr, ev ? (*ev)[1] : nullptr);
   if (rc) return rc;
   // 2. RHS + PCG start; 3. speculative iterations; 4. the stopping test,
@@ -625,7 +626,9 @@ extern "C" int tc_sim_create(tc_sim** out, const tc_sim_desc* d) {
            "tc_sim_create: need two x buffers (and two x_prev buffers with the linear guess)");
   TC_CHECK(d->n_stim >= 0 && d->n_stim <= TC_MAX_STIMULI, TC_ERR_ARG, "bad stimulus count");
   TC_CHECK(d->n_stim == 0 || d->stim_mask, TC_ERR_ARG, "stimuli need a mask");
-  TC_CHECK(d->state_stride == 0 || d->state_stride >= d->n, TC_ERR_ARG,
+  TC_CHECK(d->state_block == 0 || d->state_block == 1, TC_ERR_ARG, "state_block %d not 0 or 1",
+           d->state_block);
+  TC_CHECK(d->state_block || d->state_stride == 0 || d->state_stride >= d->n, TC_ERR_ARG,
            "tc_sim_create: state stride below n");
   TC_CHECK(d->k_history >= 0 && d->k_history <= kHistoryMax, TC_ERR_ARG,
            "k_history %d outside 0..%d", d->k_history, kHistoryMax);

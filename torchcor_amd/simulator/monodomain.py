@@ -46,6 +46,12 @@ VERBOSE_TIMING_PERIOD = 16
 
 
 class Monodomain:
+    #: the time-step runtime keeps the ionic states in blocks of 64 nodes
+    #: (all of a node block's states in one contiguous span, tc_sim_desc
+    #: state_block); the model's (S, n) states are copied in at start and
+    #: back at finish.  Same results either way.
+    block_states = True
+
     def __init__(self, ionic_model, T, dt, device=None, dtype=None, reorder="auto"):
         self.device = torch.device(device) if device is not None else default_device()
         self.dtype = dtype if dtype is not None else torch.float32
@@ -71,6 +77,7 @@ class Monodomain:
         self.node_order = None   # internal position i holds mesh node node_order[i]
         self.node_map = None     # mesh node j sits at internal position node_map[j]
         self.row_window = 0      # rows sorted by length in windows of this many (0: not)
+        self._state_blocks = None
 
     # ------------------------------------------------------------ set-up
     def load_mesh(self, path="Data/ventricle/Case_1", unit_conversion=1000):
@@ -321,8 +328,20 @@ class Monodomain:
         d.model = ion.MODEL
         self._cparams = ion.c_params()
         d.params = C.cast(C.pointer(self._cparams), C.c_void_p)
-        d.states = L.ptr(ion.states)
-        d.state_stride = ion.states.stride(0)
+        self._state_blocks = None
+        if self.block_states:
+            # (S, n) rows -> (ceil(n/64), S, 64) blocks
+            S, n = ion.states.shape
+            nb = -(-n // 64)
+            rows = torch.zeros((S, nb * 64), dtype=ion.states.dtype, device=ion.states.device)
+            rows[:, :n] = ion.states
+            self._state_blocks = rows.view(S, nb, 64).permute(1, 0, 2).contiguous()
+            d.states = L.ptr(self._state_blocks)
+            d.state_stride = 0
+            d.state_block = 1
+        else:
+            d.states = L.ptr(ion.states)
+            d.state_stride = ion.states.stride(0)
         for k, tab in enumerate(ion.c_tables()):
             d.tab[k] = L.ptr(tab)
         d.pcg = self.system._pcg
@@ -539,6 +558,11 @@ class Monodomain:
     def finish(self):
         R = self._run
         L.call("tc_sim_finish", self._sim, L.stream_handle(self.device))
+        if self._state_blocks is not None:
+            # the runtime's state blocks back into the model's (S, n) rows
+            st = self.ionic_model.states
+            S, n = st.shape
+            st.copy_(self._state_blocks.permute(1, 0, 2).reshape(S, -1)[:, :n])
         torch.cuda.synchronize(self.device)
         t, steps, xi, nfr = self.sim_status()
         self.solving_time = time.time() - R["t0"]
