@@ -382,20 +382,22 @@ def copy_ceiling(nbytes, reps=10):
     """GB/s of a plain device copy that reads nbytes/2 and writes nbytes/2
     (the library's tc_stream_copy: one 16-byte load and store per thread),
     with the 256 MB last-level cache evicted before each timed copy by a
-    1 GiB copy: what HBM delivers to a kernel that moves the same bytes
-    once.  Median of `reps` copies timed with events on the copy's stream."""
+    1 GiB read (a sum): what HBM delivers to a kernel that moves the same
+    bytes once.  The eviction only reads, so the copy does not also pay for
+    writing back dirty lines it displaces (an eviction by a 1 GiB copy
+    leaves that much dirty data and cuts the copy's rate by about a fifth).
+    Median of `reps` copies timed with events on the copy's stream."""
     import torch
     from torchcor_amd import _lib as L
     half = int(nbytes // 2) // 16 * 16
     a = torch.ones(half // 4, device="cuda:0")
     b = torch.empty_like(a)
     fa = torch.ones(1 << 28, device="cuda:0")
-    fb = torch.empty_like(fa)
     s = L.stream_handle()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ts = []
     for r in range(reps + 2):
-        L.call("tc_stream_copy", fa.data_ptr(), fb.data_ptr(), fa.numel() * 4, s)
+        fa.sum()                      # clean eviction: reads 1 GiB, writes a scalar
         e0.record()
         L.call("tc_stream_copy", a.data_ptr(), b.data_ptr(), half, s)
         e1.record()
@@ -404,7 +406,7 @@ def copy_ceiling(nbytes, reps=10):
             ts.append(e0.elapsed_time(e1) / 1e3)
     ts.sort()
     t = ts[len(ts) // 2]
-    del a, b, fa, fb
+    del a, b, fa
     torch.cuda.empty_cache()
     return round(2 * half / t / 1e9, 1)
 

@@ -637,7 +637,8 @@ typedef struct tc_serve_desc {
   void* AT;                  /* (rows) dtype, or NULL */
   void* RT;
   uint32_t* stim_mask;       /* (rows) bits into the row's system stimuli */
-  void* states[3];           /* per tc_model: (S, rows) dtype, NULL if unused */
+  void* states[3];           /* per tc_model: (cap_slices, S, 64) dtype (each slice's
+                                S states of its 64 rows together), NULL if unused */
   const void* params[3];     /* per tc_model: tc_*_params */
   const void* tab[3][3];     /* per tc_model: tables */
   double dt, dt_ionic, Cm, theta, a_tol, r_tol;

@@ -781,10 +781,14 @@ __global__ void k_sv_place_rows(int64_t row0, int64_t n, int64_t nrows, const T*
 
 template <class T>
 __global__ void k_sv_place_states(int64_t row0, int64_t n, int64_t nrows, int S,
-                                  const T* __restrict__ st_s, T* st, int64_t ss) {
+                                  const T* __restrict__ st_s, T* st) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nrows) return;
-  for (int k = 0; k < S; ++k) st[k * ss + row0 + q] = q < n ? st_s[k * n + q] : T(0);
+  // the pool's states lie in blocks of one slice: row's state k at
+  // (row / 64) * S * 64 + k * 64 + row % 64
+  const int64_t row = row0 + q;
+  T* b = st + (row >> 6) * S * 64 + (row & 63);
+  for (int k = 0; k < S; ++k) b[k * 64] = q < n ? st_s[k * n + q] : T(0);
 }
 
 template <class T>
@@ -1084,7 +1088,8 @@ tc_ionic_io sv_ionic_io(tc_serve* S, int m) {
   io.pool_sys_steps = S->hctl.steps;
   io.pool_sys_bits = S->hctl.bits;
   io.pool_sys_stim = S->hctl.stim;
-  io.state_stride = d.cap_slices * 64;
+  io.state_stride = d.cap_slices * 64;   // the per-row arrays' extent
+  io.state_block = 1;                     // states in blocks of one slice
   return io;
 }
 
@@ -1494,7 +1499,7 @@ extern "C" int tc_serve_add(tc_serve* S, const tc_serve_system* y, int* slot_out
                        (double*)d.p[1], (double*)d.Ap, (double*)d.AT, (double*)d.RT,
                        (uint32_t*)d.stim_mask);
     hipLaunchKernelGGL(k_sv_place_states<double>, gr, b, 0, s, row0, y->n, nrows, y->n_states,
-                       (const double*)y->states, (double*)d.states[y->model], d.cap_slices * 64);
+                       (const double*)y->states, (double*)d.states[y->model]);
   } else {
     hipLaunchKernelGGL(k_sv_place_entries<float>, ge, b, 0, s, y->n_entries, e0, row0, y->col,
                        y->dcol, (const float*)y->Mv, (const float*)y->Kv, (const float*)y->Av,
@@ -1513,7 +1518,7 @@ extern "C" int tc_serve_add(tc_serve* S, const tc_serve_system* y, int* slot_out
                          0, (float*)d.Dinv, (float*)d.r, (float*)d.p[0], (float*)d.p[1],
                          (float*)d.Ap, (float*)d.AT, (float*)d.RT, (uint32_t*)d.stim_mask);
     hipLaunchKernelGGL(k_sv_place_states<float>, gr, b, 0, s, row0, y->n, nrows, y->n_states,
-                       (const float*)y->states, (float*)d.states[y->model], d.cap_slices * 64);
+                       (const float*)y->states, (float*)d.states[y->model]);
   }
   TC_LAUNCH();
   // per-system scalars

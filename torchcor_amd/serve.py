@@ -99,7 +99,8 @@ class ServeEngine:
         for m, make in MODEL_CLASSES.items():
             obj = make(dt, dev, dtype)
             obj.construct_tables()
-            st = torch.zeros((len(obj.STATES), rows), dtype=dtype, device=dev)
+            # states in blocks of one slice: (slice, state, row in slice)
+            st = torch.zeros((cap_slices, len(obj.STATES), 64), dtype=dtype, device=dev)
             self.models[m] = (obj, obj.c_params(), obj.c_tables(), st)
         d = L.ServeDesc()
         P = self.pool
