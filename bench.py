@@ -82,10 +82,10 @@ def column_bytes(system):
 # WRITE_SIZE x1.0.  profiles/cube101_ttp_step_hbm_traffic.md holds the
 # counter summary and the commands.
 MEASURED_TRAFFIC_101 = {
-    "ionic": 173_187_072,
-    "rhs_init": 174_730_240,
-    "pcg_spmv": 82_765_824,
-    "pcg_update": 37_340_160,
+    "ionic": 173_189_344,
+    "rhs_init": 174_725_792,
+    "pcg_spmv": 82_761_248,
+    "pcg_update": 37_302_912,
 }
 
 
