@@ -579,6 +579,37 @@ def window(N, dev, warmup, steps, period, graph=False):
     return sim, time.perf_counter() - t0, sim.iterations[it0:], t_setup
 
 
+def size_sweep(sides, warmup, steps):
+    """node-timesteps/s of the TTP cube workload (float32, the bench's
+    stimulus and conductivities) over cube sides: the size dependence the
+    reference's own volume sweep measures (benchmark/torchcor/volume.py).
+    Kernel figures from HIP events on every 16th timed step."""
+    import torch
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    for N in sides:
+        torch.cuda.reset_peak_memory_stats(dev)
+        sim, el, its, t_setup = window(N, dev, warmup, steps, 16 if steps >= 256 else 1)
+        n = sim.n_nodes
+        kt = {k: round(ms / c * 1e3, 2) for k, (ms, c) in sim.kernel_times().items() if c}
+        S = len(sim.ionic_model.STATES)
+        ion_bytes = n * (8 + 8 * S + 8 + 8)
+        line = {"size_sweep": True, "side": N, "n_nodes": n, "nnz": sim.system.nnz,
+                "steps": steps, "warmup": warmup, "t_window_ms": [warmup * 0.01, (warmup + steps) * 0.01],
+                "ms_per_step": round(el / steps * 1e3, 4),
+                "node_timesteps_per_s": n * steps / el,
+                "pcg_iterations_per_step": sum(its) / steps + 1,
+                "kernel_avg_us": kt,
+                "ionic_GBps": round(ion_bytes / (kt["ionic"] * 1e-6) / 1e9, 1) if kt.get("ionic") else None,
+                "setup_s": round(t_setup, 2),
+                "peak_device_GB": round(torch.cuda.max_memory_allocated(dev) / 1e9, 2)}
+        print(json.dumps(line), flush=True)
+        sim.finish()
+        del sim
+        gc.collect()
+        torch.cuda.empty_cache()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -611,6 +642,10 @@ def main():
     ap.add_argument("--trace-summary", default="",
                     help="only summarise the last --steps steps of each pass in this "
                          "rocprofv3 kernel_trace.csv (no GPU needed) and exit")
+    ap.add_argument("--size-sweep", default="",
+                    help="only time the TTP cube at these comma-separated sides (e.g. "
+                         "21,41,101,201), --warmup then --steps steps each, one JSON line "
+                         "per size, and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -621,6 +656,9 @@ def main():
         return
     if args.trace_summary:
         trace_summary(args.trace_summary, args.steps)
+        return
+    if args.size_sweep:
+        size_sweep([int(v) for v in args.size_sweep.split(",")], args.warmup, args.steps)
         return
 
     import torch
