@@ -119,6 +119,25 @@ def test_auto_reorder_choice():
     assert c.node_order is None            # the cube's own numbering is already banded
 
 
+def test_auto_order_without_slice_sort_when_sets_do_not_fit(monkeypatch):
+    """When the staged split's column sets would not fit a stage, "auto"
+    keeps the plain level order (the per-entry gathers would be slowed by
+    the row grouping)."""
+    _say("[gpu] reorder auto, stage limit below the sets")
+    import torchcor_amd._lib as L
+    from torchcor_amd.core.reorder import element_pattern, level_order
+    monkeypatch.setattr(L, "STAGE_MAX_COLS", 64)
+    mesh = _shell()
+    s = _sim(mesh, "ms2v", torch.float64, 1.0, "auto")
+    s.assemble()
+    n = mesh[0].shape[0]
+    rp, col = element_pattern(mesh[1], n)
+    lo, _, bw1 = level_order(rp, col, n, np.asarray(mesh[0], dtype=np.float64))
+    assert s.row_window == 0
+    assert np.array_equal(s.node_order.cpu().numpy(), lo)
+    assert s.bandwidth_reordered == bw1 == s.system.bandwidth
+
+
 @pytest.mark.parametrize("model,tol,order", [("ttp", 1e-6, "levels"), ("ms2v", 0.5, "levels"),
                                              ("ttp", 1e-6, "rcm"), ("ttp", 1e-6, "auto")])
 def test_shell_reordered_matches_mesh_order_float64(model, tol, order):

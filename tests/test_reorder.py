@@ -27,8 +27,9 @@ from scipy.sparse.csgraph import reverse_cuthill_mckee
 torch = pytest.importorskip("torch")
 
 from torchcor_amd import meshes  # noqa: E402
-from torchcor_amd.core.reorder import (RCM, element_pattern, level_order, pick_order,  # noqa: E402
-                                       rcm_order, slice_sort)
+from torchcor_amd.core.reorder import (RCM, element_pattern, level_order,  # noqa: E402
+                                       order_bandwidth, pick_order, rcm_order, slice_sort,
+                                       window_columns)
 
 
 def _csr(n, edges):
@@ -301,3 +302,24 @@ def test_pick_order_modes():
     assert pick_order(*args, "levels") is not None
     with pytest.raises(ValueError):
         pick_order(rpt, colt, n, nodes, "bfs")
+
+
+def test_window_columns_and_bandwidth_by_brute_force():
+    nodes, elems = meshes.jittered_delaunay(12, seed=8)
+    n = nodes.shape[0]
+    rp, col = element_pattern(elems, n)
+    lo, _, b1 = level_order(rp, col, n, nodes)
+    rpt, colt = torch.as_tensor(rp), torch.as_tensor(col)
+    pos = np.empty(n, dtype=np.int64)
+    pos[lo] = np.arange(n)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    for window in (1, 64, 300, n):
+        best = 0
+        for w0 in range(0, n, window):
+            sel = (pos[rows] >= w0) & (pos[rows] < w0 + window)
+            best = max(best, len(set(pos[col[sel]].tolist())))
+        assert window_columns(rpt, colt, lo, window) == best
+        # the slice sort keeps every window's node set
+        assert window_columns(rpt, colt, slice_sort(lo, rp, window), window) == best
+    assert order_bandwidth(rpt, colt, lo) == b1 == _band_profile(rp, col, lo)[0]
+    assert order_bandwidth(rpt, colt, np.arange(n)) == _band_profile(rp, col, np.arange(n))[0]

@@ -45,6 +45,7 @@ class Stimulus(C.Structure):
 
 
 POOL_STIM = 4
+STAGE_MAX_COLS = 4096          # TC_STAGE_MAX_COLS: columns of one block's staged set
 
 
 class IonicIO(C.Structure):

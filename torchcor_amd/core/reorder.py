@@ -111,10 +111,40 @@ def pick_order(row_ptr, col, n, xyz, mode, window=0):
         return None
     if window:
         order = slice_sort(order, row_ptr, window)
-        pos = torch.empty(n, dtype=torch.int64, device=rp.device)
-        pos[torch.as_tensor(order, device=rp.device)] = torch.arange(n, device=rp.device)
-        bw1 = int((pos[cl] - pos[rows]).abs().max().item()) if cl.numel() else 0
+        bw1 = order_bandwidth(row_ptr, col, order)
     return order, bw0, bw1
+
+
+def order_bandwidth(row_ptr, col, order):
+    """Bandwidth (largest |new row - new column|) of the pattern in ``order``
+    (position -> old node index); tensors, computed on their device."""
+    rp = torch.as_tensor(row_ptr)
+    dev = rp.device
+    n = rp.shape[0] - 1
+    cl = torch.as_tensor(col, device=dev).to(torch.int64)
+    if cl.numel() == 0:
+        return 0
+    pos = torch.empty(n, dtype=torch.int64, device=dev)
+    pos[torch.as_tensor(order, device=dev)] = torch.arange(n, device=dev)
+    rows = torch.repeat_interleave(torch.arange(n, device=dev), rp.diff().to(torch.int64))
+    return int((pos[cl] - pos[rows]).abs().max().item())
+
+
+def window_columns(row_ptr, col, order, window):
+    """The most distinct columns that the rows of one window of ``window``
+    consecutive positions of ``order`` reference (the pattern in the old
+    numbering; tensors, computed on their device).  A slice sort inside
+    those windows leaves each window's node set, and so this count, as it
+    is; a staged split over the same windows needs a set this large."""
+    rp = torch.as_tensor(row_ptr)
+    dev = rp.device
+    n = rp.shape[0] - 1
+    pos = torch.empty(n, dtype=torch.int64, device=dev)
+    pos[torch.as_tensor(order, device=dev)] = torch.arange(n, device=dev)
+    rows = torch.repeat_interleave(torch.arange(n, device=dev), rp.diff().to(torch.int64))
+    w = pos[rows] // int(window)
+    key = torch.unique(w * n + pos[torch.as_tensor(col, device=dev).to(torch.int64)])
+    return int(torch.bincount(key // n).max().item()) if key.numel() else 0
 
 
 def element_pattern(elems, n):

@@ -32,7 +32,7 @@ import torch
 from .. import _lib as L
 from ..core.conductivity import Conductivity
 from ..core.mesh import MeshReader
-from ..core.reorder import pick_order
+from ..core.reorder import order_bandwidth, pick_order, slice_sort, window_columns
 from ..core.stimulation import Stimuli
 from ..core.system import P1System
 from ..ionic.base import default_device
@@ -113,13 +113,19 @@ class Monodomain:
         self.node_order = self.node_map = None
         self.row_window = 0
         n = sys_.n
-        window = P1System.sort_window(n) if (self.reorder == "auto"
-                                             and P1System.stage_gathers) else 0
-        res = pick_order(sys_.row_ptr, sys_.col, n, self.nodes, self.reorder, window)
+        res = pick_order(sys_.row_ptr, sys_.col, n, self.nodes, self.reorder)
         if res is None:
             return None
         order, self.bandwidth_natural, self.bandwidth_reordered = res
-        self.row_window = window
+        if self.reorder == "auto" and P1System.stage_gathers:
+            # rows grouped by length inside the staged split's windows, when
+            # that split's column sets fit a stage (else the passes gather per
+            # entry, which the grouping would slow)
+            window = P1System.sort_window(n)
+            if window_columns(sys_.row_ptr, sys_.col, order, window) <= L.STAGE_MAX_COLS:
+                order = slice_sort(order, sys_.row_ptr, window)
+                self.bandwidth_reordered = order_bandwidth(sys_.row_ptr, sys_.col, order)
+                self.row_window = window
         dev = self.device
         self.node_order = torch.as_tensor(order, dtype=torch.long, device=dev)
         self.node_map = torch.empty_like(self.node_order)
