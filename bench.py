@@ -288,6 +288,8 @@ def irregular_config(dev, steps, warmup, n_target=640_000, orders=(None, "rcm", 
                     "pcg_blocks": int(sim.system.pcg_blocks),
                     "staged_gathers": bool(sim.system.staged),
                     "row_window": int(sim.row_window),
+                    "stage_chunk": int(getattr(sim.system, "stage_chunk", 0))
+                    if sim.system.staged else None,
                     "stage_max_cols": int(getattr(sim.system, "stage_max_cols", 0))
                     if sim.system.staged else None,
                     "kernel_avg_us": kt,

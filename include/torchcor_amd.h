@@ -255,22 +255,32 @@ int tc_pcg_set_blocks(tc_pcg* plan, int blocks);
  * counterpart otherwise. */
 typedef struct tc_stage {
   int blocks;               /* the plan's block count the stage was built for */
-  int max_cols;             /* largest set of one block (<= TC_STAGE_MAX_COLS) */
-  const int32_t* col_ptr;   /* (blocks+1) device */
-  const int32_t* own;       /* (blocks) position of the block's first row in its set */
-  const int32_t* cols;      /* (col_ptr[blocks]) device */
+  int max_cols;             /* largest set of one unit (<= TC_STAGE_MAX_COLS) */
+  const int32_t* col_ptr;   /* (units+1) device */
+  const int32_t* own;       /* (units) position of the unit's first row in its set */
+  const int32_t* cols;      /* (col_ptr[units]) device */
   const uint16_t* lidx;     /* (padded entries) device */
+  /* Slices per unit: 0 = one unit per block (its whole slice range); else a
+   * multiple of 8, at most 16: block v's slice range (per = ceil(n_slices /
+   * blocks) slices from v * per) is staged in consecutive units of `chunk`
+   * slices, one after the other (units = blocks * ceil(per / chunk), unit
+   * v * ceil(per / chunk) + c for block v's c-th), so blocks holding more
+   * slices than one set can stage still stage. */
+  int chunk;
 } tc_stage;
 #define TC_STAGE_MAX_COLS 4096
 /* Host: the stage of a SELL pattern (slice_ptr, col: host copies, per-row
- * int32 columns) split over `blocks` blocks.  cols_host needs room for
- * slice_ptr[n_slices] entries (an upper bound); col_ptr_host (blocks+1),
- * own_host (blocks), lidx_host (slice_ptr[n_slices]).  *max_cols_host
- * receives the largest set; TC_ERR_SIZE when it exceeds TC_STAGE_MAX_COLS
- * (the pattern is then left unstaged). */
+ * int32 columns) split over `blocks` blocks, each staged in units of
+ * `chunk` slices (0: one unit per block; as tc_stage.chunk).  cols_host
+ * needs room for slice_ptr[n_slices] entries (an upper bound);
+ * col_ptr_host (units+1), own_host (units), lidx_host
+ * (slice_ptr[n_slices]).  *max_cols_host receives the largest set;
+ * TC_ERR_SIZE when it exceeds TC_STAGE_MAX_COLS (the pattern is then left
+ * unstaged). */
 int tc_sell_stage(int64_t n, int64_t n_slices, const int32_t* slice_ptr_host,
-                  const int32_t* col_host, int blocks, int32_t* col_ptr_host, int32_t* own_host,
-                  int32_t* cols_host, uint16_t* lidx_host, int32_t* max_cols_host);
+                  const int32_t* col_host, int blocks, int chunk, int32_t* col_ptr_host,
+                  int32_t* own_host, int32_t* cols_host, uint16_t* lidx_host,
+                  int32_t* max_cols_host);
 /* Attach a stage (device arrays owned by the caller; NULL detaches).  The
  * spmv passes of a time-step runtime on the plan (tc_sim_*) use it while
  * the plan's block count equals stage->blocks and its slices are not

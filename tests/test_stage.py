@@ -41,16 +41,19 @@ def _sell(n, rng, spread):
     return np.array(sp, np.int32), np.concatenate(cols).astype(np.int32)
 
 
-def _stage(n, sp, cols, G):
+def _stage(n, sp, cols, G, chunk=0):
     import torchcor_amd._lib as L
     ns = sp.shape[0] - 1
-    col_ptr = np.empty(G + 1, np.int32)
-    own = np.empty(G, np.int32)
+    per = -(-ns // G)
+    units = G * (-(-per // chunk) if chunk else 1)
+    col_ptr = np.empty(units + 1, np.int32)
+    own = np.empty(units, np.int32)
     out = np.empty(cols.shape[0], np.int32)
     lidx = np.empty(cols.shape[0], np.uint16)
     mx = C.c_int32(0)
-    rc = L.lib.tc_sell_stage(n, ns, sp.ctypes.data, cols.ctypes.data, G, col_ptr.ctypes.data,
-                             own.ctypes.data, out.ctypes.data, lidx.ctypes.data, C.byref(mx))
+    rc = L.lib.tc_sell_stage(n, ns, sp.ctypes.data, cols.ctypes.data, G, chunk,
+                             col_ptr.ctypes.data, own.ctypes.data, out.ctypes.data,
+                             lidx.ctypes.data, C.byref(mx))
     return rc, col_ptr, own, out, lidx, mx.value
 
 
@@ -96,3 +99,45 @@ def test_stage_rejects_bad_arguments():
     cols = np.full(64, 99, np.int32)   # column out of range for n = 10
     rc, *_ = _stage(10, sp, cols, 1)
     assert rc == L.TC_ERR_ARG
+
+
+@pytest.mark.parametrize("n,G,chunk,spread", [(5000, 3, 8, 300), (64 * 41 + 9, 2, 16, 60),
+                                              (3000, 5, 16, 100)])
+def test_stage_units_match_definition(n, G, chunk, spread):
+    """Blocks staged in units of `chunk` slices: unit c of block v covers
+    slices v*per + c*chunk .. (clipped to the block), its set the distinct
+    columns of those slices' entries."""
+    import torchcor_amd._lib as L
+    rng = np.random.default_rng(n + G + chunk)
+    sp, cols = _sell(n, rng, spread)
+    rc, col_ptr, own, out, lidx, mx = _stage(n, sp, cols, G, chunk)
+    assert rc == 0, L.lib.tc_last_error()
+    ns = sp.shape[0] - 1
+    per = -(-ns // G)
+    upb = -(-per // chunk)
+    biggest, covered = 0, 0
+    for v in range(G):
+        b0, b1 = v * per, min(ns, (v + 1) * per)
+        for c in range(upb):
+            u = v * upb + c
+            s0, s1 = b0 + c * chunk, min(b0 + (c + 1) * chunk, b1)
+            got = out[col_ptr[u]:col_ptr[u + 1]]
+            if s0 >= s1:
+                assert got.size == 0
+                continue
+            covered += s1 - s0
+            ent = cols[sp[s0]:sp[s1]]
+            want = np.unique(ent)
+            assert np.array_equal(got, want), (v, c)
+            assert want[own[u]] == s0 * 64
+            assert np.array_equal(want[lidx[sp[s0]:sp[s1]]], ent)
+            biggest = max(biggest, want.size)
+    assert covered == ns and mx == biggest
+
+
+def test_stage_rejects_bad_chunk():
+    import torchcor_amd._lib as L
+    rng = np.random.default_rng(3)
+    sp, cols = _sell(500, rng, 20)
+    assert _stage(500, sp, cols, 2, 12)[0] == L.TC_ERR_ARG
+    assert _stage(500, sp, cols, 2, 24)[0] == L.TC_ERR_ARG

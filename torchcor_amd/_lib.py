@@ -37,7 +37,8 @@ class Sell(C.Structure):
 
 class Stage(C.Structure):
     _fields_ = [("blocks", C.c_int), ("max_cols", C.c_int), ("col_ptr", C.c_void_p),
-                ("own", C.c_void_p), ("cols", C.c_void_p), ("lidx", C.c_void_p)]
+                ("own", C.c_void_p), ("cols", C.c_void_p), ("lidx", C.c_void_p),
+                ("chunk", C.c_int)]
 
 
 class Stimulus(C.Structure):
@@ -184,7 +185,7 @@ FUNCS = {
     "tc_pcg_solve": (p, p, p, p, p, dbl, dbl, i32, i32, INT_P, p),
     "tc_pcg_set_blocks": (p, i32),
     "tc_pcg_set_stage": (p, C.POINTER(Stage)),
-    "tc_sell_stage": (i64, i64, p, p, i32, p, p, p, p, C.POINTER(C.c_int32)),
+    "tc_sell_stage": (i64, i64, p, p, i32, i32, p, p, p, p, C.POINTER(C.c_int32)),
     "tc_pcg_set_timing": (p, i32),
     "tc_pcg_get_timing": (p, C.POINTER(dbl), I64_P),
     "tc_pcg_solve_static": (SELL_P, p, p, i32, p, p, p, p, dbl, dbl, i32, i32, INT_P, p),

@@ -30,14 +30,40 @@ class P1System:
     #: runtime's spmv pass (tc_sell_stage) when every block's column set fits
     stage_gathers = True
 
+    #: force the staged split (blocks, slices per unit) instead of choosing
+    #: it (None); for tests of the unit loop on small meshes
+    stage_override = None
+
     @staticmethod
-    def sort_window(n):
-        """Rows of one block of the staged split of n rows (8 slices a block,
-        at most 1024 blocks): the window in which ``slice_sort`` groups rows
-        by length, so that each block keeps its own nodes."""
+    def stage_split(n):
+        """(blocks, chunk) of the staged split for n rows: 8 slices a block
+        while that needs at most 1024 blocks (chunk 0: each block stages its
+        whole range at once); beyond, at most 1024 blocks (a multiple of 8)
+        of a multiple of 8 slices each, staged in units of 8 slices (sets of
+        512 rows, which keep the RHS pass's x, w and x_prev within one
+        block's LDS)."""
         from .. import autotune
         ns = -(-int(n) // 64)
-        return -(-ns // autotune.pcg_blocks(n, 8)) * 64
+        g8 = autotune.pcg_blocks(n, 8)
+        if -(-ns // g8) <= 16:
+            return g8, 0
+        per = 8 * -(-ns // (8 * 1024))
+        G = -(-ns // per)
+        up = -(-G // 8) * 8
+        if -(-ns // up) == per:
+            G = up
+        if -(-ns // G) != per:
+            return g8, 0
+        return G, 8
+
+    @staticmethod
+    def sort_window(n):
+        """Rows of one staged unit of the staged split of n rows: the window
+        in which ``slice_sort`` groups rows by length, so that each unit
+        keeps its own nodes."""
+        G, chunk = P1System.stage_split(n)
+        ns = -(-int(n) // 64)
+        return (chunk or -(-ns // G)) * 64
 
     def __init__(self, nodes, elems, device=None):
         device = torch.device(device) if device is not None else elems.device
@@ -176,8 +202,10 @@ class P1System:
         split references more than TC_STAGE_MAX_COLS columns, the split of
         8 slices per block (more, smaller blocks) is tried and, if it fits,
         becomes the plan's; rows sorted by length inside that split's block
-        windows (``row_window``) try it first.  False when the operator has
-        shared slices or no split fits (the plan then gathers per entry)."""
+        windows (``row_window``) try it first.  Meshes whose blocks hold more
+        than 16 slices even at 1024 blocks stage each block's range in units
+        of 8 slices (``stage_split``).  False when the operator has shared
+        slices or no split fits (the plan then gathers per entry)."""
         import numpy as np
         from .. import autotune
         if self.slice_tag is not None:
@@ -186,16 +214,23 @@ class P1System:
         sp = self.slice_ptr.cpu().numpy()
         cl = self.scol.cpu().numpy()
         col_ptr = own = None
-        cands = [int(self.pcg_blocks), autotune.pcg_blocks(self.n, 8)]
+        split = self.stage_split(self.n)
+        cands = [(int(self.pcg_blocks), 0), (autotune.pcg_blocks(self.n, 8), 0)]
         if self.row_window and self.row_window == self.sort_window(self.n):
-            cands.insert(0, autotune.pcg_blocks(self.n, 8))
-        for G in dict.fromkeys(cands):
-            col_ptr = np.empty(G + 1, np.int32)
-            own = np.empty(G, np.int32)
+            cands.insert(0, split)
+        if split[1]:
+            cands.append(split)
+        if self.stage_override is not None:
+            cands = [tuple(self.stage_override)]
+        for G, chunk in dict.fromkeys(cands):
+            per = -(-ns // G)
+            units = G * (-(-per // chunk) if chunk else 1)
+            col_ptr = np.empty(units + 1, np.int32)
+            own = np.empty(units, np.int32)
             cols = np.empty(self.padded, np.int32)
             lidx = np.empty(self.padded, np.uint16)
             mx = C.c_int32(0)
-            rc = L.lib.tc_sell_stage(self.n, ns, sp.ctypes.data, cl.ctypes.data, G,
+            rc = L.lib.tc_sell_stage(self.n, ns, sp.ctypes.data, cl.ctypes.data, G, chunk,
                                      col_ptr.ctypes.data, own.ctypes.data, cols.ctypes.data,
                                      lidx.ctypes.data, C.byref(mx))
             if rc != L.TC_ERR_SIZE:
@@ -209,12 +244,13 @@ class P1System:
         dev = self.device
         self.st_col_ptr = torch.from_numpy(col_ptr).to(dev)
         self.st_own = torch.from_numpy(own).to(dev)
-        self.st_cols = torch.from_numpy(cols[:int(col_ptr[G])].copy()).to(dev)
+        self.st_cols = torch.from_numpy(cols[:int(col_ptr[units])].copy()).to(dev)
         self.st_lidx = torch.from_numpy(lidx.view(np.int16)).to(dev)
         st = L.Stage(G, int(mx.value), L.ptr(self.st_col_ptr), L.ptr(self.st_own),
-                     L.ptr(self.st_cols), L.ptr(self.st_lidx))
+                     L.ptr(self.st_cols), L.ptr(self.st_lidx), chunk)
         L.call("tc_pcg_set_stage", plan, C.byref(st))
         self.stage_max_cols = int(mx.value)
+        self.stage_chunk = chunk
         return True
 
     def _tune_plan(self, plan):

@@ -257,31 +257,42 @@ extern "C" int tc_pcg_set_stage(tc_pcg* P, const tc_stage* st) {
     return TC_OK;
   }
   TC_CHECK(st->col_ptr && st->own && st->cols && st->lidx && st->blocks >= 1 &&
-               st->max_cols >= 1 && st->max_cols <= TC_STAGE_MAX_COLS,
-           TC_ERR_ARG, "tc_pcg_set_stage: bad stage (max_cols 1..%d)", TC_STAGE_MAX_COLS);
+               st->max_cols >= 1 && st->max_cols <= TC_STAGE_MAX_COLS &&
+               (st->chunk == 0 || st->chunk == 8 || st->chunk == 16),
+           TC_ERR_ARG, "tc_pcg_set_stage: bad stage (max_cols 1..%d, chunk 0, 8 or 16)",
+           TC_STAGE_MAX_COLS);
   P->stage = *st;
   ++P->version;
   return TC_OK;
 }
 
-// Host: the distinct columns of each block's rows (ascending), each row's
-// first-row position and every entry's position in its block's set.  A
-// block's columns lie in a window [lo, hi] of the row range; a marker array
+// Host: the distinct columns of each unit's rows (ascending; a unit is a
+// block's slice range, or a run of `chunk` slices of it), each unit's
+// first-row position and every entry's position in its unit's set.  A
+// unit's columns lie in a window [lo, hi] of the row range; a marker array
 // over that window finds the set in one pass over the entries and one over
 // the window.
 extern "C" int tc_sell_stage(int64_t n, int64_t ns, const int32_t* sptr, const int32_t* col,
-                             int G, int32_t* col_ptr, int32_t* own, int32_t* cols,
+                             int G, int chunk, int32_t* col_ptr, int32_t* own, int32_t* cols,
                              uint16_t* lidx, int32_t* max_cols) {
   TC_CHECK(n > 0 && ns == (n + 63) / 64 && sptr && col && G >= 1 && col_ptr && own && cols &&
                lidx && max_cols,
            TC_ERR_ARG, "tc_sell_stage: bad arguments");
+  TC_CHECK(chunk == 0 || (chunk % 8 == 0 && chunk <= 16), TC_ERR_ARG,
+           "tc_sell_stage: chunk %d is not 0, 8 or 16", chunk);
   const int64_t per = (ns + G - 1) / G;
+  // units of `chunk` slices inside each block's range (one unit per block
+  // without chunking)
+  const int64_t upb = chunk ? (per + chunk - 1) / chunk : 1;
+  const int64_t ulen = chunk ? chunk : per;
   std::vector<int32_t> pos;
   int64_t total = 0;
   int32_t mx = 0;
-  for (int v = 0; v < G; ++v) {
+  for (int64_t v = 0; v < (int64_t)G * upb; ++v) {
     col_ptr[v] = (int32_t)total;
-    const int64_t s0 = (int64_t)v * per, s1 = s0 + per < ns ? s0 + per : ns;
+    const int64_t b0 = v / upb * per, b1 = b0 + per < ns ? b0 + per : ns;
+    const int64_t s0 = b0 + v % upb * ulen;
+    const int64_t s1 = s0 + ulen < b1 ? s0 + ulen : b1;
     own[v] = 0;
     if (s0 >= s1) continue;
     const int64_t e0 = sptr[s0], e1 = sptr[s1];
@@ -315,7 +326,7 @@ extern "C" int tc_sell_stage(int64_t n, int64_t ns, const int32_t* sptr, const i
       }
     const int32_t r0 = (int32_t)(s0 * 64);
     TC_CHECK(r0 >= lo && r0 <= hi && pos[r0 - lo] >= 0, TC_ERR_ARG,
-             "tc_sell_stage: block %d does not reference its first row", v);
+             "tc_sell_stage: unit %lld does not reference its first row", (long long)v);
     own[v] = pos[r0 - lo];
     if (cnt > mx) mx = cnt;
     if (cnt > TC_STAGE_MAX_COLS) break;  // refused below
@@ -323,7 +334,7 @@ extern "C" int tc_sell_stage(int64_t n, int64_t ns, const int32_t* sptr, const i
       for (int64_t e = e0; e < e1; ++e) lidx[e] = (uint16_t)pos[col[e] - lo];
     total += cnt;
   }
-  col_ptr[G] = (int32_t)total;
+  col_ptr[(int64_t)G * upb] = (int32_t)total;
   *max_cols = mx;
   TC_CHECK(mx <= TC_STAGE_MAX_COLS, TC_ERR_SIZE,
            "tc_sell_stage: a block references %d columns (at most %d)", mx, TC_STAGE_MAX_COLS);

@@ -558,11 +558,11 @@ k_pcg_spmv(int64_t n, const int32_t* __restrict__ sptr, const CI* __restrict__ s
 // r, p gathers are in flight together.
 constexpr int kStageR = (TC_STAGE_MAX_COLS + kPcgBlock - 1) / kPcgBlock;
 
-template <class T>
+template <class T, bool UNITS>
 __global__ void __launch_bounds__(kPcgBlock)
 k_pcg_spmv_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* __restrict__ lidx,
                   const int32_t* __restrict__ st_ptr, const int32_t* __restrict__ st_own,
-                  const int32_t* __restrict__ st_col, const T* __restrict__ Av,
+                  const int32_t* __restrict__ st_col, int chunk, const T* __restrict__ Av,
                   const T* __restrict__ Dinv, const T* __restrict__ r,
                   const T* __restrict__ pin, T* __restrict__ pout, T* __restrict__ Ap, int k,
                   PcgCtl* ctl, const double* __restrict__ part0, const double* __restrict__ part2,
@@ -573,26 +573,37 @@ k_pcg_spmv_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* _
   const Split sp = block_slices(ns);
   const int vb = block_vindex();
   const int lane = threadIdx.x & 63, wv = wave_id();
-  const int32_t u0 = st_ptr[vb], nu = st_ptr[vb + 1] - u0;
+  const int G = gridDim.x;
+  // the block's units (runs of `chunk` slices, or its whole range), staged
+  // one after the other
+  // (UNITS false: one unit per block, the chunk argument unused)
+  const int64_t per = (ns + G - 1) / G;
+  const int upb = UNITS ? (int)((per + chunk - 1) / chunk) : 1;
+  const int64_t ulen = UNITS ? chunk : per;
   const int halted_early = ctl->converged | ctl->stall;
   T dv[kStageR], rv[kStageR], qv[kStageR];
-  if (!halted_early && nu > 0) {
-    int32_t cv[kStageR];
+  {
+    // the set of unit 0: Dinv, r and the previous p of its columns
+    // (registers), in flight during the fold
+    const int64_t ui = (int64_t)vb * upb;
+    const int32_t u0 = st_ptr[ui], nu = st_ptr[ui + 1] - u0;
+    if (!halted_early && nu > 0) {
+      int32_t cv[kStageR];
 #pragma unroll
-    for (int q = 0; q < kStageR; ++q) {
-      const int t = (int)threadIdx.x + q * kPcgBlock;
-      cv[q] = st_col[u0 + (t < nu ? t : nu - 1)];
-    }
+      for (int q = 0; q < kStageR; ++q) {
+        const int t = (int)threadIdx.x + q * kPcgBlock;
+        cv[q] = st_col[u0 + (t < nu ? t : nu - 1)];
+      }
 #pragma unroll
-    for (int q = 0; q < kStageR; ++q) {
-      dv[q] = Dinv[cv[q]];
-      rv[q] = r[cv[q]];
-      qv[q] = k == 0 ? T(0) : pin[cv[q]];
+      for (int q = 0; q < kStageR; ++q) {
+        dv[q] = Dinv[cv[q]];
+        rv[q] = r[cv[q]];
+        qv[q] = k == 0 ? T(0) : pin[cv[q]];
+      }
     }
   }
   __shared__ int halted;
   if (threadIdx.x == 0) halted = ctl->converged | ctl->stall;
-  const int G = gridDim.x;
   const double rz_prev = ctl->rz[(k - 1) & 1];
   double v[2];
   fold_partials<2>(k == 0 ? part0 : part2, G, v);
@@ -618,46 +629,85 @@ k_pcg_spmv_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* _
     ctl->n_iter = k;
     ctl->iter = k;
   }
-#pragma unroll
-  for (int q = 0; q < kStageR; ++q) {
-    const int t = (int)threadIdx.x + q * kPcgBlock;
-    if (t < nu) {
-      if (k == 0) ps[t] = (T)((double)dv[q] * (double)rv[q]);
-      else ps[t] = (T)pdir(beta, (double)qv[q], (double)dv[q], (double)rv[q]);
-    }
-  }
-  __syncthreads();
-  const int64_t r0 = sp.s0 * 64;
-  const int32_t own = st_own[vb];
   double red[1] = {0.0};
-  for (int64_t s = sp.s0 + wv; s < sp.s1; s += kWaves) {
-    const int64_t i = s * 64 + lane;
-    const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
-    const uint16_t* li = lidx + b + lane;
-    const T* a = Av + b + lane;
-    double acc = 0.0;
-    for (int j0 = 0; j0 < wd; j0 += kChunk) {
-      uint16_t lj[kChunk];
-      T aj[kChunk];
+  // p' of unit c's set into LDS from its Dinv, r, p (d, rr, q)
+  auto form_set = [&](int32_t nu, const T (&d)[kStageR], const T (&rr)[kStageR],
+                      const T (&q)[kStageR]) {
 #pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        const int jj = j0 + u < wd ? j0 + u : wd - 1;
-        lj[u] = li[jj * 64];
-        aj[u] = a[jj * 64];
-      }
-#pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        const double t = acc + (double)aj[u] * (double)ps[lj[u]];
-        acc = j0 + u < wd ? t : acc;
+    for (int t0 = 0; t0 < kStageR; ++t0) {
+      const int t = (int)threadIdx.x + t0 * kPcgBlock;
+      if (t < nu) {
+        if (k == 0) ps[t] = (T)((double)d[t0] * (double)rr[t0]);
+        else ps[t] = (T)pdir(beta, (double)q[t0], (double)d[t0], (double)rr[t0]);
       }
     }
-    if (i < n) {
-      const T pi = ps[own + (int32_t)(i - r0)];
-      const T api = (T)acc;
-      pout[i] = pi;
-      Ap[i] = api;
-      red[0] += (double)pi * (double)api;
+  };
+  // the rows of unit c: wave w takes its slices us0 + w, us0 + w + 8, ...;
+  // with units of a multiple of 8 slices each lane's rows come in the order
+  // of the unstaged pass (s0 + w, s0 + w + 8, ...)
+  auto unit_rows = [&](int c) {
+    const int64_t ui = (int64_t)vb * upb + c;
+    const int64_t us0 = sp.s0 + c * ulen;
+    const int64_t us1 = us0 + ulen < sp.s1 ? us0 + ulen : sp.s1;
+    const int64_t r0 = us0 * 64;
+    const int32_t own = st_own[ui];
+    for (int64_t s = us0 + wv; s < us1; s += kWaves) {
+      const int64_t i = s * 64 + lane;
+      const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
+      const uint16_t* li = lidx + b + lane;
+      const T* a = Av + b + lane;
+      double acc = 0.0;
+      for (int j0 = 0; j0 < wd; j0 += kChunk) {
+        uint16_t lj[kChunk];
+        T aj[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          const int jj = j0 + u < wd ? j0 + u : wd - 1;
+          lj[u] = li[jj * 64];
+          aj[u] = a[jj * 64];
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          const double t = acc + (double)aj[u] * (double)ps[lj[u]];
+          acc = j0 + u < wd ? t : acc;
+        }
+      }
+      if (i < n) {
+        const T pi = ps[own + (int32_t)(i - r0)];
+        const T api = (T)acc;
+        pout[i] = pi;
+        Ap[i] = api;
+        red[0] += (double)pi * (double)api;
+      }
     }
+  };
+  // unit 0 from the set loaded before the fold, then the later units (each
+  // loading its own set once the previous unit's rows are done)
+  form_set(st_ptr[(int64_t)vb * upb + 1] - st_ptr[(int64_t)vb * upb], dv, rv, qv);
+  __syncthreads();
+  unit_rows(0);
+  for (int c = 1; c < upb; ++c) {
+    __syncthreads();
+    T d2[kStageR], r2[kStageR], q2[kStageR];
+    const int64_t ui = (int64_t)vb * upb + c;
+    const int32_t u0 = st_ptr[ui], nu = st_ptr[ui + 1] - u0;
+    if (nu > 0) {
+      int32_t cv[kStageR];
+#pragma unroll
+      for (int q = 0; q < kStageR; ++q) {
+        const int t = (int)threadIdx.x + q * kPcgBlock;
+        cv[q] = st_col[u0 + (t < nu ? t : nu - 1)];
+      }
+#pragma unroll
+      for (int q = 0; q < kStageR; ++q) {
+        d2[q] = Dinv[cv[q]];
+        r2[q] = r[cv[q]];
+        q2[q] = k == 0 ? T(0) : pin[cv[q]];
+      }
+    }
+    form_set(nu, d2, r2, q2);
+    __syncthreads();
+    unit_rows(c);
   }
   store_partials<1>(red, part1);
 }
@@ -968,11 +1018,11 @@ __host__ __device__ inline size_t rhs_stage_bytes(int nu) {
 // per wave halves each block's chain of dependent loads).  The 16-wave form
 // adds wave w + 8's per-lane sums to wave w's and folds the first 8 waves,
 // which is the 8-wave form's order of additions: the same partial sums.
-template <class T, class XT, class WT, bool LG, int NT>
+template <class T, class XT, class WT, bool LG, int NT, bool UNITS>
 __global__ void __launch_bounds__(NT)
 k_rhs_init_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* __restrict__ lidx,
                   const int32_t* __restrict__ st_ptr, const int32_t* __restrict__ st_own,
-                  const int32_t* __restrict__ st_col, const T* __restrict__ Mv,
+                  const int32_t* __restrict__ st_col, int chunk, const T* __restrict__ Mv,
                   const T* __restrict__ Kv, const T* __restrict__ Dinv, const WT* __restrict__ w,
                   const XT* __restrict__ x, const T* __restrict__ xp, XT* __restrict__ xnext,
                   T* __restrict__ xpnext, T* __restrict__ r, double coef, double Cm,
@@ -982,122 +1032,143 @@ k_rhs_init_staged(int64_t n, const int32_t* __restrict__ sptr, const uint16_t* _
   const Split sp = block_slices(ns);
   const int vb = block_vindex();
   const int lane = threadIdx.x & 63, wv = wave_id();
-  const int32_t u0 = st_ptr[vb], nu = st_ptr[vb + 1] - u0;
-  XT* xs = reinterpret_cast<XT*>(stage_lds);
-  WT* ws = reinterpret_cast<WT*>(stage_lds + (((size_t)nu * sizeof(XT) + 15) & ~(size_t)15));
-  T* ps = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(ws) +
-                               (((size_t)nu * sizeof(WT) + 15) & ~(size_t)15));
+  const int G = gridDim.x;
+  // the block's units (runs of `chunk` slices, or its whole range), staged
+  // one after the other; with 16 waves a unit holds at most 16 slices
+  // (UNITS false: one unit per block, the chunk argument unused)
+  const int64_t per = (ns + G - 1) / G;
+  const int upb = UNITS ? (int)((per + chunk - 1) / chunk) : 1;
+  const int64_t ulen = UNITS ? chunk : per;
   __shared__ int st;
   if (threadIdx.x == 0) st = ctl->stall;
   static_assert(NT == kPcgBlock || NT == 2 * kPcgBlock, "8 or 16 waves");
   constexpr int R = (TC_STAGE_MAX_COLS + NT - 1) / NT;
   constexpr int NW = NT / 64;
-  if (nu > 0) {
-    int32_t cv[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int t = (int)threadIdx.x + q * NT;
-      cv[q] = st_col[u0 + (t < nu ? t : nu - 1)];
-    }
-    XT xv[R];
-    WT wv_[R];
-    T pv[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      xv[q] = x[cv[q]];
-      wv_[q] = w[cv[q]];
-      if (LG) pv[q] = xp[cv[q]];
-    }
-#pragma unroll
-    for (int q = 0; q < R; ++q) {
-      const int t = (int)threadIdx.x + q * NT;
-      if (t < nu) {
-        xs[t] = xv[q];
-        ws[t] = wv_[q];
-        if (LG) ps[t] = pv[q];
-      }
-    }
-  }
-  __syncthreads();
-  if (st) return;
-  const int64_t r0 = sp.s0 * 64;
-  const int32_t own = st_own[vb];
   double red[2] = {0.0, 0.0};
-  // the factors of this lane's last row's terms (0 when it had none)
-  double f_ri = 0.0, f_dri = 0.0, f_db = 0.0;
-  for (int64_t s = sp.s0 + wv; s < sp.s1; s += NW) {
-    const int64_t i = s * 64 + lane;
-    const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
-    const uint16_t* li = lidx + b + lane;
-    const T* m = Mv + b + lane;
-    const T* kk = Kv + b + lane;
-    double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
-    for (int j0 = 0; j0 < wd; j0 += kChunk) {
-      uint16_t lj[kChunk];
-      T mv[kChunk], kv[kChunk];
+  for (int c = 0; c < upb; ++c) {
+    const int64_t ui = (int64_t)vb * upb + c;
+    const int32_t u0 = st_ptr[ui], nu = st_ptr[ui + 1] - u0;
+    XT* xs = reinterpret_cast<XT*>(stage_lds);
+    WT* ws = reinterpret_cast<WT*>(stage_lds + (((size_t)nu * sizeof(XT) + 15) & ~(size_t)15));
+    T* ps = reinterpret_cast<T*>(reinterpret_cast<unsigned char*>(ws) +
+                                 (((size_t)nu * sizeof(WT) + 15) & ~(size_t)15));
+    if (c > 0) __syncthreads();  // every wave is done with the previous unit's LDS
+    if (nu > 0) {
+      int32_t cv[R];
 #pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        const int jj = j0 + u < wd ? j0 + u : wd - 1;
-        lj[u] = li[jj * 64];
-        mv[u] = m[jj * 64];
-        kv[u] = kk[jj * 64];
+      for (int q = 0; q < R; ++q) {
+        const int t = (int)threadIdx.x + q * NT;
+        cv[q] = st_col[u0 + (t < nu ? t : nu - 1)];
+      }
+      XT xv[R];
+      WT wv_[R];
+      T pv[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        xv[q] = x[cv[q]];
+        wv_[q] = w[cv[q]];
+        if (LG) pv[q] = xp[cv[q]];
       }
 #pragma unroll
-      for (int u = 0; u < kChunk; ++u) {
-        if (j0 + u >= wd) break;
-        const int q = lj[u];
-        const XT xq = xs[q];
-        double x0;
-        if (LG) x0 = (double)(XT)(2.0 * (double)xq - (double)ps[q]);
-        else x0 = (double)xq;
-        const double mj = (double)mv[u], kj = (double)kv[u];
-        am += mj * (double)ws[q];
-        if (sizeof(WT) == 8) ak += kj * (double)xq;
-        else ak += kj * (double)(T)xq;
-        mx += mj * x0;
-        kx += kj * x0;
+      for (int q = 0; q < R; ++q) {
+        const int t = (int)threadIdx.x + q * NT;
+        if (t < nu) {
+          xs[t] = xv[q];
+          ws[t] = wv_[q];
+          if (LG) ps[t] = pv[q];
+        }
       }
     }
-    if (i < n) {
-      const int q = own + (int32_t)(i - r0);
-      const XT xi = xs[q];
-      if (LG) {
-        xnext[i] = (XT)(2.0 * (double)xi - (double)ps[q]);
-        xpnext[i] = (T)xi;
+    __syncthreads();
+    if (st) return;
+    const int64_t us0 = sp.s0 + c * ulen;
+    const int64_t us1 = us0 + ulen < sp.s1 ? us0 + ulen : sp.s1;
+    const int64_t r0 = us0 * 64;
+    const int32_t own = st_own[ui];
+    // the factors of this lane's last row's terms in the unit (0 when it
+    // had none)
+    double f_ri = 0.0, f_dri = 0.0, f_db = 0.0;
+    for (int64_t s = us0 + wv; s < us1; s += NW) {
+      const int64_t i = s * 64 + lane;
+      const int32_t b = sptr[s], wd = (sptr[s + 1] - b) >> 6;
+      const uint16_t* li = lidx + b + lane;
+      const T* m = Mv + b + lane;
+      const T* kk = Kv + b + lane;
+      double am = 0.0, ak = 0.0, mx = 0.0, kx = 0.0;
+      for (int j0 = 0; j0 < wd; j0 += kChunk) {
+        uint16_t lj[kChunk];
+        T mv[kChunk], kv[kChunk];
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          const int jj = j0 + u < wd ? j0 + u : wd - 1;
+          lj[u] = li[jj * 64];
+          mv[u] = m[jj * 64];
+          kv[u] = kk[jj * 64];
+        }
+#pragma unroll
+        for (int u = 0; u < kChunk; ++u) {
+          if (j0 + u >= wd) break;
+          const int q = lj[u];
+          const XT xq = xs[q];
+          double x0;
+          if (LG) x0 = (double)(XT)(2.0 * (double)xq - (double)ps[q]);
+          else x0 = (double)xq;
+          const double mj = (double)mv[u], kj = (double)kv[u];
+          am += mj * (double)ws[q];
+          if (sizeof(WT) == 8) ak += kj * (double)xq;
+          else ak += kj * (double)(T)xq;
+          mx += mj * x0;
+          kx += kj * x0;
+        }
       }
-      double bi = am - coef * ak;
-      if (sizeof(WT) != 8) bi = (double)(T)bi;
-      const double d = (double)Dinv[i];
-      const T ri = (T)(bi - ax0(Cm, mx, tdt, kx));
-      r[i] = ri;
-      f_ri = (double)ri;
-      f_dri = d * (double)ri;
-      f_db = d * bi;
-      red[0] = __builtin_fma(f_ri, f_dri, red[0]);
-      red[1] = __builtin_fma(f_db, f_db, red[1]);
+      if (i < n) {
+        const int q = own + (int32_t)(i - r0);
+        const XT xi = xs[q];
+        if (LG) {
+          xnext[i] = (XT)(2.0 * (double)xi - (double)ps[q]);
+          xpnext[i] = (T)xi;
+        }
+        double bi = am - coef * ak;
+        if (sizeof(WT) != 8) bi = (double)(T)bi;
+        const double d = (double)Dinv[i];
+        const T ri = (T)(bi - ax0(Cm, mx, tdt, kx));
+        r[i] = ri;
+        f_ri = (double)ri;
+        f_dri = d * (double)ri;
+        f_db = d * bi;
+        if (NW == kWaves || wv < kWaves) {
+          red[0] = __builtin_fma(f_ri, f_dri, red[0]);
+          red[1] = __builtin_fma(f_db, f_db, red[1]);
+        }
+      }
+    }
+    if constexpr (NT != kPcgBlock) {
+      // waves 8..15 (one row per lane at most in a unit) hand that row's
+      // factors to waves 0..7 (through the LDS of the set, free once every
+      // wave has read it), which fuse them into their sums as the 8-wave
+      // block's second slice of the unit does
+      double* comb = reinterpret_cast<double*>(stage_lds);
+      __syncthreads();
+      if (wv >= kWaves) {
+        double* cb = comb + ((wv - kWaves) * 64 + lane) * 3;
+        cb[0] = f_ri;
+        cb[1] = f_dri;
+        cb[2] = f_db;
+      }
+      __syncthreads();
+      if (wv < kWaves) {
+        const double* cb = comb + (wv * 64 + lane) * 3;
+        red[0] = __builtin_fma(cb[0], cb[1], red[0]);
+        red[1] = __builtin_fma(cb[2], cb[2], red[1]);
+      }
     }
   }
   if constexpr (NT == kPcgBlock) {
     store_partials<2>(red, part0);
   } else {
-    // waves 8..15 (one row per lane at most) hand that row's factors to
-    // waves 0..7 (through the LDS of the set, free once every wave has
-    // read it), which fuse them into their sums as the 8-wave block's
-    // second slice does, and fold as the 8-wave block does
-    double* comb = reinterpret_cast<double*>(stage_lds);
-    __syncthreads();
-    if (wv >= kWaves) {
-      double* c = comb + ((wv - kWaves) * 64 + lane) * 3;
-      c[0] = f_ri;
-      c[1] = f_dri;
-      c[2] = f_db;
-    }
-    __syncthreads();
+    // the first 8 waves fold as the 8-wave block does
     __shared__ double wsum[2][kWaves];
     if (wv < kWaves) {
-      const double* c = comb + (wv * 64 + lane) * 3;
-      red[0] = __builtin_fma(c[0], c[1], red[0]);
-      red[1] = __builtin_fma(c[2], c[2], red[1]);
       red[0] = wave_sum(red[0]);
       red[1] = wave_sum(red[1]);
       if (lane == 0) {

@@ -169,9 +169,11 @@ int launch_spmv_at(tc_pcg* P, int k, hipStream_t s, hipEvent_t e0, hipEvent_t e1
   T* pout = pp[(k + 1) & 1];
   if (staged(P)) {
     const tc_stage& st = P->stage;
-    launch_lds(k_pcg_spmv_staged<T>, dim3(P->G), dim3(kPcgBlock),
+    launch_lds(st.chunk ? k_pcg_spmv_staged<T, true> : k_pcg_spmv_staged<T, false>,
+               dim3(P->G), dim3(kPcgBlock),
                (size_t)st.max_cols * sizeof(T), s, e0, e1, P->n, P->A.slice_ptr, st.lidx,
-               st.col_ptr, st.own, st.cols, (const T*)P->Av, (const T*)P->Dinv, (const T*)P->r,
+               st.col_ptr, st.own, st.cols, st.chunk, (const T*)P->Av, (const T*)P->Dinv,
+               (const T*)P->r,
                (const T*)pin, pout, (T*)P->Ap, k, P->ctl, (const double*)P->part[0],
                (const double*)P->part[2], P->part[1]);
   } else if (P->A.dcol) {
@@ -320,14 +322,18 @@ void rhs_init_staged_launch(tc_sim* S, int xc, int pc, double coef, double tdt, 
   // their rows' factors over takes 3 doubles per lane of 8 waves, 12 KB,
   // of the set's LDS)
   const int64_t ns = (d.n + 63) / 64;
-  const bool wide = (ns + P->G - 1) / P->G <= 2 * kWaves;
+  // 16 waves when a block's range holds at most 16 slices, or its units
+  // 16 slices (units of 8 slices: one slice per wave of 8)
+  const bool wide = st.chunk ? st.chunk == 2 * kWaves : (ns + P->G - 1) / P->G <= 2 * kWaves;
   constexpr size_t kHandOver = 3 * kWaves * 64 * sizeof(double);
   size_t lds = rhs_stage_bytes<T, XT, WT, LG>(st.max_cols);
   if (wide && lds < kHandOver) lds = kHandOver;
-  auto kern = wide ? k_rhs_init_staged<T, XT, WT, LG, 2 * kPcgBlock>
-                   : k_rhs_init_staged<T, XT, WT, LG, kPcgBlock>;
+  auto kern = st.chunk ? (wide ? k_rhs_init_staged<T, XT, WT, LG, 2 * kPcgBlock, true>
+                               : k_rhs_init_staged<T, XT, WT, LG, kPcgBlock, true>)
+                       : (wide ? k_rhs_init_staged<T, XT, WT, LG, 2 * kPcgBlock, false>
+                               : k_rhs_init_staged<T, XT, WT, LG, kPcgBlock, false>);
   launch_lds(kern, dim3(P->G), dim3(wide ? 2 * kPcgBlock : kPcgBlock), lds, s, e0, e1, d.n,
-             P->A.slice_ptr, st.lidx, st.col_ptr, st.own, st.cols, (const T*)d.Mv,
+             P->A.slice_ptr, st.lidx, st.col_ptr, st.own, st.cols, st.chunk, (const T*)d.Mv,
              (const T*)d.Kv, (const T*)P->Dinv, (const WT*)d.w, (const XT*)d.x[xc],
              LG ? (const T*)d.xp[pc] : (const T*)nullptr, LG ? (XT*)d.x[1 - xc] : (XT*)nullptr,
              LG ? (T*)d.xp[1 - pc] : (T*)nullptr, (T*)P->r, coef, d.Cm, tdt, P->ctl, d.a_tol,
