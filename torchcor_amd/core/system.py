@@ -39,9 +39,9 @@ class P1System:
         """(blocks, chunk) of the staged split for n rows: 8 slices a block
         while that needs at most 1024 blocks (chunk 0: each block stages its
         whole range at once); beyond, at most 1024 blocks (a multiple of 8)
-        of a multiple of 8 slices each, staged in units of 8 slices (sets of
-        512 rows, which keep the RHS pass's x, w and x_prev within one
-        block's LDS)."""
+        of a multiple of 8 slices each, staged in units of 8 slices (512
+        rows: on an unstructured mesh about 3,000 columns, where units of 16
+        slices exceed the 4,096-column stage)."""
         from .. import autotune
         ns = -(-int(n) // 64)
         g8 = autotune.pcg_blocks(n, 8)

@@ -309,8 +309,9 @@ int desired_K(tc_sim* S) {
 }
 
 // Staged gathers in the RHS + PCG start pass: the plan's stage applies and
-// the set's x, w and x_prev fit the LDS of one block.
-constexpr size_t kStageLdsMax = 65536;
+// the set's x, w and x_prev fit the LDS of one block (gfx950: 160 KB a
+// workgroup, the whole CU's LDS).
+constexpr size_t kStageLdsMax = 163840;
 
 template <class T, class XT, class WT, bool LG>
 void rhs_init_staged_launch(tc_sim* S, int xc, int pc, double coef, double tdt, hipStream_t s,
