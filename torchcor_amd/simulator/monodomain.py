@@ -16,9 +16,9 @@ Node order.  When the mesh's own numbering scatters neighbours (bandwidth of
 the pattern above n/8 and at least twice what the new order gives), the
 operators and every per-node array live in a locality order on the device:
 the Cuthill-McKee level structure with each level along a Z-order curve of
-the node coordinates (``level_order``), and in "auto" mode the rows of each
-PCG block's window then sorted by length (``slice_sort``: less SELL padding,
-the same column set a block).  The mesh arrays, stimulus node ids and every
+the node coordinates (``level_order``), and in "auto" mode, when the staged
+split's column sets fit a stage, the rows of each of its windows then sorted
+by length (``slice_sort``: less SELL padding, the same column set a window).  The mesh arrays, stimulus node ids and every
 output (u, frames, AT, RT, M, K, A) stay in the caller's numbering.
 ``reorder`` selects this: "auto" (default), "levels" (always, the level
 order alone), "rcm" (always, plain reverse Cuthill-McKee) or None (never).
