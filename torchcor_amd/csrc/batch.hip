@@ -10,7 +10,7 @@
 // coalesced row of `lanes` consecutive entries.
 //
 // Reductions: every pass block writes per-instance partial sums
-// partials[block * lanes + b]; a fold kernel with one block per instance adds
+// partials[b * blocks + block]; a fold kernel with one block per instance adds
 // them in block order (deterministic) and updates that instance's scalars.
 // Instances that met the stopping rule become inactive: their lanes skip the
 // updates and the folds leave their scalars alone, so each instance follows
@@ -102,7 +102,8 @@ __device__ inline bool bat_halted(const BatCtl* ctl, int B, bool need_active) {
   return h != 0;
 }
 
-// Per-instance block sums of NV values -> part[k * G * lanes + block * lanes + b].
+// Per-instance block sums of NV values -> part[k * G * lanes + b * G + block]:
+// instance-major, so the fold of one instance reads G consecutive values.
 template <int NV>
 __device__ inline void bat_partials(double (&v)[NV], double* part, int lanes) {
   __shared__ double sh[NV][kBW][64];
@@ -121,7 +122,7 @@ __device__ inline void bat_partials(double (&v)[NV], double* part, int lanes) {
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
       for (int w = 0; w < kBW; ++w) s += sh[k][w][threadIdx.x];
-      part[k * GL + (int64_t)blockIdx.x * lanes + threadIdx.x] = s;
+      part[k * GL + (int64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
     }
   }
 }
@@ -452,8 +453,8 @@ k_bat_fold(int kind, int k, int G, int lanes, int B, BatCtl* ctl, const double* 
   for (int q = 0; q < R; ++q) {
     const int g = (int)threadIdx.x + q * kBB;
     const int gc = g < G ? g : G - 1;
-    const double t0 = part[(int64_t)gc * lanes + b];
-    const double t1 = kind != 1 ? part[GL + (int64_t)gc * lanes + b] : 0.0;
+    const double t0 = part[(int64_t)b * G + gc];
+    const double t1 = kind != 1 ? part[GL + (int64_t)b * G + gc] : 0.0;
     x0[q] = g < G ? t0 : 0.0;
     x1[q] = g < G ? t1 : 0.0;
   }
