@@ -596,12 +596,15 @@ def size_sweep(sides, warmup, steps):
         kt = {k: round(ms / c * 1e3, 2) for k, (ms, c) in sim.kernel_times().items() if c}
         S = len(sim.ionic_model.STATES)
         ion_bytes = n * (8 + 8 * S + 8 + 8)
+        B = alg_bytes(n, sim.system.nnz, S, column_bytes(sim.system),
+                      xb=sim._X[0].element_size(), wb=sim._w.element_size())
         line = {"size_sweep": True, "side": N, "n_nodes": n, "nnz": sim.system.nnz,
                 "steps": steps, "warmup": warmup, "t_window_ms": [warmup * 0.01, (warmup + steps) * 0.01],
                 "ms_per_step": round(el / steps * 1e3, 4),
                 "node_timesteps_per_s": n * steps / el,
                 "pcg_iterations_per_step": sum(its) / steps + 1,
                 "kernel_avg_us": kt,
+                "kernel_alg_bytes": B,
                 "ionic_GBps": round(ion_bytes / (kt["ionic"] * 1e-6) / 1e9, 1) if kt.get("ionic") else None,
                 "setup_s": round(t_setup, 2),
                 "peak_device_GB": round(torch.cuda.max_memory_allocated(dev) / 1e9, 2)}
